@@ -1,0 +1,242 @@
+/*
+ * sitk_hip.h -- C ABI of libsitk_hip, an MI355X (gfx950) implementation of the
+ * registration metric-evaluation path of sitk-rs (physwkim/sitk-rs).
+ *
+ * The surface mirrors the reference's device backend, `crate::cuda`
+ * (crates/sitk/src/cuda/mod.rs:35-56): device-resident images and masks with an
+ * upload / to_host residency contract, the bit-exact recursive Gaussian, and
+ * resident mean-squares and Mattes metrics that are built once per pyramid level
+ * and evaluated against any number of point maps. Citations are relative to
+ * crates/sitk/ of the reference.
+ *
+ * Conventions (the reference's, src/cuda/mod.rs:3-18, buffer.rs:12-149):
+ *   - Every function returns an int status (SITKHIP_OK = 0 or a sitkhip_status
+ *     code mirroring a CudaError variant, src/cuda/error.rs:11-108); the message
+ *     of the last failure on the calling thread is sitkhip_last_error().
+ *   - Every operation is synchronous: it has finished on the device when it
+ *     returns, as the reference syncs its one stream before returning.
+ *   - Handles own device memory; *_free releases it. A resident metric takes
+ *     private device-to-device copies of its images and masks, so the caller may
+ *     free them after creation (buffer.rs:143-149, mean_squares.rs:252-253).
+ *   - Inputs are validated on the host before any launch, and refusals are named
+ *     (stage count 1..8, sample index in grid, bins >= 5, resident.rs:598-703).
+ *   - Volumes are 3-D, dimension-0 fastest, float32 on the device
+ *     (image.rs:28-56). Geometry is size/spacing/origin/direction (row-major).
+ *   - One process-wide backend on one device (backend.rs:154-160).
+ */
+#ifndef SITK_HIP_H
+#define SITK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+/* Mirrors CudaError (src/cuda/error.rs:11-108) plus the registration
+ * refusals the resident Mattes constructor raises (registration/mattes.rs:152-168). */
+typedef enum {
+    SITKHIP_OK = 0,
+    SITKHIP_ERR_NO_DEVICE = 1,              /* CudaError::NoDevice */
+    SITKHIP_ERR_DRIVER = 2,                 /* CudaError::Driver (HIP runtime) */
+    SITKHIP_ERR_UNSUPPORTED_PIXEL_TYPE = 3, /* CudaError::UnsupportedPixelType */
+    SITKHIP_ERR_UNSUPPORTED_GEOMETRY = 4,   /* CudaError::UnsupportedGeometry */
+    SITKHIP_ERR_DEGENERATE_INPUT = 5,       /* CudaError::DegenerateInput */
+    SITKHIP_ERR_MASKED_EXPLICIT_POINTS = 6, /* CudaError::MaskedExplicitPoints */
+    SITKHIP_ERR_SAMPLE_INDEX_OUT_OF_GRID = 7, /* CudaError::SampleIndexOutOfGrid */
+    SITKHIP_ERR_POINT_MAP_STAGE_COUNT = 8,  /* CudaError::PointMapStageCount */
+    SITKHIP_ERR_HISTOGRAM_SHAPE = 9,        /* CudaError::HistogramShape */
+    SITKHIP_ERR_NOT_SCALAR = 10,            /* CudaError::NotScalar */
+    SITKHIP_ERR_INVALID_ARGUMENT = 11,      /* null handle/pointer, length mismatch */
+    SITKHIP_ERR_TOO_FEW_BINS = 12,          /* RegistrationError::TooFewHistogramBins */
+    SITKHIP_ERR_CONSTANT_INTENSITY = 13,    /* RegistrationError::ConstantIntensity */
+    SITKHIP_ERR_TOO_MANY_BINS = 14,         /* bins above SITKHIP_MAX_BINS */
+    SITKHIP_ERR_NOT_BUILT = 15              /* library has no code object for this device */
+} sitkhip_status;
+
+const char *sitkhip_status_name(int status);
+/* Message of the last failure on this thread ("" if none). */
+const char *sitkhip_last_error(void);
+
+/* --------------------------------------------------------------- backend */
+/* backend() (src/cuda/backend.rs:154-160): select and initialise `device`
+ * once per process. Idempotent; a second call with another device fails. */
+int sitkhip_backend_init(int device);
+/* The HIP stream every operation of this library is enqueued on, as a
+ * hipStream_t cast to void* (for event timing by a caller). */
+int sitkhip_backend_stream(void **stream);
+int sitkhip_device_name(char *buf, size_t len);
+
+/* ------------------------------------------------------------ pixel types */
+/* The scalar PixelId variants the upload cast accepts (image.rs:93-106). */
+typedef enum {
+    SITKHIP_UINT8 = 0, SITKHIP_INT8 = 1, SITKHIP_UINT16 = 2, SITKHIP_INT16 = 3,
+    SITKHIP_UINT32 = 4, SITKHIP_INT32 = 5, SITKHIP_UINT64 = 6, SITKHIP_INT64 = 7,
+    SITKHIP_FLOAT32 = 8, SITKHIP_FLOAT64 = 9
+} sitkhip_pixel_type;
+
+/* Geometry {size, spacing, origin, direction} (src/cuda/image.rs:122-155). */
+typedef struct {
+    size_t size[3];
+    double spacing[3];
+    double origin[3];
+    double direction[9];
+} sitkhip_geometry;
+
+/* ----------------------------------------------------------------- images */
+typedef struct sitkhip_image_s *sitkhip_image;
+
+/* DeviceImage::upload (image.rs:182-268): copy `len` = prod(size) voxels of
+ * `pixel_type` and cast each on the device as (float)(double)x. */
+int sitkhip_image_upload(int pixel_type, const void *data, size_t len,
+                         const sitkhip_geometry *geometry, sitkhip_image *out);
+/* DeviceImage::filled (image.rs:305-331). */
+int sitkhip_image_filled(const sitkhip_geometry *geometry, float value, sitkhip_image *out);
+/* DeviceImage::to_host (image.rs:269-284): `len` must equal the voxel count. */
+int sitkhip_image_to_host(sitkhip_image img, float *out, size_t len);
+int sitkhip_image_geometry(sitkhip_image img, sitkhip_geometry *out);
+size_t sitkhip_image_len(sitkhip_image img);
+/* A process-unique identity (image.rs:364-368). */
+uint64_t sitkhip_image_id(sitkhip_image img);
+/* Device address of the float32 voxels (valid until sitkhip_image_free). */
+const float *sitkhip_image_device_ptr(sitkhip_image img);
+void sitkhip_image_free(sitkhip_image img);
+
+/* ------------------------------------------------------------------ masks */
+typedef struct sitkhip_mask_s *sitkhip_mask;
+
+/* DeviceMask::upload (mask.rs:93-117): nonzero is inside. */
+int sitkhip_mask_upload(const uint8_t *data, size_t len, const sitkhip_geometry *geometry,
+                        sitkhip_mask *out);
+/* DeviceMask::from_device_image (mask.rs:118-150): voxel != 0.0. */
+int sitkhip_mask_from_image(sitkhip_image img, sitkhip_mask *out);
+/* DeviceMask::intersect (mask.rs:151-186): same geometry required. */
+int sitkhip_mask_intersect(sitkhip_mask a, sitkhip_mask b, sitkhip_mask *out);
+int sitkhip_mask_to_host(sitkhip_mask m, uint8_t *out, size_t len);
+void sitkhip_mask_free(sitkhip_mask m);
+
+/* ---------------------------------------------------------------- pyramid */
+/* recursive_gaussian (src/cuda/ops/pyramid.rs:217-293): ZeroOrder Deriche
+ * smoothing with physical `sigma` per axis, axes in order 0,1,2, sigma <= 0
+ * skips an axis, a smoothed axis needs >= 4 voxels. Bit-exact against the
+ * reference's host filter (src/filters/recursive_gaussian.rs:171-204,472-528):
+ * FP64 recursion, one narrowing to float at the end. Output geometry = input. */
+int sitkhip_recursive_gaussian(sitkhip_image in, const double sigma[3], sitkhip_image *out);
+
+/* ------------------------------------------------------- resident metrics */
+#define SITKHIP_MAX_STAGES 8 /* MAX_STAGES (src/cuda/ops/resident.rs:74) */
+#define SITKHIP_MAX_BINS 255 /* fixed Parzen bins are held as uint8 per sample */
+
+typedef enum { SITKHIP_METRIC_MEAN_SQUARES = 0, SITKHIP_METRIC_MATTES = 1 } sitkhip_metric_kind;
+/* Per-sample arithmetic of the sampler. F32 (default): point map and every
+ * discrete decision (inside test, cell floor, mask round) in FP64, the
+ * trilinear weights / value / gradient and the Parzen window in FP32.
+ * F64: everything in FP64 in the host's operation order. Reductions are FP64
+ * (moments) or 64-bit fixed point (histogram) in both. */
+typedef enum { SITKHIP_PRECISION_F32 = 0, SITKHIP_PRECISION_F64 = 1 } sitkhip_precision;
+
+/* MattesBins (src/cuda/ops/mattes.rs:129-143): derived by the host from the
+ * value ranges, handed to the device. */
+typedef struct {
+    size_t bins;
+    size_t padding;
+    double fixed_bin_size;
+    double moving_bin_size;
+    double fixed_normalized_min;
+    double moving_normalized_min;
+    double moving_true_min;
+    double moving_true_max;
+} sitkhip_mattes_bins;
+
+typedef struct sitkhip_metric_s *sitkhip_metric;
+
+/* ResidentMetric::from_device_masked / ResidentMattes::from_device_masked
+ * (mean_squares.rs:258-312, ops/mattes.rs:526-572).
+ *   fixed_indices == NULL: FixedPoints::Grid (every voxel, dim-0-fastest);
+ *   otherwise FixedPoints::Indices: sample s is fixed voxel fixed_indices[s]
+ *     (duplicates count twice; every index must name a voxel, else
+ *     SITKHIP_ERR_SAMPLE_INDEX_OUT_OF_GRID).
+ *   fixed_mask / moving_mask may be NULL.
+ * The metric copies everything it needs; the handles may be freed after. */
+int sitkhip_metric_create(int kind, sitkhip_image fixed, const int64_t *fixed_indices,
+                          size_t n_indices, sitkhip_mask fixed_mask, sitkhip_image moving,
+                          sitkhip_mask moving_mask, int precision, sitkhip_metric *out);
+void sitkhip_metric_free(sitkhip_metric m);
+/* sample_count (mean_squares.rs:313-318): samples the metric walks (after the
+ * fixed mask). */
+int sitkhip_metric_sample_count(sitkhip_metric m, size_t *out);
+/* volume_bytes (mean_squares.rs:319-328): device bytes the metric holds. */
+int sitkhip_metric_volume_bytes(sitkhip_metric m, size_t *out);
+/* value_ranges (ops/mattes.rs:587-600): (min, max) of the fixed sample set and
+ * of the moving volume over its mask; (0,0) when empty. */
+int sitkhip_metric_value_ranges(sitkhip_metric m, double fixed_range[2], double moving_range[2]);
+
+/* MattesGeometry::new (registration/mattes.rs:152-186) on the host: refuses
+ * bins < 5 and constant ranges by name. padding is 2. */
+int sitkhip_mattes_bins_from_ranges(const double fixed_range[2], const double moving_range[2],
+                                    size_t bins, sitkhip_mattes_bins *out);
+/* Install the histogram geometry the Mattes evaluations below use (bins <=
+ * SITKHIP_MAX_BINS). Precomputes every sample's fixed Parzen bin in FP64. */
+int sitkhip_mattes_set_bins(sitkhip_metric m, const sitkhip_mattes_bins *bins);
+
+/* Point maps: `stages` is nstage * 12 doubles, each a PointStage (matrix
+ * row-major 3x3 then offset[3], resident.rs:60-65), applied in order
+ * p <- M p + t (matrix_offset.rs:102-109). 1 <= nstage <= 8. */
+
+/* ResidentMattes::joint_histogram (ops/mattes.rs:697-730): unnormalised joint
+ * histogram (bins^2, row-major [fixed*bins + moving]), fixed marginal (bins)
+ * and valid-sample count. Accumulated in 64-bit fixed point (2^-32 units),
+ * returned as double. */
+int sitkhip_mattes_joint_histogram(sitkhip_metric m, const double *stages, int nstage,
+                                   double *joint, double *marginal, size_t *valid);
+/* ResidentMattes::derivative_moments (ops/mattes.rs:737-841): the 12
+ * pRatio-weighted moments A[3], B[3][3] (row-major [d][e]) of the second pass,
+ * against the caller's pratio table (bins^2, pRatio * n_factor). */
+int sitkhip_mattes_derivative_moments(sitkhip_metric m, const double *stages, int nstage,
+                                      const double *pratio, double moments[12]);
+/* Fused value: histogram pass + the Mattes tail (mattes.rs:273-328) on the
+ * device. value = -MI, or DBL_MAX with valid = 0 for a degenerate histogram. */
+int sitkhip_mattes_value(sitkhip_metric m, const double *stages, int nstage, double *value,
+                         size_t *valid);
+/* Fused value and derivative moments: histogram pass, device tail, derivative
+ * pass, with no host round trip between them. */
+int sitkhip_mattes_value_and_moments(sitkhip_metric m, const double *stages, int nstage,
+                                     double *value, double moments[12], size_t *valid);
+
+/* ResidentMetric::evaluate (mean_squares.rs:329-439): the 14 moments
+ * [sq, S0[3], S1[3][3] row-major [d][e], count] of the mean-squares metric,
+ * with diff = M - F. */
+int sitkhip_ms_moments(sitkhip_metric m, const double *stages, int nstage, double moments[14]);
+/* Value-only mean squares: sum of diff^2 and the valid count. */
+int sitkhip_ms_value(sitkhip_metric m, const double *stages, int nstage, double *sq,
+                     size_t *valid);
+
+/* ------------------------------------------------- host contractions */
+/* The Jacobian's affine decomposition J(x) = J0 + sum_e x_e C_e
+ * (registration/cuda.rs:294-363): j0 is 3 x nparams row-major, c is
+ * 3 (e) x 3 (d) x nparams. */
+/* contract (registration/cuda.rs:381-412): value = sq/count, deriv_k =
+ * (2/count) (J0 . S0 + C . S1). count 0 gives DBL_MAX and a zero derivative. */
+int sitkhip_contract_ms(const double moments[14], const double *j0, const double *c,
+                        int nparams, double *value, double *deriv, size_t *valid);
+/* contract_mattes (registration/cuda.rs:505-524). */
+int sitkhip_contract_mattes(const double moments[12], const double *j0, const double *c,
+                            int nparams, double *deriv);
+
+/* -------------------------------------------------------------- profiling */
+/* When enabled, every kernel launch of this library is bracketed by HIP events
+ * on the library stream and its duration accumulated per kernel name. */
+int sitkhip_profile_enable(int on);
+int sitkhip_profile_reset(void);
+/* Accumulated (count, total milliseconds) of kernels whose name starts with
+ * `prefix`; returns the number of distinct kernel names matched. */
+int sitkhip_profile_query(const char *prefix, uint64_t *count, double *total_ms);
+/* Writes "name count total_ms\n" lines into buf. */
+int sitkhip_profile_dump(char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,116 @@
+// Shared host-side plumbing of libsitk_hip: status codes, the per-thread error
+// message, the process-wide backend (device + stream), device allocations and
+// the optional per-kernel event profiler.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/sitk_hip.h"
+
+namespace sitkhip {
+
+// A failure carried out of the C++ layer to the C ABI: a status code (one of
+// sitkhip_status) and a message, the way CudaError carries its variant and
+// text (crates/sitk/src/cuda/error.rs:11-108).
+struct Error {
+    int status;
+    std::string message;
+};
+
+[[noreturn]] void fail(int status, const std::string &message);
+void set_last_error(const std::string &message);
+
+// Throws Error{SITKHIP_ERR_DRIVER} naming the call and the HIP error string.
+void check_hip(hipError_t e, const char *what, const char *file, int line);
+#define HIP_OK(call) ::sitkhip::check_hip((call), #call, __FILE__, __LINE__)
+
+// backend() (crates/sitk/src/cuda/backend.rs:154-160): one device, one stream,
+// initialised once per process. Throws SITKHIP_ERR_NO_DEVICE without a GPU.
+struct Backend {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    std::string name;
+};
+Backend &backend();
+
+// RAII device allocation (crates/sitk/src/cuda/buffer.rs:12-149). A zero-length
+// request allocates one element, so a handle is never null
+// (crates/sitk/src/cuda/ops/resident.rs:544-546).
+template <typename T>
+struct DeviceBuffer {
+    T *ptr = nullptr;
+    size_t len = 0;
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t n) { alloc(n); }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    DeviceBuffer(DeviceBuffer &&o) noexcept : ptr(o.ptr), len(o.len) { o.ptr = nullptr; o.len = 0; }
+    DeviceBuffer &operator=(DeviceBuffer &&o) noexcept {
+        if (this != &o) { release(); ptr = o.ptr; len = o.len; o.ptr = nullptr; o.len = 0; }
+        return *this;
+    }
+    ~DeviceBuffer() { release(); }
+    void alloc(size_t n) {
+        release();
+        size_t bytes = (n == 0 ? 1 : n) * sizeof(T);
+        HIP_OK(hipMalloc(reinterpret_cast<void **>(&ptr), bytes));
+        len = n;
+    }
+    void release() {
+        if (ptr) { (void)hipFree(ptr); ptr = nullptr; }
+        len = 0;
+    }
+    size_t bytes() const { return len * sizeof(T); }
+    void upload(const T *host, size_t n) {
+        HIP_OK(hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, backend().stream));
+    }
+    void download(T *host, size_t n) const {
+        HIP_OK(hipMemcpyAsync(host, ptr, n * sizeof(T), hipMemcpyDeviceToHost, backend().stream));
+    }
+};
+
+void sync();
+
+// ---------------------------------------------------------------- profiler
+// When enabled, launches made through Launch bracket themselves with events on
+// the library stream; durations are folded per kernel name at query time.
+bool profiling_enabled();
+void profile_record(const char *name, hipEvent_t start, hipEvent_t stop);
+hipEvent_t profile_event();
+
+struct LaunchScope {
+    const char *name;
+    hipEvent_t start = nullptr;
+    explicit LaunchScope(const char *n) : name(n) {
+        if (profiling_enabled()) {
+            start = profile_event();
+            HIP_OK(hipEventRecord(start, backend().stream));
+        }
+    }
+    ~LaunchScope() noexcept(false) {
+        if (start) {
+            hipEvent_t stop = profile_event();
+            HIP_OK(hipEventRecord(stop, backend().stream));
+            profile_record(name, start, stop);
+        }
+    }
+};
+
+// Checks the launch itself (bad configuration) right after it is enqueued.
+void check_launch(const char *name);
+
+// Number of workgroups for a grid-stride kernel over `n` items, capped.
+inline unsigned grid_for(size_t n, unsigned block, unsigned cap) {
+    size_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return static_cast<unsigned>(g);
+}
+
+}  // namespace sitkhip

@@ -1,0 +1,292 @@
+// Recursive (Deriche/Farneback 4th-order IIR) Gaussian smoothing on the device,
+// bit-exact against the reference's host filter.
+//
+// Reference: crates/sitk/src/filters/recursive_gaussian.rs:171-204 (axis loop),
+// :472-528 (filter_line), crates/sitk/src/core/deriche.rs:187-347 (the 20
+// coefficients), crates/sitk/src/cuda/ops/pyramid.rs:217-293 (device entry).
+//
+// Bit-exactness rests on three things, all kept here:
+//   1. The recursion runs in FP64 with every multiply and add rounded separately
+//      (this file is compiled with -ffp-contract=off) and in the reference's
+//      left-to-right term order, one line per lane.
+//   2. The coefficients are computed on the host with the platform libm, in the
+//      reference's expression order, and handed to the kernel.
+//   3. Intermediate axes stay FP64; the only narrowing is (float) of the last
+//      smoothed axis' output, which equals the reference's single `as f32`.
+//
+// Layout: volumes are dimension-0 fastest. A smoothed axis is one launch. Axis 1
+// and 2 lanes own lines that are adjacent in dimension 0, so every step of the
+// recursion is one coalesced 64-lane access. Axis 0 lines are contiguous, so
+// lanes would stride by a whole row; that axis is staged through LDS in chunks
+// of CHUNK elements so global traffic stays row-contiguous.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "common.hpp"
+#include "image.hpp"
+
+namespace sitkhip {
+
+// The 20 coefficients in to_array order (deriche.rs:290-296):
+// [n0..n3, d1..d4, m1..m4, bn1..bn4, bm1..bm4].
+struct IirCoef {
+    double c[20];
+};
+
+// ----------------------------------------------------------- host: coefficients
+// Poles and the ZeroOrder Farneback column (deriche.rs:112-117, 135-140).
+namespace {
+constexpr double kW1 = 0.6681, kL1 = -1.3932, kW2 = 2.0787, kL2 = -1.3732;
+constexpr double kA1 = 1.3530, kB1 = 1.8151, kA2 = -0.3531, kB2 = 0.0902;
+}  // namespace
+
+// ZeroOrder coefficients for index-space sigma `sigmad` (deriche.rs:187-280
+// with compute_d_coefficients :304-319 and compute_n_coefficients :325-347).
+// The same products and sums as the reference, in the same order.
+IirCoef zero_order_coefficients(double sigmad) {
+    const double cos1 = std::cos(kW1 / sigmad), cos2 = std::cos(kW2 / sigmad);
+    const double sin1 = std::sin(kW1 / sigmad), sin2 = std::sin(kW2 / sigmad);
+    const double e1 = std::exp(kL1 / sigmad), e2 = std::exp(kL2 / sigmad);
+
+    const double d4 = e1 * e1 * e2 * e2;
+    const double d3 = -2.0 * cos1 * e1 * e2 * e2 - 2.0 * cos2 * e2 * e1 * e1;
+    const double d2 = 4.0 * cos2 * cos1 * e1 * e2 + e1 * e1 + e2 * e2;
+    const double d1 = -2.0 * (e2 * cos2 + e1 * cos1);
+    const double sd = 1.0 + d1 + d2 + d3 + d4;
+
+    double n0 = kA1 + kA2;
+    double n1 = e2 * (kB2 * sin2 - (kA2 + 2.0 * kA1) * cos2) +
+                e1 * (kB1 * sin1 - (kA1 + 2.0 * kA2) * cos1);
+    double n2 = ((kA1 + kA2) * cos2 * cos1 - kB1 * cos2 * sin1 - kB2 * cos1 * sin2) * 2.0 * e1 * e2 +
+                kA2 * e1 * e1 + kA1 * e2 * e2;
+    double n3 = e2 * e1 * e1 * (kB2 * sin2 - kA2 * cos2) + e1 * e2 * e2 * (kB1 * sin1 - kA1 * cos1);
+    const double sn_raw = n0 + n1 + n2 + n3;
+
+    // Unity DC gain.
+    const double alpha0 = 2.0 * sn_raw / sd - n0;
+    n0 = n0 / alpha0;
+    n1 = n1 / alpha0;
+    n2 = n2 / alpha0;
+    n3 = n3 / alpha0;
+
+    // Symmetric anti-causal numerator and the boundary coefficients.
+    const double m1 = n1 - d1 * n0, m2 = n2 - d2 * n0, m3 = n3 - d3 * n0, m4 = -d4 * n0;
+    const double sn = n0 + n1 + n2 + n3;
+    const double sm = m1 + m2 + m3 + m4;
+
+    IirCoef k;
+    double *c = k.c;
+    c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+    c[4] = d1; c[5] = d2; c[6] = d3; c[7] = d4;
+    c[8] = m1; c[9] = m2; c[10] = m3; c[11] = m4;
+    c[12] = d1 * sn / sd; c[13] = d2 * sn / sd; c[14] = d3 * sn / sd; c[15] = d4 * sn / sd;
+    c[16] = d1 * sm / sd; c[17] = d2 * sm / sd; c[18] = d3 * sm / sd; c[19] = d4 * sm / sd;
+    return k;
+}
+
+// ------------------------------------------------------------------- device
+template <typename T>
+__device__ __forceinline__ double widen(T v) { return static_cast<double>(v); }
+
+template <typename T>
+__device__ __forceinline__ T narrow(double v);
+template <>
+__device__ __forceinline__ double narrow<double>(double v) { return v; }
+template <>
+__device__ __forceinline__ float narrow<float>(double v) { return static_cast<float>(v); }
+
+// The line geometry of one smoothed axis.
+struct LineSet {
+    long long nlines;
+    int len;             // elements per line
+    long long stride;    // element stride along the line
+    long long n0, n01;   // for decoding a line index
+    int axis;
+};
+
+__device__ __forceinline__ long long line_start(const LineSet &L, long long l) {
+    if (L.axis == 0) return l * L.n0;                                 // l = i1 + i2*n1
+    if (L.axis == 1) return (l % L.n0) + (l / L.n0) * L.n01;          // l = i0 + i2*n0
+    return l;                                                         // l = i0 + i1*n0
+}
+
+// One line per lane, straight from global memory (axes 1 and 2: adjacent lanes
+// touch adjacent addresses at every step).
+//
+// Causal pass: writes outs[k] to tmp. Anti-causal pass: walks back, and writes
+// dst[k] = narrow(outs[k] + scratch[k]) as soon as scratch[k] exists -- the
+// same single addition the reference's final loop performs (:525-527).
+// tmp may alias dst when DstT is double (each element is read before it is
+// overwritten, by the same lane).
+template <typename SrcT, typename DstT>
+__global__ __launch_bounds__(256) void iir_lines(const SrcT *__restrict__ src, double *tmp,
+                                                 DstT *dst, LineSet L, IirCoef k) {
+    const long long l = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (l >= L.nlines) return;
+    const long long s0 = line_start(L, l);
+    const long long st = L.stride;
+    const int ln = L.len;
+    const double *c = k.c;
+    const double n0 = c[0], n1 = c[1], n2 = c[2], n3 = c[3];
+    const double d1 = c[4], d2 = c[5], d3 = c[6], d4 = c[7];
+    const double m1 = c[8], m2 = c[9], m3 = c[10], m4 = c[11];
+
+    // ---- causal (recursive_gaussian.rs:476-493)
+    const double v1 = widen(src[s0]);
+    const double x1 = widen(src[s0 + st]);
+    const double x2 = widen(src[s0 + 2 * st]);
+    const double x3 = widen(src[s0 + 3 * st]);
+    double o0 = v1 * n0 + v1 * n1 + v1 * n2 + v1 * n3;
+    double o1 = x1 * n0 + v1 * n1 + v1 * n2 + v1 * n3;
+    double o2 = x2 * n0 + x1 * n1 + v1 * n2 + v1 * n3;
+    double o3 = x3 * n0 + x2 * n1 + x1 * n2 + v1 * n3;
+    o0 -= v1 * c[12] + v1 * c[13] + v1 * c[14] + v1 * c[15];
+    o1 -= o0 * d1 + v1 * c[13] + v1 * c[14] + v1 * c[15];
+    o2 -= o1 * d1 + o0 * d2 + v1 * c[14] + v1 * c[15];
+    o3 -= o2 * d1 + o1 * d2 + o0 * d3 + v1 * c[15];
+    tmp[s0] = o0;
+    tmp[s0 + st] = o1;
+    tmp[s0 + 2 * st] = o2;
+    tmp[s0 + 3 * st] = o3;
+    // window: a = data[i-1], b = data[i-2], cc = data[i-3]; p1..p4 = outs[i-1..i-4]
+    double a = x3, b = x2, cc = x1;
+    double p1 = o3, p2 = o2, p3 = o1, p4 = o0;
+    for (int i = 4; i < ln; ++i) {
+        const double x = widen(src[s0 + i * st]);
+        double o = x * n0 + a * n1 + b * n2 + cc * n3;
+        o -= p1 * d1 + p2 * d2 + p3 * d3 + p4 * d4;
+        tmp[s0 + i * st] = o;
+        cc = b; b = a; a = x;
+        p4 = p3; p3 = p2; p2 = p1; p1 = o;
+    }
+
+    // ---- anti-causal (recursive_gaussian.rs:495-522) and the roll-in (:525-527)
+    const long long e = s0 + static_cast<long long>(ln - 1) * st;
+    const double v2 = widen(src[e]);
+    const double y1 = widen(src[e - st]);      // data[ln-2]
+    const double y2 = widen(src[e - 2 * st]);  // data[ln-3]
+    double s_1 = v2 * m1 + v2 * m2 + v2 * m3 + v2 * m4;          // scratch[ln-1]
+    double s_2 = v2 * m1 + v2 * m2 + v2 * m3 + v2 * m4;          // data[ln-1] is v2
+    double s_3 = y1 * m1 + v2 * m2 + v2 * m3 + v2 * m4;          // scratch[ln-3]
+    double s_4 = y2 * m1 + y1 * m2 + v2 * m3 + v2 * m4;          // scratch[ln-4]
+    s_1 -= v2 * c[16] + v2 * c[17] + v2 * c[18] + v2 * c[19];
+    s_2 -= s_1 * d1 + v2 * c[17] + v2 * c[18] + v2 * c[19];
+    s_3 -= s_2 * d1 + s_1 * d2 + v2 * c[18] + v2 * c[19];
+    s_4 -= s_3 * d1 + s_2 * d2 + s_1 * d3 + v2 * c[19];
+    dst[e] = narrow<DstT>(tmp[e] + s_1);
+    dst[e - st] = narrow<DstT>(tmp[e - st] + s_2);
+    dst[e - 2 * st] = narrow<DstT>(tmp[e - 2 * st] + s_3);
+    dst[e - 3 * st] = narrow<DstT>(tmp[e - 3 * st] + s_4);
+    // window for index j: data[j+1..j+4] = (w1, w2, w3, w4); scratch[j+1..j+4] = (q1..q4)
+    double w2 = y2, w3 = y1, w4 = v2;    // data[ln-3], data[ln-2], data[ln-1]
+    double q1 = s_4, q2 = s_3, q3 = s_2, q4 = s_1;
+    for (int j = ln - 5; j >= 0; --j) {
+        const double w1 = widen(src[s0 + (j + 1) * st]);  // data[j+1]
+        double s = w1 * m1 + w2 * m2 + w3 * m3 + w4 * m4;
+        s -= q1 * d1 + q2 * d2 + q3 * d3 + q4 * d4;
+        dst[s0 + j * st] = narrow<DstT>(tmp[s0 + j * st] + s);
+        w4 = w3; w3 = w2; w2 = w1;
+        q4 = q3; q3 = q2; q2 = q1; q1 = s;
+    }
+}
+
+static LineSet lines_for(const sitkhip_geometry &g, int axis) {
+    LineSet L;
+    long long n0 = static_cast<long long>(g.size[0]);
+    long long n1 = static_cast<long long>(g.size[1]);
+    long long n2 = static_cast<long long>(g.size[2]);
+    long long n = n0 * n1 * n2;
+    L.axis = axis;
+    L.len = static_cast<int>(g.size[axis]);
+    L.nlines = n / static_cast<long long>(g.size[axis]);
+    L.n0 = n0;
+    L.n01 = n0 * n1;
+    L.stride = axis == 0 ? 1 : (axis == 1 ? n0 : n0 * n1);
+    return L;
+}
+
+template <typename SrcT, typename DstT>
+static void launch_axis(const SrcT *src, double *tmp, DstT *dst, const sitkhip_geometry &g, int axis,
+                        const IirCoef &k) {
+    LineSet L = lines_for(g, axis);
+    unsigned blocks = static_cast<unsigned>((L.nlines + 255) / 256);
+    static const char *names[3] = {"gauss_iir_axis0", "gauss_iir_axis1", "gauss_iir_axis2"};
+    LaunchScope scope(names[axis]);
+    hipLaunchKernelGGL((iir_lines<SrcT, DstT>), dim3(blocks), dim3(256), 0, backend().stream, src,
+                       tmp, dst, L, k);
+    check_launch(names[axis]);
+}
+
+__global__ void copy_f32(const float *__restrict__ s, float *__restrict__ d, size_t n) {
+    size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        d[i] = s[i];
+}
+
+// recursive_gaussian (pyramid.rs:217-293 / recursive_gaussian.rs:171-204).
+Image *recursive_gaussian(const Image &in, const double sigma[3]) {
+    for (int d = 0; d < 3; ++d) {
+        if (!(sigma[d] >= 0.0))
+            fail(SITKHIP_ERR_INVALID_ARGUMENT, "recursive_gaussian: sigma must be >= 0");
+    }
+    int axes[3];
+    int na = 0;
+    for (int d = 0; d < 3; ++d) {
+        if (sigma[d] <= 0.0) continue;
+        if (in.geom.size[d] < 4)
+            fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY,
+                 "recursive_gaussian: axis " + std::to_string(d) + " has " +
+                     std::to_string(in.geom.size[d]) + " voxels; the recursion needs at least 4");
+        axes[na++] = d;
+    }
+    const sitkhip_geometry &g = in.geom;
+    size_t n = in.len();
+    Image *out = image_like(g);
+    try {
+        if (na == 0) {
+            LaunchScope scope("copy_f32");
+            hipLaunchKernelGGL(copy_f32, dim3(grid_for(n, 256, 4096)), dim3(256), 0,
+                               backend().stream, in.buf.ptr, out->buf.ptr, n);
+            check_launch("copy_f32");
+            sync();
+            return out;
+        }
+        DeviceBuffer<double> A(n);
+        DeviceBuffer<double> B(na >= 2 ? n : 0);
+        const float *src32 = in.buf.ptr;
+        if (na == 1) {
+            IirCoef k = zero_order_coefficients(sigma[axes[0]] / g.spacing[axes[0]]);
+            launch_axis<float, float>(src32, A.ptr, out->buf.ptr, g, axes[0], k);
+        } else {
+            // first smoothed axis: f32 -> A (causal outputs land in A itself)
+            IirCoef k0 = zero_order_coefficients(sigma[axes[0]] / g.spacing[axes[0]]);
+            launch_axis<float, double>(src32, A.ptr, A.ptr, g, axes[0], k0);
+            double *cur = A.ptr, *other = B.ptr;
+            for (int a = 1; a < na - 1; ++a) {
+                IirCoef k = zero_order_coefficients(sigma[axes[a]] / g.spacing[axes[a]]);
+                launch_axis<double, double>(cur, other, other, g, axes[a], k);
+                double *t = cur; cur = other; other = t;
+            }
+            IirCoef kl = zero_order_coefficients(sigma[axes[na - 1]] / g.spacing[axes[na - 1]]);
+            launch_axis<double, float>(cur, other, out->buf.ptr, g, axes[na - 1], kl);
+        }
+        sync();
+    } catch (...) {
+        delete out;
+        throw;
+    }
+    return out;
+}
+
+}  // namespace sitkhip
+
+extern "C" int sitkhip_recursive_gaussian(sitkhip_image in, const double sigma[3],
+                                          sitkhip_image *out) {
+    return sitkhip::run_guarded([&] {
+        auto img = reinterpret_cast<sitkhip::Image *>(in);
+        if (!img || !sigma || !out)
+            sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "null image, sigma or out-pointer");
+        *out = reinterpret_cast<sitkhip_image>(sitkhip::recursive_gaussian(*img, sigma));
+    });
+}

@@ -1,0 +1,1101 @@
+// Resident mean-squares and Mattes mutual-information metrics.
+//
+// Reference (crates/sitk/):
+//   host metrics   src/registration/metric.rs:1065-1223 (mean squares),
+//                  src/registration/mattes.rs:152-328, 442-676 (Mattes)
+//   device surface src/cuda/ops/mean_squares.rs:186-439 (ResidentMetric),
+//                  src/cuda/ops/mattes.rs:500-934 (ResidentMattes),
+//                  src/cuda/ops/resident.rs:420-773 (sample sets, partials)
+//
+// Design (one process, one GPU, everything resident in HBM):
+//   * The sample set is built once per level: a grid (no list) or a sorted list
+//     of flat fixed indices (uint32). A fixed mask is applied at build time, as
+//     the host's FixedSamples does, so the kernels never see it.
+//   * Per sample, the fixed side is pre-reduced to what the metric reads: the
+//     float32 value (mean squares) or the FP64-computed Parzen bin as uint8
+//     (Mattes), in sample order, so the per-iteration passes stream them.
+//   * Mattes value pass: every workgroup scatters the four cubic-B-spline taps of
+//     each sample into an LDS joint histogram of 64-bit fixed-point integers
+//     (2^-32 units) and integer marginal counts, then folds its non-zero bins
+//     into one global histogram with 64-bit integer atomics. Integer addition is
+//     associative, so the histogram is identical on every run and launch shape.
+//   * Mattes tail: one workgroup turns the integer histogram into -MI and the
+//     pRatio table in FP64 on the device, and clears the histogram for the next
+//     evaluation.
+//   * Mattes derivative pass: pRatio in LDS; each sample's weight
+//     w = sum_taps pRatio * B3'(arg) scales 12 FP64 moments A[d] = sum w dM_d,
+//     B[d][e] = sum w dM_d x_e (the moment form of src/cuda/ops/mattes.rs:75-91).
+//   * Mean squares: 14 FP64 moments in one pass (mean_squares.rs:101-145).
+//   * FP64 moments are reduced per lane, per wavefront, per workgroup into a
+//     fixed number of partials, then folded in workgroup order by one
+//     workgroup: a fixed reduction tree, so results are identical run to run.
+//   * Workgroup b handles a contiguous slice of the samples; slices are
+//     assigned so that the workgroups one XCD runs (b mod 8) hold adjacent
+//     slices and share that XCD's L2 for the moving-volume gathers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "image.hpp"
+#include "sampler.hpp"
+
+namespace sitkhip {
+
+constexpr int kBlock = 256;
+constexpr unsigned kGrid = 1024;      // fixed: the reduction tree must not depend on n
+constexpr int kLdsMaxBins = 90;       // bins^2 * 8 B of LDS per workgroup (<= 64 KB)
+constexpr double kFix = 4294967296.0; // 2^32 fixed-point units per histogram unit
+constexpr double kInvFix = 1.0 / 4294967296.0;
+constexpr uint8_t kNoBin = 255;
+
+struct SampleSet {
+    const uint32_t *idx;  // nullptr: the whole fixed grid in order
+    long long n;
+};
+
+struct MattesParams {
+    int bins;
+    double mbs, mnmin, inv_mbs;  // moving bin size, normalised min, 1/bin size
+    double mmin, mmax;           // moving true range (reject outside)
+    float mmin_f, mmax_f;
+};
+
+// ------------------------------------------------------------- cubic B-spline
+// itk::BSplineKernelFunction<3> and its derivative in the reference's
+// expression order (registration/mattes.rs:96-124).
+template <typename R>
+__device__ __forceinline__ R bspline3(R u) {
+    const R a = fabs(u);
+    if (a < R(1)) {
+        const R sq = a * a;
+        return (R(4) - R(6) * sq + R(3) * sq * a) / R(6);
+    } else if (a < R(2)) {
+        const R sq = a * a;
+        return (R(8) - R(12) * a + R(6) * sq - sq * a) / R(6);
+    }
+    return R(0);
+}
+
+template <typename R>
+__device__ __forceinline__ R bspline3_deriv(R u) {
+    if (u >= R(0) && u < R(1)) return R(-2) * u + R(1.5) * u * u;
+    if (u > R(-1) && u < R(0)) return R(-2) * u - R(1.5) * u * u;
+    if (u >= R(1) && u < R(2)) return R(-2) + R(2) * u - R(0.5) * u * u;
+    if (u > R(-2) && u <= R(-1)) return R(2) + R(2) * u + R(0.5) * u * u;
+    return R(0);
+}
+
+// Workgroup slice of the samples: contiguous, XCD-grouped (see file comment).
+__device__ __forceinline__ void block_slice(long long n, long long &begin, long long &end,
+                                            unsigned &logical) {
+    const unsigned G = gridDim.x;
+    const unsigned b = blockIdx.x;
+    logical = (b % 8u) * (G / 8u) + b / 8u;
+    const long long per = (n + G - 1) / G;
+    begin = static_cast<long long>(logical) * per;
+    end = begin + per;
+    if (end > n) end = n;
+}
+
+__device__ __forceinline__ uint32_t sample_flat(const SampleSet &S, long long s) {
+    return S.idx ? S.idx[s] : static_cast<uint32_t>(s);
+}
+
+// ----------------------------------------------------- per-sample evaluation
+// Locates sample s and interpolates the moving volume there. Returns false for
+// a dropped sample. x: fixed physical point.
+template <typename R, bool GRAD>
+__device__ __forceinline__ bool eval_sample(const FixedGrid &F, const MovingVol &M,
+                                            const Stages &S, uint32_t flat, double x[3], R &mv,
+                                            R gp[3]) {
+    double p[3];
+    fixed_point(F, flat, x);
+    map_point(S, x, p);
+    Cell cell;
+    if (!locate(M, p, cell)) return false;
+    float v[8];
+    corners(M, cell, v);
+    R gi[3];
+    Trilinear<R>::template eval<GRAD>(cell, v, mv, gi);
+    if (GRAD) physical_gradient<R>(M, gi, gp);
+    return true;
+}
+
+// Parzen window of the moving value: the clamped bin index and the four tap
+// arguments (m - term) for m = idx-1 .. idx+2 (mattes.rs:193-206, 464-481).
+template <typename R>
+struct Parzen;
+
+template <>
+struct Parzen<double> {
+    __device__ __forceinline__ static int taps(const MattesParams &P, double mv, double arg[4]) {
+        const double term = mv / P.mbs - P.mnmin;
+        long long idx = static_cast<long long>(term);
+        if (idx < 2) idx = 2;
+        else if (idx > P.bins - 3) idx = P.bins - 3;
+        const int start = static_cast<int>(idx) - 1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) arg[k] = static_cast<double>(start + k) - term;
+        return start;
+    }
+    __device__ __forceinline__ static bool in_range(const MattesParams &P, double mv) {
+        return !(mv < P.mmin || mv > P.mmax);
+    }
+};
+
+template <>
+struct Parzen<float> {
+    __device__ __forceinline__ static int taps(const MattesParams &P, float mv, float arg[4]) {
+        const double term = static_cast<double>(mv) * P.inv_mbs - P.mnmin;
+        long long idx = static_cast<long long>(term);
+        if (idx < 2) idx = 2;
+        else if (idx > P.bins - 3) idx = P.bins - 3;
+        const int start = static_cast<int>(idx) - 1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) arg[k] = static_cast<float>(static_cast<double>(start + k) - term);
+        return start;
+    }
+    __device__ __forceinline__ static bool in_range(const MattesParams &P, float mv) {
+        return !(mv < P.mmin_f || mv > P.mmax_f);
+    }
+};
+
+// B3 >= 0 (a rounding residue below zero becomes 0), scaled to 2^-32 units.
+template <typename R>
+__device__ __forceinline__ unsigned long long to_fixed(R v) {
+    const double d = static_cast<double>(v);
+    return d > 0.0 ? static_cast<unsigned long long>(llrint(d * kFix)) : 0ull;
+}
+
+// ------------------------------------------------------------ FP64 reduction
+// Sum NSLOT per-lane values over the workgroup (fixed shuffle + LDS tree) and
+// write them to partials[logical * NSLOT + k].
+template <int NSLOT>
+__device__ __forceinline__ void block_reduce_store(double (&acc)[NSLOT], double *partials,
+                                                   unsigned logical) {
+    __shared__ double red[kBlock / 64][NSLOT];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) {
+        double v = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NSLOT) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
+        partials[static_cast<size_t>(logical) * NSLOT + threadIdx.x] = t;
+    }
+}
+
+// Fold G partials in logical-workgroup (= sample) order, one lane per slot
+// group: out[k] = sum_b partials[b*NSLOT + k]. Launched as one workgroup.
+template <int NSLOT>
+__global__ __launch_bounds__(kBlock) void fold_partials(const double *__restrict__ partials,
+                                                        unsigned G, double *__restrict__ out) {
+    // 256 lanes: lane t sums partials of blocks t, t+256, ... for every slot,
+    // then a fixed tree over the 256 lane sums.
+    __shared__ double red[kBlock];
+    for (int k = 0; k < NSLOT; ++k) {
+        double t = 0.0;
+        for (unsigned b = threadIdx.x; b < G; b += kBlock) t += partials[static_cast<size_t>(b) * NSLOT + k];
+        red[threadIdx.x] = t;
+        __syncthreads();
+        for (int w = kBlock / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[k] = red[0];
+        __syncthreads();
+    }
+}
+
+// -------------------------------------------------------------- mean squares
+// 14 moments: [sq, S0[3], S1[3][3], count] with diff = M - F
+// (mean_squares.rs:101-145; contraction registration/cuda.rs:381-412).
+template <typename R, bool MOMENTS>
+__global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Stages S, SampleSet SS,
+                                                  const float *__restrict__ fval,
+                                                  double *__restrict__ partials) {
+    constexpr int NS = MOMENTS ? 14 : 2;
+    double acc[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+    long long begin, end;
+    unsigned logical;
+    block_slice(SS.n, begin, end, logical);
+    for (long long s = begin + threadIdx.x; s < end; s += kBlock) {
+        const uint32_t flat = sample_flat(SS, s);
+        double x[3];
+        R mv, g[3];
+        if (!eval_sample<R, MOMENTS>(F, M, S, flat, x, mv, g)) continue;
+        const double diff = static_cast<double>(mv) - static_cast<double>(fval[s]);
+        acc[0] = fma(diff, diff, acc[0]);
+        if (MOMENTS) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double dg = diff * static_cast<double>(g[d]);
+                acc[1 + d] += dg;
+#pragma unroll
+                for (int e = 0; e < 3; ++e) acc[4 + d * 3 + e] = fma(dg, x[e], acc[4 + d * 3 + e]);
+            }
+            acc[13] += 1.0;
+        } else {
+            acc[1] += 1.0;
+        }
+    }
+    block_reduce_store<NS>(acc, partials, logical);
+}
+
+// ------------------------------------------------------------------- Mattes
+// Pass 1: joint histogram (fixed point) + fixed-marginal counts.
+template <typename R, bool LDS>
+__global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, Stages S,
+                                                      SampleSet SS, const uint8_t *__restrict__ fbin,
+                                                      MattesParams P,
+                                                      unsigned long long *__restrict__ g_joint,
+                                                      unsigned long long *__restrict__ g_marg) {
+    extern __shared__ unsigned long long s_hist[];  // bins^2 joint, then bins marginal
+    const int B = P.bins;
+    const int nb = B * B + B;
+    unsigned long long *joint = LDS ? s_hist : g_joint;
+    unsigned long long *marg = LDS ? s_hist + B * B : g_marg;
+    if (LDS) {
+        for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
+        __syncthreads();
+    }
+    long long begin, end;
+    unsigned logical;
+    block_slice(SS.n, begin, end, logical);
+    for (long long s = begin + threadIdx.x; s < end; s += kBlock) {
+        const int fb = fbin[s];
+        if (fb == kNoBin) continue;
+        const uint32_t flat = sample_flat(SS, s);
+        double x[3];
+        R mv, g[3];
+        if (!eval_sample<R, false>(F, M, S, flat, x, mv, g)) continue;
+        if (!Parzen<R>::in_range(P, mv)) continue;
+        R arg[4];
+        const int m0 = Parzen<R>::taps(P, mv, arg);
+        unsigned long long *row = joint + fb * B + m0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const unsigned long long q = to_fixed(bspline3<R>(arg[k]));
+            if (q) atomicAdd(row + k, q);
+        }
+        atomicAdd(marg + fb, 1ull);
+    }
+    if (LDS) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < nb; i += kBlock) {
+            const unsigned long long v = s_hist[i];
+            if (v) atomicAdd(i < B * B ? g_joint + i : g_marg + (i - B * B), v);
+        }
+    }
+}
+
+struct TailOut {
+    double value;
+    double valid;      // as double for the host read-back
+    double joint_sum;
+    double pad;
+};
+
+// The Mattes tail on the device (mattes.rs:273-328): one workgroup.
+// Reads the integer histogram, writes value / valid and the pRatio table
+// (pRatio * n_factor, zero where the value sum skips the bin), then clears the
+// histogram for the next evaluation.
+__global__ __launch_bounds__(1024) void mattes_tail(unsigned long long *__restrict__ g_joint,
+                                                    unsigned long long *__restrict__ g_marg, int B,
+                                                    double mbs, double *__restrict__ pratio,
+                                                    float *__restrict__ pratio_f,
+                                                    TailOut *__restrict__ out) {
+    __shared__ unsigned long long s_u[1024];
+    __shared__ double s_d[1024];
+    __shared__ double s_mm[256];
+    __shared__ double s_logfm[256];
+    __shared__ double s_fm[256];
+    const int t = threadIdx.x;
+    const int nbin = B * B;
+
+    // valid = sum of marginal counts; joint mass = exact integer sum.
+    unsigned long long cnt = 0, mass = 0;
+    for (int i = t; i < B; i += 1024) cnt += g_marg[i];
+    for (int i = t; i < nbin; i += 1024) mass += g_joint[i];
+    s_u[t] = cnt;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w) s_u[t] += s_u[t + w];
+        __syncthreads();
+    }
+    const unsigned long long valid = s_u[0];
+    __syncthreads();
+    s_u[t] = mass;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w) s_u[t] += s_u[t + w];
+        __syncthreads();
+    }
+    const double joint_sum = static_cast<double>(s_u[0]) * kInvFix;
+    __syncthreads();
+
+    const bool degenerate = valid == 0 || joint_sum < DBL_EPSILON;
+    const double inv_sum = degenerate ? 0.0 : 1.0 / joint_sum;
+    const double n_factor = degenerate ? 0.0 : 1.0 / (mbs * static_cast<double>(valid));
+
+    // Marginals (fixed: counts * inv_sum; moving: column sums of the normalised joint).
+    for (int f = t; f < B; f += 1024) {
+        const double fm = static_cast<double>(g_marg[f]) * inv_sum;
+        s_fm[f] = fm;
+        s_logfm[f] = fm > DBL_EPSILON ? log(fm) : 0.0;
+    }
+    for (int m = t; m < B; m += 1024) {
+        double mm = 0.0;
+        for (int f = 0; f < B; ++f) mm += static_cast<double>(g_joint[f * B + m]) * kInvFix * inv_sum;
+        s_mm[m] = mm;
+    }
+    __syncthreads();
+
+    double part = 0.0;
+    for (int i = t; i < nbin; i += 1024) {
+        const int f = i / B, m = i - f * B;
+        const double jp = static_cast<double>(g_joint[i]) * kInvFix * inv_sum;
+        double pr_out = 0.0;
+        if (!degenerate && s_fm[f] > DBL_EPSILON && s_mm[m] > DBL_EPSILON && jp > DBL_EPSILON) {
+            const double pr = log(jp / s_mm[m]);
+            part += jp * (pr - s_logfm[f]);
+            pr_out = pr * n_factor;
+        }
+        pratio[i] = pr_out;
+        pratio_f[i] = static_cast<float>(pr_out);
+    }
+    s_d[t] = part;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w) s_d[t] += s_d[t + w];
+        __syncthreads();
+    }
+    if (t == 0) {
+        out->value = degenerate ? DBL_MAX : -s_d[0];
+        out->valid = static_cast<double>(valid);
+        out->joint_sum = joint_sum;
+    }
+    // Clear for the next evaluation (a kernel, not a memset, so the sequence can
+    // be captured in a graph).
+    __syncthreads();
+    for (int i = t; i < nbin; i += 1024) g_joint[i] = 0ull;
+    for (int i = t; i < B; i += 1024) g_marg[i] = 0ull;
+}
+
+// Pass 2: the 12 pRatio-weighted moments.
+template <typename R, bool LDS>
+__global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M, Stages S,
+                                                       SampleSet SS, const uint8_t *__restrict__ fbin,
+                                                       MattesParams P,
+                                                       const R *__restrict__ g_pratio,
+                                                       double *__restrict__ partials) {
+    extern __shared__ unsigned char s_raw[];
+    R *s_pr = reinterpret_cast<R *>(s_raw);
+    const int B = P.bins;
+    const R *pr = LDS ? s_pr : g_pratio;
+    if (LDS) {
+        for (int i = threadIdx.x; i < B * B; i += kBlock) s_pr[i] = g_pratio[i];
+        __syncthreads();
+    }
+    double acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+    long long begin, end;
+    unsigned logical;
+    block_slice(SS.n, begin, end, logical);
+    for (long long s = begin + threadIdx.x; s < end; s += kBlock) {
+        const int fb = fbin[s];
+        if (fb == kNoBin) continue;
+        const uint32_t flat = sample_flat(SS, s);
+        double x[3];
+        R mv, g[3];
+        if (!eval_sample<R, true>(F, M, S, flat, x, mv, g)) continue;
+        if (!Parzen<R>::in_range(P, mv)) continue;
+        R arg[4];
+        const int m0 = Parzen<R>::taps(P, mv, arg);
+        const R *row = pr + fb * B + m0;
+        R w = R(0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w += row[k] * bspline3_deriv<R>(arg[k]);
+        const double wd = static_cast<double>(w);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double wg = wd * static_cast<double>(g[d]);
+            acc[d] += wg;
+#pragma unroll
+            for (int e = 0; e < 3; ++e) acc[3 + d * 3 + e] = fma(wg, x[e], acc[3 + d * 3 + e]);
+        }
+    }
+    block_reduce_store<12>(acc, partials, logical);
+}
+
+// Fixed Parzen bin per sample, FP64 (mattes.rs:193-206 with the fixed axis'
+// geometry): idx = trunc(v / binSize - normalizedMin) clamped to [2, B-3].
+__global__ void fixed_bins(SampleSet SS, const float *__restrict__ fval_per_sample, int B,
+                           double fbs, double fnmin, uint8_t *__restrict__ out) {
+    size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         s < static_cast<size_t>(SS.n); s += stride) {
+        const double v = static_cast<double>(fval_per_sample[s]);
+        const double term = v / fbs - fnmin;
+        long long idx = static_cast<long long>(term);
+        if (idx < 2) idx = 2;
+        else if (idx > B - 3) idx = B - 3;
+        out[s] = static_cast<uint8_t>(idx);
+    }
+}
+
+__global__ void gather_values(const uint32_t *__restrict__ idx, long long n,
+                              const float *__restrict__ vol, float *__restrict__ out) {
+    size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         s < static_cast<size_t>(n); s += stride)
+        out[s] = vol[idx[s]];
+}
+
+// (min, max, count) partials over values (optionally masked), folded on host
+// in block order. min/max are selections, exact in any order.
+__global__ __launch_bounds__(kBlock) void range_partials(const float *__restrict__ v,
+                                                         const uint8_t *__restrict__ mask, long long n,
+                                                         double *__restrict__ partials) {
+    __shared__ double smin[kBlock], smax[kBlock], scnt[kBlock];
+    double lo = INFINITY, hi = -INFINITY, c = 0.0;
+    size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         i < static_cast<size_t>(n); i += stride) {
+        if (mask && !mask[i]) continue;
+        const double x = static_cast<double>(v[i]);
+        lo = fmin(lo, x);
+        hi = fmax(hi, x);
+        c += 1.0;
+    }
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    scnt[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+            scnt[threadIdx.x] += scnt[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x * 3 + 0] = smin[0];
+        partials[blockIdx.x * 3 + 1] = smax[0];
+        partials[blockIdx.x * 3 + 2] = scnt[0];
+    }
+}
+
+// ------------------------------------------------------------------ host side
+struct Metric {
+    int kind = 0;
+    int precision = 0;
+    FixedGrid F{};
+    MovingVol Mv{};
+    SampleSet SS{};
+    DeviceBuffer<float> moving;
+    DeviceBuffer<uint8_t> mmask;
+    DeviceBuffer<uint32_t> idx;
+    DeviceBuffer<float> fval;     // per-sample fixed values (MS, and the range/bins source)
+    DeviceBuffer<uint8_t> fbin;   // per-sample fixed Parzen bins (Mattes)
+    DeviceBuffer<double> partials;
+    DeviceBuffer<double> moments;  // 14
+    // Mattes
+    bool bins_set = false;
+    sitkhip_mattes_bins bins{};
+    MattesParams P{};
+    DeviceBuffer<unsigned long long> joint, marg;
+    DeviceBuffer<double> pratio;
+    DeviceBuffer<float> pratio_f;
+    DeviceBuffer<TailOut> tail;
+    double fixed_range[2] = {0, 0};
+    double moving_range[2] = {0, 0};
+    size_t nsamples = 0;
+    size_t voxels_fixed = 0;
+};
+
+static void range_of(const float *v, const uint8_t *mask, long long n, double out[2]) {
+    const unsigned G = 512;
+    DeviceBuffer<double> part(G * 3);
+    {
+        LaunchScope scope("range_partials");
+        hipLaunchKernelGGL(range_partials, dim3(G), dim3(kBlock), 0, backend().stream, v, mask, n,
+                           part.ptr);
+        check_launch("range_partials");
+    }
+    std::vector<double> h(G * 3);
+    part.download(h.data(), h.size());
+    sync();
+    double lo = INFINITY, hi = -INFINITY, c = 0.0;
+    for (unsigned b = 0; b < G; ++b) {
+        lo = std::fmin(lo, h[b * 3]);
+        hi = std::fmax(hi, h[b * 3 + 1]);
+        c += h[b * 3 + 2];
+    }
+    if (c == 0.0) { out[0] = 0.0; out[1] = 0.0; }
+    else { out[0] = lo; out[1] = hi; }
+}
+
+static void inverse3(const double m[9], double out[9]) {
+    // Gauss-Jordan with partial pivoting, dividing rows by the pivot
+    // (src/core/matrix.rs:44-102), so a diagonal geometry gives diag(1/spacing).
+    double a[3][6];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) a[r][c] = m[r * 3 + c];
+        for (int c = 0; c < 3; ++c) a[r][3 + c] = r == c ? 1.0 : 0.0;
+    }
+    for (int col = 0; col < 3; ++col) {
+        int piv = col;
+        double best = std::fabs(a[col][col]);
+        for (int r = col + 1; r < 3; ++r)
+            if (std::fabs(a[r][col]) > best) { best = std::fabs(a[r][col]); piv = r; }
+        if (best < 1e-12) fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "singular direction * spacing matrix");
+        if (piv != col)
+            for (int c = 0; c < 6; ++c) std::swap(a[col][c], a[piv][c]);
+        const double diag = a[col][col];
+        for (int c = 0; c < 6; ++c) a[col][c] /= diag;
+        for (int r = 0; r < 3; ++r) {
+            if (r == col) continue;
+            const double f = a[r][col];
+            if (f == 0.0) continue;
+            for (int c = 0; c < 6; ++c) a[r][c] -= f * a[col][c];
+        }
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) out[r * 3 + c] = a[r][3 + c];
+}
+
+static void index_to_physical(const sitkhip_geometry &g, double out[9]) {
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) out[r * 3 + c] = g.direction[r * 3 + c] * g.spacing[c];
+}
+
+Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
+                      const Mask *fmask, const Image &moving, const Mask *mmask, int precision) {
+    if (kind != SITKHIP_METRIC_MEAN_SQUARES && kind != SITKHIP_METRIC_MATTES)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "unknown metric kind");
+    if (precision != SITKHIP_PRECISION_F32 && precision != SITKHIP_PRECISION_F64)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "unknown precision");
+    const sitkhip_geometry &fg = fixed.geom;
+    const sitkhip_geometry &mg = moving.geom;
+    for (int d = 0; d < 3; ++d) {
+        if (fg.size[d] >= (1u << 31) || mg.size[d] >= (1u << 31))
+            fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "axis longer than 2^31 voxels");
+    }
+    const size_t nf = fixed.len();
+    if (nf >= (1ull << 32)) fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "fixed grid above 2^32 voxels");
+    if (fmask && !same_geometry(fmask->geom, fg))
+        fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "fixed mask geometry differs from the fixed image");
+    if (mmask && !same_geometry(mmask->geom, mg))
+        fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "moving mask geometry differs from the moving image");
+
+    auto *m = new Metric();
+    try {
+        m->kind = kind;
+        m->precision = precision;
+        m->voxels_fixed = nf;
+        // Fixed grid geometry.
+        for (int d = 0; d < 3; ++d) {
+            m->F.size[d] = static_cast<uint32_t>(fg.size[d]);
+            m->F.origin[d] = fg.origin[d];
+        }
+        m->F.div0 = make_fastdiv(m->F.size[0]);
+        m->F.div1 = make_fastdiv(m->F.size[1]);
+        index_to_physical(fg, m->F.i2p);
+
+        // Sample list (host-side validation and mask filter, as FixedSamples does).
+        std::vector<uint8_t> hmask;
+        if (fmask) {
+            hmask.resize(nf);
+            fmask->buf.download(hmask.data(), nf);
+            sync();
+        }
+        std::vector<uint32_t> list;
+        bool grid = indices == nullptr;
+        if (!grid) {
+            list.reserve(n_indices);
+            for (size_t s = 0; s < n_indices; ++s) {
+                const int64_t v = indices[s];
+                if (v < 0 || static_cast<uint64_t>(v) >= nf)
+                    fail(SITKHIP_ERR_SAMPLE_INDEX_OUT_OF_GRID,
+                         "fixed sample index " + std::to_string(v) + " is outside the fixed grid (" +
+                             std::to_string(nf) + " voxels)");
+                if (fmask && !hmask[static_cast<size_t>(v)]) continue;
+                list.push_back(static_cast<uint32_t>(v));
+            }
+            // Ascending order: the metric is a sum over the sample multiset, so
+            // the order only changes rounding of the FP64 moments (the histogram
+            // is integer) and it turns the gathers into near-streaming reads.
+            std::sort(list.begin(), list.end());
+        } else if (fmask) {
+            for (size_t s = 0; s < nf; ++s)
+                if (hmask[s]) list.push_back(static_cast<uint32_t>(s));
+            grid = false;
+        }
+        if (grid) {
+            m->SS.idx = nullptr;
+            m->SS.n = static_cast<long long>(nf);
+        } else {
+            m->idx.alloc(list.size());
+            m->idx.upload(list.data(), list.size());
+            m->SS.idx = m->idx.ptr;
+            m->SS.n = static_cast<long long>(list.size());
+        }
+        m->nsamples = static_cast<size_t>(m->SS.n);
+
+        // Per-sample fixed values (grid: a private copy of the fixed volume).
+        m->fval.alloc(m->nsamples);
+        if (grid) {
+            HIP_OK(hipMemcpyAsync(m->fval.ptr, fixed.buf.ptr, nf * sizeof(float),
+                                  hipMemcpyDeviceToDevice, backend().stream));
+        } else if (m->nsamples > 0) {
+            LaunchScope scope("gather_values");
+            hipLaunchKernelGGL(gather_values, dim3(grid_for(m->nsamples, 256, 4096)), dim3(256), 0,
+                               backend().stream, m->idx.ptr, m->SS.n, fixed.buf.ptr, m->fval.ptr);
+            check_launch("gather_values");
+        }
+
+        // Moving volume and mask: private device copies.
+        const size_t nm = moving.len();
+        m->moving.alloc(nm);
+        HIP_OK(hipMemcpyAsync(m->moving.ptr, moving.buf.ptr, nm * sizeof(float),
+                              hipMemcpyDeviceToDevice, backend().stream));
+        if (mmask) {
+            m->mmask.alloc(nm);
+            HIP_OK(hipMemcpyAsync(m->mmask.ptr, mmask->buf.ptr, nm, hipMemcpyDeviceToDevice,
+                                  backend().stream));
+        }
+        m->Mv.vol = m->moving.ptr;
+        m->Mv.mask = mmask ? m->mmask.ptr : nullptr;
+        for (int d = 0; d < 3; ++d) {
+            m->Mv.size[d] = static_cast<int>(mg.size[d]);
+            m->Mv.origin[d] = mg.origin[d];
+        }
+        m->Mv.stride1 = static_cast<long long>(mg.size[0]);
+        m->Mv.stride2 = static_cast<long long>(mg.size[0] * mg.size[1]);
+        double mi2p[9];
+        index_to_physical(mg, mi2p);
+        inverse3(mi2p, m->Mv.p2i);
+
+        m->partials.alloc(kGrid * 14);
+        m->moments.alloc(14);
+        sync();
+
+        // Value ranges (ops/mattes.rs:587-589): fixed over the sample set, moving
+        // over its mask.
+        range_of(m->fval.ptr, nullptr, m->SS.n, m->fixed_range);
+        range_of(m->moving.ptr, m->Mv.mask, static_cast<long long>(nm), m->moving_range);
+    } catch (...) {
+        delete m;
+        throw;
+    }
+    return m;
+}
+
+static void check_stages(int nstage, const double *stages) {
+    if (nstage < 1 || nstage > kMaxStages)
+        fail(SITKHIP_ERR_POINT_MAP_STAGE_COUNT, "the point map has " + std::to_string(nstage) +
+                                                    " stages; the device replays 1..=8");
+    if (!stages) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null stages");
+}
+
+static Stages make_stages(const double *stages, int nstage) {
+    check_stages(nstage, stages);
+    Stages S;
+    S.n = nstage;
+    std::memset(S.s, 0, sizeof(S.s));
+    std::memcpy(S.s, stages, sizeof(double) * 12 * nstage);
+    return S;
+}
+
+void mattes_bins_from_ranges(const double fr[2], const double mr[2], size_t bins,
+                             sitkhip_mattes_bins *out) {
+    // MattesGeometry::new (mattes.rs:152-186).
+    const size_t padding = 2;
+    if (bins < 2 * padding + 1)
+        fail(SITKHIP_ERR_TOO_FEW_BINS, "Mattes needs at least 5 histogram bins, got " +
+                                           std::to_string(bins));
+    if (fr[1] - fr[0] <= DBL_EPSILON)
+        fail(SITKHIP_ERR_CONSTANT_INTENSITY, "the fixed image is constant over the sample set");
+    if (mr[1] - mr[0] <= DBL_EPSILON)
+        fail(SITKHIP_ERR_CONSTANT_INTENSITY, "the moving image is constant over its mask");
+    const double denom = static_cast<double>(bins - 2 * padding);
+    out->bins = bins;
+    out->padding = padding;
+    out->fixed_bin_size = (fr[1] - fr[0]) / denom;
+    out->moving_bin_size = (mr[1] - mr[0]) / denom;
+    out->fixed_normalized_min = fr[0] / out->fixed_bin_size - static_cast<double>(padding);
+    out->moving_normalized_min = mr[0] / out->moving_bin_size - static_cast<double>(padding);
+    out->moving_true_min = mr[0];
+    out->moving_true_max = mr[1];
+}
+
+void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b) {
+    if (m.kind != SITKHIP_METRIC_MATTES) fail(SITKHIP_ERR_INVALID_ARGUMENT, "not a Mattes metric");
+    if (b.bins < 5) fail(SITKHIP_ERR_TOO_FEW_BINS, "Mattes needs at least 5 histogram bins");
+    if (b.bins > SITKHIP_MAX_BINS)
+        fail(SITKHIP_ERR_TOO_MANY_BINS, "at most " + std::to_string(SITKHIP_MAX_BINS) + " bins");
+    if (b.padding != 2) fail(SITKHIP_ERR_HISTOGRAM_SHAPE, "padding must be 2");
+    if (!(b.moving_bin_size > 0.0) || !(b.fixed_bin_size > 0.0))
+        fail(SITKHIP_ERR_HISTOGRAM_SHAPE, "bin sizes must be positive");
+    m.bins = b;
+    const int B = static_cast<int>(b.bins);
+    m.P.bins = B;
+    m.P.mbs = b.moving_bin_size;
+    m.P.inv_mbs = 1.0 / b.moving_bin_size;
+    m.P.mnmin = b.moving_normalized_min;
+    m.P.mmin = b.moving_true_min;
+    m.P.mmax = b.moving_true_max;
+    m.P.mmin_f = static_cast<float>(b.moving_true_min);
+    m.P.mmax_f = static_cast<float>(b.moving_true_max);
+    // float bounds must not shrink the accepted range
+    if (static_cast<double>(m.P.mmin_f) > b.moving_true_min)
+        m.P.mmin_f = std::nextafter(m.P.mmin_f, -INFINITY);
+    if (static_cast<double>(m.P.mmax_f) < b.moving_true_max)
+        m.P.mmax_f = std::nextafter(m.P.mmax_f, INFINITY);
+    m.fbin.alloc(m.nsamples);
+    if (m.nsamples > 0) {
+        LaunchScope scope("fixed_bins");
+        hipLaunchKernelGGL(fixed_bins, dim3(grid_for(m.nsamples, 256, 4096)), dim3(256), 0,
+                           backend().stream, m.SS, m.fval.ptr, B, b.fixed_bin_size,
+                           b.fixed_normalized_min, m.fbin.ptr);
+        check_launch("fixed_bins");
+    }
+    m.joint.alloc(static_cast<size_t>(B) * B);
+    m.marg.alloc(static_cast<size_t>(B));
+    m.pratio.alloc(static_cast<size_t>(B) * B);
+    m.pratio_f.alloc(static_cast<size_t>(B) * B);
+    m.tail.alloc(1);
+    HIP_OK(hipMemsetAsync(m.joint.ptr, 0, m.joint.bytes(), backend().stream));
+    HIP_OK(hipMemsetAsync(m.marg.ptr, 0, m.marg.bytes(), backend().stream));
+    m.bins_set = true;
+    sync();
+}
+
+static void require_bins(const Metric &m) {
+    if (m.kind != SITKHIP_METRIC_MATTES) fail(SITKHIP_ERR_INVALID_ARGUMENT, "not a Mattes metric");
+    if (!m.bins_set)
+        fail(SITKHIP_ERR_HISTOGRAM_SHAPE, "the Mattes histogram geometry has not been set");
+}
+
+template <typename R>
+static void launch_hist(Metric &m, const Stages &S) {
+    const int B = m.P.bins;
+    const bool lds = B <= kLdsMaxBins;
+    const size_t shm = lds ? static_cast<size_t>(B * B + B) * sizeof(unsigned long long) : 0;
+    LaunchScope scope("mattes_hist");
+    if (lds)
+        hipLaunchKernelGGL((mattes_hist<R, true>), dim3(kGrid), dim3(kBlock), shm, backend().stream,
+                           m.F, m.Mv, S, m.SS, m.fbin.ptr, m.P, m.joint.ptr, m.marg.ptr);
+    else
+        hipLaunchKernelGGL((mattes_hist<R, false>), dim3(kGrid), dim3(kBlock), 0, backend().stream,
+                           m.F, m.Mv, S, m.SS, m.fbin.ptr, m.P, m.joint.ptr, m.marg.ptr);
+    check_launch("mattes_hist");
+}
+
+static void launch_tail(Metric &m) {
+    LaunchScope scope("mattes_tail");
+    hipLaunchKernelGGL(mattes_tail, dim3(1), dim3(1024), 0, backend().stream, m.joint.ptr,
+                       m.marg.ptr, m.P.bins, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, m.tail.ptr);
+    check_launch("mattes_tail");
+}
+
+template <typename R>
+static void launch_deriv(Metric &m, const Stages &S, const R *pratio) {
+    const int B = m.P.bins;
+    const size_t shm = static_cast<size_t>(B) * B * sizeof(R);
+    const bool lds = shm + 1024 <= 64 * 1024;  // plus the reduction's static LDS
+    {
+        LaunchScope scope("mattes_deriv");
+        if (lds)
+            hipLaunchKernelGGL((mattes_deriv<R, true>), dim3(kGrid), dim3(kBlock), shm,
+                               backend().stream, m.F, m.Mv, S, m.SS, m.fbin.ptr, m.P, pratio,
+                               m.partials.ptr);
+        else
+            hipLaunchKernelGGL((mattes_deriv<R, false>), dim3(kGrid), dim3(kBlock), 0,
+                               backend().stream, m.F, m.Mv, S, m.SS, m.fbin.ptr, m.P, pratio,
+                               m.partials.ptr);
+        check_launch("mattes_deriv");
+    }
+    LaunchScope scope("fold_partials");
+    hipLaunchKernelGGL((fold_partials<12>), dim3(1), dim3(kBlock), 0, backend().stream,
+                       m.partials.ptr, kGrid, m.moments.ptr);
+    check_launch("fold_partials");
+}
+
+void mattes_joint_histogram(Metric &m, const double *stages, int nstage, double *joint,
+                            double *marginal, size_t *valid) {
+    require_bins(m);
+    Stages S = make_stages(stages, nstage);
+    const int B = m.P.bins;
+    if (m.precision == SITKHIP_PRECISION_F64) launch_hist<double>(m, S);
+    else launch_hist<float>(m, S);
+    std::vector<unsigned long long> hj(static_cast<size_t>(B) * B), hm(B);
+    m.joint.download(hj.data(), hj.size());
+    m.marg.download(hm.data(), hm.size());
+    HIP_OK(hipMemsetAsync(m.joint.ptr, 0, m.joint.bytes(), backend().stream));
+    HIP_OK(hipMemsetAsync(m.marg.ptr, 0, m.marg.bytes(), backend().stream));
+    sync();
+    unsigned long long v = 0;
+    for (int i = 0; i < B * B; ++i)
+        if (joint) joint[i] = static_cast<double>(hj[i]) * kInvFix;
+    for (int i = 0; i < B; ++i) {
+        if (marginal) marginal[i] = static_cast<double>(hm[i]);
+        v += hm[i];
+    }
+    if (valid) *valid = static_cast<size_t>(v);
+}
+
+void mattes_derivative_moments(Metric &m, const double *stages, int nstage, const double *pratio,
+                               double moments[12]) {
+    require_bins(m);
+    if (!pratio) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null pratio");
+    Stages S = make_stages(stages, nstage);
+    const int B = m.P.bins;
+    m.pratio.upload(pratio, static_cast<size_t>(B) * B);
+    if (m.precision == SITKHIP_PRECISION_F64) {
+        launch_deriv<double>(m, S, m.pratio.ptr);
+    } else {
+        std::vector<float> pf(static_cast<size_t>(B) * B);
+        for (size_t i = 0; i < pf.size(); ++i) pf[i] = static_cast<float>(pratio[i]);
+        m.pratio_f.upload(pf.data(), pf.size());
+        launch_deriv<float>(m, S, m.pratio_f.ptr);
+    }
+    m.moments.download(moments, 12);
+    sync();
+}
+
+static void value_pass(Metric &m, const Stages &S) {
+    if (m.precision == SITKHIP_PRECISION_F64) launch_hist<double>(m, S);
+    else launch_hist<float>(m, S);
+    launch_tail(m);
+}
+
+void mattes_value(Metric &m, const double *stages, int nstage, double *value, size_t *valid) {
+    require_bins(m);
+    Stages S = make_stages(stages, nstage);
+    value_pass(m, S);
+    TailOut t;
+    m.tail.download(&t, 1);
+    sync();
+    if (value) *value = t.value;
+    if (valid) *valid = static_cast<size_t>(t.valid);
+}
+
+void mattes_value_and_moments(Metric &m, const double *stages, int nstage, double *value,
+                              double moments[12], size_t *valid) {
+    require_bins(m);
+    Stages S = make_stages(stages, nstage);
+    value_pass(m, S);
+    if (m.precision == SITKHIP_PRECISION_F64) launch_deriv<double>(m, S, m.pratio.ptr);
+    else launch_deriv<float>(m, S, m.pratio_f.ptr);
+    TailOut t;
+    m.tail.download(&t, 1);
+    if (moments) m.moments.download(moments, 12);
+    sync();
+    if (value) *value = t.value;
+    if (valid) *valid = static_cast<size_t>(t.valid);
+}
+
+template <typename R, bool MOM>
+static void launch_ms(Metric &m, const Stages &S) {
+    constexpr int NS = MOM ? 14 : 2;
+    {
+        LaunchScope scope(MOM ? "ms_moments" : "ms_value");
+        hipLaunchKernelGGL((ms_pass<R, MOM>), dim3(kGrid), dim3(kBlock), 0, backend().stream, m.F,
+                           m.Mv, S, m.SS, m.fval.ptr, m.partials.ptr);
+        check_launch("ms_pass");
+    }
+    LaunchScope scope("fold_partials");
+    hipLaunchKernelGGL((fold_partials<NS>), dim3(1), dim3(kBlock), 0, backend().stream,
+                       m.partials.ptr, kGrid, m.moments.ptr);
+    check_launch("fold_partials");
+}
+
+void ms_moments(Metric &m, const double *stages, int nstage, double out[14]) {
+    if (m.kind != SITKHIP_METRIC_MEAN_SQUARES)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "not a mean-squares metric");
+    Stages S = make_stages(stages, nstage);
+    if (m.precision == SITKHIP_PRECISION_F64) launch_ms<double, true>(m, S);
+    else launch_ms<float, true>(m, S);
+    m.moments.download(out, 14);
+    sync();
+}
+
+void ms_value(Metric &m, const double *stages, int nstage, double *sq, size_t *valid) {
+    if (m.kind != SITKHIP_METRIC_MEAN_SQUARES)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "not a mean-squares metric");
+    Stages S = make_stages(stages, nstage);
+    if (m.precision == SITKHIP_PRECISION_F64) launch_ms<double, false>(m, S);
+    else launch_ms<float, false>(m, S);
+    double h[2];
+    m.moments.download(h, 2);
+    sync();
+    if (sq) *sq = h[0];
+    if (valid) *valid = static_cast<size_t>(h[1]);
+}
+
+}  // namespace sitkhip
+
+// =================================================================== C ABI
+using namespace sitkhip;
+
+extern "C" {
+
+int sitkhip_metric_create(int kind, sitkhip_image fixed, const int64_t *fixed_indices,
+                          size_t n_indices, sitkhip_mask fixed_mask, sitkhip_image moving,
+                          sitkhip_mask moving_mask, int precision, sitkhip_metric *out) {
+    return run_guarded([&] {
+        auto f = reinterpret_cast<Image *>(fixed);
+        auto mv = reinterpret_cast<Image *>(moving);
+        if (!f || !mv || !out) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null image or out-pointer");
+        *out = reinterpret_cast<sitkhip_metric>(
+            metric_create(kind, *f, fixed_indices, n_indices, reinterpret_cast<Mask *>(fixed_mask),
+                          *mv, reinterpret_cast<Mask *>(moving_mask), precision));
+    });
+}
+
+void sitkhip_metric_free(sitkhip_metric m) { delete reinterpret_cast<Metric *>(m); }
+
+static Metric &metric_ref(sitkhip_metric h) {
+    auto m = reinterpret_cast<Metric *>(h);
+    if (!m) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null metric");
+    return *m;
+}
+
+int sitkhip_metric_sample_count(sitkhip_metric h, size_t *out) {
+    return run_guarded([&] {
+        if (!out) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null out-pointer");
+        *out = metric_ref(h).nsamples;
+    });
+}
+
+int sitkhip_metric_volume_bytes(sitkhip_metric h, size_t *out) {
+    return run_guarded([&] {
+        if (!out) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null out-pointer");
+        Metric &m = metric_ref(h);
+        *out = m.moving.bytes() + m.mmask.bytes() + m.idx.bytes() + m.fval.bytes() + m.fbin.bytes();
+    });
+}
+
+int sitkhip_metric_value_ranges(sitkhip_metric h, double fixed_range[2], double moving_range[2]) {
+    return run_guarded([&] {
+        Metric &m = metric_ref(h);
+        if (fixed_range) { fixed_range[0] = m.fixed_range[0]; fixed_range[1] = m.fixed_range[1]; }
+        if (moving_range) { moving_range[0] = m.moving_range[0]; moving_range[1] = m.moving_range[1]; }
+    });
+}
+
+int sitkhip_mattes_bins_from_ranges(const double fixed_range[2], const double moving_range[2],
+                                    size_t bins, sitkhip_mattes_bins *out) {
+    return run_guarded([&] {
+        if (!fixed_range || !moving_range || !out)
+            fail(SITKHIP_ERR_INVALID_ARGUMENT, "null argument");
+        mattes_bins_from_ranges(fixed_range, moving_range, bins, out);
+    });
+}
+
+int sitkhip_mattes_set_bins(sitkhip_metric h, const sitkhip_mattes_bins *bins) {
+    return run_guarded([&] {
+        if (!bins) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null bins");
+        mattes_set_bins(metric_ref(h), *bins);
+    });
+}
+
+int sitkhip_mattes_joint_histogram(sitkhip_metric h, const double *stages, int nstage,
+                                   double *joint, double *marginal, size_t *valid) {
+    return run_guarded([&] { mattes_joint_histogram(metric_ref(h), stages, nstage, joint, marginal, valid); });
+}
+
+int sitkhip_mattes_derivative_moments(sitkhip_metric h, const double *stages, int nstage,
+                                      const double *pratio, double moments[12]) {
+    return run_guarded([&] {
+        if (!moments) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null moments");
+        mattes_derivative_moments(metric_ref(h), stages, nstage, pratio, moments);
+    });
+}
+
+int sitkhip_mattes_value(sitkhip_metric h, const double *stages, int nstage, double *value,
+                         size_t *valid) {
+    return run_guarded([&] { mattes_value(metric_ref(h), stages, nstage, value, valid); });
+}
+
+int sitkhip_mattes_value_and_moments(sitkhip_metric h, const double *stages, int nstage,
+                                     double *value, double moments[12], size_t *valid) {
+    return run_guarded(
+        [&] { mattes_value_and_moments(metric_ref(h), stages, nstage, value, moments, valid); });
+}
+
+int sitkhip_ms_moments(sitkhip_metric h, const double *stages, int nstage, double moments[14]) {
+    return run_guarded([&] {
+        if (!moments) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null moments");
+        ms_moments(metric_ref(h), stages, nstage, moments);
+    });
+}
+
+int sitkhip_ms_value(sitkhip_metric h, const double *stages, int nstage, double *sq, size_t *valid) {
+    return run_guarded([&] { ms_value(metric_ref(h), stages, nstage, sq, valid); });
+}
+
+// contract (registration/cuda.rs:381-412).
+int sitkhip_contract_ms(const double mom[14], const double *j0, const double *c, int nparams,
+                        double *value, double *deriv, size_t *valid) {
+    return run_guarded([&] {
+        if (!mom || !j0 || !c || nparams <= 0) fail(SITKHIP_ERR_INVALID_ARGUMENT, "bad arguments");
+        const double count = mom[13];
+        if (valid) *valid = static_cast<size_t>(count);
+        if (count == 0.0) {
+            if (value) *value = DBL_MAX;
+            if (deriv)
+                for (int k = 0; k < nparams; ++k) deriv[k] = 0.0;
+            return;
+        }
+        const double inv = 1.0 / count;
+        if (deriv) {
+            for (int k = 0; k < nparams; ++k) {
+                double g = 0.0;
+                for (int d = 0; d < 3; ++d) {
+                    g += j0[d * nparams + k] * mom[1 + d];
+                    for (int e = 0; e < 3; ++e) g += c[(e * 3 + d) * nparams + k] * mom[4 + d * 3 + e];
+                }
+                deriv[k] = 2.0 * g * inv;
+            }
+        }
+        if (value) *value = mom[0] * inv;
+    });
+}
+
+// contract_mattes (registration/cuda.rs:505-524).
+int sitkhip_contract_mattes(const double mom[12], const double *j0, const double *c, int nparams,
+                            double *deriv) {
+    return run_guarded([&] {
+        if (!mom || !j0 || !c || !deriv || nparams <= 0)
+            fail(SITKHIP_ERR_INVALID_ARGUMENT, "bad arguments");
+        for (int k = 0; k < nparams; ++k) {
+            double g = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                g += j0[d * nparams + k] * mom[d];
+                for (int e = 0; e < 3; ++e) g += c[(e * 3 + d) * nparams + k] * mom[3 + d * 3 + e];
+            }
+            deriv[k] = g;
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,254 @@
+// The per-sample chain shared by every metric kernel: fixed voxel -> physical
+// point -> point map -> moving continuous index -> mask / inside tests ->
+// trilinear value (and gradient) of the moving volume.
+//
+// Reference chain (crates/sitk/):
+//   point      src/registration/scales.rs:259-271 (origin-first fold)
+//   point map  src/transform/matrix_offset.rs:102-109, src/core/matrix.rs:9-21
+//   index      src/registration/metric.rs:795-807
+//   mask gate  src/registration/metric.rs:817-831, src/core/coord.rs:189-191
+//   inside     src/transform/interpolator.rs:29-34
+//   trilinear  src/transform/interpolator.rs:58-149
+//   gradient   src/registration/metric.rs:848-864
+//
+// Everything up to and including the cell floor is FP64 in the reference's
+// operation order (this code is compiled with -ffp-contract=off), so every
+// discrete decision -- inside, mask, which cell -- is the reference's own. The
+// interpolation itself is templated on the arithmetic type R: double follows the
+// reference term for term; float computes weights, value and gradient in FP32
+// from the FP64 fraction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sitkhip {
+
+// Unsigned division by a runtime constant via multiply-high (Granlund &
+// Montgomery 1994): q = (umulhi(n, m) + n) >> l for 32-bit n and d >= 1.
+struct FastDiv {
+    uint32_t d, m, l;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f;
+    f.d = d;
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    f.l = l;
+    f.m = static_cast<uint32_t>(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+    return f;
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) {
+    uint64_t t = static_cast<uint64_t>(__umulhi(n, f.m)) + n;
+    return static_cast<uint32_t>(t >> f.l);
+}
+
+constexpr int kMaxStages = 8;
+
+struct Stages {
+    int n;
+    double s[kMaxStages * 12];  // per stage: matrix row-major [9], offset [3]
+};
+
+struct FixedGrid {
+    uint32_t size[3];
+    FastDiv div0, div1;  // flat -> (i0, i1, i2)
+    double origin[3];
+    double i2p[9];
+};
+
+struct MovingVol {
+    const float *vol;
+    const uint8_t *mask;  // nullptr: no moving mask
+    int size[3];
+    long long stride1, stride2;
+    double origin[3];
+    double p2i[9];
+};
+
+// VirtualGrid::write_point for flat voxel `flat` (origin first, then the three
+// columns in order).
+__device__ __forceinline__ void fixed_point(const FixedGrid &F, uint32_t flat, double x[3]) {
+    uint32_t q0 = fdiv(flat, F.div0);
+    uint32_t i0 = flat - q0 * F.size[0];
+    uint32_t i2 = fdiv(q0, F.div1);
+    uint32_t i1 = q0 - i2 * F.size[1];
+    const double id0 = static_cast<double>(i0), id1 = static_cast<double>(i1),
+                 id2 = static_cast<double>(i2);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double acc = F.origin[r];
+        acc += F.i2p[r * 3 + 0] * id0;
+        acc += F.i2p[r * 3 + 1] * id1;
+        acc += F.i2p[r * 3 + 2] * id2;
+        x[r] = acc;
+    }
+}
+
+// replay_stages: p <- mat_vec(M, p) + t per stage, mat_vec seeded at 0.0.
+__device__ __forceinline__ void map_point(const Stages &S, const double x[3], double p[3]) {
+    double q0 = x[0], q1 = x[1], q2 = x[2];
+    for (int s = 0; s < S.n; ++s) {
+        const double *m = S.s + 12 * s;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        a0 += m[0] * q0; a0 += m[1] * q1; a0 += m[2] * q2;
+        a1 += m[3] * q0; a1 += m[4] * q1; a1 += m[5] * q2;
+        a2 += m[6] * q0; a2 += m[7] * q1; a2 += m[8] * q2;
+        q0 = a0 + m[9];
+        q1 = a1 + m[10];
+        q2 = a2 + m[11];
+    }
+    p[0] = q0; p[1] = q1; p[2] = q2;
+}
+
+// The located sample: which cell of the moving volume and where inside it.
+struct Cell {
+    int base[3];
+    double frac[3];
+};
+
+// continuous_index + mask_allows + is_inside + floor. Returns false for a
+// sample the reference drops (outside the buffer or on a zero mask voxel).
+__device__ __forceinline__ bool locate(const MovingVol &M, const double p[3], Cell &cell) {
+    double c[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double acc = -0.0;  // Rust's f64 sum seeds at -0.0
+        acc += M.p2i[r * 3 + 0] * (p[0] - M.origin[0]);
+        acc += M.p2i[r * 3 + 1] * (p[1] - M.origin[1]);
+        acc += M.p2i[r * 3 + 2] * (p[2] - M.origin[2]);
+        c[r] = acc;
+    }
+    if (M.mask) {
+        long long flat = 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double r = floor(c[d] + 0.5);  // RoundHalfIntegerUp
+            if (!(r >= 0.0) || r >= static_cast<double>(M.size[d])) return false;
+            long long ri = static_cast<long long>(r);
+            flat += d == 0 ? ri : (d == 1 ? ri * M.stride1 : ri * M.stride2);
+        }
+        if (M.mask[flat] == 0) return false;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (!(c[d] >= -0.5 && c[d] < static_cast<double>(M.size[d]) - 0.5)) return false;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double f = floor(c[d]);
+        cell.base[d] = static_cast<int>(f);
+        cell.frac[d] = c[d] - f;
+    }
+    return true;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// The 8 corner values, in corner order (bit d of the corner selects base+1 on
+// axis d), with neighbour indices clamped into the volume.
+__device__ __forceinline__ void corners(const MovingVol &M, const Cell &cell, float v[8]) {
+    int x0 = clampi(cell.base[0], 0, M.size[0] - 1), x1 = clampi(cell.base[0] + 1, 0, M.size[0] - 1);
+    int y0 = clampi(cell.base[1], 0, M.size[1] - 1), y1 = clampi(cell.base[1] + 1, 0, M.size[1] - 1);
+    int z0 = clampi(cell.base[2], 0, M.size[2] - 1), z1 = clampi(cell.base[2] + 1, 0, M.size[2] - 1);
+    const long long r00 = y0 * M.stride1 + z0 * M.stride2;
+    const long long r10 = y1 * M.stride1 + z0 * M.stride2;
+    const long long r01 = y0 * M.stride1 + z1 * M.stride2;
+    const long long r11 = y1 * M.stride1 + z1 * M.stride2;
+    v[0] = M.vol[r00 + x0];
+    v[1] = M.vol[r00 + x1];
+    v[2] = M.vol[r10 + x0];
+    v[3] = M.vol[r10 + x1];
+    v[4] = M.vol[r01 + x0];
+    v[5] = M.vol[r01 + x1];
+    v[6] = M.vol[r11 + x0];
+    v[7] = M.vol[r11 + x1];
+}
+
+// Trilinear value and index-space gradient. Primary template: FP64 in the
+// reference's order (interpolator.rs:103-149): weight = ((1*w0)*w1)*w2,
+// value += weight*b, grad[j] += (sign*w_without_j)*b, every operation rounded.
+template <typename R>
+struct Trilinear;
+
+template <>
+struct Trilinear<double> {
+    template <bool GRAD>
+    __device__ __forceinline__ static void eval(const Cell &cell, const float vf[8], double &value,
+                                                double g[3]) {
+        const double f0 = cell.frac[0], f1 = cell.frac[1], f2 = cell.frac[2];
+        const double u0 = 1.0 - f0, u1 = 1.0 - f1, u2 = 1.0 - f2;
+        double v = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+#pragma unroll
+        for (int corner = 0; corner < 8; ++corner) {
+            const double w0 = (corner & 1) ? f0 : u0;
+            const double w1 = (corner & 2) ? f1 : u1;
+            const double w2 = (corner & 4) ? f2 : u2;
+            const double b = static_cast<double>(vf[corner]);
+            const double weight = (w0 * w1) * w2;
+            v += weight * b;
+            if (GRAD) {
+                const double s0 = (corner & 1) ? 1.0 : -1.0;
+                const double s1 = (corner & 2) ? 1.0 : -1.0;
+                const double s2 = (corner & 4) ? 1.0 : -1.0;
+                g0 += s0 * (w1 * w2) * b;
+                g1 += s1 * (w0 * w2) * b;
+                g2 += s2 * (w0 * w1) * b;
+            }
+        }
+        value = v;
+        if (GRAD) { g[0] = g0; g[1] = g1; g[2] = g2; }
+    }
+};
+
+template <>
+struct Trilinear<float> {
+    template <bool GRAD>
+    __device__ __forceinline__ static void eval(const Cell &cell, const float vf[8], float &value,
+                                                float g[3]) {
+        const float f0 = static_cast<float>(cell.frac[0]);
+        const float f1 = static_cast<float>(cell.frac[1]);
+        const float f2 = static_cast<float>(cell.frac[2]);
+        const float u0 = static_cast<float>(1.0 - cell.frac[0]);
+        const float u1 = static_cast<float>(1.0 - cell.frac[1]);
+        const float u2 = static_cast<float>(1.0 - cell.frac[2]);
+        // Separable form: interpolate along axis 0 first, then 1, then 2.
+        const float a00 = fmaf(f0, vf[1], u0 * vf[0]);
+        const float a10 = fmaf(f0, vf[3], u0 * vf[2]);
+        const float a01 = fmaf(f0, vf[5], u0 * vf[4]);
+        const float a11 = fmaf(f0, vf[7], u0 * vf[6]);
+        const float b0 = fmaf(f1, a10, u1 * a00);
+        const float b1 = fmaf(f1, a11, u1 * a01);
+        value = fmaf(f2, b1, u2 * b0);
+        if (GRAD) {
+            // d/dc0: differences along axis 0, weighted over axes 1, 2.
+            const float d00 = vf[1] - vf[0], d10 = vf[3] - vf[2];
+            const float d01 = vf[5] - vf[4], d11 = vf[7] - vf[6];
+            const float e0 = fmaf(f1, d10, u1 * d00);
+            const float e1 = fmaf(f1, d11, u1 * d01);
+            g[0] = fmaf(f2, e1, u2 * e0);
+            // d/dc1: differences along axis 1 of the axis-0 interpolants.
+            g[1] = fmaf(f2, a11 - a01, u2 * (a10 - a00));
+            // d/dc2
+            g[2] = b1 - b0;
+        }
+    }
+};
+
+// Index-space gradient -> physical gradient: g_d = sum_j gi_j * P[j][d].
+template <typename R>
+__device__ __forceinline__ void physical_gradient(const MovingVol &M, const R gi[3], R gp[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        R acc = static_cast<R>(-0.0);
+        acc += gi[0] * static_cast<R>(M.p2i[0 * 3 + d]);
+        acc += gi[1] * static_cast<R>(M.p2i[1 * 3 + d]);
+        acc += gi[2] * static_cast<R>(M.p2i[2 * 3 + d]);
+        gp[d] = acc;
+    }
+}
+
+}  // namespace sitkhip

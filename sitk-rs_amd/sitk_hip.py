@@ -1,0 +1,451 @@
+"""ctypes binding of libsitk_hip (include/sitk_hip.h).
+
+Mirrors the reference's device surface `crate::cuda` (crates/sitk/src/cuda/mod.rs:35-56)
+with the same names: ``DeviceImage``, ``DeviceMask``, ``recursive_gaussian``,
+``ResidentMetric`` (mean squares) and ``ResidentMattes``. Errors raise ``CudaError``
+carrying the status name of the C ABI, mirroring the reference's ``CudaError``
+variants (crates/sitk/src/cuda/error.rs:11-108).
+
+This module only marshals arguments: every computation runs in the HIP library.
+Loading fails loudly if ``libsitk_hip.so`` has not been built.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsitk_hip.so")
+
+OK = 0
+PIXEL = {
+    np.dtype(np.uint8): 0, np.dtype(np.int8): 1, np.dtype(np.uint16): 2, np.dtype(np.int16): 3,
+    np.dtype(np.uint32): 4, np.dtype(np.int32): 5, np.dtype(np.uint64): 6, np.dtype(np.int64): 7,
+    np.dtype(np.float32): 8, np.dtype(np.float64): 9,
+}
+MEAN_SQUARES, MATTES = 0, 1
+PRECISION_F32, PRECISION_F64 = 0, 1
+MAX_STAGES = 8
+
+
+class CudaError(RuntimeError):
+    """A refused or failed device operation: ``.status`` is the C status name."""
+
+    def __init__(self, status: str, message: str):
+        super().__init__(f"{status}: {message}")
+        self.status = status
+        self.message = message
+
+
+class Geometry(C.Structure):
+    _fields_ = [
+        ("size", C.c_size_t * 3),
+        ("spacing", C.c_double * 3),
+        ("origin", C.c_double * 3),
+        ("direction", C.c_double * 9),
+    ]
+
+    @staticmethod
+    def make(size, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), direction=None):
+        g = Geometry()
+        g.size[:] = [int(v) for v in size]
+        g.spacing[:] = [float(v) for v in spacing]
+        g.origin[:] = [float(v) for v in origin]
+        d = np.eye(3) if direction is None else np.asarray(direction, dtype=np.float64)
+        g.direction[:] = [float(v) for v in d.reshape(9)]
+        return g
+
+
+class MattesBins(C.Structure):
+    _fields_ = [
+        ("bins", C.c_size_t),
+        ("padding", C.c_size_t),
+        ("fixed_bin_size", C.c_double),
+        ("moving_bin_size", C.c_double),
+        ("fixed_normalized_min", C.c_double),
+        ("moving_normalized_min", C.c_double),
+        ("moving_true_min", C.c_double),
+        ("moving_true_max", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it was never built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, szp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_size_t)
+        gp = C.POINTER(Geometry)
+        L.sitkhip_status_name.restype = C.c_char_p
+        L.sitkhip_status_name.argtypes = [C.c_int]
+        L.sitkhip_last_error.restype = C.c_char_p
+        L.sitkhip_backend_init.argtypes = [C.c_int]
+        L.sitkhip_backend_stream.argtypes = [C.POINTER(vp)]
+        L.sitkhip_device_name.argtypes = [C.c_char_p, C.c_size_t]
+        L.sitkhip_image_upload.argtypes = [C.c_int, vp, C.c_size_t, gp, C.POINTER(vp)]
+        L.sitkhip_image_filled.argtypes = [gp, C.c_float, C.POINTER(vp)]
+        L.sitkhip_image_to_host.argtypes = [vp, C.POINTER(C.c_float), C.c_size_t]
+        L.sitkhip_image_geometry.argtypes = [vp, gp]
+        L.sitkhip_image_len.argtypes = [vp]
+        L.sitkhip_image_len.restype = C.c_size_t
+        L.sitkhip_image_id.argtypes = [vp]
+        L.sitkhip_image_id.restype = C.c_uint64
+        L.sitkhip_image_device_ptr.argtypes = [vp]
+        L.sitkhip_image_device_ptr.restype = C.c_void_p
+        L.sitkhip_image_free.argtypes = [vp]
+        L.sitkhip_image_free.restype = None
+        L.sitkhip_mask_upload.argtypes = [C.POINTER(C.c_uint8), C.c_size_t, gp, C.POINTER(vp)]
+        L.sitkhip_mask_from_image.argtypes = [vp, C.POINTER(vp)]
+        L.sitkhip_mask_intersect.argtypes = [vp, vp, C.POINTER(vp)]
+        L.sitkhip_mask_to_host.argtypes = [vp, C.POINTER(C.c_uint8), C.c_size_t]
+        L.sitkhip_mask_free.argtypes = [vp]
+        L.sitkhip_mask_free.restype = None
+        L.sitkhip_recursive_gaussian.argtypes = [vp, dp, C.POINTER(vp)]
+        L.sitkhip_metric_create.argtypes = [C.c_int, vp, C.POINTER(C.c_int64), C.c_size_t, vp, vp,
+                                            vp, C.c_int, C.POINTER(vp)]
+        L.sitkhip_metric_free.argtypes = [vp]
+        L.sitkhip_metric_free.restype = None
+        L.sitkhip_metric_sample_count.argtypes = [vp, szp]
+        L.sitkhip_metric_volume_bytes.argtypes = [vp, szp]
+        L.sitkhip_metric_value_ranges.argtypes = [vp, dp, dp]
+        L.sitkhip_mattes_bins_from_ranges.argtypes = [dp, dp, C.c_size_t, C.POINTER(MattesBins)]
+        L.sitkhip_mattes_set_bins.argtypes = [vp, C.POINTER(MattesBins)]
+        L.sitkhip_mattes_joint_histogram.argtypes = [vp, dp, C.c_int, dp, dp, szp]
+        L.sitkhip_mattes_derivative_moments.argtypes = [vp, dp, C.c_int, dp, dp]
+        L.sitkhip_mattes_value.argtypes = [vp, dp, C.c_int, dp, szp]
+        L.sitkhip_mattes_value_and_moments.argtypes = [vp, dp, C.c_int, dp, dp, szp]
+        L.sitkhip_ms_moments.argtypes = [vp, dp, C.c_int, dp]
+        L.sitkhip_ms_value.argtypes = [vp, dp, C.c_int, dp, szp]
+        L.sitkhip_contract_ms.argtypes = [dp, dp, dp, C.c_int, dp, dp, szp]
+        L.sitkhip_contract_mattes.argtypes = [dp, dp, dp, C.c_int, dp]
+        L.sitkhip_profile_enable.argtypes = [C.c_int]
+        L.sitkhip_profile_query.argtypes = [C.c_char_p, C.POINTER(C.c_uint64), dp]
+        L.sitkhip_profile_dump.argtypes = [C.c_char_p, C.c_size_t]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        L = lib()
+        raise CudaError(L.sitkhip_status_name(rc).decode(), L.sitkhip_last_error().decode())
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _stages(stages) -> tuple:
+    s = np.ascontiguousarray(stages, dtype=np.float64).reshape(-1)
+    if s.size % 12:
+        raise ValueError("stages must be a multiple of 12 doubles")
+    return s, s.size // 12
+
+
+def backend_init(device: int = 0) -> None:
+    _check(lib().sitkhip_backend_init(device))
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    _check(lib().sitkhip_device_name(buf, 256))
+    return buf.value.decode()
+
+
+def backend_stream() -> int:
+    p = C.c_void_p()
+    _check(lib().sitkhip_backend_stream(C.byref(p)))
+    return p.value or 0
+
+
+class DeviceImage:
+    """DeviceImage (crates/sitk/src/cuda/image.rs:162-413): a float32 device volume.
+
+    Arrays are (nz, ny, nx) numpy arrays (dimension 0 = x fastest in memory).
+    """
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @staticmethod
+    def upload(arr: np.ndarray, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), direction=None):
+        a = np.ascontiguousarray(arr)
+        if a.dtype not in PIXEL:
+            raise CudaError("UnsupportedPixelType", f"no upload cast for {a.dtype}")
+        nz, ny, nx = a.shape
+        g = Geometry.make((nx, ny, nz), spacing, origin, direction)
+        h = C.c_void_p()
+        _check(lib().sitkhip_image_upload(PIXEL[a.dtype], a.ctypes.data, a.size, C.byref(g),
+                                          C.byref(h)))
+        return DeviceImage(h)
+
+    @staticmethod
+    def filled(size, value: float, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0),
+               direction=None):
+        g = Geometry.make(size, spacing, origin, direction)
+        h = C.c_void_p()
+        _check(lib().sitkhip_image_filled(C.byref(g), float(value), C.byref(h)))
+        return DeviceImage(h)
+
+    def geometry(self) -> Geometry:
+        g = Geometry()
+        _check(lib().sitkhip_image_geometry(self._h, C.byref(g)))
+        return g
+
+    def __len__(self) -> int:
+        return int(lib().sitkhip_image_len(self._h))
+
+    def id(self) -> int:
+        return int(lib().sitkhip_image_id(self._h))
+
+    def device_ptr(self) -> int:
+        return int(lib().sitkhip_image_device_ptr(self._h) or 0)
+
+    def to_host(self) -> np.ndarray:
+        g = self.geometry()
+        out = np.empty(len(self), dtype=np.float32)
+        _check(lib().sitkhip_image_to_host(self._h, out.ctypes.data_as(C.POINTER(C.c_float)),
+                                           out.size))
+        return out.reshape(g.size[2], g.size[1], g.size[0])
+
+    def free(self) -> None:
+        if self._h:
+            lib().sitkhip_image_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceMask:
+    """DeviceMask (crates/sitk/src/cuda/mask.rs:82-216): 1 inside, 0 outside."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @staticmethod
+    def upload(arr: np.ndarray, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), direction=None):
+        a = np.ascontiguousarray(arr, dtype=np.uint8)
+        nz, ny, nx = a.shape
+        g = Geometry.make((nx, ny, nz), spacing, origin, direction)
+        h = C.c_void_p()
+        _check(lib().sitkhip_mask_upload(a.ctypes.data_as(C.POINTER(C.c_uint8)), a.size,
+                                         C.byref(g), C.byref(h)))
+        return DeviceMask(h)
+
+    @staticmethod
+    def from_device_image(img: DeviceImage):
+        h = C.c_void_p()
+        _check(lib().sitkhip_mask_from_image(img._h, C.byref(h)))
+        return DeviceMask(h)
+
+    def intersect(self, other: "DeviceMask") -> "DeviceMask":
+        h = C.c_void_p()
+        _check(lib().sitkhip_mask_intersect(self._h, other._h, C.byref(h)))
+        return DeviceMask(h)
+
+    def to_host(self, shape) -> np.ndarray:
+        n = int(np.prod(shape))
+        out = np.empty(n, dtype=np.uint8)
+        _check(lib().sitkhip_mask_to_host(self._h, out.ctypes.data_as(C.POINTER(C.c_uint8)), n))
+        return out.reshape(shape)
+
+    def free(self) -> None:
+        if self._h:
+            lib().sitkhip_mask_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def recursive_gaussian(src: DeviceImage, sigma) -> DeviceImage:
+    """recursive_gaussian (crates/sitk/src/cuda/ops/pyramid.rs:217)."""
+    s = np.ascontiguousarray(sigma, dtype=np.float64)
+    if s.size != 3:
+        raise CudaError("UnsupportedGeometry", "sigma must have 3 entries")
+    h = C.c_void_p()
+    _check(lib().sitkhip_recursive_gaussian(src._h, _dp(s), C.byref(h)))
+    return DeviceImage(h)
+
+
+class _Resident:
+    def __init__(self, kind, fixed: DeviceImage, moving: DeviceImage, indices=None,
+                 fixed_mask: DeviceMask | None = None, moving_mask: DeviceMask | None = None,
+                 precision: int = PRECISION_F32):
+        h = C.c_void_p()
+        if indices is None:
+            ip, n = None, 0
+        else:
+            self._idx = np.ascontiguousarray(indices, dtype=np.int64)
+            ip, n = self._idx.ctypes.data_as(C.POINTER(C.c_int64)), self._idx.size
+        _check(lib().sitkhip_metric_create(kind, fixed._h, ip, n,
+                                           fixed_mask._h if fixed_mask else None, moving._h,
+                                           moving_mask._h if moving_mask else None, precision,
+                                           C.byref(h)))
+        self._h = h
+
+    def sample_count(self) -> int:
+        v = C.c_size_t()
+        _check(lib().sitkhip_metric_sample_count(self._h, C.byref(v)))
+        return v.value
+
+    def volume_bytes(self) -> int:
+        v = C.c_size_t()
+        _check(lib().sitkhip_metric_volume_bytes(self._h, C.byref(v)))
+        return v.value
+
+    def value_ranges(self):
+        fr = np.empty(2)
+        mr = np.empty(2)
+        _check(lib().sitkhip_metric_value_ranges(self._h, _dp(fr), _dp(mr)))
+        return (float(fr[0]), float(fr[1])), (float(mr[0]), float(mr[1]))
+
+    def free(self) -> None:
+        if self._h:
+            lib().sitkhip_metric_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class ResidentMetric(_Resident):
+    """Mean squares (crates/sitk/src/cuda/ops/mean_squares.rs:186-439)."""
+
+    def __init__(self, fixed, moving, **kw):
+        super().__init__(MEAN_SQUARES, fixed, moving, **kw)
+
+    def evaluate(self, stages) -> np.ndarray:
+        """The 14 moments [sq, S0[3], S1[3][3], count]."""
+        s, n = _stages(stages)
+        out = np.empty(14)
+        _check(lib().sitkhip_ms_moments(self._h, _dp(s), n, _dp(out)))
+        return out
+
+    def value(self, stages):
+        s, n = _stages(stages)
+        sq = C.c_double()
+        valid = C.c_size_t()
+        _check(lib().sitkhip_ms_value(self._h, _dp(s), n, C.byref(sq), C.byref(valid)))
+        return sq.value, valid.value
+
+
+def mattes_bins(fixed_range, moving_range, bins: int) -> MattesBins:
+    """MattesGeometry::new (crates/sitk/src/registration/mattes.rs:152-186)."""
+    fr = np.asarray(fixed_range, dtype=np.float64)
+    mr = np.asarray(moving_range, dtype=np.float64)
+    b = MattesBins()
+    _check(lib().sitkhip_mattes_bins_from_ranges(_dp(fr), _dp(mr), bins, C.byref(b)))
+    return b
+
+
+class ResidentMattes(_Resident):
+    """Mattes MI (crates/sitk/src/cuda/ops/mattes.rs:500-934)."""
+
+    def __init__(self, fixed, moving, bins: int | None = None, **kw):
+        super().__init__(MATTES, fixed, moving, **kw)
+        self.bins = None
+        if bins is not None:
+            fr, mr = self.value_ranges()
+            self.set_bins(mattes_bins(fr, mr, bins))
+
+    def set_bins(self, b: MattesBins) -> None:
+        _check(lib().sitkhip_mattes_set_bins(self._h, C.byref(b)))
+        self.bins = b
+
+    def joint_histogram(self, stages):
+        s, n = _stages(stages)
+        B = int(self.bins.bins)
+        joint = np.empty(B * B)
+        marg = np.empty(B)
+        valid = C.c_size_t()
+        _check(lib().sitkhip_mattes_joint_histogram(self._h, _dp(s), n, _dp(joint), _dp(marg),
+                                                    C.byref(valid)))
+        return joint, marg, valid.value
+
+    def derivative_moments(self, stages, pratio) -> np.ndarray:
+        s, n = _stages(stages)
+        pr = np.ascontiguousarray(pratio, dtype=np.float64)
+        out = np.empty(12)
+        _check(lib().sitkhip_mattes_derivative_moments(self._h, _dp(s), n, _dp(pr), _dp(out)))
+        return out
+
+    def value(self, stages):
+        s, n = _stages(stages)
+        v = C.c_double()
+        valid = C.c_size_t()
+        _check(lib().sitkhip_mattes_value(self._h, _dp(s), n, C.byref(v), C.byref(valid)))
+        return v.value, valid.value
+
+    def value_and_moments(self, stages):
+        s, n = _stages(stages)
+        v = C.c_double()
+        valid = C.c_size_t()
+        out = np.empty(12)
+        _check(lib().sitkhip_mattes_value_and_moments(self._h, _dp(s), n, C.byref(v), _dp(out),
+                                                      C.byref(valid)))
+        return v.value, out, valid.value
+
+
+def contract_ms(moments, j0, c, nparams: int):
+    """contract (crates/sitk/src/registration/cuda.rs:381-412)."""
+    m = np.ascontiguousarray(moments, dtype=np.float64)
+    j0 = np.ascontiguousarray(j0, dtype=np.float64)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    v = C.c_double()
+    d = np.empty(nparams)
+    valid = C.c_size_t()
+    _check(lib().sitkhip_contract_ms(_dp(m), _dp(j0), _dp(c), nparams, C.byref(v), _dp(d),
+                                     C.byref(valid)))
+    return v.value, d, valid.value
+
+
+def contract_mattes(moments, j0, c, nparams: int) -> np.ndarray:
+    """contract_mattes (crates/sitk/src/registration/cuda.rs:505-524)."""
+    m = np.ascontiguousarray(moments, dtype=np.float64)
+    j0 = np.ascontiguousarray(j0, dtype=np.float64)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    d = np.empty(nparams)
+    _check(lib().sitkhip_contract_mattes(_dp(m), _dp(j0), _dp(c), nparams, _dp(d)))
+    return d
+
+
+def profile_enable(on: bool) -> None:
+    _check(lib().sitkhip_profile_enable(int(on)))
+
+
+def profile_reset() -> None:
+    _check(lib().sitkhip_profile_reset())
+
+
+def profile_query(prefix: str):
+    cnt = C.c_uint64()
+    ms = C.c_double()
+    n = lib().sitkhip_profile_query(prefix.encode(), C.byref(cnt), C.byref(ms))
+    if n < 0:
+        _check(-n)
+    return cnt.value, ms.value
+
+
+def profile_dump() -> str:
+    buf = C.create_string_buffer(1 << 16)
+    _check(lib().sitkhip_profile_dump(buf, 1 << 16))
+    return buf.value.decode()

@@ -1,0 +1,372 @@
+"""Pins the CPU oracle against the golden vectors the reference itself holds,
+and against what the mathematics fixes where no vector exists.
+
+Golden vectors (paths relative to physwkim/sitk-rs):
+  - synth seed 42, first five voxels: crates/sitk/tests/bench_correctness_gate.rs:162-168
+  - FNV-1a 'a' test vector: same file :170-175
+  - input checksums 64^3 / 256^3 / 512^3: same file :192-211
+  - smoothing_recursive_gaussian([2,2,2]) output checksums:
+      64^3  0x8f3da2e917366232  bench/results/rust-cuda.ndjson:5
+      256^3 0x9aaea53a51fda207  bench/results/rust-cuda.ndjson:53,
+                                crates/sitk/tests/bench_correctness_gate.rs:60-63
+      512^3 0xb60665c90074c378  bench/results/rust-cuda.ndjson:101
+      64^3 seed 0x2545F4914F6CDD1D -> 0x9e5067c73b94ee1e  crates/sitk/tests/bit_parity.rs:151-160
+  - Mattes value -1.063357897140305 (doc/upstream-findings.md:273) on the fixture
+    of crates/sitk/tests/mattes_device.rs:42-81 (volume(32,[0;3]) vs
+    inverted(32,[1.5,-0.5,0.75]), spacing [0.8,0.9,1.1], origin [-3,2,1], 50 bins,
+    translation [1,0,0]).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+SEED = 42
+
+
+def test_synth_first_five_values():
+    assert list(O.synth(SEED, 5)) == [59.0, 641.0, 384.0, 121.0, 923.0]
+
+
+def test_fnv1a64_test_vector():
+    assert O.fnv1a64(np.frombuffer(b"a", dtype=np.uint8)) == 0xAF63DC4C8601EC8C
+
+
+@pytest.mark.parametrize("dim,want", [(64, 0xA60A081F21AF857E), (256, 0xB04930CDA0BBCE53)])
+def test_input_checksums(dim, want):
+    assert O.fnv1a64(O.synth(SEED, dim ** 3)) == want
+
+
+@pytest.mark.parametrize(
+    "dim,seed,want",
+    [
+        (64, SEED, 0x8F3DA2E917366232),
+        (64, 0x2545F4914F6CDD1D, 0x9E5067C73B94EE1E),
+        (256, SEED, 0x9AAEA53A51FDA207),
+    ],
+)
+def test_recursive_gaussian_golden_checksums(dim, seed, want):
+    v = O.synth(seed, dim ** 3)
+    out = O.smoothing_recursive_gaussian(v, (dim,) * 3, (1, 1, 1), (2, 2, 2), threads=8)
+    assert O.fnv1a64(out) == want
+
+
+@pytest.mark.slow
+def test_recursive_gaussian_golden_checksum_512():
+    v = O.synth(SEED, 512 ** 3)
+    assert O.fnv1a64(v) == 0xFBF1951B8B4B69AA
+    out = O.smoothing_recursive_gaussian(v, (512,) * 3, (1, 1, 1), (2, 2, 2), threads=8)
+    assert O.fnv1a64(out) == 0xB60665C90074C378
+
+
+def test_recursive_gaussian_thread_count_invariance():
+    v = O.synth(7, 40 * 33 * 21)
+    outs = [O.fnv1a64(O.smoothing_recursive_gaussian(v, (40, 33, 21), (0.9, 1.1, 1.5),
+                                                    (1.3, 2.0, 0.7), threads=t))
+            for t in (1, 3, 8)]
+    assert len(set(outs)) == 1
+
+
+def test_deriche_zero_order_dc_gain_is_one():
+    # crates/sitk/src/core/deriche.rs:363-373: the zero-order filter has unity DC gain.
+    for sigmad in (0.5, 1.0, 2.0, 3.7, 8.0):
+        c = O.deriche_coefficients(sigmad)
+        n, d, m = c[0:4], c[4:8], c[8:12]
+        gain = (n.sum() + m.sum()) / (1.0 + d.sum())
+        assert abs(gain - 1.0) < 1e-12
+
+
+def test_constant_is_preserved_and_zero_sigma_is_identity():
+    # recursive_gaussian.rs:545-572
+    shape = (12, 10, 9)
+    const = np.full(int(np.prod(shape)), 5.0, dtype=np.float32)
+    out = O.smoothing_recursive_gaussian(const, shape[::-1], (1, 1, 1), (2, 2, 2))
+    assert np.max(np.abs(out - 5.0)) < 1e-5
+    v = O.synth(3, int(np.prod(shape)))
+    out = O.smoothing_recursive_gaussian(v, shape[::-1], (1, 1, 1), (0, 0, 0))
+    assert np.array_equal(out, v)
+
+
+def test_axis_pass_is_filter_line_on_every_line():
+    """Smoothing only axis 1 equals filter_line applied to each y-line: pins the
+    line decomposition independently of the golden checksums."""
+    nx, ny, nz = 5, 11, 4
+    v = O.synth(11, nx * ny * nz)
+    out = O.smoothing_recursive_gaussian(v, (nx, ny, nz), (1, 1, 1), (0, 1.7, 0))
+    coef = O.deriche_coefficients(1.7)
+    vol = v.reshape(nz, ny, nx).astype(np.float64)
+    want = np.empty_like(vol)
+    for z in range(nz):
+        for x in range(nx):
+            want[z, :, x] = O.filter_line(vol[z, :, x], coef)
+    assert np.array_equal(out.reshape(nz, ny, nx), want.astype(np.float32))
+
+
+def test_filter_line_is_a_linear_shift_invariant_filter_in_the_interior():
+    """Far from the borders the IIR response to a unit impulse is a symmetric,
+    positive, unit-mass Gaussian-like kernel."""
+    coef = O.deriche_coefficients(3.0)
+    x = np.zeros(201)
+    x[100] = 1.0
+    y = O.filter_line(x, coef)
+    assert abs(y.sum() - 1.0) < 1e-9
+    assert np.allclose(y[100 - 30:100], y[101:131][::-1], atol=1e-9)
+    var = np.sum(y * (np.arange(201) - 100.0) ** 2)
+    assert abs(math.sqrt(var) - 3.0) < 0.15
+
+
+def test_axis_shorter_than_four_is_refused():
+    v = O.synth(1, 3 * 8 * 8)
+    with pytest.raises(ValueError):
+        O.smoothing_recursive_gaussian(v, (3, 8, 8), (1, 1, 1), (1, 1, 1))
+    # but not when that axis is not smoothed
+    O.smoothing_recursive_gaussian(v, (3, 8, 8), (1, 1, 1), (0, 1, 1))
+
+
+# ----------------------------------------------------------------- Mattes
+def _ledger_problem():
+    f = O.fixture_volume(32, [0, 0, 0])
+    m = O.fixture_inverted(32, [1.5, -0.5, 0.75])
+    return O.MetricProblem(f, [0.8, 0.9, 1.1], [-3, 2, 1], m, [0.8, 0.9, 1.1], [-3, 2, 1])
+
+
+def test_mattes_ledger_value_is_reproduced_bit_for_bit():
+    """The reference-held host value, recorded before the reference made its
+    joint-PDF mass a Kahan sum (ledger row 2.161), so it is reproduced with the
+    naive sum of that time; the current Kahan tail is 3 ulp away."""
+    pb = _ledger_problem()
+    g = pb.mattes_geometry(50)
+    t = O.make_transform(O.TRANSLATION, [1.0, 0.0, 0.0])
+    try:
+        O.set_joint_sum_compensated(False)
+        v = pb.mattes_value(g, t)
+    finally:
+        O.set_joint_sum_compensated(True)
+    assert v == -1.063357897140305
+    vk = pb.mattes_value(g, t)
+    assert vk == -1.0633578971403057
+    assert abs(vk - v) <= 4 * np.spacing(abs(v))
+
+
+def test_mattes_matches_an_independent_numpy_restatement():
+    """Brute force on a small pair with numpy (no shared code with the oracle)."""
+    pb = _ledger_problem()
+    g = pb.mattes_geometry(50)
+    t = O.make_transform(O.TRANSLATION, [0.0, 0.0, 0.0])
+    f = pb.fixed.astype(np.float64).ravel()
+    m = pb.moving.astype(np.float64).ravel()  # identity on identical grids: mv == voxel
+    B = 50
+    fi = np.clip((f / g.fixed_bin_size - g.fixed_normalized_min).astype(np.int64), 2, B - 3)
+    mt = m / g.moving_bin_size - g.moving_normalized_min
+    mi = np.clip(mt.astype(np.int64), 2, B - 3)
+
+    def b3(u):
+        a = np.abs(u)
+        return np.where(a < 1, (4 - 6 * a * a + 3 * a ** 3) / 6,
+                        np.where(a < 2, (2 - a) ** 3 / 6, 0.0))
+
+    J = np.zeros((B, B))
+    for o in range(-1, 3):
+        np.add.at(J, (fi, mi + o), b3(mi + o - mt))
+    P = J / J.sum()
+    pf, pm = P.sum(1), P.sum(0)
+    nz = P > 0
+    mi_val = (P[nz] * np.log(P[nz] / np.outer(pf, pm)[nz])).sum()
+    assert abs(pb.mattes_value(g, t) - (-mi_val)) < 1e-12
+    joint, marg, valid = pb.mattes_histogram(g, t)
+    assert valid == 32 ** 3
+    assert np.allclose(joint.reshape(B, B), J, rtol=1e-12, atol=1e-12)
+    assert np.array_equal(marg, np.bincount(fi, minlength=B).astype(np.float64))
+
+
+def test_cubic_bspline_partition_of_unity_and_derivative():
+    # mattes.rs:821-846
+    for i in range(20):
+        frac = i / 20.0
+        s = sum(O.cubic_bspline(k - frac) for k in (-1, 0, 1, 2))
+        assert abs(s - 1.0) < 1e-12
+    h = 1e-6
+    for i in range(-40, 40):
+        u = i / 10.0
+        if abs(u - round(u)) < 1e-9:
+            continue
+        fd = (O.cubic_bspline(u + h) - O.cubic_bspline(u - h)) / (2 * h)
+        assert abs(fd - O.cubic_bspline_derivative(u)) < 1e-4
+
+
+def test_kahan_sum_is_itk_without_the_final_fold():
+    # mattes.rs:855-903: the compensated sum differs from the naive one and is closer.
+    h = np.array([(613.7 if i % 97 == 0 else 0.1) * (1.0 + (i % 13) / 7.0) for i in range(2500)])
+    naive = -0.0
+    for t in h:
+        naive += t
+    s, c = 0.0, 0.0
+    for t in h:
+        x = s + t
+        c += (s - x) + t if abs(s) >= abs(t) else (t - x) + s
+        s = x
+    ref = s + c
+    k = O.compensated_sum(h)
+    assert naive != ref and k != naive and abs(k - ref) < abs(naive - ref)
+
+
+def _interior_problem():
+    """A 16^3 fixed block sitting in the middle of the 32^3 moving fixture, so
+    small rotations never move a sample across the moving buffer's edge: the
+    analytic derivative has no term for samples entering or leaving the domain,
+    so a finite difference only agrees with it when none does."""
+    sp = [0.8, 0.9, 1.1]
+    org = np.array([-3.0, 2.0, 1.0])
+    m = O.fixture_inverted(32, [1.5, -0.5, 0.75])
+    f = np.ascontiguousarray(O.fixture_volume(32, [0, 0, 0])[8:24, 8:24, 8:24])
+    forg = org + 8 * np.array(sp)
+    return O.MetricProblem(f, sp, forg, m, sp, org), forg + 7.5 * np.array(sp)
+
+
+def test_mattes_derivative_matches_finite_difference():
+    # mattes.rs:1015-1041 (3-D version, Euler3D and translation)
+    pb, ctr = _interior_problem()
+    g = pb.mattes_geometry(50)
+    for kind, p0, steps in [
+        (O.TRANSLATION, [1.3, -0.7, 0.4], [1e-3] * 3),
+        (O.EULER3D, [0.03, -0.02, 0.05, 1.1, -0.6, 0.3], [1e-3] * 6),
+    ]:
+        t = O.make_transform(kind, p0, center=ctr)
+        _, an, _ = pb.mattes_evaluate(g, t)
+        for k in range(len(p0)):
+            pp = list(p0)
+            pm = list(p0)
+            pp[k] += steps[k]
+            pm[k] -= steps[k]
+            fd = (pb.mattes_value(g, O.make_transform(kind, pp, center=ctr)) -
+                  pb.mattes_value(g, O.make_transform(kind, pm, center=ctr))) / (2 * steps[k])
+            assert abs(fd - an[k]) < 5e-3 * max(1.0, abs(an[k])), (kind, k, fd, an[k])
+
+
+def test_mattes_moment_form_equals_dense_form():
+    """The 12-moment derivative (src/cuda/ops/mattes.rs:75-91) contracted with the
+    affine decomposition equals the dense global-support derivative; the
+    reference pins the two at 1e-9 (tests/mattes_device.rs:312-350)."""
+    from helpers import affine_form, rel_vec
+
+    pb = _ledger_problem()
+    g = pb.mattes_geometry(50)
+    for kind, p in [(O.EULER3D, [0.05, -0.03, 0.08, 1.2, -0.7, 0.4]),
+                    (O.AFFINE, [1.02, 0.03, -0.01, -0.02, 0.99, 0.04, 0.01, -0.03, 1.01,
+                                0.6, -1.1, 0.4])]:
+        t = O.make_transform(kind, p, center=[1.0, -2.0, 0.5])
+        v, dense, valid = pb.mattes_evaluate(g, t)
+        joint, marg, valid2 = pb.mattes_histogram(g, t)
+        v2, pratio = pb.mattes_tail(joint, marg, valid2, g)
+        assert v == v2 and valid == valid2
+        mom = pb.mattes_moments(g, t, pratio)
+        j0, c = affine_form(t)
+        deriv = np.zeros(t.nparams)
+        for k in range(t.nparams):
+            acc = 0.0
+            for d in range(3):
+                acc += j0[d, k] * mom[d]
+                for e in range(3):
+                    acc += c[e, d, k] * mom[3 + 3 * d + e]
+            deriv[k] = acc
+        assert rel_vec(deriv, dense) < 1e-9
+
+
+def test_mattes_refusals():
+    pb = _ledger_problem()
+    with pytest.raises(ValueError, match="too few"):
+        pb.mattes_geometry(4)
+    pb.mattes_geometry(5)
+    flat = np.full((4, 4, 4), 3.0, dtype=np.float32)
+    pc = O.MetricProblem(flat, [1, 1, 1], [0, 0, 0], pb.moving[:4, :4, :4], [1, 1, 1], [0, 0, 0])
+    with pytest.raises(ValueError, match="constant fixed"):
+        pc.mattes_geometry(50)
+
+
+# ----------------------------------------------------------------- mean squares
+def test_mean_squares_identity_is_zero_with_zero_gradient():
+    # metric.rs:1499-1509
+    f = O.fixture_volume(16, [0, 0, 0])
+    pb = O.MetricProblem(f, [1, 1, 1], [0, 0, 0], f, [1, 1, 1], [0, 0, 0])
+    v, d, valid = pb.ms_evaluate(O.make_transform(O.TRANSLATION, [0, 0, 0]))
+    assert v == 0.0 and np.all(d == 0.0) and valid == 16 ** 3
+
+
+def test_mean_squares_one_voxel_shift_is_hand_computable():
+    """Moving = fixed shifted by one voxel along x; a translation of exactly one
+    voxel re-aligns every interior sample, and the value is the mean of the
+    squared differences of the clamped border column only."""
+    n = 8
+    f = (np.arange(n ** 3, dtype=np.float32) % 17).reshape(n, n, n)
+    m = np.empty_like(f)
+    m[:, :, 1:] = f[:, :, :-1]
+    m[:, :, 0] = f[:, :, 0]
+    pb = O.MetricProblem(f, [1, 1, 1], [0, 0, 0], m, [1, 1, 1], [0, 0, 0])
+    v, valid = pb.ms_value(O.make_transform(O.TRANSLATION, [1.0, 0, 0]))
+    # samples with x = n-1 map to x = n (outside); the others read m[x+1] = f[x]
+    assert valid == n * n * (n - 1)
+    assert v == 0.0
+
+
+def test_mean_squares_derivative_matches_finite_difference():
+    pb, ctr = _interior_problem()
+    p0 = [0.02, -0.01, 0.03, 0.6, -0.4, 0.2]
+    t = O.make_transform(O.EULER3D, p0, center=ctr)
+    _, an, _ = pb.ms_evaluate(t)
+    for k in range(6):
+        # the interpolant is piecewise trilinear: steps small enough that no
+        # sample crosses a cell wall make the difference exact to rounding
+        h = 1e-6
+        pp = list(p0)
+        pm = list(p0)
+        pp[k] += h
+        pm[k] -= h
+        fp = pb.ms_value(O.make_transform(O.EULER3D, pp, center=ctr))[0]
+        fm = pb.ms_value(O.make_transform(O.EULER3D, pm, center=ctr))[0]
+        fd = (fp - fm) / (2 * h)
+        assert abs(fd - an[k]) < 1e-4 * max(1.0, abs(an[k])), (k, fd, an[k])
+
+
+# ----------------------------------------------------------------- transforms
+def test_stage_replay_is_transform_point_bitwise():
+    # src/transform/matrix_offset.rs:102-109; tests/cuda_point_map.rs:84
+    rng = np.random.default_rng(0)
+    for kind, p in [(O.TRANSLATION, [0.7, -1.3, 2.1]),
+                    (O.EULER3D, [0.05, -0.03, 0.08, 1.2, -0.7, 0.4]),
+                    (O.AFFINE, [1.02, 0.03, -0.01, -0.02, 0.99, 0.04, 0.01, -0.03, 1.01,
+                                0.6, -1.1, 0.4])]:
+        t = O.make_transform(kind, p, center=[1.0, -2.0, 0.5])
+        st = O.transform_stage(t)
+        for _ in range(200):
+            x = rng.uniform(-100, 100, 3)
+            assert np.array_equal(O.replay_stages(st, x), O.transform_point(t, x))
+
+
+def test_euler3d_jacobian_matches_finite_difference():
+    for zyx in (False, True):
+        p0 = np.array([0.3, -0.2, 0.5, 1.0, 2.0, 3.0])
+        c = [1.0, -2.0, 0.5]
+        x = np.array([4.0, -3.0, 7.0])
+        J = O.transform_jacobian(O.make_transform(O.EULER3D, p0, c, zyx), x)
+        for k in range(6):
+            h = 1e-6
+            pp, pm = p0.copy(), p0.copy()
+            pp[k] += h
+            pm[k] -= h
+            fd = (O.transform_point(O.make_transform(O.EULER3D, pp, c, zyx), x) -
+                  O.transform_point(O.make_transform(O.EULER3D, pm, c, zyx), x)) / (2 * h)
+            assert np.allclose(fd, J[:, k], atol=1e-6)
+
+
+def test_draw_samples():
+    # metric.rs:419-437
+    assert O.draw_samples(100, 0, 1.0, 0) is None
+    assert list(O.draw_samples(10, 1, 0.3, 0)) == [0, 4, 8]
+    a = O.draw_samples(1000, 2, 0.2, 5)
+    b = O.draw_samples(1000, 2, 0.2, 5)
+    assert a.size == 200 and np.array_equal(a, b) and a.min() >= 0 and a.max() < 1000
+    # SplitMix64 first output for seed 0 (Vigna's reference implementation)
+    assert O.draw_samples(1 << 63, 2, 1.0 / (1 << 63), 0)[0] == 0xE220A8397B1DCDAF % (1 << 63)
