@@ -173,6 +173,14 @@ int sitkhip_metric_volume_bytes(sitkhip_metric m, size_t *out);
  * of the moving volume over its mask; (0,0) when empty. */
 int sitkhip_metric_value_ranges(sitkhip_metric m, double fixed_range[2], double moving_range[2]);
 
+/* draw_samples (src/registration/metric.rs:101-122, 419-437), on the host: the
+ * device is handed the list and never draws. strategy 0 None (*n = SIZE_MAX,
+ * no list: use FixedPoints::Grid), 1 Regular (stride ceil(1/percentage)),
+ * 2 Random (floor(voxels * percentage) SplitMix64 draws with replacement).
+ * `out` may be NULL to query the count. */
+int sitkhip_draw_samples(size_t voxels, int strategy, double percentage, uint64_t seed,
+                         int64_t *out, size_t *n);
+
 /* MattesGeometry::new (registration/mattes.rs:152-186) on the host: refuses
  * bins < 5 and constant ranges by name. padding is 2. */
 int sitkhip_mattes_bins_from_ranges(const double fixed_range[2], const double moving_range[2],

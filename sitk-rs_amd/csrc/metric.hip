@@ -1099,3 +1099,44 @@ int sitkhip_contract_mattes(const double mom[12], const double *j0, const double
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ sampling
+// draw_samples (crates/sitk/src/registration/metric.rs:101-122, 419-437): the
+// host draws, the device is handed the list. strategy 0 None (returns
+// SIZE_MAX in *n: the identity selection needs no list), 1 Regular (stride
+// ceil(1/pct)), 2 Random (floor(voxels*pct) SplitMix64 draws, next_below =
+// u64 % voxels, with replacement). out may be NULL to query the count.
+extern "C" int sitkhip_draw_samples(size_t voxels, int strategy, double percentage, uint64_t seed,
+                                    int64_t *out, size_t *n) {
+    return sitkhip::run_guarded([&] {
+        if (!n) sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "null count out-pointer");
+        if (strategy == 0) {
+            *n = SIZE_MAX;
+            return;
+        }
+        if (voxels == 0) sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "no voxels to draw from");
+        if (strategy == 1) {
+            size_t stride = static_cast<size_t>(std::ceil(1.0 / percentage));
+            if (stride < 1) stride = 1;
+            size_t c = 0;
+            for (size_t v = 0; v < voxels; v += stride) {
+                if (out) out[c] = static_cast<int64_t>(v);
+                ++c;
+            }
+            *n = c;
+            return;
+        }
+        if (strategy != 2) sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "unknown sampling strategy");
+        const size_t count = static_cast<size_t>(static_cast<double>(voxels) * percentage);
+        uint64_t state = seed;
+        for (size_t i = 0; i < count; ++i) {
+            state += 0x9E3779B97F4A7C15ull;
+            uint64_t z = state;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            if (out) out[i] = static_cast<int64_t>(z % static_cast<uint64_t>(voxels));
+        }
+        *n = count;
+    });
+}

@@ -126,6 +126,8 @@ def lib():
         L.sitkhip_ms_value.argtypes = [vp, dp, C.c_int, dp, szp]
         L.sitkhip_contract_ms.argtypes = [dp, dp, dp, C.c_int, dp, dp, szp]
         L.sitkhip_contract_mattes.argtypes = [dp, dp, dp, C.c_int, dp]
+        L.sitkhip_draw_samples.argtypes = [C.c_size_t, C.c_int, C.c_double, C.c_uint64,
+                                           C.POINTER(C.c_int64), szp]
         L.sitkhip_profile_enable.argtypes = [C.c_int]
         L.sitkhip_profile_query.argtypes = [C.c_char_p, C.POINTER(C.c_uint64), dp]
         L.sitkhip_profile_dump.argtypes = [C.c_char_p, C.c_size_t]
@@ -426,6 +428,23 @@ def contract_mattes(moments, j0, c, nparams: int) -> np.ndarray:
     d = np.empty(nparams)
     _check(lib().sitkhip_contract_mattes(_dp(m), _dp(j0), _dp(c), nparams, _dp(d)))
     return d
+
+
+SAMPLING_NONE, SAMPLING_REGULAR, SAMPLING_RANDOM = 0, 1, 2
+
+
+def draw_samples(voxels: int, strategy: int, percentage: float, seed: int):
+    """draw_samples (crates/sitk/src/registration/metric.rs:419-437); None for
+    the identity selection."""
+    n = C.c_size_t()
+    _check(lib().sitkhip_draw_samples(voxels, strategy, percentage, C.c_uint64(seed), None,
+                                      C.byref(n)))
+    if n.value == C.c_size_t(-1).value:
+        return None
+    out = np.empty(n.value, dtype=np.int64)
+    _check(lib().sitkhip_draw_samples(voxels, strategy, percentage, C.c_uint64(seed),
+                                      out.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(n)))
+    return out
 
 
 def profile_enable(on: bool) -> None:

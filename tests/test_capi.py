@@ -102,3 +102,13 @@ def test_contractions_match_the_reference_formula():
     # no valid sample: f64::MAX and a zero derivative (registration/cuda.rs:385-392)
     v, dm, valid = S.contract_ms(np.zeros(14), np.zeros((3, 6)), np.zeros((3, 3, 6)), 6)
     assert v == np.finfo(np.float64).max and valid == 0 and not dm.any()
+
+
+def test_draw_samples_is_the_references():
+    import oracle as O
+
+    assert S.draw_samples(100, 0, 1.0, 0) is None
+    for strategy, pct, seed in [(1, 0.3, 0), (1, 0.05, 0), (2, 0.2, 5), (2, 0.25, 12345)]:
+        got = S.draw_samples(10007, strategy, pct, seed)
+        want = O.draw_samples(10007, strategy, pct, seed)
+        assert np.array_equal(got, want)
