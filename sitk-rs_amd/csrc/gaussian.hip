@@ -14,14 +14,15 @@
 //   3. Intermediate axes stay FP64; the only narrowing is (float) of the last
 //      smoothed axis' output, which equals the reference's single `as f32`.
 //
-// Layout: volumes are dimension-0 fastest. A smoothed axis is one launch. Axis 1
-// and 2 lanes own lines that are adjacent in dimension 0, so every step of the
-// recursion is one coalesced 64-lane access. Axis 0 lines are contiguous, so
-// lanes would stride by a whole row; that axis is staged through LDS in chunks
-// of CHUNK elements so global traffic stays row-contiguous.
+// Layout: volumes are dimension-0 fastest. A smoothed axis is one launch. Lines
+// along axes 1 and 2 are assigned to lanes by their dimension-0 coordinate, so
+// every step of the recursion is one coalesced access across a wavefront; axis 0
+// lines are contiguous, so there a step touches one cache line per lane and the
+// following steps hit it in cache.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "common.hpp"
 #include "image.hpp"
@@ -191,6 +192,133 @@ __global__ __launch_bounds__(256) void iir_lines(const SrcT *__restrict__ src, d
     }
 }
 
+// Two lanes per line, 32 lines per wavefront: lane l < 32 runs the causal
+// recursion of line l forward while lane l + 32 runs the anti-causal recursion
+// of the same line backward, in lock-step. At step i the causal lane produces
+// outs[i] and the anti-causal lane scratch[n-1-i]. Until the middle (i < H =
+// ceil(n/2)) each lane parks its value in LDS; after it, each lane reads its
+// partner's value for the same element (produced n-1-i steps earlier) and
+// writes the finished output outs[k] + scratch[k]. The causal outputs never
+// leave the chip, so a line costs one read of its input per lane and one write
+// of its output. Both recursions have the same shape,
+//     v = (((x*c0 + a*c1) + b*c2) + cc*c3) - (((p1*d1 + p2*d2) + p3*d3) + p4*d4),
+// with (c0..c3) = (n0..n3) for the causal lane and (m1..m4) for the anti-causal
+// lane (whose x is data[j+1] for output j), so the loop is lane-uniform.
+// Loads run PD steps ahead of their use.
+constexpr int kPairLines = 32;
+constexpr int kPD = 16;
+
+template <typename SrcT, typename DstT>
+__global__ __launch_bounds__(64) void iir_pair_lines(const SrcT *__restrict__ src,
+                                                     DstT *__restrict__ dst, LineSet L, IirCoef k,
+                                                     int H) {
+    extern __shared__ double lds[];  // [2][H][32]: role, step, line
+    const int lane = threadIdx.x;
+    const int role = lane >> 5;  // 0 causal, 1 anti-causal
+    const int li = lane & 31;
+    long long line = static_cast<long long>(blockIdx.x) * kPairLines + li;
+    const bool act = line < L.nlines;
+    if (!act) line = L.nlines - 1;  // mirror a valid line; stores are suppressed
+    const long long s0 = line_start(L, line);
+    const long long st = L.stride;
+    const int n = L.len;
+    const double *c = k.c;
+    const double d1 = c[4], d2 = c[5], d3 = c[6], d4 = c[7];
+    const double c0 = role ? c[8] : c[0];
+    const double c1 = role ? c[9] : c[1];
+    const double c2 = role ? c[10] : c[2];
+    const double c3 = role ? c[11] : c[3];
+    double *mine = lds + role * H * kPairLines;
+    const double *partner = lds + (1 - role) * H * kPairLines;
+
+    double a, b, cc, p1, p2, p3, p4;
+    double v[4];
+    if (role == 0) {
+        // causal boundary (recursive_gaussian.rs:476-487)
+        const double n0 = c[0], n1 = c[1], n2 = c[2], n3 = c[3];
+        const double v1 = widen(src[s0]);
+        const double x1 = widen(src[s0 + st]);
+        const double x2 = widen(src[s0 + 2 * st]);
+        const double x3 = widen(src[s0 + 3 * st]);
+        double o0 = v1 * n0 + v1 * n1 + v1 * n2 + v1 * n3;
+        double o1 = x1 * n0 + v1 * n1 + v1 * n2 + v1 * n3;
+        double o2 = x2 * n0 + x1 * n1 + v1 * n2 + v1 * n3;
+        double o3 = x3 * n0 + x2 * n1 + x1 * n2 + v1 * n3;
+        o0 -= v1 * c[12] + v1 * c[13] + v1 * c[14] + v1 * c[15];
+        o1 -= o0 * d1 + v1 * c[13] + v1 * c[14] + v1 * c[15];
+        o2 -= o1 * d1 + o0 * d2 + v1 * c[14] + v1 * c[15];
+        o3 -= o2 * d1 + o1 * d2 + o0 * d3 + v1 * c[15];
+        v[0] = o0; v[1] = o1; v[2] = o2; v[3] = o3;
+        a = x3; b = x2; cc = x1;
+        p1 = o3; p2 = o2; p3 = o1; p4 = o0;
+    } else {
+        // anti-causal boundary (recursive_gaussian.rs:496-510)
+        const double m1 = c[8], m2 = c[9], m3 = c[10], m4 = c[11];
+        const long long e = s0 + static_cast<long long>(n - 1) * st;
+        const double v2 = widen(src[e]);
+        const double y1 = widen(src[e - st]);
+        const double y2 = widen(src[e - 2 * st]);
+        double q1 = v2 * m1 + v2 * m2 + v2 * m3 + v2 * m4;
+        double q2 = v2 * m1 + v2 * m2 + v2 * m3 + v2 * m4;
+        double q3 = y1 * m1 + v2 * m2 + v2 * m3 + v2 * m4;
+        double q4 = y2 * m1 + y1 * m2 + v2 * m3 + v2 * m4;
+        q1 -= v2 * c[16] + v2 * c[17] + v2 * c[18] + v2 * c[19];
+        q2 -= q1 * d1 + v2 * c[17] + v2 * c[18] + v2 * c[19];
+        q3 -= q2 * d1 + q1 * d2 + v2 * c[18] + v2 * c[19];
+        q4 -= q3 * d1 + q2 * d2 + q1 * d3 + v2 * c[19];
+        v[0] = q1; v[1] = q2; v[2] = q3; v[3] = q4;  // scratch[n-1 .. n-4]
+        a = y2; b = y1; cc = v2;                     // data[n-3], data[n-2], data[n-1]
+        p1 = q4; p2 = q3; p3 = q2; p4 = q1;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) mine[i * kPairLines + li] = v[i];
+
+    // value read at step i sits at A + i*S; output of step i goes to W + i*S
+    const long long S = role ? -st : st;
+    const long long A = role ? s0 + static_cast<long long>(n) * st : s0;
+    const long long W = role ? s0 + static_cast<long long>(n - 1) * st : s0;
+
+    SrcT cur[kPD], nxt[kPD];
+#pragma unroll
+    for (int j = 0; j < kPD; ++j) {
+        const int i = 4 + j;
+        cur[j] = i < n ? src[A + i * S] : SrcT(0);
+    }
+    for (int base = 4; base < n; base += kPD) {
+#pragma unroll
+        for (int j = 0; j < kPD; ++j) {
+            const int i = base + kPD + j;
+            nxt[j] = i < n ? src[A + i * S] : SrcT(0);
+        }
+#pragma unroll
+        for (int j = 0; j < kPD; ++j) {
+            const int i = base + j;
+            if (i < n) {
+                const double x = widen(cur[j]);
+                double o = x * c0 + a * c1 + b * c2 + cc * c3;
+                o -= p1 * d1 + p2 * d2 + p3 * d3 + p4 * d4;
+                cc = b; b = a; a = x;
+                p4 = p3; p3 = p2; p2 = p1; p1 = o;
+                if (i < H) {
+                    mine[i * kPairLines + li] = o;
+                } else {
+                    if (i == H) __syncthreads();
+                    const double q = partner[(n - 1 - i) * kPairLines + li];
+                    if (act) dst[W + i * S] = narrow<DstT>(o + q);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kPD; ++j) cur[j] = nxt[j];
+    }
+    // odd n: the middle element was parked by both lanes at step H-1
+    if ((n & 1) && role == 0 && act) {
+        const int mid = (n - 1) >> 1;
+        dst[s0 + static_cast<long long>(mid) * st] =
+            narrow<DstT>(mine[mid * kPairLines + li] + partner[mid * kPairLines + li]);
+    }
+}
+
 static LineSet lines_for(const sitkhip_geometry &g, int axis) {
     LineSet L;
     long long n0 = static_cast<long long>(g.size[0]);
@@ -206,15 +334,41 @@ static LineSet lines_for(const sitkhip_geometry &g, int axis) {
     return L;
 }
 
+static size_t max_lds_per_block() {
+    static size_t v = 0;
+    if (!v) {
+        int a = 0;
+        HIP_OK(hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                     backend().device));
+        v = static_cast<size_t>(a);
+    }
+    return v;
+}
+
+// tmp: an FP64 scratch volume the single-lane kernel needs for its causal
+// outputs (it may alias dst when DstT is double); the two-lane kernel, used
+// whenever its LDS fits, needs none.
 template <typename SrcT, typename DstT>
 static void launch_axis(const SrcT *src, double *tmp, DstT *dst, const sitkhip_geometry &g, int axis,
                         const IirCoef &k) {
     LineSet L = lines_for(g, axis);
-    unsigned blocks = static_cast<unsigned>((L.nlines + 255) / 256);
     static const char *names[3] = {"gauss_iir_axis0", "gauss_iir_axis1", "gauss_iir_axis2"};
+    const int H = (L.len + 1) / 2;
+    const size_t lds = static_cast<size_t>(2) * H * kPairLines * sizeof(double);
     LaunchScope scope(names[axis]);
-    hipLaunchKernelGGL((iir_lines<SrcT, DstT>), dim3(blocks), dim3(256), 0, backend().stream, src,
-                       tmp, dst, L, k);
+    if (L.len >= 8 && lds <= max_lds_per_block()) {
+        unsigned blocks = static_cast<unsigned>((L.nlines + kPairLines - 1) / kPairLines);
+        if (lds > 64 * 1024)
+            HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void *>(&iir_pair_lines<SrcT, DstT>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(lds)));
+        hipLaunchKernelGGL((iir_pair_lines<SrcT, DstT>), dim3(blocks), dim3(64), lds,
+                           backend().stream, src, dst, L, k, H);
+    } else {
+        unsigned blocks = static_cast<unsigned>((L.nlines + 255) / 256);
+        hipLaunchKernelGGL((iir_lines<SrcT, DstT>), dim3(blocks), dim3(256), 0, backend().stream,
+                           src, tmp, dst, L, k);
+    }
     check_launch(names[axis]);
 }
 
@@ -252,8 +406,13 @@ Image *recursive_gaussian(const Image &in, const double sigma[3]) {
             sync();
             return out;
         }
-        DeviceBuffer<double> A(n);
-        DeviceBuffer<double> B(na >= 2 ? n : 0);
+        // FP64 intermediates, kept between calls (every call is synchronous and
+        // the library has one stream, so one pair of buffers serves them all).
+        static std::mutex mu;
+        static DeviceBuffer<double> A, B;
+        std::lock_guard<std::mutex> lk(mu);
+        if (A.len < n) A.alloc(n);
+        if (na >= 2 && B.len < n) B.alloc(n);
         const float *src32 = in.buf.ptr;
         if (na == 1) {
             IirCoef k = zero_order_coefficients(sigma[axes[0]] / g.spacing[axes[0]]);

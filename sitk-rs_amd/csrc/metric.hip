@@ -62,7 +62,7 @@ struct MattesParams {
     int bins;
     double mbs, mnmin, inv_mbs;  // moving bin size, normalised min, 1/bin size
     double mmin, mmax;           // moving true range (reject outside)
-    float mmin_f, mmax_f;
+    float mmin_f, mmax_f, inv_mbs_f;
 };
 
 // ------------------------------------------------------------- cubic B-spline
@@ -79,6 +79,20 @@ __device__ __forceinline__ R bspline3(R u) {
         return (R(8) - R(12) * a + R(6) * sq - sq * a) / R(6);
     }
     return R(0);
+}
+
+// FP32: the same polynomials, (2-a)^3/6 form and a multiply by 1/6 (no
+// correctly rounded division on the FP32 path).
+template <>
+__device__ __forceinline__ float bspline3<float>(float u) {
+    const float a = fabsf(u);
+    const float sq = a * a;
+    if (a < 1.0f) return fmaf(fmaf(3.0f, a, -6.0f), sq, 4.0f) * (1.0f / 6.0f);
+    if (a < 2.0f) {
+        const float t = 2.0f - a;
+        return t * t * t * (1.0f / 6.0f);
+    }
+    return 0.0f;
 }
 
 template <typename R>
@@ -121,7 +135,9 @@ __device__ __forceinline__ bool eval_sample(const FixedGrid &F, const MovingVol 
     float v[8];
     corners(M, cell, v);
     R gi[3];
-    Trilinear<R>::template eval<GRAD>(cell, v, mv, gi);
+    R fr[3] = {static_cast<R>(cell.frac[0]), static_cast<R>(cell.frac[1]),
+               static_cast<R>(cell.frac[2])};
+    Trilinear<R>::template eval<GRAD>(fr, v, mv, gi);
     if (GRAD) physical_gradient<R>(M, gi, gp);
     return true;
 }
@@ -150,14 +166,15 @@ struct Parzen<double> {
 
 template <>
 struct Parzen<float> {
+    // term = mv/binSize - normalizedMin = (mv - trueMin)/binSize + padding: the
+    // second form has no cancellation, so it is accurate in FP32.
     __device__ __forceinline__ static int taps(const MattesParams &P, float mv, float arg[4]) {
-        const double term = static_cast<double>(mv) * P.inv_mbs - P.mnmin;
-        long long idx = static_cast<long long>(term);
-        if (idx < 2) idx = 2;
-        else if (idx > P.bins - 3) idx = P.bins - 3;
-        const int start = static_cast<int>(idx) - 1;
+        const float term = fmaf(mv - P.mmin_f, P.inv_mbs_f, 2.0f);
+        int idx = static_cast<int>(term);
+        idx = idx < 2 ? 2 : (idx > P.bins - 3 ? P.bins - 3 : idx);
+        const int start = idx - 1;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) arg[k] = static_cast<float>(static_cast<double>(start + k) - term);
+        for (int k = 0; k < 4; ++k) arg[k] = static_cast<float>(start + k) - term;
         return start;
     }
     __device__ __forceinline__ static bool in_range(const MattesParams &P, float mv) {
@@ -165,16 +182,26 @@ struct Parzen<float> {
     }
 };
 
-// B3 >= 0 (a rounding residue below zero becomes 0), scaled to 2^-32 units.
+// B3 in [0, 2/3] (a rounding residue below zero becomes 0), scaled to 2^-32
+// units: every contribution fits in 32 bits.
 template <typename R>
 __device__ __forceinline__ unsigned long long to_fixed(R v) {
     const double d = static_cast<double>(v);
     return d > 0.0 ? static_cast<unsigned long long>(llrint(d * kFix)) : 0ull;
 }
+template <>
+__device__ __forceinline__ unsigned long long to_fixed<float>(float v) {
+    return v > 0.0f ? static_cast<unsigned long long>(__float2uint_rn(v * 4294967296.0f)) : 0ull;
+}
 
 // ------------------------------------------------------------ FP64 reduction
-// Sum NSLOT per-lane values over the workgroup (fixed shuffle + LDS tree) and
-// write them to partials[logical * NSLOT + k].
+// Per-workgroup reduction of NSLOT per-lane FP64 sums (fixed shuffle pattern +
+// LDS) into partials[logical * NSLOT + k]. A separate one-workgroup launch
+// folds the partials in logical-workgroup (= sample) order. Both trees are
+// fixed, so the result is identical on every run. (No cross-workgroup
+// handshake inside a pass: on a multi-XCD device an agent-scope fence in every
+// workgroup writes back and invalidates its XCD's L2 and costs more than the
+// extra launch.)
 template <int NSLOT>
 __device__ __forceinline__ void block_reduce_store(double (&acc)[NSLOT], double *partials,
                                                    unsigned logical) {
@@ -197,26 +224,86 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[NSLOT], double 
     }
 }
 
-// Fold G partials in logical-workgroup (= sample) order, one lane per slot
-// group: out[k] = sum_b partials[b*NSLOT + k]. Launched as one workgroup.
+// out[k] = sum over G partials of slot k: lane t sums rows t, t+256, ... (all
+// loads issued first), then a fixed shuffle + LDS tree. One workgroup.
 template <int NSLOT>
 __global__ __launch_bounds__(kBlock) void fold_partials(const double *__restrict__ partials,
                                                         unsigned G, double *__restrict__ out) {
-    // 256 lanes: lane t sums partials of blocks t, t+256, ... for every slot,
-    // then a fixed tree over the 256 lane sums.
-    __shared__ double red[kBlock];
-    for (int k = 0; k < NSLOT; ++k) {
-        double t = 0.0;
-        for (unsigned b = threadIdx.x; b < G; b += kBlock) t += partials[static_cast<size_t>(b) * NSLOT + k];
-        red[threadIdx.x] = t;
-        __syncthreads();
-        for (int w = kBlock / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[k] = red[0];
-        __syncthreads();
+    __shared__ double fold[NSLOT][kBlock / 64];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    double t[NSLOT];
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) t[k] = 0.0;
+    for (unsigned b0 = 0; b0 < G; b0 += kBlock) {
+        const double *row = partials + static_cast<size_t>(b0 + threadIdx.x) * NSLOT;
+        double v[NSLOT];
+#pragma unroll
+        for (int k = 0; k < NSLOT; ++k) v[k] = row[k];
+#pragma unroll
+        for (int k = 0; k < NSLOT; ++k) t[k] += v[k];
     }
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) {
+        double x = t[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) fold[k][wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NSLOT) {
+        double x = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) x += fold[threadIdx.x][w];
+        out[threadIdx.x] = x;
+    }
+}
+
+// Samples per lane per iteration: all U samples' loads are issued before any of
+// their values is used, so each lane keeps U independent gathers in flight.
+#ifndef SITKHIP_BATCH
+#define SITKHIP_BATCH 4
+#endif
+constexpr int kU = SITKHIP_BATCH;
+
+// Phase A of the batched loop: which voxel each of the U samples is and
+// whether it exists (s < end and, for Mattes, not an excluded fixed bin).
+struct Batch {
+    uint32_t flat[kU];
+    int fb[kU];
+    bool live[kU];
+};
+
+// Phase B: located samples -- the in-cell fraction in the interpolation type R
+// and the 8 corner values. Loads are unconditional with clamped addresses, so
+// every lane issues the same loads; the fixed point is recomputed later rather
+// than held across the gathers.
+template <typename R>
+struct Located {
+    R fr[kU][3];
+    float v[kU][8];
+    bool ok[kU];
+};
+
+template <typename R>
+__device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol &M,
+                                             const Stages &S, const Batch &b, Located<R> &L) {
+    Cell cell[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        double x[3], p[3];
+        fixed_point(F, b.flat[u], x);
+        map_point(S, x, p);
+        L.ok[u] = b.live[u] && locate(M, p, cell[u]);
+        if (!L.ok[u]) {
+            cell[u].base[0] = cell[u].base[1] = cell[u].base[2] = 0;
+            cell[u].frac[0] = cell[u].frac[1] = cell[u].frac[2] = 0.0;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) L.fr[u][d] = static_cast<R>(cell[u].frac[d]);
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) corners(M, cell[u], L.v[u]);
 }
 
 // -------------------------------------------------------------- mean squares
@@ -233,30 +320,138 @@ __global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Stag
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    for (long long s = begin + threadIdx.x; s < end; s += kBlock) {
-        const uint32_t flat = sample_flat(SS, s);
-        double x[3];
-        R mv, g[3];
-        if (!eval_sample<R, MOMENTS>(F, M, S, flat, x, mv, g)) continue;
-        const double diff = static_cast<double>(mv) - static_cast<double>(fval[s]);
-        acc[0] = fma(diff, diff, acc[0]);
-        if (MOMENTS) {
+    for (long long s0 = begin + threadIdx.x; s0 < end; s0 += kBlock * kU) {
+        Batch b;
+        float fv[kU];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const double dg = diff * static_cast<double>(g[d]);
-                acc[1 + d] += dg;
+        for (int u = 0; u < kU; ++u) {
+            const long long s = s0 + static_cast<long long>(u) * kBlock;
+            b.live[u] = s < end;
+            const long long sc = b.live[u] ? s : begin;
+            b.flat[u] = sample_flat(SS, sc);
+            fv[u] = fval[sc];
+        }
+        Located<R> L;
+        locate_batch<R>(F, M, S, b, L);
 #pragma unroll
-                for (int e = 0; e < 3; ++e) acc[4 + d * 3 + e] = fma(dg, x[e], acc[4 + d * 3 + e]);
+        for (int u = 0; u < kU; ++u) {
+            if (!L.ok[u]) continue;
+            R mv, gi[3], g[3];
+            Trilinear<R>::template eval<MOMENTS>(L.fr[u], L.v[u], mv, gi);
+            const double diff = static_cast<double>(mv) - static_cast<double>(fv[u]);
+            acc[0] = fma(diff, diff, acc[0]);
+            if (MOMENTS) {
+                double x[3];
+                fixed_point(F, b.flat[u], x);
+                physical_gradient<R>(M, gi, g);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double dg = diff * static_cast<double>(g[d]);
+                    acc[1 + d] += dg;
+#pragma unroll
+                    for (int e = 0; e < 3; ++e)
+                        acc[4 + d * 3 + e] = fma(dg, x[e], acc[4 + d * 3 + e]);
+                }
+                acc[13] += 1.0;
+            } else {
+                acc[1] += 1.0;
             }
-            acc[13] += 1.0;
-        } else {
-            acc[1] += 1.0;
         }
     }
     block_reduce_store<NS>(acc, partials, logical);
 }
 
 // ------------------------------------------------------------------- Mattes
+struct TailOut {
+    double value;
+    double valid;  // as double: the host reads one block of doubles
+    double joint_sum;
+    double pad;
+};
+
+// The Mattes tail (mattes.rs:273-328) on the finished integer histogram, one
+// workgroup of kTail lanes: -MI, valid, and the pRatio table (pRatio * n_factor;
+// zero where the value sum skips the bin). Clears the histogram for the next
+// evaluation. The joint mass is summed exactly in integers and rounded once.
+constexpr int kTail = 1024;
+
+__device__ void mattes_tail_block(unsigned long long *g_joint, unsigned long long *g_marg, int B,
+                                  double mbs, double *pratio, float *pratio_f, TailOut *out,
+                                  unsigned long long *s_hist) {
+    __shared__ unsigned long long s_u[kTail];
+    __shared__ double s_d[kTail];
+    __shared__ double s_mm[256], s_fm[256], s_logfm[256];
+    const int t = threadIdx.x;
+    const int nbin = B * B;
+    const unsigned long long *hj = g_joint;
+    const unsigned long long *hm = g_marg;
+    if (s_hist) {
+        // stage the histogram in LDS
+        for (int i = t; i < nbin; i += kTail) s_hist[i] = g_joint[i];
+        for (int i = t; i < B; i += kTail) s_hist[nbin + i] = g_marg[i];
+        __syncthreads();
+        hj = s_hist;
+        hm = s_hist + nbin;
+    }
+    unsigned long long mass = 0;
+    for (int i = t; i < nbin; i += kTail) mass += hj[i];
+    s_u[t] = mass;
+    __syncthreads();
+    for (int w = kTail / 2; w > 0; w >>= 1) {
+        if (t < w) s_u[t] += s_u[t + w];
+        __syncthreads();
+    }
+    const double joint_sum = static_cast<double>(s_u[0]) * kInvFix;
+    unsigned long long valid = 0;
+    for (int f = 0; f < B; ++f) valid += hm[f];  // every lane, same order
+    const bool degenerate = valid == 0 || joint_sum < DBL_EPSILON;
+    const double inv_sum = degenerate ? 0.0 : 1.0 / joint_sum;
+    const double n_factor = degenerate ? 0.0 : 1.0 / (mbs * static_cast<double>(valid));
+    // marginals: fixed = counts * inv_sum; moving = column sums of the normalised
+    // joint, accumulated over f in order by 4 lanes per column then combined.
+    if (t < B) {
+        const double fm = static_cast<double>(hm[t]) * inv_sum;
+        s_fm[t] = fm;
+        s_logfm[t] = fm > DBL_EPSILON ? log(fm) : 0.0;
+    }
+    if (t < B) {
+        double mm = 0.0;
+        for (int f = 0; f < B; ++f) mm += static_cast<double>(hj[f * B + t]) * kInvFix * inv_sum;
+        s_mm[t] = mm;
+    }
+    __syncthreads();
+    double part = 0.0;
+    for (int i = t; i < nbin; i += kTail) {
+        double pr_out = 0.0;
+        const unsigned long long q = hj[i];
+        if (q && !degenerate) {
+            const int f = i / B, m = i - f * B;
+            const double jp = static_cast<double>(q) * kInvFix * inv_sum;
+            if (s_fm[f] > DBL_EPSILON && s_mm[m] > DBL_EPSILON && jp > DBL_EPSILON) {
+                const double pr = log(jp / s_mm[m]);
+                part += jp * (pr - s_logfm[f]);
+                pr_out = pr * n_factor;
+            }
+        }
+        pratio[i] = pr_out;
+        pratio_f[i] = static_cast<float>(pr_out);
+    }
+    s_d[t] = part;
+    __syncthreads();
+    for (int w = kTail / 2; w > 0; w >>= 1) {
+        if (t < w) s_d[t] += s_d[t + w];
+        __syncthreads();
+    }
+    // every read of the histogram is done: clear it for the next evaluation
+    for (int i = t; i < nbin; i += kTail) g_joint[i] = 0ull;
+    for (int i = t; i < B; i += kTail) g_marg[i] = 0ull;
+    if (t == 0) {
+        out->value = degenerate ? DBL_MAX : -s_d[0];
+        out->valid = static_cast<double>(valid);
+        out->joint_sum = joint_sum;
+    }
+}
+
 // Pass 1: joint histogram (fixed point) + fixed-marginal counts.
 template <typename R, bool LDS>
 __global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, Stages S,
@@ -276,23 +471,35 @@ __global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, 
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    for (long long s = begin + threadIdx.x; s < end; s += kBlock) {
-        const int fb = fbin[s];
-        if (fb == kNoBin) continue;
-        const uint32_t flat = sample_flat(SS, s);
-        double x[3];
-        R mv, g[3];
-        if (!eval_sample<R, false>(F, M, S, flat, x, mv, g)) continue;
-        if (!Parzen<R>::in_range(P, mv)) continue;
-        R arg[4];
-        const int m0 = Parzen<R>::taps(P, mv, arg);
-        unsigned long long *row = joint + fb * B + m0;
+    for (long long s0 = begin + threadIdx.x; s0 < end; s0 += kBlock * kU) {
+        Batch b;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const unsigned long long q = to_fixed(bspline3<R>(arg[k]));
-            if (q) atomicAdd(row + k, q);
+        for (int u = 0; u < kU; ++u) {
+            const long long s = s0 + static_cast<long long>(u) * kBlock;
+            const bool in = s < end;
+            const long long sc = in ? s : begin;
+            b.fb[u] = fbin[sc];
+            b.flat[u] = sample_flat(SS, sc);
+            b.live[u] = in && b.fb[u] != kNoBin;
         }
-        atomicAdd(marg + fb, 1ull);
+        Located<R> L;
+        locate_batch<R>(F, M, S, b, L);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (!L.ok[u]) continue;
+            R mv, gi[3];
+            Trilinear<R>::template eval<false>(L.fr[u], L.v[u], mv, gi);
+            if (!Parzen<R>::in_range(P, mv)) continue;
+            R arg[4];
+            const int m0 = Parzen<R>::taps(P, mv, arg);
+            unsigned long long *row = joint + b.fb[u] * B + m0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const unsigned long long q = to_fixed(bspline3<R>(arg[k]));
+                if (q) atomicAdd(row + k, q);
+            }
+            atomicAdd(marg + b.fb[u], 1ull);
+        }
     }
     if (LDS) {
         __syncthreads();
@@ -303,100 +510,19 @@ __global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, 
     }
 }
 
-struct TailOut {
-    double value;
-    double valid;      // as double for the host read-back
-    double joint_sum;
-    double pad;
-};
-
-// The Mattes tail on the device (mattes.rs:273-328): one workgroup.
-// Reads the integer histogram, writes value / valid and the pRatio table
-// (pRatio * n_factor, zero where the value sum skips the bin), then clears the
-// histogram for the next evaluation.
-__global__ __launch_bounds__(1024) void mattes_tail(unsigned long long *__restrict__ g_joint,
-                                                    unsigned long long *__restrict__ g_marg, int B,
-                                                    double mbs, double *__restrict__ pratio,
-                                                    float *__restrict__ pratio_f,
-                                                    TailOut *__restrict__ out) {
-    __shared__ unsigned long long s_u[1024];
-    __shared__ double s_d[1024];
-    __shared__ double s_mm[256];
-    __shared__ double s_logfm[256];
-    __shared__ double s_fm[256];
-    const int t = threadIdx.x;
-    const int nbin = B * B;
-
-    // valid = sum of marginal counts; joint mass = exact integer sum.
-    unsigned long long cnt = 0, mass = 0;
-    for (int i = t; i < B; i += 1024) cnt += g_marg[i];
-    for (int i = t; i < nbin; i += 1024) mass += g_joint[i];
-    s_u[t] = cnt;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if (t < w) s_u[t] += s_u[t + w];
-        __syncthreads();
-    }
-    const unsigned long long valid = s_u[0];
-    __syncthreads();
-    s_u[t] = mass;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if (t < w) s_u[t] += s_u[t + w];
-        __syncthreads();
-    }
-    const double joint_sum = static_cast<double>(s_u[0]) * kInvFix;
-    __syncthreads();
-
-    const bool degenerate = valid == 0 || joint_sum < DBL_EPSILON;
-    const double inv_sum = degenerate ? 0.0 : 1.0 / joint_sum;
-    const double n_factor = degenerate ? 0.0 : 1.0 / (mbs * static_cast<double>(valid));
-
-    // Marginals (fixed: counts * inv_sum; moving: column sums of the normalised joint).
-    for (int f = t; f < B; f += 1024) {
-        const double fm = static_cast<double>(g_marg[f]) * inv_sum;
-        s_fm[f] = fm;
-        s_logfm[f] = fm > DBL_EPSILON ? log(fm) : 0.0;
-    }
-    for (int m = t; m < B; m += 1024) {
-        double mm = 0.0;
-        for (int f = 0; f < B; ++f) mm += static_cast<double>(g_joint[f * B + m]) * kInvFix * inv_sum;
-        s_mm[m] = mm;
-    }
-    __syncthreads();
-
-    double part = 0.0;
-    for (int i = t; i < nbin; i += 1024) {
-        const int f = i / B, m = i - f * B;
-        const double jp = static_cast<double>(g_joint[i]) * kInvFix * inv_sum;
-        double pr_out = 0.0;
-        if (!degenerate && s_fm[f] > DBL_EPSILON && s_mm[m] > DBL_EPSILON && jp > DBL_EPSILON) {
-            const double pr = log(jp / s_mm[m]);
-            part += jp * (pr - s_logfm[f]);
-            pr_out = pr * n_factor;
-        }
-        pratio[i] = pr_out;
-        pratio_f[i] = static_cast<float>(pr_out);
-    }
-    s_d[t] = part;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if (t < w) s_d[t] += s_d[t + w];
-        __syncthreads();
-    }
-    if (t == 0) {
-        out->value = degenerate ? DBL_MAX : -s_d[0];
-        out->valid = static_cast<double>(valid);
-        out->joint_sum = joint_sum;
-    }
-    // Clear for the next evaluation (a kernel, not a memset, so the sequence can
-    // be captured in a graph).
-    __syncthreads();
-    for (int i = t; i < nbin; i += 1024) g_joint[i] = 0ull;
-    for (int i = t; i < B; i += 1024) g_marg[i] = 0ull;
+// The tail as its own one-workgroup launch. STAGE: the histogram is staged in
+// (dynamic) LDS first; otherwise (bin counts whose histogram does not fit) it is
+// read from global memory.
+template <bool STAGE>
+__global__ __launch_bounds__(kTail) void mattes_tail_kernel(unsigned long long *g_joint,
+                                                             unsigned long long *g_marg, int B,
+                                                             double mbs, double *pratio,
+                                                             float *pratio_f, TailOut *out) {
+    extern __shared__ unsigned long long s_stage[];
+    mattes_tail_block(g_joint, g_marg, B, mbs, pratio, pratio_f, out, STAGE ? s_stage : nullptr);
 }
 
-// Pass 2: the 12 pRatio-weighted moments.
+// Pass 2: the 12 pRatio-weighted moments, folded by the last workgroup.
 template <typename R, bool LDS>
 __global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M, Stages S,
                                                        SampleSet SS, const uint8_t *__restrict__ fbin,
@@ -417,27 +543,42 @@ __global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M,
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    for (long long s = begin + threadIdx.x; s < end; s += kBlock) {
-        const int fb = fbin[s];
-        if (fb == kNoBin) continue;
-        const uint32_t flat = sample_flat(SS, s);
-        double x[3];
-        R mv, g[3];
-        if (!eval_sample<R, true>(F, M, S, flat, x, mv, g)) continue;
-        if (!Parzen<R>::in_range(P, mv)) continue;
-        R arg[4];
-        const int m0 = Parzen<R>::taps(P, mv, arg);
-        const R *row = pr + fb * B + m0;
-        R w = R(0);
+    for (long long s0 = begin + threadIdx.x; s0 < end; s0 += kBlock * kU) {
+        Batch b;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) w += row[k] * bspline3_deriv<R>(arg[k]);
-        const double wd = static_cast<double>(w);
+        for (int u = 0; u < kU; ++u) {
+            const long long s = s0 + static_cast<long long>(u) * kBlock;
+            const bool in = s < end;
+            const long long sc = in ? s : begin;
+            b.fb[u] = fbin[sc];
+            b.flat[u] = sample_flat(SS, sc);
+            b.live[u] = in && b.fb[u] != kNoBin;
+        }
+        Located<R> L;
+        locate_batch<R>(F, M, S, b, L);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double wg = wd * static_cast<double>(g[d]);
-            acc[d] += wg;
+        for (int u = 0; u < kU; ++u) {
+            if (!L.ok[u]) continue;
+            R mv, gi[3], g[3];
+            Trilinear<R>::template eval<true>(L.fr[u], L.v[u], mv, gi);
+            if (!Parzen<R>::in_range(P, mv)) continue;
+            physical_gradient<R>(M, gi, g);
+            R arg[4];
+            const int m0 = Parzen<R>::taps(P, mv, arg);
+            const R *row = pr + b.fb[u] * B + m0;
+            R w = R(0);
 #pragma unroll
-            for (int e = 0; e < 3; ++e) acc[3 + d * 3 + e] = fma(wg, x[e], acc[3 + d * 3 + e]);
+            for (int k = 0; k < 4; ++k) w += row[k] * bspline3_deriv<R>(arg[k]);
+            const double wd = static_cast<double>(w);
+            double x[3];
+            fixed_point(F, b.flat[u], x);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double wg = wd * static_cast<double>(g[d]);
+                acc[d] += wg;
+#pragma unroll
+                for (int e = 0; e < 3; ++e) acc[3 + d * 3 + e] = fma(wg, x[e], acc[3 + d * 3 + e]);
+            }
         }
     }
     block_reduce_store<12>(acc, partials, logical);
@@ -503,6 +644,11 @@ __global__ __launch_bounds__(kBlock) void range_partials(const float *__restrict
 }
 
 // ------------------------------------------------------------------ host side
+struct HostResult {
+    TailOut tail;
+    double moments[14];
+};
+
 struct Metric {
     int kind = 0;
     int precision = 0;
@@ -515,7 +661,9 @@ struct Metric {
     DeviceBuffer<float> fval;     // per-sample fixed values (MS, and the range/bins source)
     DeviceBuffer<uint8_t> fbin;   // per-sample fixed Parzen bins (Mattes)
     DeviceBuffer<double> partials;
-    DeviceBuffer<double> moments;  // 14
+    // Results land in pinned, device-mapped host memory: no copy per evaluation.
+    HostResult *res = nullptr;        // host view
+    HostResult *res_dev = nullptr;    // device view of the same memory
     // Mattes
     bool bins_set = false;
     sitkhip_mattes_bins bins{};
@@ -523,7 +671,9 @@ struct Metric {
     DeviceBuffer<unsigned long long> joint, marg;
     DeviceBuffer<double> pratio;
     DeviceBuffer<float> pratio_f;
-    DeviceBuffer<TailOut> tail;
+    ~Metric() {
+        if (res) (void)hipHostFree(res);
+    }
     double fixed_range[2] = {0, 0};
     double moving_range[2] = {0, 0};
     size_t nsamples = 0;
@@ -598,6 +748,8 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
         if (fg.size[d] >= (1u << 31) || mg.size[d] >= (1u << 31))
             fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "axis longer than 2^31 voxels");
     }
+    if (moving.len() >= (1ull << 31))
+        fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "moving volume above 2^31 voxels");
     const size_t nf = fixed.len();
     if (nf >= (1ull << 32)) fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "fixed grid above 2^32 voxels");
     if (fmask && !same_geometry(fmask->geom, fg))
@@ -694,7 +846,10 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
         inverse3(mi2p, m->Mv.p2i);
 
         m->partials.alloc(kGrid * 14);
-        m->moments.alloc(14);
+        HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&m->res), sizeof(HostResult),
+                             hipHostMallocMapped | hipHostMallocCoherent));
+        std::memset(m->res, 0, sizeof(HostResult));
+        HIP_OK(hipHostGetDevicePointer(reinterpret_cast<void **>(&m->res_dev), m->res, 0));
         sync();
 
         // Value ranges (ops/mattes.rs:587-589): fixed over the sample set, moving
@@ -764,6 +919,7 @@ void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b) {
     m.P.mmax = b.moving_true_max;
     m.P.mmin_f = static_cast<float>(b.moving_true_min);
     m.P.mmax_f = static_cast<float>(b.moving_true_max);
+    m.P.inv_mbs_f = static_cast<float>(1.0 / b.moving_bin_size);
     // float bounds must not shrink the accepted range
     if (static_cast<double>(m.P.mmin_f) > b.moving_true_min)
         m.P.mmin_f = std::nextafter(m.P.mmin_f, -INFINITY);
@@ -781,7 +937,6 @@ void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b) {
     m.marg.alloc(static_cast<size_t>(B));
     m.pratio.alloc(static_cast<size_t>(B) * B);
     m.pratio_f.alloc(static_cast<size_t>(B) * B);
-    m.tail.alloc(1);
     HIP_OK(hipMemsetAsync(m.joint.ptr, 0, m.joint.bytes(), backend().stream));
     HIP_OK(hipMemsetAsync(m.marg.ptr, 0, m.marg.bytes(), backend().stream));
     m.bins_set = true;
@@ -795,32 +950,47 @@ static void require_bins(const Metric &m) {
 }
 
 template <typename R>
-static void launch_hist(Metric &m, const Stages &S) {
+static void launch_hist(Metric &m, const Stages &S, bool tail) {
     const int B = m.P.bins;
     const bool lds = B <= kLdsMaxBins;
     const size_t shm = lds ? static_cast<size_t>(B * B + B) * sizeof(unsigned long long) : 0;
-    LaunchScope scope("mattes_hist");
+    {
+        LaunchScope scope("mattes_hist");
+        if (lds)
+            hipLaunchKernelGGL((mattes_hist<R, true>), dim3(kGrid), dim3(kBlock), shm,
+                               backend().stream, m.F, m.Mv, S, m.SS, m.fbin.ptr, m.P, m.joint.ptr,
+                               m.marg.ptr);
+        else
+            hipLaunchKernelGGL((mattes_hist<R, false>), dim3(kGrid), dim3(kBlock), 0,
+                               backend().stream, m.F, m.Mv, S, m.SS, m.fbin.ptr, m.P, m.joint.ptr,
+                               m.marg.ptr);
+        check_launch("mattes_hist");
+    }
+    if (!tail) return;
+    TailOut *out = &m.res_dev->tail;
+    LaunchScope scope("mattes_tail");
     if (lds)
-        hipLaunchKernelGGL((mattes_hist<R, true>), dim3(kGrid), dim3(kBlock), shm, backend().stream,
-                           m.F, m.Mv, S, m.SS, m.fbin.ptr, m.P, m.joint.ptr, m.marg.ptr);
+        hipLaunchKernelGGL((mattes_tail_kernel<true>), dim3(1), dim3(kTail), shm, backend().stream,
+                           m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out);
     else
-        hipLaunchKernelGGL((mattes_hist<R, false>), dim3(kGrid), dim3(kBlock), 0, backend().stream,
-                           m.F, m.Mv, S, m.SS, m.fbin.ptr, m.P, m.joint.ptr, m.marg.ptr);
-    check_launch("mattes_hist");
+        hipLaunchKernelGGL((mattes_tail_kernel<false>), dim3(1), dim3(kTail), 0, backend().stream,
+                           m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out);
+    check_launch("mattes_tail");
 }
 
-static void launch_tail(Metric &m) {
-    LaunchScope scope("mattes_tail");
-    hipLaunchKernelGGL(mattes_tail, dim3(1), dim3(1024), 0, backend().stream, m.joint.ptr,
-                       m.marg.ptr, m.P.bins, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, m.tail.ptr);
-    check_launch("mattes_tail");
+template <int NS>
+static void launch_fold(Metric &m) {
+    LaunchScope scope("fold_partials");
+    hipLaunchKernelGGL((fold_partials<NS>), dim3(1), dim3(kBlock), 0, backend().stream,
+                       m.partials.ptr, kGrid, m.res_dev->moments);
+    check_launch("fold_partials");
 }
 
 template <typename R>
 static void launch_deriv(Metric &m, const Stages &S, const R *pratio) {
     const int B = m.P.bins;
     const size_t shm = static_cast<size_t>(B) * B * sizeof(R);
-    const bool lds = shm + 1024 <= 64 * 1024;  // plus the reduction's static LDS
+    const bool lds = shm + 2048 <= 64 * 1024;  // plus the reduction's static LDS
     {
         LaunchScope scope("mattes_deriv");
         if (lds)
@@ -833,10 +1003,7 @@ static void launch_deriv(Metric &m, const Stages &S, const R *pratio) {
                                m.partials.ptr);
         check_launch("mattes_deriv");
     }
-    LaunchScope scope("fold_partials");
-    hipLaunchKernelGGL((fold_partials<12>), dim3(1), dim3(kBlock), 0, backend().stream,
-                       m.partials.ptr, kGrid, m.moments.ptr);
-    check_launch("fold_partials");
+    launch_fold<12>(m);
 }
 
 void mattes_joint_histogram(Metric &m, const double *stages, int nstage, double *joint,
@@ -844,8 +1011,8 @@ void mattes_joint_histogram(Metric &m, const double *stages, int nstage, double 
     require_bins(m);
     Stages S = make_stages(stages, nstage);
     const int B = m.P.bins;
-    if (m.precision == SITKHIP_PRECISION_F64) launch_hist<double>(m, S);
-    else launch_hist<float>(m, S);
+    if (m.precision == SITKHIP_PRECISION_F64) launch_hist<double>(m, S, false);
+    else launch_hist<float>(m, S, false);
     std::vector<unsigned long long> hj(static_cast<size_t>(B) * B), hm(B);
     m.joint.download(hj.data(), hj.size());
     m.marg.download(hm.data(), hm.size());
@@ -877,23 +1044,21 @@ void mattes_derivative_moments(Metric &m, const double *stages, int nstage, cons
         m.pratio_f.upload(pf.data(), pf.size());
         launch_deriv<float>(m, S, m.pratio_f.ptr);
     }
-    m.moments.download(moments, 12);
     sync();
+    std::memcpy(moments, m.res->moments, 12 * sizeof(double));
 }
 
 static void value_pass(Metric &m, const Stages &S) {
-    if (m.precision == SITKHIP_PRECISION_F64) launch_hist<double>(m, S);
-    else launch_hist<float>(m, S);
-    launch_tail(m);
+    if (m.precision == SITKHIP_PRECISION_F64) launch_hist<double>(m, S, true);
+    else launch_hist<float>(m, S, true);
 }
 
 void mattes_value(Metric &m, const double *stages, int nstage, double *value, size_t *valid) {
     require_bins(m);
     Stages S = make_stages(stages, nstage);
     value_pass(m, S);
-    TailOut t;
-    m.tail.download(&t, 1);
     sync();
+    const TailOut &t = m.res->tail;
     if (value) *value = t.value;
     if (valid) *valid = static_cast<size_t>(t.valid);
 }
@@ -905,27 +1070,22 @@ void mattes_value_and_moments(Metric &m, const double *stages, int nstage, doubl
     value_pass(m, S);
     if (m.precision == SITKHIP_PRECISION_F64) launch_deriv<double>(m, S, m.pratio.ptr);
     else launch_deriv<float>(m, S, m.pratio_f.ptr);
-    TailOut t;
-    m.tail.download(&t, 1);
-    if (moments) m.moments.download(moments, 12);
     sync();
+    const TailOut &t = m.res->tail;
+    if (moments) std::memcpy(moments, m.res->moments, 12 * sizeof(double));
     if (value) *value = t.value;
     if (valid) *valid = static_cast<size_t>(t.valid);
 }
 
 template <typename R, bool MOM>
 static void launch_ms(Metric &m, const Stages &S) {
-    constexpr int NS = MOM ? 14 : 2;
     {
         LaunchScope scope(MOM ? "ms_moments" : "ms_value");
         hipLaunchKernelGGL((ms_pass<R, MOM>), dim3(kGrid), dim3(kBlock), 0, backend().stream, m.F,
                            m.Mv, S, m.SS, m.fval.ptr, m.partials.ptr);
         check_launch("ms_pass");
     }
-    LaunchScope scope("fold_partials");
-    hipLaunchKernelGGL((fold_partials<NS>), dim3(1), dim3(kBlock), 0, backend().stream,
-                       m.partials.ptr, kGrid, m.moments.ptr);
-    check_launch("fold_partials");
+    launch_fold<MOM ? 14 : 2>(m);
 }
 
 void ms_moments(Metric &m, const double *stages, int nstage, double out[14]) {
@@ -934,8 +1094,8 @@ void ms_moments(Metric &m, const double *stages, int nstage, double out[14]) {
     Stages S = make_stages(stages, nstage);
     if (m.precision == SITKHIP_PRECISION_F64) launch_ms<double, true>(m, S);
     else launch_ms<float, true>(m, S);
-    m.moments.download(out, 14);
     sync();
+    std::memcpy(out, m.res->moments, 14 * sizeof(double));
 }
 
 void ms_value(Metric &m, const double *stages, int nstage, double *sq, size_t *valid) {
@@ -944,11 +1104,9 @@ void ms_value(Metric &m, const double *stages, int nstage, double *sq, size_t *v
     Stages S = make_stages(stages, nstage);
     if (m.precision == SITKHIP_PRECISION_F64) launch_ms<double, false>(m, S);
     else launch_ms<float, false>(m, S);
-    double h[2];
-    m.moments.download(h, 2);
     sync();
-    if (sq) *sq = h[0];
-    if (valid) *valid = static_cast<size_t>(h[1]);
+    if (sq) *sq = m.res->moments[0];
+    if (valid) *valid = static_cast<size_t>(m.res->moments[1]);
 }
 
 }  // namespace sitkhip

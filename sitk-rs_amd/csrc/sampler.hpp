@@ -150,14 +150,16 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 
 // The 8 corner values, in corner order (bit d of the corner selects base+1 on
 // axis d), with neighbour indices clamped into the volume.
+// Offsets are 32-bit: a moving volume holds fewer than 2^31 voxels.
 __device__ __forceinline__ void corners(const MovingVol &M, const Cell &cell, float v[8]) {
-    int x0 = clampi(cell.base[0], 0, M.size[0] - 1), x1 = clampi(cell.base[0] + 1, 0, M.size[0] - 1);
-    int y0 = clampi(cell.base[1], 0, M.size[1] - 1), y1 = clampi(cell.base[1] + 1, 0, M.size[1] - 1);
-    int z0 = clampi(cell.base[2], 0, M.size[2] - 1), z1 = clampi(cell.base[2] + 1, 0, M.size[2] - 1);
-    const long long r00 = y0 * M.stride1 + z0 * M.stride2;
-    const long long r10 = y1 * M.stride1 + z0 * M.stride2;
-    const long long r01 = y0 * M.stride1 + z1 * M.stride2;
-    const long long r11 = y1 * M.stride1 + z1 * M.stride2;
+    const int x0 = clampi(cell.base[0], 0, M.size[0] - 1), x1 = clampi(cell.base[0] + 1, 0, M.size[0] - 1);
+    const int y0 = clampi(cell.base[1], 0, M.size[1] - 1), y1 = clampi(cell.base[1] + 1, 0, M.size[1] - 1);
+    const int z0 = clampi(cell.base[2], 0, M.size[2] - 1), z1 = clampi(cell.base[2] + 1, 0, M.size[2] - 1);
+    const int s1 = static_cast<int>(M.stride1), s2 = static_cast<int>(M.stride2);
+    const int r00 = y0 * s1 + z0 * s2;
+    const int r10 = y1 * s1 + z0 * s2;
+    const int r01 = y0 * s1 + z1 * s2;
+    const int r11 = y1 * s1 + z1 * s2;
     v[0] = M.vol[r00 + x0];
     v[1] = M.vol[r00 + x1];
     v[2] = M.vol[r10 + x0];
@@ -168,18 +170,19 @@ __device__ __forceinline__ void corners(const MovingVol &M, const Cell &cell, fl
     v[7] = M.vol[r11 + x1];
 }
 
-// Trilinear value and index-space gradient. Primary template: FP64 in the
-// reference's order (interpolator.rs:103-149): weight = ((1*w0)*w1)*w2,
-// value += weight*b, grad[j] += (sign*w_without_j)*b, every operation rounded.
+// Trilinear value and index-space gradient from the in-cell fraction `fr` and
+// the 8 corner values. double: the reference's order (interpolator.rs:103-149),
+// weight = ((1*w0)*w1)*w2, value += weight*b, grad[j] += (sign*w_without_j)*b,
+// every operation rounded. float: the separable form with FMAs.
 template <typename R>
 struct Trilinear;
 
 template <>
 struct Trilinear<double> {
     template <bool GRAD>
-    __device__ __forceinline__ static void eval(const Cell &cell, const float vf[8], double &value,
-                                                double g[3]) {
-        const double f0 = cell.frac[0], f1 = cell.frac[1], f2 = cell.frac[2];
+    __device__ __forceinline__ static void eval(const double fr[3], const float vf[8],
+                                                double &value, double g[3]) {
+        const double f0 = fr[0], f1 = fr[1], f2 = fr[2];
         const double u0 = 1.0 - f0, u1 = 1.0 - f1, u2 = 1.0 - f2;
         double v = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
 #pragma unroll
@@ -207,14 +210,10 @@ struct Trilinear<double> {
 template <>
 struct Trilinear<float> {
     template <bool GRAD>
-    __device__ __forceinline__ static void eval(const Cell &cell, const float vf[8], float &value,
+    __device__ __forceinline__ static void eval(const float fr[3], const float vf[8], float &value,
                                                 float g[3]) {
-        const float f0 = static_cast<float>(cell.frac[0]);
-        const float f1 = static_cast<float>(cell.frac[1]);
-        const float f2 = static_cast<float>(cell.frac[2]);
-        const float u0 = static_cast<float>(1.0 - cell.frac[0]);
-        const float u1 = static_cast<float>(1.0 - cell.frac[1]);
-        const float u2 = static_cast<float>(1.0 - cell.frac[2]);
+        const float f0 = fr[0], f1 = fr[1], f2 = fr[2];
+        const float u0 = 1.0f - f0, u1 = 1.0f - f1, u2 = 1.0f - f2;
         // Separable form: interpolate along axis 0 first, then 1, then 2.
         const float a00 = fmaf(f0, vf[1], u0 * vf[0]);
         const float a10 = fmaf(f0, vf[3], u0 * vf[2]);

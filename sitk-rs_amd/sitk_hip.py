@@ -18,7 +18,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsitk_hip.so")
+# SITK_HIP_LIBRARY points the binding at another build of the same library.
+LIB_PATH = os.environ.get("SITK_HIP_LIBRARY", os.path.join(_HERE, "libsitk_hip.so"))
 
 OK = 0
 PIXEL = {
