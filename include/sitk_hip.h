@@ -125,6 +125,28 @@ void sitkhip_mask_free(sitkhip_mask m);
  * FP64 recursion, one narrowing to float at the end. Output geometry = input. */
 int sitkhip_recursive_gaussian(sitkhip_image in, const double sigma[3], sitkhip_image *out);
 
+/* ------------------------------------------------------ pyramid placement */
+/* shrunk_geometry (src/cuda/ops/pyramid.rs:355-420, ITK ShrinkImageFilter): size
+ * max(1, size / f), spacing * f, origin moved by Direction * diag(spacing) *
+ * delta with delta = (size - 1)/2 - f (out - 1)/2. */
+int sitkhip_shrunk_geometry(const sitkhip_geometry *in, const size_t factors[3],
+                            sitkhip_geometry *out);
+/* shrink (src/cuda/ops/pyramid.rs:422-470): gather every f-th voxel from the
+ * rounded centre offset. */
+int sitkhip_shrink(sitkhip_image in, const size_t factors[3], sitkhip_image *out);
+/* resample_linear[_through] (src/cuda/ops/pyramid.rs:677-738): the input
+ * sampled at the point-map image of every output voxel, linear, default value
+ * outside. stages NULL / nstage 0: the identity map. */
+int sitkhip_resample_linear(sitkhip_image in, const sitkhip_geometry *out_geometry,
+                            const double *stages, int nstage, double default_value,
+                            sitkhip_image *out);
+/* resample_nearest[_through] (src/cuda/ops/pyramid.rs:739-790), round half up. */
+int sitkhip_resample_nearest(sitkhip_image in, const sitkhip_geometry *out_geometry,
+                             const double *stages, int nstage, double default_value,
+                             sitkhip_image *out);
+int sitkhip_mask_resample_nearest(sitkhip_mask in, const sitkhip_geometry *out_geometry,
+                                  const double *stages, int nstage, sitkhip_mask *out);
+
 /* ------------------------------------------------------- resident metrics */
 #define SITKHIP_MAX_STAGES 8 /* MAX_STAGES (src/cuda/ops/resident.rs:74) */
 #define SITKHIP_MAX_BINS 255 /* fixed Parzen bins are held as uint8 per sample */
@@ -232,6 +254,74 @@ int sitkhip_contract_ms(const double moments[14], const double *j0, const double
 /* contract_mattes (registration/cuda.rs:505-524). */
 int sitkhip_contract_mattes(const double moments[12], const double *j0, const double *c,
                             int nparams, double *deriv);
+
+/* ----------------------------------------------------------- registration */
+/* ImageRegistrationMethod::execute_on_device (src/registration/method.rs:
+ * 2698-2879): a multi-level registration with every volume resident on the
+ * device; the optimizers (src/registration/optimizer.rs:271-817) and the
+ * physical-shift scales / learning-rate estimator (src/registration/
+ * scales.rs:225-871) run on the host, one metric evaluation per probe. */
+#define SITKHIP_MAX_LEVELS 8
+
+typedef enum {
+    SITKHIP_TRANSFORM_TRANSLATION = 0, /* t[3] */
+    SITKHIP_TRANSFORM_EULER3D = 1,     /* angle x, y, z (rad), t[3]; about center */
+    SITKHIP_TRANSFORM_AFFINE = 2       /* A row-major [9], t[3]; about center */
+} sitkhip_transform_kind;
+
+typedef struct {
+    int kind;
+    int compute_zyx; /* Euler3D composition Rz.Ry.Rx instead of Rz.Rx.Ry */
+    double center[3];
+    double params[12];
+} sitkhip_transform;
+
+typedef enum { SITKHIP_OPTIMIZER_REGULAR_STEP = 0, SITKHIP_OPTIMIZER_LINE_SEARCH = 1 } sitkhip_optimizer;
+typedef enum { SITKHIP_LR_MANUAL = 0, SITKHIP_LR_ONCE = 1, SITKHIP_LR_EACH_ITERATION = 2 } sitkhip_lr_mode;
+typedef enum { SITKHIP_SCALES_UNIT = 0, SITKHIP_SCALES_PHYSICAL_SHIFT = 1 } sitkhip_scales_mode;
+typedef enum {
+    SITKHIP_STOP_MAX_ITERATIONS = 0,
+    SITKHIP_STOP_GRADIENT_CONVERGED = 1,
+    SITKHIP_STOP_STEP_TOO_SMALL = 2,
+    SITKHIP_STOP_CONVERGED = 3
+} sitkhip_stop_reason;
+
+typedef struct {
+    int metric;                 /* sitkhip_metric_kind */
+    int bins;                   /* Mattes histogram bins */
+    int sampling;               /* 0 every voxel, 1 regular, 2 random (SplitMix64) */
+    double sampling_percentage;
+    uint64_t seed;
+    int precision;              /* sitkhip_precision */
+    int optimizer;              /* sitkhip_optimizer */
+    double learning_rate;       /* manual rate (RSGD: initial step length) */
+    double minimum_step;        /* RSGD stop */
+    int iterations;             /* per level */
+    double relaxation;          /* RSGD step relaxation on reversal (0.5) */
+    double gradient_tolerance;  /* RSGD stationary-point stop (1e-4) */
+    int learning_rate_mode;     /* sitkhip_lr_mode */
+    int scales_mode;            /* sitkhip_scales_mode */
+    int levels;
+    size_t shrink[SITKHIP_MAX_LEVELS][3];
+    double sigma[SITKHIP_MAX_LEVELS]; /* physical smoothing sigma per level, all axes */
+} sitkhip_registration_config;
+
+typedef struct {
+    double value;               /* metric value at the result, last level */
+    size_t valid_points;
+    int iterations;             /* summed over levels */
+    int level_iterations[SITKHIP_MAX_LEVELS];
+    int level_stop[SITKHIP_MAX_LEVELS];
+    double level_value[SITKHIP_MAX_LEVELS];
+    size_t evaluations;         /* value + derivative evaluations */
+    size_t value_evaluations;   /* value-only evaluations (line-search probes) */
+    double seconds;             /* wall clock of the whole run */
+} sitkhip_registration_result;
+
+/* transform: the initial transform in, the registered one out. */
+int sitkhip_register(sitkhip_image fixed, sitkhip_image moving,
+                     const sitkhip_registration_config *config, sitkhip_transform *transform,
+                     sitkhip_registration_result *result);
 
 /* -------------------------------------------------------------- profiling */
 /* When enabled, every kernel launch of this library is bracketed by HIP events

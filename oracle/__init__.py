@@ -129,6 +129,8 @@ def lib():
         L.orc_ms_evaluate.restype = C.c_double
         L.orc_ms_value.argtypes = [pp, tp, szp]
         L.orc_ms_value.restype = C.c_double
+        L.orc_resample_linear.argtypes = [fp, szp, dp, dp, dp, szp, dp, dp, dp, tp, C.c_double, fp]
+        L.orc_shrunk_geometry.argtypes = [szp, dp, dp, dp, szp, szp, dp, dp]
         L.orc_fixture_volume.argtypes = [C.c_size_t, dp, fp]
         L.orc_fixture_inverted.argtypes = [C.c_size_t, dp, fp]
         L.orc_fixture_commensurate.argtypes = [C.c_size_t, fp]
@@ -423,3 +425,27 @@ def set_joint_sum_compensated(on: bool) -> None:
 def compensated_sum(v) -> float:
     a = np.ascontiguousarray(v, dtype=np.float64)
     return lib().orc_compensated_sum(_dp(a), a.size)
+
+
+def resample_linear(vol, in_spacing, in_origin, out_size, out_spacing, out_origin, t,
+                    default=0.0, in_dir=None, out_dir=None):
+    """vol: (nz, ny, nx) float32; out_size (nx, ny, nz). Returns (nz', ny', nx')."""
+    v = np.ascontiguousarray(vol, dtype=np.float32)
+    nz, ny, nx = v.shape
+    eye = (C.c_double * 9)(*np.eye(3).reshape(9))
+    idir = eye if in_dir is None else (C.c_double * 9)(*np.asarray(in_dir, float).reshape(9))
+    odir = eye if out_dir is None else (C.c_double * 9)(*np.asarray(out_dir, float).reshape(9))
+    out = np.empty(int(np.prod(out_size)), dtype=np.float32)
+    lib().orc_resample_linear(_fp(v), _szp((nx, ny, nz)), _d3(in_spacing), _d3(in_origin), idir,
+                              _szp(out_size), _d3(out_spacing), _d3(out_origin), odir,
+                              C.byref(t), default, _fp(out))
+    return out.reshape(out_size[2], out_size[1], out_size[0])
+
+
+def shrunk_geometry(size, spacing, origin, factors, direction=None):
+    d = (C.c_double * 9)(*(np.eye(3) if direction is None else np.asarray(direction, float)).reshape(9))
+    osz = (C.c_size_t * 3)()
+    osp = (C.c_double * 3)()
+    oor = (C.c_double * 3)()
+    lib().orc_shrunk_geometry(_szp(size), _d3(spacing), _d3(origin), d, _szp(factors), osz, osp, oor)
+    return list(osz), list(osp), list(oor)

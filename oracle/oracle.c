@@ -1191,3 +1191,61 @@ void orc_fixture_commensurate(size_t n, float *out) {
         for (size_t j = 0; j < n; ++j)
             for (size_t i = 0; i < n; ++i) out[o++] = (float)((i * 7 + j * 13 + k * 3) % 64);
 }
+
+/* ------------------------------------------------------------------------- */
+/* ResampleImageFilter::execute (src/transform/resample.rs:226-290) with the
+ * linear interpolator, Float32 in and out: out index -> physical (origin
+ * first) -> transform_point -> input continuous index (mat_vec seeded 0.0)
+ * -> linear_at, default outside. Geometry arrays: size[3], spacing[3],
+ * origin[3], direction[9]. */
+void orc_resample_linear(const float *in, const size_t in_size[3], const double in_spacing[3],
+                         const double in_origin[3], const double in_dir[9],
+                         const size_t out_size[3], const double out_spacing[3],
+                         const double out_origin[3], const double out_dir[9],
+                         const orc_transform *t, double dflt, float *out) {
+    double o2p[9], i2p[9], p2i[9];
+    orc_index_to_physical_matrix(out_dir, out_spacing, o2p);
+    orc_index_to_physical_matrix(in_dir, in_spacing, i2p);
+    orc_invert3(i2p, p2i);
+    orc_problem pb;
+    memset(&pb, 0, sizeof(pb));
+    for (int d = 0; d < 3; ++d) { pb.msize[d] = in_size[d]; pb.morigin[d] = in_origin[d]; }
+    pb.mvals = in;
+    size_t n = out_size[0] * out_size[1] * out_size[2];
+    size_t idx[3] = {0, 0, 0};
+    for (size_t o = 0; o < n; ++o) {
+        double phys[3], mapped[3], diff[3], c[3];
+        for (int r = 0; r < 3; ++r) {
+            double acc = out_origin[r];
+            for (int k = 0; k < 3; ++k) acc += o2p[r * 3 + k] * (double)idx[k];
+            phys[r] = acc;
+        }
+        orc_transform_point(t, phys, mapped);
+        for (int d = 0; d < 3; ++d) diff[d] = mapped[d] - in_origin[d];
+        mat_vec3(p2i, diff, c);
+        double v;
+        out[o] = (float)(linear_at(&pb, c, &v) ? v : dflt);
+        for (int d = 0; d < 3; ++d) {
+            if (++idx[d] < out_size[d]) break;
+            idx[d] = 0;
+        }
+    }
+}
+
+/* shrink geometry (src/filters/shrink.rs:51-91) */
+void orc_shrunk_geometry(const size_t size[3], const double spacing[3], const double origin[3],
+                         const double dir[9], const size_t f[3], size_t out_size[3],
+                         double out_spacing[3], double out_origin[3]) {
+    double delta[3];
+    for (int d = 0; d < 3; ++d) {
+        size_t s = size[d] / f[d];
+        out_size[d] = s < 1 ? 1 : s;
+        out_spacing[d] = spacing[d] * (double)f[d];
+        delta[d] = ((double)size[d] - 1.0) / 2.0 - (double)f[d] * ((double)out_size[d] - 1.0) / 2.0;
+    }
+    for (int i = 0; i < 3; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < 3; ++j) acc += dir[i * 3 + j] * spacing[j] * delta[j];
+        out_origin[i] = origin[i] + acc;
+    }
+}

@@ -188,6 +188,17 @@ double orc_ms_evaluate(const orc_problem *pb, const orc_transform *t,
                        double *deriv, size_t *valid);
 double orc_ms_value(const orc_problem *pb, const orc_transform *t, size_t *valid);
 
+/* ---- pyramid placement (src/transform/resample.rs:226-290,
+ *      src/filters/shrink.rs:51-91) ---- */
+void orc_resample_linear(const float *in, const size_t in_size[3], const double in_spacing[3],
+                         const double in_origin[3], const double in_dir[9],
+                         const size_t out_size[3], const double out_spacing[3],
+                         const double out_origin[3], const double out_dir[9],
+                         const orc_transform *t, double dflt, float *out);
+void orc_shrunk_geometry(const size_t size[3], const double spacing[3], const double origin[3],
+                         const double dir[9], const size_t f[3], size_t out_size[3],
+                         double out_spacing[3], double out_origin[3]);
+
 /* ---- fixtures of tests/mattes_device.rs:42-100 ---- */
 void orc_fixture_volume(size_t n, const double shift[3], float *out);
 void orc_fixture_inverted(size_t n, const double shift[3], float *out);

@@ -42,6 +42,7 @@
 
 #include "common.hpp"
 #include "image.hpp"
+#include "metric.hpp"
 #include "sampler.hpp"
 
 namespace sitkhip {
@@ -702,7 +703,7 @@ static void range_of(const float *v, const uint8_t *mask, long long n, double ou
     else { out[0] = lo; out[1] = hi; }
 }
 
-static void inverse3(const double m[9], double out[9]) {
+void invert3_host(const double m[9], double out[9]) {
     // Gauss-Jordan with partial pivoting, dividing rows by the pivot
     // (src/core/matrix.rs:44-102), so a diagonal geometry gives diag(1/spacing).
     double a[3][6];
@@ -843,7 +844,7 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
         m->Mv.stride2 = static_cast<long long>(mg.size[0] * mg.size[1]);
         double mi2p[9];
         index_to_physical(mg, mi2p);
-        inverse3(mi2p, m->Mv.p2i);
+        invert3_host(mi2p, m->Mv.p2i);
 
         m->partials.alloc(kGrid * 14);
         HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&m->res), sizeof(HostResult),
@@ -861,6 +862,13 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
         throw;
     }
     return m;
+}
+
+void metric_destroy(Metric *m) { delete m; }
+size_t metric_sample_count(const Metric &m) { return m.nsamples; }
+void metric_value_ranges(const Metric &m, double fr[2], double mr[2]) {
+    fr[0] = m.fixed_range[0]; fr[1] = m.fixed_range[1];
+    mr[0] = m.moving_range[0]; mr[1] = m.moving_range[1];
 }
 
 static void check_stages(int nstage, const double *stages) {

@@ -1,0 +1,34 @@
+// Host entry points of the resident metrics (defined in metric.hip), for the
+// registration driver in the same library.
+#pragma once
+
+#include "common.hpp"
+#include "image.hpp"
+
+namespace sitkhip {
+
+struct Metric;
+
+Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
+                      const Mask *fmask, const Image &moving, const Mask *mmask, int precision);
+void metric_destroy(Metric *m);
+size_t metric_sample_count(const Metric &m);
+void metric_value_ranges(const Metric &m, double fixed_range[2], double moving_range[2]);
+void mattes_bins_from_ranges(const double fr[2], const double mr[2], size_t bins,
+                             sitkhip_mattes_bins *out);
+void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b);
+void mattes_value(Metric &m, const double *stages, int nstage, double *value, size_t *valid);
+void mattes_value_and_moments(Metric &m, const double *stages, int nstage, double *value,
+                              double moments[12], size_t *valid);
+void ms_moments(Metric &m, const double *stages, int nstage, double out[14]);
+void ms_value(Metric &m, const double *stages, int nstage, double *sq, size_t *valid);
+void invert3_host(const double m[9], double out[9]);
+
+// Pyramid placement (pyramid.hip).
+void shrunk_geometry(const sitkhip_geometry &g, const size_t f[3], sitkhip_geometry &out,
+                     long long offset[3]);
+Image *resample_linear(const Image &in, const sitkhip_geometry &og, const double *stages,
+                       int nstage, double dflt);
+Image *recursive_gaussian(const Image &in, const double sigma[3]);
+
+}  // namespace sitkhip

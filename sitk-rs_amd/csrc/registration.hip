@@ -1,0 +1,489 @@
+// The registration driver: a multi-level ImageRegistrationMethod run with every
+// volume resident on the device (the reference's execute_on_device,
+// crates/sitk/src/registration/method.rs:2698-2879, with its level preparation
+// :2416-2569 / :2919-2963 and optimizer dispatch `drive` :3140-3529).
+//
+// The optimizers and the scales / learning-rate estimator are host logic, as in
+// the reference: each iteration hands the device one point-map stage, the device
+// evaluates the metric (value + 12 or 14 moments) and the host contracts the
+// moments with the transform's Jacobian decomposition.
+//   RegularStepGradientDescent      src/registration/optimizer.rs:271-450
+//   GradientDescentLineSearch       src/registration/optimizer.rs:462-817
+//   WindowConvergenceMonitor        src/registration/convergence.rs
+//   physical-shift scales / rates   src/registration/scales.rs:225-871
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "image.hpp"
+#include "metric.hpp"
+#include "transform.hpp"
+
+namespace sitkhip {
+
+// ITK's CompensatedSummation: the running sum without the final residual
+// (src/core/compensated.rs:115-128).
+static double compensated(const std::vector<double> &v) {
+    double sum = 0.0, comp = 0.0;
+    for (double x : v) {
+        const double ci = x - comp;
+        const double t = sum + ci;
+        comp = (t - sum) - ci;
+        sum = t;
+    }
+    return sum;
+}
+
+// ------------------------------------------------------------ scales estimator
+// RegistrationParameterScalesFromPhysicalShift for a global linear transform:
+// both sample sets are the 8 corners of the virtual grid (scales.rs:489-515),
+// with the Jacobians cached once (they do not depend on the parameters).
+struct ScalesEstimator {
+    int np = 0;
+    double max_step = 1.0;           // minimum virtual spacing
+    double delta = 0.01;             // small parameter variation
+    std::vector<double> jac;         // 8 x (3 x np)
+
+    void build(const sitkhip_geometry &grid, const Transform &t) {
+        np = t.nparams;
+        max_step = std::min(grid.spacing[0], std::min(grid.spacing[1], grid.spacing[2]));
+        jac.assign(static_cast<size_t>(8) * 3 * np, 0.0);
+        double i2p[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) i2p[r * 3 + c] = grid.direction[r * 3 + c] * grid.spacing[c];
+        for (int i = 0; i < 8; ++i) {
+            double idx[3], x[3];
+            for (int d = 0; d < 3; ++d)
+                idx[d] = ((i >> d) & 1) ? static_cast<double>(grid.size[d] - 1) : 0.0;
+            for (int r = 0; r < 3; ++r) {
+                double v = grid.origin[r];
+                for (int c = 0; c < 3; ++c) v += i2p[r * 3 + c] * idx[c];
+                x[r] = v;
+            }
+            t.jacobian(x, jac.data() + static_cast<size_t>(i) * 3 * np);
+        }
+    }
+
+    // shift_scales (scales.rs:659-718)
+    std::vector<double> scales() const {
+        std::vector<double> shifts(np, 0.0);
+        for (int s = 0; s < 8; ++s) {
+            const double *j = jac.data() + static_cast<size_t>(s) * 3 * np;
+            for (int p = 0; p < np; ++p) {
+                double sq = 0.0;
+                for (int r = 0; r < 3; ++r) sq += j[r * np + p] * j[r * np + p];
+                shifts[p] = std::max(shifts[p], delta * std::sqrt(sq));
+            }
+        }
+        double min_nz = INFINITY;
+        for (double s : shifts)
+            if (s > DBL_EPSILON) min_nz = std::min(min_nz, s);
+        if (!std::isfinite(min_nz)) return std::vector<double>(np, 1.0);
+        const double inv_d2 = 1.0 / (delta * delta);
+        std::vector<double> out(np);
+        for (int p = 0; p < np; ++p) {
+            const double base = shifts[p] <= DBL_EPSILON ? min_nz * min_nz : shifts[p] * shifts[p];
+            out[p] = base * inv_d2;
+        }
+        return out;
+    }
+
+    double max_shift(const std::vector<double> &d) const {
+        double m = 0.0;
+        for (int s = 0; s < 8; ++s) {
+            const double *j = jac.data() + static_cast<size_t>(s) * 3 * np;
+            double sq = 0.0;
+            for (int r = 0; r < 3; ++r) {
+                double v = 0.0;
+                for (int p = 0; p < np; ++p) v += j[r * np + p] * d[p];
+                sq += v * v;
+            }
+            m = std::max(m, sq);
+        }
+        return std::sqrt(m);
+    }
+
+    // shift_step_scale (scales.rs:777-792) and estimate_learning_rate (:818-826)
+    double learning_rate(const std::vector<double> &step) const {
+        double mx = 0.0;
+        for (double v : step) mx = std::max(mx, std::fabs(v));
+        double scale = 0.0;
+        if (mx > DBL_EPSILON) {
+            const double factor = delta / mx;
+            std::vector<double> small(step.size());
+            for (size_t k = 0; k < step.size(); ++k) small[k] = step[k] * factor;
+            scale = max_shift(small) / factor;
+        }
+        return scale <= DBL_EPSILON ? 1.0 : max_step / scale;
+    }
+};
+
+// ------------------------------------------------------------ the objective
+struct LevelObjective {
+    Metric *metric = nullptr;
+    int kind = SITKHIP_METRIC_MATTES;
+    Transform t;
+    size_t evaluations = 0, value_evaluations = 0;
+    size_t last_valid = 0;
+
+    // Value and parameter derivative at p.
+    double value_and_gradient(const std::vector<double> &p, std::vector<double> &g) {
+        t.set_parameters(p.data());
+        double st[12];
+        t.stage(st);
+        double j0[36], c[108];
+        t.affine_form(j0, c);
+        const int np = t.nparams;
+        g.assign(np, 0.0);
+        ++evaluations;
+        if (kind == SITKHIP_METRIC_MATTES) {
+            double value, mom[12];
+            size_t valid;
+            mattes_value_and_moments(*metric, st, 1, &value, mom, &valid);
+            last_valid = valid;
+            for (int k = 0; k < np; ++k) {
+                double acc = 0.0;
+                for (int d = 0; d < 3; ++d) {
+                    acc += j0[d * np + k] * mom[d];
+                    for (int e = 0; e < 3; ++e) acc += c[(e * 3 + d) * np + k] * mom[3 + d * 3 + e];
+                }
+                g[k] = acc;
+            }
+            return value;
+        }
+        double mom[14];
+        ms_moments(*metric, st, 1, mom);
+        last_valid = static_cast<size_t>(mom[13]);
+        if (mom[13] == 0.0) return DBL_MAX;
+        const double inv = 1.0 / mom[13];
+        for (int k = 0; k < np; ++k) {
+            double acc = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                acc += j0[d * np + k] * mom[1 + d];
+                for (int e = 0; e < 3; ++e) acc += c[(e * 3 + d) * np + k] * mom[4 + d * 3 + e];
+            }
+            g[k] = 2.0 * acc * inv;
+        }
+        return mom[0] * inv;
+    }
+
+    double value(const std::vector<double> &p) {
+        t.set_parameters(p.data());
+        double st[12];
+        t.stage(st);
+        ++value_evaluations;
+        if (kind == SITKHIP_METRIC_MATTES) {
+            double v;
+            size_t valid;
+            mattes_value(*metric, st, 1, &v, &valid);
+            return v;
+        }
+        double sq;
+        size_t valid;
+        ms_value(*metric, st, 1, &sq, &valid);
+        return valid == 0 ? DBL_MAX : sq / static_cast<double>(valid);
+    }
+};
+
+// --------------------------------------------------------------- optimizers
+struct OptResult {
+    std::vector<double> params;
+    double value;
+    int iterations;
+    int stop;
+};
+
+// RegularStepGradientDescentOptimizer::optimize_with_lr (optimizer.rs:353-450).
+static OptResult regular_step(const sitkhip_registration_config &cfg, const std::vector<double> &scales,
+                              std::vector<double> p, LevelObjective &obj,
+                              const std::function<double(const std::vector<double> &)> &lr_of) {
+    const size_t n = p.size();
+    std::vector<double> grad;
+    double value = obj.value_and_gradient(p, grad);
+    std::vector<double> prev(n, 0.0), scaled(n);
+    double relaxation = 1.0;
+    int stop = SITKHIP_STOP_MAX_ITERATIONS;
+    int taken = 0;
+    for (;;) {
+        for (size_t k = 0; k < n; ++k) scaled[k] = grad[k] / scales[k];
+        std::vector<double> sq(n), dot(n);
+        for (size_t k = 0; k < n; ++k) { sq[k] = scaled[k] * scaled[k]; dot[k] = scaled[k] * prev[k]; }
+        const double gm = std::sqrt(compensated(sq));
+        if (gm < cfg.gradient_tolerance) { stop = SITKHIP_STOP_GRADIENT_CONVERGED; break; }
+        if (compensated(dot) < 0.0) relaxation *= cfg.relaxation;
+        const double step = relaxation * lr_of(scaled);
+        if (step < cfg.minimum_step) { stop = SITKHIP_STOP_STEP_TOO_SMALL; break; }
+        if (taken >= cfg.iterations) break;
+        const double factor = step / gm;
+        for (size_t k = 0; k < n; ++k) p[k] -= factor * scaled[k];
+        prev = scaled;
+        ++taken;
+        value = obj.value_and_gradient(p, grad);
+    }
+    return {p, value, taken, stop};
+}
+
+// WindowConvergenceMonitor (convergence.rs): the negated slope of a line fit to
+// the last `window` values normalised by the running total |energy|.
+struct ConvergenceMonitor {
+    size_t window;
+    std::deque<double> values;
+    double total = 0.0;
+    explicit ConvergenceMonitor(size_t w) : window(w) {}
+    void add(double v) {
+        values.push_back(v);
+        if (values.size() > window) values.pop_front();
+        total += std::fabs(v);
+    }
+    bool value(double &cv) const {
+        if (values.size() < window) return false;
+        if (total == 0.0) { cv = 0.0; return true; }
+        const double w = static_cast<double>(window);
+        double o0 = 0, o1 = 0, d0 = 0, d1 = 0;
+        size_t n = 0;
+        for (double e : values) {
+            const double x = static_cast<double>(n) / (w - 1.0);
+            const double a = 1.0 - x, b = x, y = e / total, dd = a * a + b * b;
+            o0 += a * a;
+            o1 += b * b;
+            d0 += a * a * a * y / dd;
+            d1 += b * b * b * y / dd;
+            ++n;
+        }
+        const double c0 = o0 != 0.0 ? d0 / o0 : 0.0;
+        const double c1 = o1 != 0.0 ? d1 / o1 : 0.0;
+        cv = c0 - c1;
+        return true;
+    }
+};
+
+// golden_section_search (optimizer.rs:475-545), iterative form of the same
+// recursion.
+static double golden_section(double eps, unsigned max_it, double a, double b, double c,
+                             const std::function<double(double)> &f) {
+    const double phi = 1.618034, resphi = 2.0 - phi;
+    bool have_b = false;
+    double fb = 0.0;
+    unsigned it = 0;
+    for (;;) {
+        if (it > max_it) return (c + a) / 2.0;
+        ++it;
+        const double x = (c - b > b - a) ? b + resphi * (c - b) : b - resphi * (b - a);
+        if (std::fabs(c - a) < eps * (std::fabs(b) + std::fabs(x))) return (c + a) / 2.0;
+        const double fx = f(x);
+        if (!have_b) { fb = f(b); have_b = true; }
+        if (fx < fb) {
+            if (c - b > b - a) { a = b; b = x; } else { c = b; b = x; }
+            fb = fx;
+        } else {
+            if (c - b > b - a) { c = x; } else { a = x; }
+        }
+    }
+}
+
+// GradientDescentLineSearchOptimizer::run (optimizer.rs:724-817).
+static OptResult line_search(const sitkhip_registration_config &cfg, const std::vector<double> &scales,
+                             std::vector<double> p, LevelObjective &obj, double base_lr) {
+    const size_t n = p.size();
+    ConvergenceMonitor mon(10);
+    std::vector<double> grad;
+    double value = obj.value_and_gradient(p, grad);
+    double best = value;
+    std::vector<double> best_p = p;
+    int stop = SITKHIP_STOP_MAX_ITERATIONS;
+    int taken = 0;
+    for (;;) {
+        if (value < best) { best = value; best_p = p; }
+        mon.add(value);
+        double cv;
+        if (mon.value(cv) && cv <= 1e-6) { stop = SITKHIP_STOP_CONVERGED; break; }
+        if (taken >= cfg.iterations) break;
+        std::vector<double> d(n);
+        for (size_t k = 0; k < n; ++k) d[k] = grad[k] / scales[k];
+        auto line = [&](double x) {
+            std::vector<double> trial(n);
+            for (size_t k = 0; k < n; ++k) trial[k] = p[k] - x * d[k];
+            return obj.value(trial);
+        };
+        const double lr = golden_section(0.01, 20, base_lr * 0.0, base_lr, base_lr * 5.0, line);
+        base_lr = lr;
+        for (size_t k = 0; k < n; ++k) p[k] -= lr * d[k];
+        ++taken;
+        value = obj.value_and_gradient(p, grad);
+    }
+    return {best_p, best, taken, stop};
+}
+
+// ------------------------------------------------------------------- driver
+struct ImageHolder {
+    Image *img = nullptr;
+    bool owned = false;
+    ~ImageHolder() { if (owned) delete img; }
+};
+
+void register_images(const Image &fixed, const Image &moving, const sitkhip_registration_config &cfg,
+                     sitkhip_transform &tr, sitkhip_registration_result &res) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (cfg.levels < 1 || cfg.levels > SITKHIP_MAX_LEVELS)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "levels must be 1..8");
+    if (tr.kind != SITKHIP_TRANSFORM_TRANSLATION && tr.kind != SITKHIP_TRANSFORM_EULER3D &&
+        tr.kind != SITKHIP_TRANSFORM_AFFINE)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "unsupported transform kind");
+    if (cfg.optimizer != SITKHIP_OPTIMIZER_REGULAR_STEP && cfg.optimizer != SITKHIP_OPTIMIZER_LINE_SEARCH)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "unsupported optimizer");
+    std::memset(&res, 0, sizeof(res));
+    Transform T;
+    T.kind = tr.kind;
+    T.compute_zyx = tr.compute_zyx != 0;
+    for (int d = 0; d < 3; ++d) T.center[d] = tr.center[d];
+    T.recompute();
+    std::vector<double> p(tr.params, tr.params + T.nparams);
+    T.set_parameters(p.data());
+
+    for (int level = 0; level < cfg.levels; ++level) {
+        const double s = cfg.sigma[level];
+        const double sig[3] = {s, s, s};
+        const size_t *f = cfg.shrink[level];
+        const bool smooth = s > 0.0;
+        const bool native = f[0] == 1 && f[1] == 1 && f[2] == 1;
+        // prepare_level (method.rs:2416-2569): smooth both at full resolution,
+        // place the fixed onto the shrunk grid, keep the moving at full size.
+        ImageHolder sf, sm, fl;
+        sf.img = smooth ? recursive_gaussian(fixed, sig) : const_cast<Image *>(&fixed);
+        sf.owned = smooth;
+        sm.img = smooth ? recursive_gaussian(moving, sig) : const_cast<Image *>(&moving);
+        sm.owned = smooth;
+        if (native) {
+            fl.img = sf.img;
+        } else {
+            sitkhip_geometry grid;
+            long long off[3];
+            shrunk_geometry(fixed.geom, f, grid, off);
+            fl.img = resample_linear(*sf.img, grid, nullptr, 0, 0.0);
+            fl.owned = true;
+        }
+        const Image &fixed_level = *fl.img;
+        const size_t nvox = fixed_level.len();
+        std::vector<int64_t> draws;
+        const int64_t *idx = nullptr;
+        size_t nidx = 0;
+        if (cfg.sampling == 1 || cfg.sampling == 2) {
+            size_t cnt = 0;
+            if (cfg.sampling == 1) {
+                size_t stride = static_cast<size_t>(std::ceil(1.0 / cfg.sampling_percentage));
+                if (stride < 1) stride = 1;
+                for (size_t v = 0; v < nvox; v += stride) draws.push_back(static_cast<int64_t>(v));
+            } else {
+                cnt = static_cast<size_t>(static_cast<double>(nvox) * cfg.sampling_percentage);
+                draws.resize(cnt);
+                uint64_t state = cfg.seed;
+                for (size_t i = 0; i < cnt; ++i) {
+                    state += 0x9E3779B97F4A7C15ull;
+                    uint64_t z = state;
+                    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                    z ^= z >> 31;
+                    draws[i] = static_cast<int64_t>(z % static_cast<uint64_t>(nvox));
+                }
+            }
+            idx = draws.data();
+            nidx = draws.size();
+        }
+        std::unique_ptr<Metric, void (*)(Metric *)> metric(
+            metric_create(cfg.metric, fixed_level, idx, nidx, nullptr, *sm.img, nullptr, cfg.precision),
+            metric_destroy);
+        if (cfg.metric == SITKHIP_METRIC_MATTES) {
+            double fr[2], mr[2];
+            metric_value_ranges(*metric, fr, mr);
+            sitkhip_mattes_bins b;
+            mattes_bins_from_ranges(fr, mr, static_cast<size_t>(cfg.bins), &b);
+            mattes_set_bins(*metric, b);
+        }
+        LevelObjective obj;
+        obj.metric = metric.get();
+        obj.kind = cfg.metric;
+        obj.t = T;
+
+        ScalesEstimator est;
+        est.build(fixed_level.geom, T);
+        const int np = T.nparams;
+        std::vector<double> scales = cfg.scales_mode == SITKHIP_SCALES_PHYSICAL_SHIFT
+                                         ? est.scales()
+                                         : std::vector<double>(np, 1.0);
+        auto scaled_of = [&](const std::vector<double> &g) {
+            std::vector<double> out(np);
+            for (int k = 0; k < np; ++k) out[k] = g[k] / scales[k];
+            return out;
+        };
+        OptResult r;
+        if (cfg.optimizer == SITKHIP_OPTIMIZER_REGULAR_STEP) {
+            if (cfg.learning_rate_mode == SITKHIP_LR_ONCE) {
+                std::vector<double> g0;
+                obj.value_and_gradient(p, g0);
+                const std::vector<double> s0 = scaled_of(g0);
+                double gm0 = 0.0;
+                for (double v : s0) gm0 += v * v;
+                gm0 = std::sqrt(gm0);
+                const double lr = est.learning_rate(s0) * gm0;
+                r = regular_step(cfg, scales, p, obj, [lr](const std::vector<double> &) { return lr; });
+            } else if (cfg.learning_rate_mode == SITKHIP_LR_EACH_ITERATION) {
+                r = regular_step(cfg, scales, p, obj, [&](const std::vector<double> &sg) {
+                    double gm = 0.0;
+                    for (double v : sg) gm += v * v;
+                    return est.learning_rate(sg) * std::sqrt(gm);
+                });
+            } else {
+                const double lr = cfg.learning_rate;
+                r = regular_step(cfg, scales, p, obj, [lr](const std::vector<double> &) { return lr; });
+            }
+        } else {
+            double lr = cfg.learning_rate;
+            if (cfg.learning_rate_mode != SITKHIP_LR_MANUAL) {
+                std::vector<double> g0;
+                obj.value_and_gradient(p, g0);
+                lr = est.learning_rate(scaled_of(g0));
+            }
+            r = line_search(cfg, scales, p, obj, lr);
+        }
+        p = r.params;
+        // the level's result is evaluated at the returned parameters (method.rs:3504-3512)
+        std::vector<double> gf;
+        const double vf = obj.value_and_gradient(p, gf);
+        if (obj.last_valid == 0) fail(SITKHIP_ERR_DEGENERATE_INPUT, "no valid samples at the result");
+        res.level_value[level] = vf;
+        res.level_iterations[level] = r.iterations;
+        res.level_stop[level] = r.stop;
+        res.iterations += r.iterations;
+        res.evaluations += obj.evaluations;
+        res.value_evaluations += obj.value_evaluations;
+        res.value = vf;
+        res.valid_points = obj.last_valid;
+        T.set_parameters(p.data());
+    }
+    for (int k = 0; k < T.nparams; ++k) tr.params[k] = p[k];
+    res.seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace sitkhip
+
+extern "C" int sitkhip_register(sitkhip_image fixed, sitkhip_image moving,
+                                const sitkhip_registration_config *config,
+                                sitkhip_transform *transform, sitkhip_registration_result *result) {
+    return sitkhip::run_guarded([&] {
+        auto f = reinterpret_cast<sitkhip::Image *>(fixed);
+        auto m = reinterpret_cast<sitkhip::Image *>(moving);
+        if (!f || !m || !config || !transform || !result)
+            sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "null argument");
+        sitkhip::register_images(*f, *m, *config, *transform, *result);
+    });
+}

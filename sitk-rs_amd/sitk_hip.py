@@ -73,6 +73,59 @@ class MattesBins(C.Structure):
     ]
 
 
+MAX_LEVELS = 8
+TRANSLATION, EULER3D, AFFINE = 0, 1, 2
+REGULAR_STEP, LINE_SEARCH = 0, 1
+LR_MANUAL, LR_ONCE, LR_EACH_ITERATION = 0, 1, 2
+SCALES_UNIT, SCALES_PHYSICAL_SHIFT = 0, 1
+STOP_NAMES = {0: "MaxIterations", 1: "GradientConverged", 2: "StepTooSmall", 3: "Converged"}
+
+
+class TransformSpec(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("compute_zyx", C.c_int),
+        ("center", C.c_double * 3),
+        ("params", C.c_double * 12),
+    ]
+
+
+class RegistrationConfig(C.Structure):
+    _fields_ = [
+        ("metric", C.c_int),
+        ("bins", C.c_int),
+        ("sampling", C.c_int),
+        ("sampling_percentage", C.c_double),
+        ("seed", C.c_uint64),
+        ("precision", C.c_int),
+        ("optimizer", C.c_int),
+        ("learning_rate", C.c_double),
+        ("minimum_step", C.c_double),
+        ("iterations", C.c_int),
+        ("relaxation", C.c_double),
+        ("gradient_tolerance", C.c_double),
+        ("learning_rate_mode", C.c_int),
+        ("scales_mode", C.c_int),
+        ("levels", C.c_int),
+        ("shrink", (C.c_size_t * 3) * MAX_LEVELS),
+        ("sigma", C.c_double * MAX_LEVELS),
+    ]
+
+
+class RegistrationResult(C.Structure):
+    _fields_ = [
+        ("value", C.c_double),
+        ("valid_points", C.c_size_t),
+        ("iterations", C.c_int),
+        ("level_iterations", C.c_int * MAX_LEVELS),
+        ("level_stop", C.c_int * MAX_LEVELS),
+        ("level_value", C.c_double * MAX_LEVELS),
+        ("evaluations", C.c_size_t),
+        ("value_evaluations", C.c_size_t),
+        ("seconds", C.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -127,6 +180,13 @@ def lib():
         L.sitkhip_ms_value.argtypes = [vp, dp, C.c_int, dp, szp]
         L.sitkhip_contract_ms.argtypes = [dp, dp, dp, C.c_int, dp, dp, szp]
         L.sitkhip_contract_mattes.argtypes = [dp, dp, dp, C.c_int, dp]
+        L.sitkhip_shrunk_geometry.argtypes = [gp, szp, gp]
+        L.sitkhip_shrink.argtypes = [vp, szp, C.POINTER(vp)]
+        L.sitkhip_resample_linear.argtypes = [vp, gp, dp, C.c_int, C.c_double, C.POINTER(vp)]
+        L.sitkhip_resample_nearest.argtypes = [vp, gp, dp, C.c_int, C.c_double, C.POINTER(vp)]
+        L.sitkhip_mask_resample_nearest.argtypes = [vp, gp, dp, C.c_int, C.POINTER(vp)]
+        L.sitkhip_register.argtypes = [vp, vp, C.POINTER(RegistrationConfig),
+                                       C.POINTER(TransformSpec), C.POINTER(RegistrationResult)]
         L.sitkhip_draw_samples.argtypes = [C.c_size_t, C.c_int, C.c_double, C.c_uint64,
                                            C.POINTER(C.c_int64), szp]
         L.sitkhip_profile_enable.argtypes = [C.c_int]
@@ -469,3 +529,83 @@ def profile_dump() -> str:
     buf = C.create_string_buffer(1 << 16)
     _check(lib().sitkhip_profile_dump(buf, 1 << 16))
     return buf.value.decode()
+
+
+# ------------------------------------------------------------- pyramid placement
+def shrunk_geometry(geometry: Geometry, factors) -> Geometry:
+    """shrunk_geometry (crates/sitk/src/cuda/ops/pyramid.rs:355-420)."""
+    out = Geometry()
+    f = (C.c_size_t * 3)(*[int(v) for v in factors])
+    _check(lib().sitkhip_shrunk_geometry(C.byref(geometry), f, C.byref(out)))
+    return out
+
+
+def shrink(img: DeviceImage, factors) -> DeviceImage:
+    f = (C.c_size_t * 3)(*[int(v) for v in factors])
+    h = C.c_void_p()
+    _check(lib().sitkhip_shrink(img._h, f, C.byref(h)))
+    return DeviceImage(h)
+
+
+def _stage_arg(stages):
+    if stages is None:
+        return None, 0, None
+    s = np.ascontiguousarray(stages, dtype=np.float64).reshape(-1)
+    return _dp(s), s.size // 12, s
+
+
+def resample_linear(img: DeviceImage, out_geometry: Geometry, stages=None,
+                    default_value: float = 0.0) -> DeviceImage:
+    """resample_linear[_through] (crates/sitk/src/cuda/ops/pyramid.rs:677-738)."""
+    sp, n, keep = _stage_arg(stages)
+    h = C.c_void_p()
+    _check(lib().sitkhip_resample_linear(img._h, C.byref(out_geometry), sp, n, default_value,
+                                         C.byref(h)))
+    return DeviceImage(h)
+
+
+def resample_nearest(img: DeviceImage, out_geometry: Geometry, stages=None,
+                     default_value: float = 0.0) -> DeviceImage:
+    sp, n, keep = _stage_arg(stages)
+    h = C.c_void_p()
+    _check(lib().sitkhip_resample_nearest(img._h, C.byref(out_geometry), sp, n, default_value,
+                                          C.byref(h)))
+    return DeviceImage(h)
+
+
+# -------------------------------------------------------------------- registration
+def registration_config(metric=MATTES, bins=50, sampling=0, sampling_percentage=1.0, seed=0,
+                        precision=PRECISION_F32, optimizer=REGULAR_STEP, learning_rate=1.0,
+                        minimum_step=1e-4, iterations=100, relaxation=0.5,
+                        gradient_tolerance=1e-4, learning_rate_mode=LR_ONCE,
+                        scales_mode=SCALES_PHYSICAL_SHIFT, shrink_factors=((1, 1, 1),),
+                        sigmas=(0.0,)) -> RegistrationConfig:
+    c = RegistrationConfig()
+    c.metric, c.bins, c.sampling = metric, bins, sampling
+    c.sampling_percentage, c.seed, c.precision = sampling_percentage, seed, precision
+    c.optimizer, c.learning_rate, c.minimum_step = optimizer, learning_rate, minimum_step
+    c.iterations, c.relaxation, c.gradient_tolerance = iterations, relaxation, gradient_tolerance
+    c.learning_rate_mode, c.scales_mode = learning_rate_mode, scales_mode
+    if len(shrink_factors) != len(sigmas) or not 1 <= len(sigmas) <= MAX_LEVELS:
+        raise ValueError("one shrink factor triple and one sigma per level (1..8 levels)")
+    c.levels = len(sigmas)
+    for i, (f, s) in enumerate(zip(shrink_factors, sigmas)):
+        c.shrink[i][:] = [int(v) for v in f]
+        c.sigma[i] = float(s)
+    return c
+
+
+def register(fixed: DeviceImage, moving: DeviceImage, config: RegistrationConfig, kind: int,
+             params, center=(0.0, 0.0, 0.0), compute_zyx=False):
+    """ImageRegistrationMethod::execute_on_device
+    (crates/sitk/src/registration/method.rs:2698-2879). Returns (params, result)."""
+    t = TransformSpec()
+    t.kind = kind
+    t.compute_zyx = int(compute_zyx)
+    t.center[:] = [float(v) for v in center]
+    for i, v in enumerate(params):
+        t.params[i] = float(v)
+    r = RegistrationResult()
+    _check(lib().sitkhip_register(fixed._h, moving._h, C.byref(config), C.byref(t), C.byref(r)))
+    n = {TRANSLATION: 3, EULER3D: 6, AFFINE: 12}[kind]
+    return np.array(t.params[:n]), r
