@@ -438,6 +438,22 @@ void orc_transform_jacobian(const orc_transform *t, const double point[3], doubl
         j[3] = 1.0;
         j[10] = 1.0;
         j[17] = 1.0;
+    } else if (t->kind == ORC_BSPLINE) {
+        /* bspline.rs:661-703: column d*ncp + flat holds the tap weight in row d;
+         * a point outside the valid region has a zero Jacobian. */
+        long start[3];
+        double w[3][4];
+        if (!bspline_support(t, point, start, w)) return;
+        size_t gs0 = t->grid_size[0], gs1 = t->grid_size[1];
+        size_t ncp = t->grid_size[0] * t->grid_size[1] * t->grid_size[2];
+        for (int k = 0; k < 4; ++k)
+            for (int jj = 0; jj < 4; ++jj)
+                for (int i = 0; i < 4; ++i) {
+                    double wt = w[0][i] * w[1][jj] * w[2][k];
+                    size_t cp = (size_t)(start[0] + i) +
+                                gs0 * ((size_t)(start[1] + jj) + gs1 * (size_t)(start[2] + k));
+                    for (int d = 0; d < 3; ++d) j[(size_t)d * (size_t)np + d * ncp + cp] = wt;
+                }
     }
 }
 

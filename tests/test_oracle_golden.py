@@ -370,3 +370,127 @@ def test_draw_samples():
     assert a.size == 200 and np.array_equal(a, b) and a.min() >= 0 and a.max() < 1000
     # SplitMix64 first output for seed 0 (Vigna's reference implementation)
     assert O.draw_samples(1 << 63, 2, 1.0 / (1 << 63), 0)[0] == 0xE220A8397B1DCDAF % (1 << 63)
+
+
+# ------------------------------------------------- pyramid placement (oracle)
+def test_shrunk_geometry_preserves_the_physical_centre():
+    # crates/sitk/src/filters/shrink.rs:51-91: out = max(1, size / f), spacing * f,
+    # origin shifted by delta so the physical centre of the grid is kept.
+    size, sp, org = (37, 29, 23), (0.9, 1.1, 1.5), (-3.0, 2.0, 1.0)
+    for f in [(1, 1, 1), (2, 2, 2), (4, 4, 4), (3, 5, 7), (100, 1, 2)]:
+        osz, osp, oor = O.shrunk_geometry(size, sp, org, f)
+        for d in range(3):
+            assert osz[d] == max(1, size[d] // f[d])
+            assert osp[d] == sp[d] * f[d]
+            c_in = org[d] + sp[d] * (size[d] - 1) / 2
+            c_out = oor[d] + osp[d] * (osz[d] - 1) / 2
+            assert abs(c_in - c_out) < 1e-12
+        if f == (1, 1, 1):
+            assert osz == list(size) and oor == list(org)
+
+
+def test_linear_resample_identity_shift_and_linear_fields():
+    # crates/sitk/src/transform/resample.rs:226-290 with the linear interpolator
+    # (src/transform/interpolator.rs:58-91).
+    v = O.synth(3, 12 * 10 * 8).reshape(8, 10, 12)
+    sp, org = (1.0, 0.8, 1.2), (1.0, -2.0, 3.0)
+    ident = O.make_transform(O.AFFINE, [1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0])
+    same = O.resample_linear(v, sp, org, (12, 10, 8), sp, org, ident)
+    assert np.array_equal(same, v)
+    # a whole-voxel translation shifts the data; voxels mapped outside get the default
+    t = O.make_transform(O.TRANSLATION, [2 * sp[0], 0.0, -1 * sp[2]])
+    sh = O.resample_linear(v, sp, org, (12, 10, 8), sp, org, t, default=-5.0)
+    assert np.array_equal(sh[1:, :, :10], v[:-1, :, 2:])
+    assert np.all(sh[:, :, 10:] == -5.0) and np.all(sh[0] == -5.0)
+    # trilinear interpolation reproduces an affine field exactly (to rounding)
+    z, y, x = np.meshgrid(np.arange(8), np.arange(10), np.arange(12), indexing="ij")
+    ramp = (3.0 * x - 2.0 * y + 0.5 * z + 7.0).astype(np.float32)
+    e = O.make_transform(O.EULER3D, [0.05, -0.03, 0.08, 0.3, -0.2, 0.1], center=[6, 2, 7])
+    out = O.resample_linear(ramp, sp, org, (6, 5, 4), (1.3, 1.1, 1.4), (3.0, 0.0, 5.0), e,
+                            default=np.nan)
+    for k, j, i in [(0, 0, 0), (1, 2, 3), (3, 4, 5), (2, 1, 4)]:
+        p = np.array([3.0 + 1.3 * i, 0.0 + 1.1 * j, 5.0 + 1.4 * k])
+        q = O.transform_point(e, p)
+        c = (q - np.array(org)) / np.array(sp)
+        if np.all(c >= 0) and np.all(c <= np.array([11, 9, 7])):
+            assert abs(out[k, j, i] - (3.0 * c[0] - 2.0 * c[1] + 0.5 * c[2] + 7.0)) < 1e-4
+
+
+# ---------------------------------------------- B-spline transform (oracle)
+def _bspline(coeff_fn, mesh=(3, 3, 3)):
+    size, sp, org = (16, 16, 16), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)
+    ncp = (mesh[0] + 3) * (mesh[1] + 3) * (mesh[2] + 3)
+    c = np.zeros(3 * ncp)
+    coeff_fn(c, ncp)
+    return O.make_bspline(size, sp, org, np.eye(3), mesh, c), c, ncp
+
+
+def test_bspline_zero_is_identity_and_constant_is_a_translation():
+    # crates/sitk/src/transform/bspline.rs:561 (transform_point): the cubic
+    # B-spline weights are a partition of unity over the 4^3 support.
+    t0, _, _ = _bspline(lambda c, n: None)
+    tc, _, _ = _bspline(lambda c, n: (c.__setitem__(slice(0, n), 1.5),
+                                      c.__setitem__(slice(n, 2 * n), -0.25),
+                                      c.__setitem__(slice(2 * n, 3 * n), 2.0)))
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        p = rng.uniform(0.5, 14.5, 3)
+        assert np.array_equal(O.transform_point(t0, p), p)
+        assert np.allclose(O.transform_point(tc, p), p + [1.5, -0.25, 2.0], atol=1e-12)
+
+
+def test_bspline_jacobian_is_the_derivative_of_the_point_map():
+    # bspline.rs:675 (sparse_jacobian_wrt_parameters): the map is linear in the
+    # coefficients, so each column is the displacement of a unit coefficient.
+    rng = np.random.default_rng(2)
+    t, c, ncp = _bspline(lambda c, n: c.__setitem__(slice(None), rng.normal(0, 0.5, c.size)))
+    p = np.array([5.3, 7.9, 4.4])
+    J = O.transform_jacobian(t, p)
+    base = O.transform_point(t, p)
+    for k in rng.choice(3 * ncp, 40, replace=False):
+        c[k] += 1.0
+        moved = O.transform_point(t, p)
+        c[k] -= 1.0
+        assert np.allclose(moved - base, J[:, k], atol=1e-12)
+    # the weights of each row sum to one
+    for d in range(3):
+        assert abs(J[d, d * ncp:(d + 1) * ncp].sum() - 1.0) < 1e-12
+
+
+def test_bspline_sparse_mattes_derivative_matches_finite_difference():
+    # crates/sitk/src/registration/mattes.rs:722-796, pinned by FD as in
+    # mattes.rs:952-1012.
+    n = 16
+    f = O.fixture_volume(n, [0, 0, 0])
+    m = O.fixture_volume(n, [0.6, -0.4, 0.3])
+    sp, org = [1.0, 1.0, 1.0], [0.0, 0.0, 0.0]
+    pb = O.MetricProblem(f, sp, org, m, sp, org)
+    g = pb.mattes_geometry(20)
+    rng = np.random.default_rng(3)
+    t, c, ncp = _bspline(lambda c, n_: c.__setitem__(slice(None), rng.normal(0, 0.2, c.size)),
+                         mesh=(2, 2, 2))
+    v0, d0, _ = pb.mattes_evaluate(g, t)
+    assert len(d0) == 3 * ncp
+    big = np.argsort(-np.abs(d0))[:6]
+    for k in big:
+        h = 1e-5
+        c[k] += h
+        vp = pb.mattes_value(g, t)
+        c[k] -= 2 * h
+        vm = pb.mattes_value(g, t)
+        c[k] += h
+        fd = (vp - vm) / (2 * h)
+        assert abs(fd - d0[k]) < 5e-3 * max(1.0, abs(d0[k])), (k, fd, d0[k])
+
+
+def test_bspline_grid_geometry_is_the_references():
+    # bspline.rs:229-262 and its own pin (bspline.rs tests, "fixed parameters are
+    # the control-point grid geometry"): gridSize = mesh + 3, gridSpacing =
+    # physDim / mesh, gridOrigin = origin - gridSpacing. Restated in 3-D with a
+    # domain of physical size (4, 8, 6) at origin (1, 2, 3) and mesh (2, 4, 3).
+    t = O.make_bspline((4, 8, 6), (1.0, 1.0, 1.0), (1.0, 2.0, 3.0), np.eye(3), (2, 4, 3),
+                       np.zeros(3 * 5 * 7 * 6))
+    assert list(t.grid_size) == [5, 7, 6]
+    assert list(t.grid_spacing) == [2.0, 2.0, 2.0]
+    assert list(t.grid_origin) == [-1.0, 0.0, 1.0]
+    assert t.nparams == 3 * 5 * 7 * 6
