@@ -323,6 +323,48 @@ int sitkhip_register(sitkhip_image fixed, sitkhip_image moving,
                      const sitkhip_registration_config *config, sitkhip_transform *transform,
                      sitkhip_registration_result *result);
 
+/* ---------------------------------------------------------------- batches */
+/* Many resident metrics evaluated together. No counterpart in the reference,
+ * whose device metrics evaluate one pair per call (src/cuda/ops/mattes.rs:
+ * 697-841, mean_squares.rs:329-439); it is what a batched registration loop
+ * calls in place of one evaluate per pair. One evaluation is one replay of a
+ * captured HIP graph: a persistent value pass over the tiles of every pair
+ * (Mattes histograms and mean-squares moments), the Mattes tails, a persistent
+ * derivative pass over the Mattes tiles, and a per-pair fold. Per pair the
+ * result equals the single-pair evaluation: the Mattes histogram (and so its
+ * value) exactly, the FP64 moments up to the order of their sum. */
+typedef struct sitkhip_batch_s *sitkhip_batch;
+
+/* The metrics (same precision; Mattes ones with bins set, at most 64 bins)
+ * must outlive the batch and must not be evaluated on their own while it is in
+ * use (they share histogram buffers). */
+int sitkhip_batch_create(const sitkhip_metric *metrics, size_t n, sitkhip_batch *out);
+void sitkhip_batch_free(sitkhip_batch b);
+/* tile_samples: samples per tile (0: 4096); grid: persistent workgroups, a
+ * multiple of 8 (0: 4 per CU); use_graph 0 launches the passes directly. */
+int sitkhip_batch_set_tiling(sitkhip_batch b, size_t tile_samples, unsigned grid, int use_graph);
+int sitkhip_batch_get_tiling(sitkhip_batch b, size_t *tile_samples, unsigned *grid);
+/* Times every candidate (tile, grid) on `stages` and keeps the fastest. */
+int sitkhip_batch_autotune(sitkhip_batch b, const double *stages, size_t *tile_samples,
+                           unsigned *grid, double *ms_per_evaluation);
+/* stages: n x 12 doubles, one PointStage per pair; active: n flags or NULL
+ * (all). Per active pair: values[i] (-MI or mean squares; DBL_MAX when no
+ * sample is valid), valid[i], and moments[14 i ..] (12 Mattes moments then two
+ * zeros, or the 14 mean-squares moments). Inactive pairs are not evaluated
+ * and their outputs are left untouched. */
+int sitkhip_batch_evaluate(sitkhip_batch b, const double *stages, const uint8_t *active,
+                           double *values, double *moments, size_t *valid);
+
+/* n registrations run together: per level every pair's level is prepared as
+ * sitkhip_register prepares it, then each optimizer iteration evaluates every
+ * still-running pair in one batched evaluation. RegularStepGradientDescent
+ * only. configs, transforms and results are arrays of n. autotune != 0 picks
+ * the tiling per level with sitkhip_batch_autotune. */
+int sitkhip_register_batch(size_t n, const sitkhip_image *fixed, const sitkhip_image *moving,
+                           const sitkhip_registration_config *configs,
+                           sitkhip_transform *transforms, sitkhip_registration_result *results,
+                           int autotune);
+
 /* -------------------------------------------------------------- profiling */
 /* When enabled, every kernel launch of this library is bracketed by HIP events
  * on the library stream and its duration accumulated per kernel name. */

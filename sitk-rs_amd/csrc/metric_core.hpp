@@ -1,0 +1,530 @@
+// Device building blocks shared by the single-pair metric kernels
+// (metric.hip) and the batched multi-pair kernels (batch.hip): sample sets, the
+// Parzen window, the 64-bit fixed-point histogram, the per-range sample loops of
+// the three passes, the Mattes tail, the FP64 reduction tree and the host-side
+// state of one resident metric.
+//
+// Reference (crates/sitk/): src/registration/mattes.rs:92-328, 442-676,
+// src/registration/metric.rs:1065-1223, src/cuda/ops/mattes.rs:75-91,
+// src/cuda/ops/mean_squares.rs:101-145.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "common.hpp"
+#include "sampler.hpp"
+
+namespace sitkhip {
+
+constexpr int kBlock = 256;
+constexpr unsigned kGrid = 1024;      // fixed: the reduction tree must not depend on n
+constexpr int kLdsMaxBins = 90;       // bins^2 * 8 B of LDS per workgroup (<= 64 KB)
+constexpr double kFix = 4294967296.0; // 2^32 fixed-point units per histogram unit
+constexpr double kInvFix = 1.0 / 4294967296.0;
+constexpr uint8_t kNoBin = 255;
+
+struct SampleSet {
+    const uint32_t *idx;  // nullptr: the whole fixed grid in order
+    long long n;
+};
+
+struct MattesParams {
+    int bins;
+    double mbs, mnmin, inv_mbs;  // moving bin size, normalised min, 1/bin size
+    double mmin, mmax;           // moving true range (reject outside)
+    float mmin_f, mmax_f, inv_mbs_f;
+};
+
+// ------------------------------------------------------------- cubic B-spline
+// itk::BSplineKernelFunction<3> and its derivative in the reference's
+// expression order (registration/mattes.rs:96-124).
+template <typename R>
+__device__ __forceinline__ R bspline3(R u) {
+    const R a = fabs(u);
+    if (a < R(1)) {
+        const R sq = a * a;
+        return (R(4) - R(6) * sq + R(3) * sq * a) / R(6);
+    } else if (a < R(2)) {
+        const R sq = a * a;
+        return (R(8) - R(12) * a + R(6) * sq - sq * a) / R(6);
+    }
+    return R(0);
+}
+
+// FP32: the same polynomials, (2-a)^3/6 form and a multiply by 1/6 (no
+// correctly rounded division on the FP32 path).
+template <>
+__device__ __forceinline__ float bspline3<float>(float u) {
+    const float a = fabsf(u);
+    const float sq = a * a;
+    if (a < 1.0f) return fmaf(fmaf(3.0f, a, -6.0f), sq, 4.0f) * (1.0f / 6.0f);
+    if (a < 2.0f) {
+        const float t = 2.0f - a;
+        return t * t * t * (1.0f / 6.0f);
+    }
+    return 0.0f;
+}
+
+template <typename R>
+__device__ __forceinline__ R bspline3_deriv(R u) {
+    if (u >= R(0) && u < R(1)) return R(-2) * u + R(1.5) * u * u;
+    if (u > R(-1) && u < R(0)) return R(-2) * u - R(1.5) * u * u;
+    if (u >= R(1) && u < R(2)) return R(-2) + R(2) * u - R(0.5) * u * u;
+    if (u > R(-2) && u <= R(-1)) return R(2) + R(2) * u + R(0.5) * u * u;
+    return R(0);
+}
+
+// Workgroup slice of the samples: contiguous, XCD-grouped (see file comment).
+__device__ __forceinline__ void block_slice(long long n, long long &begin, long long &end,
+                                            unsigned &logical) {
+    const unsigned G = gridDim.x;
+    const unsigned b = blockIdx.x;
+    logical = (b % 8u) * (G / 8u) + b / 8u;
+    const long long per = (n + G - 1) / G;
+    begin = static_cast<long long>(logical) * per;
+    end = begin + per;
+    if (end > n) end = n;
+}
+
+__device__ __forceinline__ uint32_t sample_flat(const SampleSet &S, long long s) {
+    return S.idx ? S.idx[s] : static_cast<uint32_t>(s);
+}
+// Parzen window of the moving value: the clamped bin index and the four tap
+// arguments (m - term) for m = idx-1 .. idx+2 (mattes.rs:193-206, 464-481).
+template <typename R>
+struct Parzen;
+
+template <>
+struct Parzen<double> {
+    __device__ __forceinline__ static int taps(const MattesParams &P, double mv, double arg[4]) {
+        const double term = mv / P.mbs - P.mnmin;
+        long long idx = static_cast<long long>(term);
+        if (idx < 2) idx = 2;
+        else if (idx > P.bins - 3) idx = P.bins - 3;
+        const int start = static_cast<int>(idx) - 1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) arg[k] = static_cast<double>(start + k) - term;
+        return start;
+    }
+    __device__ __forceinline__ static bool in_range(const MattesParams &P, double mv) {
+        return !(mv < P.mmin || mv > P.mmax);
+    }
+};
+
+template <>
+struct Parzen<float> {
+    // term = mv/binSize - normalizedMin = (mv - trueMin)/binSize + padding: the
+    // second form has no cancellation, so it is accurate in FP32.
+    __device__ __forceinline__ static int taps(const MattesParams &P, float mv, float arg[4]) {
+        const float term = fmaf(mv - P.mmin_f, P.inv_mbs_f, 2.0f);
+        int idx = static_cast<int>(term);
+        idx = idx < 2 ? 2 : (idx > P.bins - 3 ? P.bins - 3 : idx);
+        const int start = idx - 1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) arg[k] = static_cast<float>(start + k) - term;
+        return start;
+    }
+    __device__ __forceinline__ static bool in_range(const MattesParams &P, float mv) {
+        return !(mv < P.mmin_f || mv > P.mmax_f);
+    }
+};
+
+// B3 in [0, 2/3] (a rounding residue below zero becomes 0), scaled to 2^-32
+// units: every contribution fits in 32 bits.
+template <typename R>
+__device__ __forceinline__ unsigned long long to_fixed(R v) {
+    const double d = static_cast<double>(v);
+    return d > 0.0 ? static_cast<unsigned long long>(llrint(d * kFix)) : 0ull;
+}
+template <>
+__device__ __forceinline__ unsigned long long to_fixed<float>(float v) {
+    return v > 0.0f ? static_cast<unsigned long long>(__float2uint_rn(v * 4294967296.0f)) : 0ull;
+}
+// ------------------------------------------------------------ FP64 reduction
+// Per-workgroup reduction of NSLOT per-lane FP64 sums (fixed shuffle pattern +
+// LDS) into partials[logical * NSLOT + k]. A separate one-workgroup launch
+// folds the partials in logical-workgroup (= sample) order. Both trees are
+// fixed, so the result is identical on every run. (No cross-workgroup
+// handshake inside a pass: on a multi-XCD device an agent-scope fence in every
+// workgroup writes back and invalidates its XCD's L2 and costs more than the
+// extra launch.)
+template <int NSLOT>
+__device__ __forceinline__ void block_reduce_store(double (&acc)[NSLOT], double *partials,
+                                                   unsigned logical) {
+    __shared__ double red[kBlock / 64][NSLOT];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) {
+        double v = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NSLOT) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
+        partials[static_cast<size_t>(logical) * NSLOT + threadIdx.x] = t;
+    }
+}
+
+// out[k] = sum over G partials of slot k: lane t sums rows t, t+256, ... (all
+// loads issued first), then a fixed shuffle + LDS tree. One workgroup.
+template <int NSLOT>
+__global__ __launch_bounds__(kBlock) void fold_partials(const double *__restrict__ partials,
+                                                        unsigned G, double *__restrict__ out) {
+    __shared__ double fold[NSLOT][kBlock / 64];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    double t[NSLOT];
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) t[k] = 0.0;
+    for (unsigned b0 = 0; b0 < G; b0 += kBlock) {
+        const double *row = partials + static_cast<size_t>(b0 + threadIdx.x) * NSLOT;
+        double v[NSLOT];
+#pragma unroll
+        for (int k = 0; k < NSLOT; ++k) v[k] = row[k];
+#pragma unroll
+        for (int k = 0; k < NSLOT; ++k) t[k] += v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) {
+        double x = t[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) fold[k][wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NSLOT) {
+        double x = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) x += fold[threadIdx.x][w];
+        out[threadIdx.x] = x;
+    }
+}
+
+// Samples per lane per iteration: all U samples' loads are issued before any of
+// their values is used, so each lane keeps U independent gathers in flight.
+#ifndef SITKHIP_BATCH
+#define SITKHIP_BATCH 4
+#endif
+constexpr int kU = SITKHIP_BATCH;
+
+// Phase A of the batched loop: which voxel each of the U samples is and
+// whether it exists (s < end and, for Mattes, not an excluded fixed bin).
+struct Batch {
+    uint32_t flat[kU];
+    int fb[kU];
+    bool live[kU];
+};
+
+// Phase B: located samples -- the in-cell fraction in the interpolation type R
+// and the 8 corner values. Loads are unconditional with clamped addresses, so
+// every lane issues the same loads; the fixed point is recomputed later rather
+// than held across the gathers.
+template <typename R>
+struct Located {
+    R fr[kU][3];
+    float v[kU][8];
+    bool ok[kU];
+};
+
+template <typename R, typename Map>
+__device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol &M,
+                                             const Map &S, const Batch &b, Located<R> &L) {
+    Cell cell[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        double x[3], p[3];
+        fixed_point(F, b.flat[u], x);
+        map_point(S, x, p);
+        L.ok[u] = b.live[u] && locate(M, p, cell[u]);
+        if (!L.ok[u]) {
+            cell[u].base[0] = cell[u].base[1] = cell[u].base[2] = 0;
+            cell[u].frac[0] = cell[u].frac[1] = cell[u].frac[2] = 0.0;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) L.fr[u][d] = static_cast<R>(cell[u].frac[d]);
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) corners(M, cell[u], L.v[u]);
+}
+// ------------------------------------------------ per-range sample loops
+// Each processes the samples [begin, end) of one sample set with the kBlock
+// lanes of the calling workgroup, kU samples per lane per step (every lane
+// issues its kU gathers before using any of them). `Map` is the point map:
+// Stages (1..8 stages, kernel argument) or OneStage (a single stage).
+
+// Mean squares: [sq, S0[3], S1[3][3], count] with diff = M - F
+// (mean_squares.rs:101-145; contraction registration/cuda.rs:381-412), or
+// [sq, count] without MOMENTS.
+template <typename R, bool MOMENTS, typename Map>
+__device__ __forceinline__ void ms_range(const FixedGrid &F, const MovingVol &M, const Map &S,
+                                         const SampleSet &SS, const float *__restrict__ fval,
+                                         long long begin, long long end,
+                                         double (&acc)[MOMENTS ? 14 : 2]) {
+    for (long long s0 = begin + threadIdx.x; s0 < end; s0 += kBlock * kU) {
+        Batch b;
+        float fv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const long long s = s0 + static_cast<long long>(u) * kBlock;
+            b.live[u] = s < end;
+            const long long sc = b.live[u] ? s : begin;
+            b.flat[u] = sample_flat(SS, sc);
+            fv[u] = fval[sc];
+        }
+        Located<R> L;
+        locate_batch<R>(F, M, S, b, L);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (!L.ok[u]) continue;
+            R mv, gi[3], g[3];
+            Trilinear<R>::template eval<MOMENTS>(L.fr[u], L.v[u], mv, gi);
+            const double diff = static_cast<double>(mv) - static_cast<double>(fv[u]);
+            acc[0] = fma(diff, diff, acc[0]);
+            if (MOMENTS) {
+                double x[3];
+                fixed_point(F, b.flat[u], x);
+                physical_gradient<R>(M, gi, g);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double dg = diff * static_cast<double>(g[d]);
+                    acc[1 + d] += dg;
+#pragma unroll
+                    for (int e = 0; e < 3; ++e)
+                        acc[4 + d * 3 + e] = fma(dg, x[e], acc[4 + d * 3 + e]);
+                }
+                acc[13] += 1.0;
+            } else {
+                acc[1] += 1.0;
+            }
+        }
+    }
+}
+
+// Mattes value pass (mattes.rs:442-486): per valid sample, the four cubic
+// B-spline taps of the moving Parzen window into row `fixed bin` of `joint`
+// (64-bit fixed point) and one count into marg[fixed bin].
+template <typename R, typename Map>
+__device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &M, const Map &S,
+                                           const SampleSet &SS, const uint8_t *__restrict__ fbin,
+                                           const MattesParams &P, unsigned long long *joint,
+                                           unsigned long long *marg, long long begin,
+                                           long long end) {
+    const int B = P.bins;
+    for (long long s0 = begin + threadIdx.x; s0 < end; s0 += kBlock * kU) {
+        Batch b;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const long long s = s0 + static_cast<long long>(u) * kBlock;
+            const bool in = s < end;
+            const long long sc = in ? s : begin;
+            b.fb[u] = fbin[sc];
+            b.flat[u] = sample_flat(SS, sc);
+            b.live[u] = in && b.fb[u] != kNoBin;
+        }
+        Located<R> L;
+        locate_batch<R>(F, M, S, b, L);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (!L.ok[u]) continue;
+            R mv, gi[3];
+            Trilinear<R>::template eval<false>(L.fr[u], L.v[u], mv, gi);
+            if (!Parzen<R>::in_range(P, mv)) continue;
+            R arg[4];
+            const int m0 = Parzen<R>::taps(P, mv, arg);
+            unsigned long long *row = joint + b.fb[u] * B + m0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const unsigned long long q = to_fixed(bspline3<R>(arg[k]));
+                if (q) atomicAdd(row + k, q);
+            }
+            atomicAdd(marg + b.fb[u], 1ull);
+        }
+    }
+}
+
+// Mattes derivative pass: per valid sample w = sum_taps pRatio * B3'(arg)
+// and the 12 moments A[d] += w dM_d, B[d][e] += w dM_d x_e
+// (src/cuda/ops/mattes.rs:75-91).
+template <typename R, typename Map>
+__device__ __forceinline__ void deriv_range(const FixedGrid &F, const MovingVol &M, const Map &S,
+                                            const SampleSet &SS, const uint8_t *__restrict__ fbin,
+                                            const MattesParams &P, const R *pr, long long begin,
+                                            long long end, double (&acc)[12]) {
+    const int B = P.bins;
+    for (long long s0 = begin + threadIdx.x; s0 < end; s0 += kBlock * kU) {
+        Batch b;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const long long s = s0 + static_cast<long long>(u) * kBlock;
+            const bool in = s < end;
+            const long long sc = in ? s : begin;
+            b.fb[u] = fbin[sc];
+            b.flat[u] = sample_flat(SS, sc);
+            b.live[u] = in && b.fb[u] != kNoBin;
+        }
+        Located<R> L;
+        locate_batch<R>(F, M, S, b, L);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (!L.ok[u]) continue;
+            R mv, gi[3], g[3];
+            Trilinear<R>::template eval<true>(L.fr[u], L.v[u], mv, gi);
+            if (!Parzen<R>::in_range(P, mv)) continue;
+            physical_gradient<R>(M, gi, g);
+            R arg[4];
+            const int m0 = Parzen<R>::taps(P, mv, arg);
+            const R *row = pr + b.fb[u] * B + m0;
+            R w = R(0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w += row[k] * bspline3_deriv<R>(arg[k]);
+            const double wd = static_cast<double>(w);
+            double x[3];
+            fixed_point(F, b.flat[u], x);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double wg = wd * static_cast<double>(g[d]);
+                acc[d] += wg;
+#pragma unroll
+                for (int e = 0; e < 3; ++e) acc[3 + d * 3 + e] = fma(wg, x[e], acc[3 + d * 3 + e]);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------- Mattes
+struct TailOut {
+    double value;
+    double valid;  // as double: the host reads one block of doubles
+    double joint_sum;
+    double pad;
+};
+
+// The Mattes tail (mattes.rs:273-328) on the finished integer histogram, one
+// workgroup of kTail lanes: -MI, valid, and the pRatio table (pRatio * n_factor;
+// zero where the value sum skips the bin). Clears the histogram for the next
+// evaluation. The joint mass is summed exactly in integers and rounded once.
+constexpr int kTail = 1024;
+
+__device__ void mattes_tail_block(unsigned long long *g_joint, unsigned long long *g_marg, int B,
+                                  double mbs, double *pratio, float *pratio_f, TailOut *out,
+                                  unsigned long long *s_hist) {
+    __shared__ unsigned long long s_u[kTail];
+    __shared__ double s_d[kTail];
+    __shared__ double s_mm[256], s_fm[256], s_logfm[256];
+    const int t = threadIdx.x;
+    const int nbin = B * B;
+    const unsigned long long *hj = g_joint;
+    const unsigned long long *hm = g_marg;
+    if (s_hist) {
+        // stage the histogram in LDS
+        for (int i = t; i < nbin; i += kTail) s_hist[i] = g_joint[i];
+        for (int i = t; i < B; i += kTail) s_hist[nbin + i] = g_marg[i];
+        __syncthreads();
+        hj = s_hist;
+        hm = s_hist + nbin;
+    }
+    unsigned long long mass = 0;
+    for (int i = t; i < nbin; i += kTail) mass += hj[i];
+    s_u[t] = mass;
+    __syncthreads();
+    for (int w = kTail / 2; w > 0; w >>= 1) {
+        if (t < w) s_u[t] += s_u[t + w];
+        __syncthreads();
+    }
+    const double joint_sum = static_cast<double>(s_u[0]) * kInvFix;
+    unsigned long long valid = 0;
+    for (int f = 0; f < B; ++f) valid += hm[f];  // every lane, same order
+    const bool degenerate = valid == 0 || joint_sum < DBL_EPSILON;
+    const double inv_sum = degenerate ? 0.0 : 1.0 / joint_sum;
+    const double n_factor = degenerate ? 0.0 : 1.0 / (mbs * static_cast<double>(valid));
+    // marginals: fixed = counts * inv_sum; moving = column sums of the normalised
+    // joint, accumulated over f in order by 4 lanes per column then combined.
+    if (t < B) {
+        const double fm = static_cast<double>(hm[t]) * inv_sum;
+        s_fm[t] = fm;
+        s_logfm[t] = fm > DBL_EPSILON ? log(fm) : 0.0;
+    }
+    if (t < B) {
+        double mm = 0.0;
+        for (int f = 0; f < B; ++f) mm += static_cast<double>(hj[f * B + t]) * kInvFix * inv_sum;
+        s_mm[t] = mm;
+    }
+    __syncthreads();
+    double part = 0.0;
+    for (int i = t; i < nbin; i += kTail) {
+        double pr_out = 0.0;
+        const unsigned long long q = hj[i];
+        if (q && !degenerate) {
+            const int f = i / B, m = i - f * B;
+            const double jp = static_cast<double>(q) * kInvFix * inv_sum;
+            if (s_fm[f] > DBL_EPSILON && s_mm[m] > DBL_EPSILON && jp > DBL_EPSILON) {
+                const double pr = log(jp / s_mm[m]);
+                part += jp * (pr - s_logfm[f]);
+                pr_out = pr * n_factor;
+            }
+        }
+        pratio[i] = pr_out;
+        pratio_f[i] = static_cast<float>(pr_out);
+    }
+    s_d[t] = part;
+    __syncthreads();
+    for (int w = kTail / 2; w > 0; w >>= 1) {
+        if (t < w) s_d[t] += s_d[t + w];
+        __syncthreads();
+    }
+    // every read of the histogram is done: clear it for the next evaluation
+    for (int i = t; i < nbin; i += kTail) g_joint[i] = 0ull;
+    for (int i = t; i < B; i += kTail) g_marg[i] = 0ull;
+    if (t == 0) {
+        out->value = degenerate ? DBL_MAX : -s_d[0];
+        out->valid = static_cast<double>(valid);
+        out->joint_sum = joint_sum;
+    }
+}
+// ------------------------------------------------------------------ host side
+struct HostResult {
+    TailOut tail;
+    double moments[14];
+};
+
+struct Metric {
+    int kind = 0;
+    int precision = 0;
+    FixedGrid F{};
+    MovingVol Mv{};
+    SampleSet SS{};
+    DeviceBuffer<float> moving;
+    DeviceBuffer<uint8_t> mmask;
+    DeviceBuffer<uint32_t> idx;
+    DeviceBuffer<float> fval;     // per-sample fixed values (MS, and the range/bins source)
+    DeviceBuffer<uint8_t> fbin;   // per-sample fixed Parzen bins (Mattes)
+    DeviceBuffer<double> partials;
+    // Results land in pinned, device-mapped host memory: no copy per evaluation.
+    HostResult *res = nullptr;        // host view
+    HostResult *res_dev = nullptr;    // device view of the same memory
+    // Mattes
+    bool bins_set = false;
+    sitkhip_mattes_bins bins{};
+    MattesParams P{};
+    DeviceBuffer<unsigned long long> joint, marg;
+    DeviceBuffer<double> pratio;
+    DeviceBuffer<float> pratio_f;
+    ~Metric() {
+        if (res) (void)hipHostFree(res);
+    }
+    double fixed_range[2] = {0, 0};
+    double moving_range[2] = {0, 0};
+    size_t nsamples = 0;
+    size_t voxels_fixed = 0;
+};
+
+}  // namespace sitkhip

@@ -23,6 +23,7 @@
 #include <memory>
 #include <vector>
 
+#include "batch.hpp"
 #include "common.hpp"
 #include "image.hpp"
 #include "metric.hpp"
@@ -128,6 +129,42 @@ struct ScalesEstimator {
 };
 
 // ------------------------------------------------------------ the objective
+// The parameter gradient from the metric's moments and the transform's
+// Jacobian decomposition (registration/cuda.rs:381-412, 505-524). Mattes:
+// g_k = sum_d J0[d][k] A_d + sum_de C_e[d][k] B_de. Mean squares: the same
+// contraction of S0, S1 times 2/count; returns the value sq/count (DBL_MAX and a
+// zero gradient with no valid sample).
+static double contract_objective(int kind, double mattes_value, const double *mom,
+                                 const Transform &t, std::vector<double> &g, size_t &valid) {
+    double j0[36], c[108];
+    t.affine_form(j0, c);
+    const int np = t.nparams;
+    g.assign(np, 0.0);
+    if (kind == SITKHIP_METRIC_MATTES) {
+        for (int k = 0; k < np; ++k) {
+            double acc = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                acc += j0[d * np + k] * mom[d];
+                for (int e = 0; e < 3; ++e) acc += c[(e * 3 + d) * np + k] * mom[3 + d * 3 + e];
+            }
+            g[k] = acc;
+        }
+        return mattes_value;
+    }
+    valid = static_cast<size_t>(mom[13]);
+    if (mom[13] == 0.0) return DBL_MAX;
+    const double inv = 1.0 / mom[13];
+    for (int k = 0; k < np; ++k) {
+        double acc = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            acc += j0[d * np + k] * mom[1 + d];
+            for (int e = 0; e < 3; ++e) acc += c[(e * 3 + d) * np + k] * mom[4 + d * 3 + e];
+        }
+        g[k] = 2.0 * acc * inv;
+    }
+    return mom[0] * inv;
+}
+
 struct LevelObjective {
     Metric *metric = nullptr;
     int kind = SITKHIP_METRIC_MATTES;
@@ -140,40 +177,17 @@ struct LevelObjective {
         t.set_parameters(p.data());
         double st[12];
         t.stage(st);
-        double j0[36], c[108];
-        t.affine_form(j0, c);
-        const int np = t.nparams;
-        g.assign(np, 0.0);
         ++evaluations;
         if (kind == SITKHIP_METRIC_MATTES) {
             double value, mom[12];
             size_t valid;
             mattes_value_and_moments(*metric, st, 1, &value, mom, &valid);
             last_valid = valid;
-            for (int k = 0; k < np; ++k) {
-                double acc = 0.0;
-                for (int d = 0; d < 3; ++d) {
-                    acc += j0[d * np + k] * mom[d];
-                    for (int e = 0; e < 3; ++e) acc += c[(e * 3 + d) * np + k] * mom[3 + d * 3 + e];
-                }
-                g[k] = acc;
-            }
-            return value;
+            return contract_objective(kind, value, mom, t, g, last_valid);
         }
         double mom[14];
         ms_moments(*metric, st, 1, mom);
-        last_valid = static_cast<size_t>(mom[13]);
-        if (mom[13] == 0.0) return DBL_MAX;
-        const double inv = 1.0 / mom[13];
-        for (int k = 0; k < np; ++k) {
-            double acc = 0.0;
-            for (int d = 0; d < 3; ++d) {
-                acc += j0[d * np + k] * mom[1 + d];
-                for (int e = 0; e < 3; ++e) acc += c[(e * 3 + d) * np + k] * mom[4 + d * 3 + e];
-            }
-            g[k] = 2.0 * acc * inv;
-        }
-        return mom[0] * inv;
+        return contract_objective(kind, 0.0, mom, t, g, last_valid);
     }
 
     double value(const std::vector<double> &p) {
@@ -324,97 +338,120 @@ static OptResult line_search(const sitkhip_registration_config &cfg, const std::
 }
 
 // ------------------------------------------------------------------- driver
+static void check_config(const sitkhip_registration_config &cfg, int kind) {
+    if (cfg.levels < 1 || cfg.levels > SITKHIP_MAX_LEVELS)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "levels must be 1..8");
+    if (kind != SITKHIP_TRANSFORM_TRANSLATION && kind != SITKHIP_TRANSFORM_EULER3D &&
+        kind != SITKHIP_TRANSFORM_AFFINE)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "unsupported transform kind");
+    if (cfg.optimizer != SITKHIP_OPTIMIZER_REGULAR_STEP && cfg.optimizer != SITKHIP_OPTIMIZER_LINE_SEARCH)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "unsupported optimizer");
+}
+
+static Transform initial_transform(const sitkhip_transform &tr) {
+    Transform T;
+    T.kind = tr.kind;
+    T.compute_zyx = tr.compute_zyx != 0;
+    for (int d = 0; d < 3; ++d) T.center[d] = tr.center[d];
+    T.recompute();
+    T.set_parameters(tr.params);
+    return T;
+}
+
+// One level's metric, built the way prepare_level does it (method.rs:
+// 2416-2569): both images smoothed at full resolution, the fixed placed onto
+// the shrunk grid, the moving kept at full size; then the level's sample draw
+// (metric.rs:419-437) and, for Mattes, the histogram geometry from the value
+// ranges.
+struct LevelInputs {
+    std::unique_ptr<Metric, void (*)(Metric *)> metric{nullptr, metric_destroy};
+    sitkhip_geometry grid{};
+};
+
 struct ImageHolder {
     Image *img = nullptr;
     bool owned = false;
     ~ImageHolder() { if (owned) delete img; }
 };
 
+static LevelInputs prepare_level(const Image &fixed, const Image &moving,
+                                 const sitkhip_registration_config &cfg, int level) {
+    const double s = cfg.sigma[level];
+    const double sig[3] = {s, s, s};
+    const size_t *f = cfg.shrink[level];
+    const bool smooth = s > 0.0;
+    const bool native = f[0] == 1 && f[1] == 1 && f[2] == 1;
+    ImageHolder sf, sm, fl;
+    sf.img = smooth ? recursive_gaussian(fixed, sig) : const_cast<Image *>(&fixed);
+    sf.owned = smooth;
+    sm.img = smooth ? recursive_gaussian(moving, sig) : const_cast<Image *>(&moving);
+    sm.owned = smooth;
+    if (native) {
+        fl.img = sf.img;
+    } else {
+        sitkhip_geometry grid;
+        long long off[3];
+        shrunk_geometry(fixed.geom, f, grid, off);
+        fl.img = resample_linear(*sf.img, grid, nullptr, 0, 0.0);
+        fl.owned = true;
+    }
+    const Image &fixed_level = *fl.img;
+    const size_t nvox = fixed_level.len();
+    std::vector<int64_t> draws;
+    const int64_t *idx = nullptr;
+    size_t nidx = 0;
+    if (cfg.sampling == 1 || cfg.sampling == 2) {
+        if (cfg.sampling == 1) {
+            size_t stride = static_cast<size_t>(std::ceil(1.0 / cfg.sampling_percentage));
+            if (stride < 1) stride = 1;
+            for (size_t v = 0; v < nvox; v += stride) draws.push_back(static_cast<int64_t>(v));
+        } else {
+            const size_t cnt = static_cast<size_t>(static_cast<double>(nvox) * cfg.sampling_percentage);
+            draws.resize(cnt);
+            uint64_t state = cfg.seed;
+            for (size_t i = 0; i < cnt; ++i) {
+                state += 0x9E3779B97F4A7C15ull;
+                uint64_t z = state;
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                z ^= z >> 31;
+                draws[i] = static_cast<int64_t>(z % static_cast<uint64_t>(nvox));
+            }
+        }
+        idx = draws.data();
+        nidx = draws.size();
+    }
+    LevelInputs out;
+    out.grid = fixed_level.geom;
+    out.metric.reset(
+        metric_create(cfg.metric, fixed_level, idx, nidx, nullptr, *sm.img, nullptr, cfg.precision));
+    if (cfg.metric == SITKHIP_METRIC_MATTES) {
+        double fr[2], mr[2];
+        metric_value_ranges(*out.metric, fr, mr);
+        sitkhip_mattes_bins b;
+        mattes_bins_from_ranges(fr, mr, static_cast<size_t>(cfg.bins), &b);
+        mattes_set_bins(*out.metric, b);
+    }
+    return out;
+}
+
 void register_images(const Image &fixed, const Image &moving, const sitkhip_registration_config &cfg,
                      sitkhip_transform &tr, sitkhip_registration_result &res) {
     const auto t0 = std::chrono::steady_clock::now();
-    if (cfg.levels < 1 || cfg.levels > SITKHIP_MAX_LEVELS)
-        fail(SITKHIP_ERR_INVALID_ARGUMENT, "levels must be 1..8");
-    if (tr.kind != SITKHIP_TRANSFORM_TRANSLATION && tr.kind != SITKHIP_TRANSFORM_EULER3D &&
-        tr.kind != SITKHIP_TRANSFORM_AFFINE)
-        fail(SITKHIP_ERR_INVALID_ARGUMENT, "unsupported transform kind");
-    if (cfg.optimizer != SITKHIP_OPTIMIZER_REGULAR_STEP && cfg.optimizer != SITKHIP_OPTIMIZER_LINE_SEARCH)
-        fail(SITKHIP_ERR_INVALID_ARGUMENT, "unsupported optimizer");
+    check_config(cfg, tr.kind);
     std::memset(&res, 0, sizeof(res));
-    Transform T;
-    T.kind = tr.kind;
-    T.compute_zyx = tr.compute_zyx != 0;
-    for (int d = 0; d < 3; ++d) T.center[d] = tr.center[d];
-    T.recompute();
+    Transform T = initial_transform(tr);
     std::vector<double> p(tr.params, tr.params + T.nparams);
-    T.set_parameters(p.data());
 
     for (int level = 0; level < cfg.levels; ++level) {
-        const double s = cfg.sigma[level];
-        const double sig[3] = {s, s, s};
-        const size_t *f = cfg.shrink[level];
-        const bool smooth = s > 0.0;
-        const bool native = f[0] == 1 && f[1] == 1 && f[2] == 1;
-        // prepare_level (method.rs:2416-2569): smooth both at full resolution,
-        // place the fixed onto the shrunk grid, keep the moving at full size.
-        ImageHolder sf, sm, fl;
-        sf.img = smooth ? recursive_gaussian(fixed, sig) : const_cast<Image *>(&fixed);
-        sf.owned = smooth;
-        sm.img = smooth ? recursive_gaussian(moving, sig) : const_cast<Image *>(&moving);
-        sm.owned = smooth;
-        if (native) {
-            fl.img = sf.img;
-        } else {
-            sitkhip_geometry grid;
-            long long off[3];
-            shrunk_geometry(fixed.geom, f, grid, off);
-            fl.img = resample_linear(*sf.img, grid, nullptr, 0, 0.0);
-            fl.owned = true;
-        }
-        const Image &fixed_level = *fl.img;
-        const size_t nvox = fixed_level.len();
-        std::vector<int64_t> draws;
-        const int64_t *idx = nullptr;
-        size_t nidx = 0;
-        if (cfg.sampling == 1 || cfg.sampling == 2) {
-            size_t cnt = 0;
-            if (cfg.sampling == 1) {
-                size_t stride = static_cast<size_t>(std::ceil(1.0 / cfg.sampling_percentage));
-                if (stride < 1) stride = 1;
-                for (size_t v = 0; v < nvox; v += stride) draws.push_back(static_cast<int64_t>(v));
-            } else {
-                cnt = static_cast<size_t>(static_cast<double>(nvox) * cfg.sampling_percentage);
-                draws.resize(cnt);
-                uint64_t state = cfg.seed;
-                for (size_t i = 0; i < cnt; ++i) {
-                    state += 0x9E3779B97F4A7C15ull;
-                    uint64_t z = state;
-                    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-                    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-                    z ^= z >> 31;
-                    draws[i] = static_cast<int64_t>(z % static_cast<uint64_t>(nvox));
-                }
-            }
-            idx = draws.data();
-            nidx = draws.size();
-        }
-        std::unique_ptr<Metric, void (*)(Metric *)> metric(
-            metric_create(cfg.metric, fixed_level, idx, nidx, nullptr, *sm.img, nullptr, cfg.precision),
-            metric_destroy);
-        if (cfg.metric == SITKHIP_METRIC_MATTES) {
-            double fr[2], mr[2];
-            metric_value_ranges(*metric, fr, mr);
-            sitkhip_mattes_bins b;
-            mattes_bins_from_ranges(fr, mr, static_cast<size_t>(cfg.bins), &b);
-            mattes_set_bins(*metric, b);
-        }
+        LevelInputs in = prepare_level(fixed, moving, cfg, level);
         LevelObjective obj;
-        obj.metric = metric.get();
+        obj.metric = in.metric.get();
         obj.kind = cfg.metric;
         obj.t = T;
 
         ScalesEstimator est;
-        est.build(fixed_level.geom, T);
+        est.build(in.grid, T);
         const int np = T.nparams;
         std::vector<double> scales = cfg.scales_mode == SITKHIP_SCALES_PHYSICAL_SHIFT
                                          ? est.scales()
@@ -474,6 +511,161 @@ void register_images(const Image &fixed, const Image &moving, const sitkhip_regi
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// ------------------------------------------------------------ batched driver
+// RegularStepGradientDescent (optimizer.rs:353-450) as a state machine that
+// consumes one evaluation at a time, so that the evaluations of many
+// registrations can be made together. The sequence of parameters is the one
+// regular_step() above produces (with LR Once the first evaluation also serves
+// the rate estimate: it is made at the same parameters).
+struct RsgdRun {
+    const sitkhip_registration_config *cfg = nullptr;
+    Transform T;
+    ScalesEstimator est;
+    std::vector<double> scales, p, prev, grad;
+    double lr = 0.0, relaxation = 1.0, value = DBL_MAX;
+    size_t valid = 0, evaluations = 0;
+    int taken = 0, stop = SITKHIP_STOP_MAX_ITERATIONS;
+    bool started = false;
+
+    // Takes the evaluation (value, grad) at p; returns true when p has moved
+    // and must be evaluated again, false when the level is finished.
+    bool advance() {
+        const size_t n = p.size();
+        std::vector<double> scaled(n), sq(n), dot(n);
+        for (size_t k = 0; k < n; ++k) scaled[k] = grad[k] / scales[k];
+        if (!started) {
+            started = true;
+            if (cfg->learning_rate_mode == SITKHIP_LR_ONCE) {
+                double gm0 = 0.0;
+                for (double v : scaled) gm0 += v * v;
+                lr = est.learning_rate(scaled) * std::sqrt(gm0);
+            } else if (cfg->learning_rate_mode == SITKHIP_LR_MANUAL) {
+                lr = cfg->learning_rate;
+            }
+        }
+        for (size_t k = 0; k < n; ++k) { sq[k] = scaled[k] * scaled[k]; dot[k] = scaled[k] * prev[k]; }
+        const double gm = std::sqrt(compensated(sq));
+        if (gm < cfg->gradient_tolerance) { stop = SITKHIP_STOP_GRADIENT_CONVERGED; return false; }
+        if (compensated(dot) < 0.0) relaxation *= cfg->relaxation;
+        double rate = lr;
+        if (cfg->learning_rate_mode == SITKHIP_LR_EACH_ITERATION) {
+            double g2 = 0.0;
+            for (double v : scaled) g2 += v * v;
+            rate = est.learning_rate(scaled) * std::sqrt(g2);
+        }
+        const double step = relaxation * rate;
+        if (step < cfg->minimum_step) { stop = SITKHIP_STOP_STEP_TOO_SMALL; return false; }
+        if (taken >= cfg->iterations) { stop = SITKHIP_STOP_MAX_ITERATIONS; return false; }
+        const double factor = step / gm;
+        for (size_t k = 0; k < n; ++k) p[k] -= factor * scaled[k];
+        prev = scaled;
+        ++taken;
+        return true;
+    }
+};
+
+void register_batch(size_t n, const Image *const *fixed, const Image *const *moving,
+                    const sitkhip_registration_config *cfg, sitkhip_transform *tr,
+                    sitkhip_registration_result *res, bool autotune) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (n == 0) fail(SITKHIP_ERR_INVALID_ARGUMENT, "an empty batch");
+    int max_levels = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (!fixed[i] || !moving[i]) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null image in batch");
+        check_config(cfg[i], tr[i].kind);
+        if (cfg[i].optimizer != SITKHIP_OPTIMIZER_REGULAR_STEP)
+            fail(SITKHIP_ERR_INVALID_ARGUMENT,
+                 "batched registration runs RegularStepGradientDescent (pair " + std::to_string(i) + ")");
+        if (cfg[i].precision != cfg[0].precision)
+            fail(SITKHIP_ERR_INVALID_ARGUMENT, "every pair of a batch has the same precision");
+        max_levels = std::max(max_levels, cfg[i].levels);
+        std::memset(&res[i], 0, sizeof(res[i]));
+    }
+    std::vector<Transform> Ts(n);
+    std::vector<std::vector<double>> P(n);
+    for (size_t i = 0; i < n; ++i) {
+        Ts[i] = initial_transform(tr[i]);
+        P[i].assign(tr[i].params, tr[i].params + Ts[i].nparams);
+    }
+    for (int level = 0; level < max_levels; ++level) {
+        std::vector<size_t> who;
+        for (size_t i = 0; i < n; ++i)
+            if (level < cfg[i].levels) who.push_back(i);
+        const size_t m = who.size();
+        std::vector<LevelInputs> inputs;
+        inputs.reserve(m);
+        std::vector<Metric *> metrics(m);
+        for (size_t k = 0; k < m; ++k) {
+            const size_t i = who[k];
+            inputs.push_back(prepare_level(*fixed[i], *moving[i], cfg[i], level));
+            metrics[k] = inputs.back().metric.get();
+        }
+        std::unique_ptr<BatchEngine, void (*)(BatchEngine *)> eng(batch_create(metrics.data(), m),
+                                                                   batch_destroy);
+        std::vector<RsgdRun> runs(m);
+        std::vector<double> stages(12 * m);
+        std::vector<uint8_t> active(m, 1);
+        for (size_t k = 0; k < m; ++k) {
+            const size_t i = who[k];
+            RsgdRun &r = runs[k];
+            r.cfg = &cfg[i];
+            r.T = Ts[i];
+            r.p = P[i];
+            r.prev.assign(r.p.size(), 0.0);
+            r.est.build(inputs[k].grid, r.T);
+            r.scales = cfg[i].scales_mode == SITKHIP_SCALES_PHYSICAL_SHIFT
+                           ? r.est.scales()
+                           : std::vector<double>(r.T.nparams, 1.0);
+            r.T.set_parameters(r.p.data());
+            r.T.stage(&stages[12 * k]);
+        }
+        if (autotune) batch_autotune(*eng, stages.data(), nullptr);
+        std::vector<BatchOutput> out(m);
+        for (;;) {
+            batch_evaluate(*eng, stages.data(), active.data(), out);
+            bool any = false;
+            for (size_t k = 0; k < m; ++k) {
+                if (!active[k]) continue;
+                RsgdRun &r = runs[k];
+                ++r.evaluations;
+                size_t valid = out[k].valid;
+                r.value = contract_objective(cfg[who[k]].metric, out[k].value, out[k].moments, r.T,
+                                             r.grad, valid);
+                r.valid = out[k].valid;
+                if (r.advance()) {
+                    r.T.set_parameters(r.p.data());
+                    r.T.stage(&stages[12 * k]);
+                    any = true;
+                } else {
+                    active[k] = 0;
+                }
+            }
+            if (!any) break;
+        }
+        for (size_t k = 0; k < m; ++k) {
+            const size_t i = who[k];
+            RsgdRun &r = runs[k];
+            if (r.valid == 0)
+                fail(SITKHIP_ERR_DEGENERATE_INPUT,
+                     "pair " + std::to_string(i) + ": no valid samples at the result");
+            res[i].level_value[level] = r.value;
+            res[i].level_iterations[level] = r.taken;
+            res[i].level_stop[level] = r.stop;
+            res[i].iterations += r.taken;
+            res[i].evaluations += r.evaluations;
+            res[i].value = r.value;
+            res[i].valid_points = r.valid;
+            P[i] = r.p;
+            Ts[i].set_parameters(P[i].data());
+        }
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (size_t i = 0; i < n; ++i) {
+        for (int k = 0; k < Ts[i].nparams; ++k) tr[i].params[k] = P[i][k];
+        res[i].seconds = secs;
+    }
+}
+
 }  // namespace sitkhip
 
 extern "C" int sitkhip_register(sitkhip_image fixed, sitkhip_image moving,
@@ -485,5 +677,19 @@ extern "C" int sitkhip_register(sitkhip_image fixed, sitkhip_image moving,
         if (!f || !m || !config || !transform || !result)
             sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "null argument");
         sitkhip::register_images(*f, *m, *config, *transform, *result);
+    });
+}
+
+extern "C" int sitkhip_register_batch(size_t n, const sitkhip_image *fixed,
+                                      const sitkhip_image *moving,
+                                      const sitkhip_registration_config *configs,
+                                      sitkhip_transform *transforms,
+                                      sitkhip_registration_result *results, int autotune) {
+    return sitkhip::run_guarded([&] {
+        if (!fixed || !moving || !configs || !transforms || !results)
+            sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "null argument");
+        sitkhip::register_batch(n, reinterpret_cast<sitkhip::Image *const *>(fixed),
+                                reinterpret_cast<sitkhip::Image *const *>(moving), configs,
+                                transforms, results, autotune != 0);
     });
 }

@@ -27,6 +27,9 @@ void set_last_error(const std::string &m) { g_last_error = m; }
 
 void check_hip(hipError_t e, const char *what, const char *file, int line) {
     if (e == hipSuccess) return;
+    // A failed call also sets the runtime's last-error slot; clear it so the
+    // next launch check reports only its own launch.
+    (void)hipGetLastError();
     char buf[512];
     std::snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s at %s:%d", static_cast<int>(e),
                   hipGetErrorString(e), what, file, line);

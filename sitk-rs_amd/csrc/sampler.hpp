@@ -104,6 +104,23 @@ __device__ __forceinline__ void map_point(const Stages &S, const double x[3], do
     p[0] = q0; p[1] = q1; p[2] = q2;
 }
 
+// A single stage held in registers (the batched kernels, one stage per pair):
+// the same expression as one step of the replay above.
+struct OneStage {
+    double m[12];
+};
+
+__device__ __forceinline__ void map_point(const OneStage &S, const double x[3], double p[3]) {
+    const double *m = S.m;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    a0 += m[0] * x[0]; a0 += m[1] * x[1]; a0 += m[2] * x[2];
+    a1 += m[3] * x[0]; a1 += m[4] * x[1]; a1 += m[5] * x[2];
+    a2 += m[6] * x[0]; a2 += m[7] * x[1]; a2 += m[8] * x[2];
+    p[0] = a0 + m[9];
+    p[1] = a1 + m[10];
+    p[2] = a2 + m[11];
+}
+
 // The located sample: which cell of the moving volume and where inside it.
 struct Cell {
     int base[3];

@@ -189,6 +189,17 @@ def lib():
                                        C.POINTER(TransformSpec), C.POINTER(RegistrationResult)]
         L.sitkhip_draw_samples.argtypes = [C.c_size_t, C.c_int, C.c_double, C.c_uint64,
                                            C.POINTER(C.c_int64), szp]
+        L.sitkhip_batch_create.argtypes = [C.POINTER(vp), C.c_size_t, C.POINTER(vp)]
+        L.sitkhip_batch_free.argtypes = [vp]
+        L.sitkhip_batch_free.restype = None
+        L.sitkhip_batch_set_tiling.argtypes = [vp, C.c_size_t, C.c_uint, C.c_int]
+        L.sitkhip_batch_get_tiling.argtypes = [vp, szp, C.POINTER(C.c_uint)]
+        L.sitkhip_batch_autotune.argtypes = [vp, dp, szp, C.POINTER(C.c_uint), dp]
+        L.sitkhip_batch_evaluate.argtypes = [vp, dp, C.POINTER(C.c_uint8), dp, dp, szp]
+        L.sitkhip_register_batch.argtypes = [C.c_size_t, C.POINTER(vp), C.POINTER(vp),
+                                             C.POINTER(RegistrationConfig),
+                                             C.POINTER(TransformSpec),
+                                             C.POINTER(RegistrationResult), C.c_int]
         L.sitkhip_profile_enable.argtypes = [C.c_int]
         L.sitkhip_profile_query.argtypes = [C.c_char_p, C.POINTER(C.c_uint64), dp]
         L.sitkhip_profile_dump.argtypes = [C.c_char_p, C.c_size_t]
@@ -609,3 +620,83 @@ def register(fixed: DeviceImage, moving: DeviceImage, config: RegistrationConfig
     _check(lib().sitkhip_register(fixed._h, moving._h, C.byref(config), C.byref(t), C.byref(r)))
     n = {TRANSLATION: 3, EULER3D: 6, AFFINE: 12}[kind]
     return np.array(t.params[:n]), r
+
+
+def _transform_spec(kind, params, center=(0.0, 0.0, 0.0), compute_zyx=False) -> TransformSpec:
+    t = TransformSpec()
+    t.kind = kind
+    t.compute_zyx = int(compute_zyx)
+    t.center[:] = [float(v) for v in center]
+    for i, v in enumerate(params):
+        t.params[i] = float(v)
+    return t
+
+
+# ------------------------------------------------------------------------ batches
+class Batch:
+    """Many resident metrics evaluated together (one captured graph per
+    evaluation). The metrics must outlive the batch."""
+
+    def __init__(self, metrics):
+        self._metrics = list(metrics)  # keep them alive
+        n = len(self._metrics)
+        arr = (C.c_void_p * n)(*[m._h for m in self._metrics])
+        h = C.c_void_p()
+        _check(lib().sitkhip_batch_create(arr, n, C.byref(h)))
+        self._h = h
+        self.n = n
+
+    def set_tiling(self, tile=0, grid=0, use_graph=True) -> None:
+        _check(lib().sitkhip_batch_set_tiling(self._h, tile, grid, int(use_graph)))
+
+    def tiling(self):
+        t, g = C.c_size_t(), C.c_uint()
+        _check(lib().sitkhip_batch_get_tiling(self._h, C.byref(t), C.byref(g)))
+        return t.value, g.value
+
+    def autotune(self, stages):
+        st = np.ascontiguousarray(stages, dtype=np.float64).reshape(self.n, 12)
+        t, g, ms = C.c_size_t(), C.c_uint(), C.c_double()
+        _check(lib().sitkhip_batch_autotune(self._h, _dp(st), C.byref(t), C.byref(g), C.byref(ms)))
+        return t.value, g.value, ms.value
+
+    def evaluate(self, stages, active=None):
+        """Returns (values[n], moments[n, 14], valid[n])."""
+        st = np.ascontiguousarray(stages, dtype=np.float64).reshape(self.n, 12)
+        vals = np.full(self.n, np.nan)
+        mom = np.zeros((self.n, 14))
+        valid = np.zeros(self.n, dtype=np.uint64)
+        ap = None
+        if active is not None:
+            self._act = np.ascontiguousarray(active, dtype=np.uint8)
+            ap = self._act.ctypes.data_as(C.POINTER(C.c_uint8))
+        _check(lib().sitkhip_batch_evaluate(self._h, _dp(st), ap, _dp(vals), _dp(mom),
+                                            valid.ctypes.data_as(C.POINTER(C.c_size_t))))
+        return vals, mom, valid
+
+    def free(self) -> None:
+        if self._h:
+            lib().sitkhip_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def register_batch(fixed, moving, configs, kinds, params, centers, autotune=False,
+                   compute_zyx=False):
+    """n registrations run together, one batched evaluation per iteration.
+    Returns (list of parameter arrays, list of RegistrationResult)."""
+    n = len(fixed)
+    fa = (C.c_void_p * n)(*[f._h for f in fixed])
+    ma = (C.c_void_p * n)(*[m._h for m in moving])
+    ca = (RegistrationConfig * n)(*configs)
+    ta = (TransformSpec * n)(*[_transform_spec(k, p, c, compute_zyx)
+                               for k, p, c in zip(kinds, params, centers)])
+    ra = (RegistrationResult * n)()
+    _check(lib().sitkhip_register_batch(n, fa, ma, ca, ta, ra, int(autotune)))
+    npar = {TRANSLATION: 3, EULER3D: 6, AFFINE: 12}
+    return [np.array(ta[i].params[:npar[kinds[i]]]) for i in range(n)], [ra[i] for i in range(n)]
