@@ -87,9 +87,10 @@ def main() -> int:
     df = S.DeviceImage.upload(fixed.data, spacing=fixed.spacing, origin=fixed.origin)
     dm = S.DeviceImage.upload(moving.data, spacing=moving.spacing, origin=moving.origin)
     metric = S.ResidentMattes(df, dm, bins=50, indices=idx, precision=prec)
-    # the evaluation point: the identity affine about the volume centre, the
-    # state every registration of this pair starts from
-    I = W.AffineMap(np.eye(3), np.zeros(3), fixed.center())
+    # the evaluation point: the registration's solution (the inverse of the
+    # warp that made the moving image), where a converging optimizer spends its
+    # iterations
+    I = T_gt.inverse()
     stage = I.stage().reshape(1, 12)
 
     # secondary: the level-2 pyramid smoothing of one volume (sigma 2 mm)

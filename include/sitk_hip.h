@@ -365,6 +365,62 @@ int sitkhip_register_batch(size_t n, const sitkhip_image *fixed, const sitkhip_i
                            sitkhip_transform *transforms, sitkhip_registration_result *results,
                            int autotune);
 
+/* ------------------------------------------------------------- generators */
+/* Seeded synthetic volume (no counterpart in the reference; the benchmark
+ * pairs' generator): the sum over `nblobs` Gaussian blobs (physical centres
+ * [3 b ..], amplitudes, sigmas; each blob ends at 4.5 sigma along every axis)
+ * plus N(0, noise_sigma) voxel noise drawn per voxel from SplitMix64(seed, voxel
+ * index) by Box-Muller. Identity direction only. */
+int sitkhip_synth_blobs(const sitkhip_geometry *geometry, int nblobs, const double *centres,
+                        const double *amplitudes, const double *sigmas, double noise_sigma,
+                        uint64_t seed, sitkhip_image *out);
+
+/* ------------------------------------------------------ learned initializer */
+/* A small 3-D CNN that predicts an affine start for a registration from the
+ * image pair downsampled to 64^3. No counterpart in the reference (its
+ * initializers are geometric, src/registration/initializer.rs); it is what a
+ * registration loop calls before the optimizer. Architecture: five 3^3
+ * convolutions (2-16-32-64-64-128 channels, stride 2 from the second, ReLU),
+ * global average pool, fc 128-128 (ReLU), fc 128-12. Outputs: the 9 entries of
+ * (A - I) and t / 10 mm of x -> A (x - c) + t + c about the fixed image's
+ * centre c (fixed to moving, the registration's own direction). Convolutions
+ * (forward and backward) run as BF16 implicit GEMMs on the matrix cores with
+ * FP32 accumulation; weights and Adam state are FP32. */
+typedef struct sitkhip_cnn_s *sitkhip_cnn;
+
+/* He-initialised weights from `seed` (SplitMix64); batches of up to max_batch. */
+int sitkhip_cnn_create(int max_batch, uint64_t seed, sitkhip_cnn *out);
+void sitkhip_cnn_free(sitkhip_cnn net);
+int sitkhip_cnn_param_count(sitkhip_cnn net, size_t *n);
+/* Flat FP32 parameters: per conv layer W[co][27][ci] (ci of the first layer
+ * padded to 8, the padding zero) then bias[co]; fc1 W[128][128], b[128]; fc2
+ * W[12][128], b[12]. */
+int sitkhip_cnn_get_params(sitkhip_cnn net, float *out, size_t n);
+int sitkhip_cnn_set_params(sitkhip_cnn net, const float *in, size_t n);
+/* Gradients of the last backward pass, same layout. */
+int sitkhip_cnn_get_grads(sitkhip_cnn net, float *out, size_t n);
+/* Input slot `slot`: both images linearly resampled onto 64^3 grids spanning
+ * their own extents, z-scored, packed as BF16. */
+int sitkhip_cnn_set_input(sitkhip_cnn net, int slot, sitkhip_image fixed, sitkhip_image moving);
+/* The same from two host 64^3 float volumes (x fastest). */
+int sitkhip_cnn_set_input_raw(sitkhip_cnn net, int slot, const float *fixed64,
+                              const float *moving64);
+/* Forward pass over slots 0..batch-1; out: batch x 12 (may be NULL). */
+int sitkhip_cnn_forward(sitkhip_cnn net, int batch, float *out);
+/* Backward pass of the last forward from dL/dout (batch x 12). */
+int sitkhip_cnn_backward(sitkhip_cnn net, int batch, const float *dout);
+/* One Adam step on the gradients of the last backward pass. */
+int sitkhip_cnn_adam(sitkhip_cnn net, float lr, float beta1, float beta2, float eps);
+/* forward + mean-squared error against targets (batch x 12) + backward + Adam
+ * (beta 0.9 / 0.999, eps 1e-8). */
+int sitkhip_cnn_train_step(sitkhip_cnn net, int batch, const float *targets, float lr,
+                           float *loss);
+/* Activation `index` (0 the packed input, l + 1 the output of conv l) of the
+ * last forward, or with gradient != 0 the gradient w.r.t. it from the last
+ * backward (after the ReLU mask), as floats; n must be its element count. */
+int sitkhip_cnn_activation(sitkhip_cnn net, int index, int gradient, int batch, float *out,
+                           size_t n);
+
 /* -------------------------------------------------------------- profiling */
 /* When enabled, every kernel launch of this library is bracketed by HIP events
  * on the library stream and its duration accumulated per kernel name. */

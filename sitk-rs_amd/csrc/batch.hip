@@ -58,6 +58,7 @@ struct PairDesc {
 
 struct PairControl {
     double stage[12];
+    IndexMap X;
     int active;
     int pad;
 };
@@ -110,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void batch_pass1(const PairDesc *__restrict
             const int nb = B * B + B;
             for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
             __syncthreads();
-            hist_range<R>(d.F, d.M, S, d.SS, d.fbin, d.P, s_hist, s_hist + B * B, begin, end);
+            hist_range<R>(d.F, d.M, S, c.X, d.SS, d.fbin, d.P, s_hist, s_hist + B * B, begin, end);
             __syncthreads();
             for (int i = threadIdx.x; i < nb; i += kBlock) {
                 const unsigned long long v = s_hist[i];
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(kBlock) void batch_pass1(const PairDesc *__restrict
             double acc[14];
 #pragma unroll
             for (int k = 0; k < 14; ++k) acc[k] = 0.0;
-            ms_range<R, true>(d.F, d.M, S, d.SS, d.fval, begin, end, acc);
+            ms_range<R, true>(d.F, d.M, S, c.X, d.SS, d.fval, begin, end, acc);
             block_reduce_store<14>(acc, d.partials, static_cast<unsigned>(tl.index));
         }
         __syncthreads();  // the next tile reuses the LDS
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void batch_pass2(const PairDesc *__restrict
         double acc[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) acc[k] = 0.0;
-        deriv_range<R>(d.F, d.M, S, d.SS, d.fbin, d.P, s_pr, begin, end, acc);
+        deriv_range<R>(d.F, d.M, S, c.X, d.SS, d.fbin, d.P, s_pr, begin, end, acc);
         block_reduce_store<12>(acc, d.partials, static_cast<unsigned>(tl.index));
         __syncthreads();
     }
@@ -476,6 +477,7 @@ static void set_control(BatchEngine &b, const double *stages, const uint8_t *act
     if (!stages) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null stages");
     for (size_t i = 0; i < b.n; ++i) {
         std::memcpy(b.ctl_host[i].stage, stages + 12 * i, 12 * sizeof(double));
+        b.ctl_host[i].X = make_index_map(b.hdesc[i].F, b.hdesc[i].M, stages + 12 * i, 1);
         b.ctl_host[i].active = active ? (active[i] ? 1 : 0) : 1;
     }
 }

@@ -50,7 +50,7 @@ namespace sitkhip {
 
 // -------------------------------------------------------------- mean squares
 template <typename R, bool MOMENTS>
-__global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Stages S, SampleSet SS,
+__global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Stages S, IndexMap X, SampleSet SS,
                                                   const float *__restrict__ fval,
                                                   double *__restrict__ partials) {
     constexpr int NS = MOMENTS ? 14 : 2;
@@ -60,14 +60,14 @@ __global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Stag
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    ms_range<R, MOMENTS>(F, M, S, SS, fval, begin, end, acc);
+    ms_range<R, MOMENTS>(F, M, S, X, SS, fval, begin, end, acc);
     block_reduce_store<NS>(acc, partials, logical);
 }
 
 // Pass 1: joint histogram (fixed point) + fixed-marginal counts.
 template <typename R, bool LDS>
 __global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, Stages S,
-                                                      SampleSet SS, const uint8_t *__restrict__ fbin,
+                                                      IndexMap X, SampleSet SS, const uint8_t *__restrict__ fbin,
                                                       MattesParams P,
                                                       unsigned long long *__restrict__ g_joint,
                                                       unsigned long long *__restrict__ g_marg) {
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, 
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    hist_range<R>(F, M, S, SS, fbin, P, joint, marg, begin, end);
+    hist_range<R>(F, M, S, X, SS, fbin, P, joint, marg, begin, end);
     if (LDS) {
         __syncthreads();
         for (int i = threadIdx.x; i < nb; i += kBlock) {
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kTail) void mattes_tail_kernel(unsigned long long *
 // Pass 2: the 12 pRatio-weighted moments, folded by the last workgroup.
 template <typename R, bool LDS>
 __global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M, Stages S,
-                                                       SampleSet SS, const uint8_t *__restrict__ fbin,
+                                                       IndexMap X, SampleSet SS, const uint8_t *__restrict__ fbin,
                                                        MattesParams P,
                                                        const R *__restrict__ g_pratio,
                                                        double *__restrict__ partials) {
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M,
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    deriv_range<R>(F, M, S, SS, fbin, P, pr, begin, end, acc);
+    deriv_range<R>(F, M, S, X, SS, fbin, P, pr, begin, end, acc);
     block_reduce_store<12>(acc, partials, logical);
 }
 
@@ -475,11 +475,11 @@ static void launch_hist(Metric &m, const Stages &S, bool tail) {
         LaunchScope scope("mattes_hist");
         if (lds)
             hipLaunchKernelGGL((mattes_hist<R, true>), dim3(kGrid), dim3(kBlock), shm,
-                               backend().stream, m.F, m.Mv, S, m.SS, m.fbin.ptr, m.P, m.joint.ptr,
+                               backend().stream, m.F, m.Mv, S, make_index_map(m.F, m.Mv, S.s, S.n), m.SS, m.fbin.ptr, m.P, m.joint.ptr,
                                m.marg.ptr);
         else
             hipLaunchKernelGGL((mattes_hist<R, false>), dim3(kGrid), dim3(kBlock), 0,
-                               backend().stream, m.F, m.Mv, S, m.SS, m.fbin.ptr, m.P, m.joint.ptr,
+                               backend().stream, m.F, m.Mv, S, make_index_map(m.F, m.Mv, S.s, S.n), m.SS, m.fbin.ptr, m.P, m.joint.ptr,
                                m.marg.ptr);
         check_launch("mattes_hist");
     }
@@ -512,11 +512,11 @@ static void launch_deriv(Metric &m, const Stages &S, const R *pratio) {
         LaunchScope scope("mattes_deriv");
         if (lds)
             hipLaunchKernelGGL((mattes_deriv<R, true>), dim3(kGrid), dim3(kBlock), shm,
-                               backend().stream, m.F, m.Mv, S, m.SS, m.fbin.ptr, m.P, pratio,
+                               backend().stream, m.F, m.Mv, S, make_index_map(m.F, m.Mv, S.s, S.n), m.SS, m.fbin.ptr, m.P, pratio,
                                m.partials.ptr);
         else
             hipLaunchKernelGGL((mattes_deriv<R, false>), dim3(kGrid), dim3(kBlock), 0,
-                               backend().stream, m.F, m.Mv, S, m.SS, m.fbin.ptr, m.P, pratio,
+                               backend().stream, m.F, m.Mv, S, make_index_map(m.F, m.Mv, S.s, S.n), m.SS, m.fbin.ptr, m.P, pratio,
                                m.partials.ptr);
         check_launch("mattes_deriv");
     }
@@ -599,7 +599,7 @@ static void launch_ms(Metric &m, const Stages &S) {
     {
         LaunchScope scope(MOM ? "ms_moments" : "ms_value");
         hipLaunchKernelGGL((ms_pass<R, MOM>), dim3(kGrid), dim3(kBlock), 0, backend().stream, m.F,
-                           m.Mv, S, m.SS, m.fval.ptr, m.partials.ptr);
+                           m.Mv, S, make_index_map(m.F, m.Mv, S.s, S.n), m.SS, m.fval.ptr, m.partials.ptr);
         check_launch("ms_pass");
     }
     launch_fold<MOM ? 14 : 2>(m);

@@ -235,16 +235,90 @@ struct Located {
     bool ok[kU];
 };
 
+// c = A idx + b for the fixed grid F, the point-map stages (nstage x 12) and
+// the moving geometry M, composed in extended precision on the host.
+inline IndexMap make_index_map(const FixedGrid &F, const MovingVol &M, const double *stages,
+                               int nstage) {
+    long double Mt[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, t[3] = {0, 0, 0};
+    for (int s = 0; s < nstage; ++s) {
+        const double *m = stages + 12 * s;
+        long double nM[3][3], nt[3];
+        for (int r = 0; r < 3; ++r) {
+            nt[r] = m[9 + r];
+            for (int c = 0; c < 3; ++c) {
+                nM[r][c] = 0;
+                for (int k = 0; k < 3; ++k) nM[r][c] += static_cast<long double>(m[r * 3 + k]) * Mt[k][c];
+                nt[r] += static_cast<long double>(m[r * 3 + c]) * t[c];
+            }
+        }
+        for (int r = 0; r < 3; ++r) {
+            t[r] = nt[r];
+            for (int c = 0; c < 3; ++c) Mt[r][c] = nM[r][c];
+        }
+    }
+    // q = Mt o_f + t - o_m ; A = P Mt I2P ; b = P q
+    long double q[3], MI[3][3];
+    for (int r = 0; r < 3; ++r) {
+        q[r] = t[r] - static_cast<long double>(M.origin[r]);
+        for (int k = 0; k < 3; ++k) q[r] += Mt[r][k] * static_cast<long double>(F.origin[k]);
+        for (int c = 0; c < 3; ++c) {
+            MI[r][c] = 0;
+            for (int k = 0; k < 3; ++k) MI[r][c] += Mt[r][k] * static_cast<long double>(F.i2p[k * 3 + c]);
+        }
+    }
+    IndexMap X;
+    for (int r = 0; r < 3; ++r) {
+        long double br = 0;
+        for (int k = 0; k < 3; ++k) br += static_cast<long double>(M.p2i[r * 3 + k]) * q[k];
+        X.b[r] = static_cast<double>(br);
+        for (int c = 0; c < 3; ++c) {
+            long double a = 0;
+            for (int k = 0; k < 3; ++k) a += static_cast<long double>(M.p2i[r * 3 + k]) * MI[k][c];
+            X.A[r * 3 + c] = static_cast<double>(a);
+        }
+    }
+    return X;
+}
+
+// The fixed physical point for the moment sums of the FP32 path: the same
+// affine map as the sampler's point, with FMAs (the moments only accumulate
+// it; no decision depends on it).
+__device__ __forceinline__ void fixed_point_fma(const FixedGrid &F, uint32_t flat, double x[3]) {
+    uint32_t i0, i1, i2;
+    split_index(F, flat, i0, i1, i2);
+    const double d0 = static_cast<double>(i0), d1 = static_cast<double>(i1),
+                 d2 = static_cast<double>(i2);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        x[r] = fma(F.i2p[r * 3 + 2], d2, fma(F.i2p[r * 3 + 1], d1, fma(F.i2p[r * 3 + 0], d0, F.origin[r])));
+}
+
+// R = float: the composed index map decides, and the reference's FP64 chain
+// runs only for samples within kGuard of a decision boundary. R = double: the
+// chain always (the F64 path follows the reference term for term).
 template <typename R, typename Map>
 __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol &M,
-                                             const Map &S, const Batch &b, Located<R> &L) {
+                                             const Map &S, const IndexMap &X, const Batch &b,
+                                             Located<R> &L) {
     Cell cell[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
-        double x[3], p[3];
-        fixed_point(F, b.flat[u], x);
-        map_point(S, x, p);
-        L.ok[u] = b.live[u] && locate(M, p, cell[u]);
+        bool done = false;
+        if constexpr (sizeof(R) == 4) {
+            uint32_t i0, i1, i2;
+            split_index(F, b.flat[u], i0, i1, i2);
+            bool ok = false;
+            if (locate_composed(M, X, i0, i1, i2, ok, cell[u])) {
+                L.ok[u] = b.live[u] && ok;
+                done = true;
+            }
+        }
+        if (!done) {
+            double x[3], p[3];
+            fixed_point(F, b.flat[u], x);
+            map_point(S, x, p);
+            L.ok[u] = b.live[u] && locate(M, p, cell[u]);
+        }
         if (!L.ok[u]) {
             cell[u].base[0] = cell[u].base[1] = cell[u].base[2] = 0;
             cell[u].frac[0] = cell[u].frac[1] = cell[u].frac[2] = 0.0;
@@ -255,6 +329,7 @@ __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol
 #pragma unroll
     for (int u = 0; u < kU; ++u) corners(M, cell[u], L.v[u]);
 }
+
 // ------------------------------------------------ per-range sample loops
 // Each processes the samples [begin, end) of one sample set with the kBlock
 // lanes of the calling workgroup, kU samples per lane per step (every lane
@@ -266,7 +341,7 @@ __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol
 // [sq, count] without MOMENTS.
 template <typename R, bool MOMENTS, typename Map>
 __device__ __forceinline__ void ms_range(const FixedGrid &F, const MovingVol &M, const Map &S,
-                                         const SampleSet &SS, const float *__restrict__ fval,
+                                         const IndexMap &X, const SampleSet &SS, const float *__restrict__ fval,
                                          long long begin, long long end,
                                          double (&acc)[MOMENTS ? 14 : 2]) {
     for (long long s0 = begin + threadIdx.x; s0 < end; s0 += kBlock * kU) {
@@ -281,7 +356,7 @@ __device__ __forceinline__ void ms_range(const FixedGrid &F, const MovingVol &M,
             fv[u] = fval[sc];
         }
         Located<R> L;
-        locate_batch<R>(F, M, S, b, L);
+        locate_batch<R>(F, M, S, X, b, L);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!L.ok[u]) continue;
@@ -291,7 +366,8 @@ __device__ __forceinline__ void ms_range(const FixedGrid &F, const MovingVol &M,
             acc[0] = fma(diff, diff, acc[0]);
             if (MOMENTS) {
                 double x[3];
-                fixed_point(F, b.flat[u], x);
+                if constexpr (sizeof(R) == 4) fixed_point_fma(F, b.flat[u], x);
+                else fixed_point(F, b.flat[u], x);
                 physical_gradient<R>(M, gi, g);
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
@@ -314,7 +390,7 @@ __device__ __forceinline__ void ms_range(const FixedGrid &F, const MovingVol &M,
 // (64-bit fixed point) and one count into marg[fixed bin].
 template <typename R, typename Map>
 __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &M, const Map &S,
-                                           const SampleSet &SS, const uint8_t *__restrict__ fbin,
+                                           const IndexMap &X, const SampleSet &SS, const uint8_t *__restrict__ fbin,
                                            const MattesParams &P, unsigned long long *joint,
                                            unsigned long long *marg, long long begin,
                                            long long end) {
@@ -331,7 +407,7 @@ __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &
             b.live[u] = in && b.fb[u] != kNoBin;
         }
         Located<R> L;
-        locate_batch<R>(F, M, S, b, L);
+        locate_batch<R>(F, M, S, X, b, L);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!L.ok[u]) continue;
@@ -356,7 +432,7 @@ __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &
 // (src/cuda/ops/mattes.rs:75-91).
 template <typename R, typename Map>
 __device__ __forceinline__ void deriv_range(const FixedGrid &F, const MovingVol &M, const Map &S,
-                                            const SampleSet &SS, const uint8_t *__restrict__ fbin,
+                                            const IndexMap &X, const SampleSet &SS, const uint8_t *__restrict__ fbin,
                                             const MattesParams &P, const R *pr, long long begin,
                                             long long end, double (&acc)[12]) {
     const int B = P.bins;
@@ -372,7 +448,7 @@ __device__ __forceinline__ void deriv_range(const FixedGrid &F, const MovingVol 
             b.live[u] = in && b.fb[u] != kNoBin;
         }
         Located<R> L;
-        locate_batch<R>(F, M, S, b, L);
+        locate_batch<R>(F, M, S, X, b, L);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!L.ok[u]) continue;
@@ -388,7 +464,8 @@ __device__ __forceinline__ void deriv_range(const FixedGrid &F, const MovingVol 
             for (int k = 0; k < 4; ++k) w += row[k] * bspline3_deriv<R>(arg[k]);
             const double wd = static_cast<double>(w);
             double x[3];
-            fixed_point(F, b.flat[u], x);
+            if constexpr (sizeof(R) == 4) fixed_point_fma(F, b.flat[u], x);
+            else fixed_point(F, b.flat[u], x);
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const double wg = wd * static_cast<double>(g[d]);

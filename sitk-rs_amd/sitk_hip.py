@@ -200,6 +200,23 @@ def lib():
                                              C.POINTER(RegistrationConfig),
                                              C.POINTER(TransformSpec),
                                              C.POINTER(RegistrationResult), C.c_int]
+        fp = C.POINTER(C.c_float)
+        L.sitkhip_synth_blobs.argtypes = [gp, C.c_int, dp, dp, dp, C.c_double, C.c_uint64,
+                                          C.POINTER(vp)]
+        L.sitkhip_cnn_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
+        L.sitkhip_cnn_free.argtypes = [vp]
+        L.sitkhip_cnn_free.restype = None
+        L.sitkhip_cnn_param_count.argtypes = [vp, szp]
+        L.sitkhip_cnn_get_params.argtypes = [vp, fp, C.c_size_t]
+        L.sitkhip_cnn_set_params.argtypes = [vp, fp, C.c_size_t]
+        L.sitkhip_cnn_get_grads.argtypes = [vp, fp, C.c_size_t]
+        L.sitkhip_cnn_set_input.argtypes = [vp, C.c_int, vp, vp]
+        L.sitkhip_cnn_set_input_raw.argtypes = [vp, C.c_int, fp, fp]
+        L.sitkhip_cnn_forward.argtypes = [vp, C.c_int, fp]
+        L.sitkhip_cnn_backward.argtypes = [vp, C.c_int, fp]
+        L.sitkhip_cnn_adam.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.sitkhip_cnn_train_step.argtypes = [vp, C.c_int, fp, C.c_float, fp]
+        L.sitkhip_cnn_activation.argtypes = [vp, C.c_int, C.c_int, C.c_int, fp, C.c_size_t]
         L.sitkhip_profile_enable.argtypes = [C.c_int]
         L.sitkhip_profile_query.argtypes = [C.c_char_p, C.POINTER(C.c_uint64), dp]
         L.sitkhip_profile_dump.argtypes = [C.c_char_p, C.c_size_t]
@@ -700,3 +717,103 @@ def register_batch(fixed, moving, configs, kinds, params, centers, autotune=Fals
     _check(lib().sitkhip_register_batch(n, fa, ma, ca, ta, ra, int(autotune)))
     npar = {TRANSLATION: 3, EULER3D: 6, AFFINE: 12}
     return [np.array(ta[i].params[:npar[kinds[i]]]) for i in range(n)], [ra[i] for i in range(n)]
+
+
+# ------------------------------------------------------------- learned initializer
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class CNN:
+    """The learned initializer network (64^3 x 2 input, 12 affine outputs)."""
+
+    EDGE = 64
+    CHANNELS = (8, 16, 32, 64, 64, 128)
+
+    def __init__(self, max_batch: int = 16, seed: int = 0):
+        h = C.c_void_p()
+        _check(lib().sitkhip_cnn_create(max_batch, seed, C.byref(h)))
+        self._h = h
+        self.max_batch = max_batch
+        n = C.c_size_t()
+        _check(lib().sitkhip_cnn_param_count(self._h, C.byref(n)))
+        self.nparams = n.value
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparams, dtype=np.float32)
+        _check(lib().sitkhip_cnn_get_params(self._h, _fp(out), out.size))
+        return out
+
+    def set_params(self, p) -> None:
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().sitkhip_cnn_set_params(self._h, _fp(a), a.size))
+
+    def grads(self) -> np.ndarray:
+        out = np.empty(self.nparams, dtype=np.float32)
+        _check(lib().sitkhip_cnn_get_grads(self._h, _fp(out), out.size))
+        return out
+
+    def set_input(self, slot: int, fixed: DeviceImage, moving: DeviceImage) -> None:
+        _check(lib().sitkhip_cnn_set_input(self._h, slot, fixed._h, moving._h))
+
+    def set_input_raw(self, slot: int, fixed64, moving64) -> None:
+        f = np.ascontiguousarray(fixed64, dtype=np.float32)
+        m = np.ascontiguousarray(moving64, dtype=np.float32)
+        if f.size != self.EDGE ** 3 or m.size != self.EDGE ** 3:
+            raise ValueError("64^3 volumes expected")
+        _check(lib().sitkhip_cnn_set_input_raw(self._h, slot, _fp(f), _fp(m)))
+
+    def forward(self, batch: int) -> np.ndarray:
+        out = np.empty((batch, 12), dtype=np.float32)
+        _check(lib().sitkhip_cnn_forward(self._h, batch, _fp(out)))
+        return out
+
+    def backward(self, batch: int, dout) -> None:
+        d = np.ascontiguousarray(dout, dtype=np.float32).reshape(batch, 12)
+        _check(lib().sitkhip_cnn_backward(self._h, batch, _fp(d)))
+
+    def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8) -> None:
+        _check(lib().sitkhip_cnn_adam(self._h, lr, beta1, beta2, eps))
+
+    def train_step(self, batch: int, targets, lr=1e-3) -> float:
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(batch, 12)
+        loss = C.c_float()
+        _check(lib().sitkhip_cnn_train_step(self._h, batch, _fp(t), lr, C.byref(loss)))
+        return loss.value
+
+    def activation(self, index: int, batch: int, gradient=False) -> np.ndarray:
+        e = self.EDGE
+        for l in range(index):
+            e //= (1 if l == 0 else 2)
+        c = self.CHANNELS[index]
+        out = np.empty((batch, e, e, e, c), dtype=np.float32)
+        _check(lib().sitkhip_cnn_activation(self._h, index, int(gradient), batch, _fp(out), out.size))
+        return out
+
+    def free(self) -> None:
+        if self._h:
+            lib().sitkhip_cnn_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------- generators
+def synth_blobs(size, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), centres=(), amplitudes=(),
+                sigmas=(), noise=0.0, seed=0) -> DeviceImage:
+    """Sum of Gaussian blobs plus seeded voxel noise, generated on the device."""
+    g = Geometry.make(size, spacing, origin)
+    c = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1)
+    a = np.ascontiguousarray(amplitudes, dtype=np.float64)
+    s = np.ascontiguousarray(sigmas, dtype=np.float64)
+    nb = a.size
+    if c.size != 3 * nb or s.size != nb:
+        raise ValueError("one centre (3), amplitude and sigma per blob")
+    h = C.c_void_p()
+    _check(lib().sitkhip_synth_blobs(C.byref(g), nb, _dp(c) if nb else None, _dp(a) if nb else None,
+                                     _dp(s) if nb else None, float(noise), int(seed), C.byref(h)))
+    return DeviceImage(h)

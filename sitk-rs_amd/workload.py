@@ -172,3 +172,20 @@ def config_large_pair(seed: int = 2):
     T_gt = random_affine(rng, 10.0, 0.05, 10.0, fixed.center())
     moving = resample_linear(fixed, T_gt)
     return fixed, moving, T_gt
+
+
+def config_batched_pairs(n: int = 512, seed: int = 3):
+    """`n` independent rigid pairs: per-pair edge lengths drawn in [32, 128]
+    per axis, spacing 1 mm, the blob generator (40 blobs), each with its own
+    Euler3D perturbation (rotations +-15 deg, translations +-15 mm about the
+    volume centre). Even pairs use mean squares, odd pairs Mattes MI.
+    Returns a list of dicts {fixed: Volume, T_gt: AffineMap, metric: "ms"|"mattes"};
+    the moving image of a pair is fixed resampled through T_gt."""
+    rng = np.random.default_rng(seed)
+    pairs = []
+    for i in range(n):
+        size = tuple(int(v) for v in rng.integers(32, 129, 3))
+        vol = blob_volume(size, (1.0, 1.0, 1.0), seed=seed * 1_000_003 + i, nblobs=40)
+        _, T_gt = random_euler(rng, 15.0, 15.0, vol.center())
+        pairs.append({"fixed": vol, "T_gt": T_gt, "metric": "ms" if i % 2 == 0 else "mattes"})
+    return pairs

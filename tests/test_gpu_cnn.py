@@ -1,0 +1,140 @@
+"""Learned-initializer CNN kernels against the FP64 oracle (oracle/cnn.py).
+
+Each layer is checked on the device's own inputs (its BF16 activations and the
+BF16-rounded weights), so the comparison isolates that layer's kernel:
+  forward conv  : |device - oracle| <= 1 bfloat16 step of the output (2^-8 relative)
+                  plus 1e-3 of the layer's largest magnitude (FP32 accumulation order)
+  weight / bias : relative 1e-3 of the gradient's norm (same BF16 inputs, FP32 vs
+  gradients       FP64 accumulation)
+  input gradient: 2^-8 relative + 1e-3 of the largest magnitude (BF16 store)
+  MLP head      : FP32 vs FP64, 1e-4 relative
+Training is checked to lower the loss and to be deterministic run to run.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import cnn as N
+
+pytestmark = pytest.mark.gpu
+
+B = 2
+
+
+def _volumes(seed):
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(*(np.arange(64, dtype=np.float32),) * 3, indexing="ij")
+    out = []
+    for _ in range(2 * B):
+        v = np.zeros((64, 64, 64), np.float32)
+        for _ in range(6):
+            c = rng.uniform(8, 56, 3)
+            s = rng.uniform(3, 8)
+            v += rng.uniform(50, 200) * np.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2)
+                                               / (2 * s * s))
+        v += rng.normal(0, 3, v.shape).astype(np.float32)
+        out.append(v)
+    return out
+
+
+@pytest.fixture(scope="module")
+def net_run(hip):
+    net = hip.CNN(max_batch=4, seed=7)
+    vols = _volumes(1)
+    for b in range(B):
+        net.set_input_raw(b, vols[2 * b], vols[2 * b + 1])
+    out = net.forward(B)
+    acts = [net.activation(i, B) for i in range(6)]
+    offs, n = N.layout()
+    assert n == net.nparams
+    p = N.unflatten(net.params().astype(np.float64), offs)
+    rng = np.random.default_rng(2)
+    dout = rng.normal(0, 0.1, (B, 12)).astype(np.float32)
+    net.backward(B, dout)
+    grads = N.unflatten(net.grads().astype(np.float64), offs)
+    dacts = [net.activation(i, B, gradient=True) for i in range(1, 6)]
+    return dict(net=net, out=out, acts=acts, p=p, dout=dout, grads=grads, dacts=dacts)
+
+
+def _close(dev, ref, rel_step=2.0 ** -8, floor=1e-3):
+    dev = np.asarray(dev, np.float64)
+    ref = np.asarray(ref, np.float64)
+    tol = rel_step * np.abs(ref) + floor * max(np.max(np.abs(ref)), 1e-30)
+    bad = np.abs(dev - ref) > tol
+    return not bad.any(), float(np.max(np.abs(dev - ref) - tol))
+
+
+def test_input_packing_is_z_scored_bf16(net_run):
+    x = net_run["acts"][0]
+    assert x.shape == (B, 64, 64, 64, 8)
+    assert not x[..., 2:].any()
+    for b in range(B):
+        for c in range(2):
+            ch = x[b, ..., c].astype(np.float64)
+            assert abs(ch.mean()) < 1e-2 and abs(ch.std() - 1.0) < 1e-2
+
+
+@pytest.mark.parametrize("layer", range(5))
+def test_conv_forward_layer(net_run, layer):
+    p, acts = net_run["p"], net_run["acts"]
+    w = N.bf16(p[f"w{layer}"])
+    ref = np.maximum(N.conv_forward(acts[layer].astype(np.float64), w, p[f"b{layer}"],
+                                    N.STRIDES[layer]), 0.0)
+    ok, excess = _close(acts[layer + 1], ref)
+    assert ok, excess
+
+
+def test_head_forward(net_run):
+    p, acts = net_run["p"], net_run["acts"]
+    last = acts[5].astype(np.float64)
+    feat = last.reshape(B, -1, 128).mean(axis=1)
+    _, out = N.head_forward(feat, p)
+    assert np.allclose(net_run["out"], out, rtol=1e-4, atol=1e-4 * np.abs(out).max())
+
+
+def test_head_backward(net_run):
+    p, acts, g = net_run["p"], net_run["acts"], net_run["grads"]
+    last = acts[5].astype(np.float64)
+    feat = last.reshape(B, -1, 128).mean(axis=1)
+    h, _ = N.head_forward(feat, p)
+    gw1, gb1, gw2, gb2, dfeat = N.head_backward(feat, h, net_run["dout"].astype(np.float64), p)
+    for dev, ref in [(g["fw1"], gw1), (g["fb1"], gb1), (g["fw2"], gw2), (g["fb2"], gb2)]:
+        assert np.linalg.norm(dev - ref) <= 1e-4 * max(np.linalg.norm(ref), 1e-30)
+    # the pooled-feature gradient reaches conv5's output as dfeat / 64 (before its mask)
+    d5 = net_run["dacts"][4].astype(np.float64)
+    want = np.broadcast_to(dfeat[:, None, None, None, :] / 64.0, d5.shape) * (acts[5] > 0)
+    ok, excess = _close(d5, want)
+    assert ok, excess
+
+
+@pytest.mark.parametrize("layer", range(5))
+def test_conv_backward_layer(net_run, layer):
+    p, acts, g, dacts = net_run["p"], net_run["acts"], net_run["grads"], net_run["dacts"]
+    dy = dacts[layer].astype(np.float64)   # gradient at conv `layer`'s output, masked
+    w = N.bf16(p[f"w{layer}"])
+    dw, db, dx = N.conv_backward(acts[layer].astype(np.float64), w, dy, N.STRIDES[layer])
+    assert np.linalg.norm(g[f"w{layer}"] - dw) <= 1e-3 * max(np.linalg.norm(dw), 1e-30)
+    assert np.linalg.norm(g[f"b{layer}"] - db) <= 1e-3 * max(np.linalg.norm(db), 1e-30)
+    if layer > 0:
+        # the device stores dx as BF16 and masks it with the previous ReLU
+        dev = dacts[layer - 1].astype(np.float64)
+        ref = dx * (acts[layer] > 0)
+        ok, excess = _close(dev, ref)
+        assert ok, excess
+
+
+def test_training_lowers_the_loss_and_is_deterministic(hip):
+    vols = _volumes(5)
+    rng = np.random.default_rng(6)
+    targets = rng.normal(0, 0.3, (B, 12)).astype(np.float32)
+    runs = []
+    for _ in range(2):
+        net = hip.CNN(max_batch=B, seed=11)
+        for b in range(B):
+            net.set_input_raw(b, vols[2 * b], vols[2 * b + 1])
+        losses = [net.train_step(B, targets, lr=1e-3) for _ in range(60)]
+        runs.append((losses, net.params()))
+    losses, params = runs[0]
+    assert losses[-1] < 0.2 * losses[0], (losses[0], losses[-1])
+    assert np.array_equal(params, runs[1][1])
+    assert losses == runs[1][0]
