@@ -365,6 +365,28 @@ int sitkhip_register_batch(size_t n, const sitkhip_image *fixed, const sitkhip_i
                            sitkhip_transform *transforms, sitkhip_registration_result *results,
                            int autotune);
 
+/* ---------------------------------------------------------------- B-spline */
+/* Mattes MI through a cubic B-spline transform: the value and the
+ * sparse-support derivative (src/registration/mattes.rs:722-796, transform
+ * src/transform/bspline.rs:485-703), which the reference's device backend
+ * refuses (registration/device.rs:79-100). Built on a Mattes metric with bins
+ * set (the metric must outlive it and is used by it); the control grid is the
+ * transform's fixed parameters (bspline.rs:229-322): grid_size = mesh + 3,
+ * origin, spacing, direction. Coefficients are 3 x ncp doubles [d][cp], cp =
+ * x + gs0 (y + gs1 z). The derivative is deterministic (no atomics). */
+typedef struct sitkhip_bspline_s *sitkhip_bspline;
+
+int sitkhip_bspline_create(sitkhip_metric metric, const size_t grid_size[3],
+                           const double grid_origin[3], const double grid_spacing[3],
+                           const double grid_direction[9], sitkhip_bspline *out);
+void sitkhip_bspline_free(sitkhip_bspline b);
+int sitkhip_bspline_parameter_count(sitkhip_bspline b, size_t *n);
+int sitkhip_bspline_mattes_value(sitkhip_bspline b, const double *coeffs, double *value,
+                                 size_t *valid);
+/* deriv: 3 x ncp doubles, zero where no valid sample supports a point. */
+int sitkhip_bspline_mattes_value_and_derivative(sitkhip_bspline b, const double *coeffs,
+                                                double *value, double *deriv, size_t *valid);
+
 /* ------------------------------------------------------------- generators */
 /* Seeded synthetic volume (no counterpart in the reference; the benchmark
  * pairs' generator): the sum over `nblobs` Gaussian blobs (physical centres

@@ -466,6 +466,20 @@ static void require_bins(const Metric &m) {
         fail(SITKHIP_ERR_HISTOGRAM_SHAPE, "the Mattes histogram geometry has not been set");
 }
 
+void launch_tail(Metric &m, TailOut *out) {
+    const int B = m.P.bins;
+    const bool lds = B <= kLdsMaxBins;
+    const size_t shm = lds ? static_cast<size_t>(B * B + B) * sizeof(unsigned long long) : 0;
+    LaunchScope scope("mattes_tail");
+    if (lds)
+        hipLaunchKernelGGL((mattes_tail_kernel<true>), dim3(1), dim3(kTail), shm, backend().stream,
+                           m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out);
+    else
+        hipLaunchKernelGGL((mattes_tail_kernel<false>), dim3(1), dim3(kTail), 0, backend().stream,
+                           m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out);
+    check_launch("mattes_tail");
+}
+
 template <typename R>
 static void launch_hist(Metric &m, const Stages &S, bool tail) {
     const int B = m.P.bins;
@@ -484,15 +498,7 @@ static void launch_hist(Metric &m, const Stages &S, bool tail) {
         check_launch("mattes_hist");
     }
     if (!tail) return;
-    TailOut *out = &m.res_dev->tail;
-    LaunchScope scope("mattes_tail");
-    if (lds)
-        hipLaunchKernelGGL((mattes_tail_kernel<true>), dim3(1), dim3(kTail), shm, backend().stream,
-                           m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out);
-    else
-        hipLaunchKernelGGL((mattes_tail_kernel<false>), dim3(1), dim3(kTail), 0, backend().stream,
-                           m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out);
-    check_launch("mattes_tail");
+    launch_tail(m, &m.res_dev->tail);
 }
 
 template <int NS>

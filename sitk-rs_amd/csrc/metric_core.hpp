@@ -293,6 +293,12 @@ __device__ __forceinline__ void fixed_point_fma(const FixedGrid &F, uint32_t fla
         x[r] = fma(F.i2p[r * 3 + 2], d2, fma(F.i2p[r * 3 + 1], d1, fma(F.i2p[r * 3 + 0], d0, F.origin[r])));
 }
 
+// Whether a point map is affine (the composed index map applies to it).
+template <typename Map>
+struct MapTraits {
+    static constexpr bool affine = true;
+};
+
 // R = float: the composed index map decides, and the reference's FP64 chain
 // runs only for samples within kGuard of a decision boundary. R = double: the
 // chain always (the F64 path follows the reference term for term).
@@ -304,7 +310,7 @@ __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
         bool done = false;
-        if constexpr (sizeof(R) == 4) {
+        if constexpr (sizeof(R) == 4 && MapTraits<Map>::affine) {
             uint32_t i0, i1, i2;
             split_index(F, b.flat[u], i0, i1, i2);
             bool ok = false;
@@ -603,5 +609,8 @@ struct Metric {
     size_t nsamples = 0;
     size_t voxels_fixed = 0;
 };
+
+// The Mattes tail of metric m into `out` (one workgroup; metric.hip).
+void launch_tail(Metric &m, TailOut *out);
 
 }  // namespace sitkhip

@@ -201,6 +201,12 @@ def lib():
                                              C.POINTER(TransformSpec),
                                              C.POINTER(RegistrationResult), C.c_int]
         fp = C.POINTER(C.c_float)
+        L.sitkhip_bspline_create.argtypes = [vp, szp, dp, dp, dp, C.POINTER(vp)]
+        L.sitkhip_bspline_free.argtypes = [vp]
+        L.sitkhip_bspline_free.restype = None
+        L.sitkhip_bspline_parameter_count.argtypes = [vp, szp]
+        L.sitkhip_bspline_mattes_value.argtypes = [vp, dp, dp, szp]
+        L.sitkhip_bspline_mattes_value_and_derivative.argtypes = [vp, dp, dp, dp, szp]
         L.sitkhip_synth_blobs.argtypes = [gp, C.c_int, dp, dp, dp, C.c_double, C.c_uint64,
                                           C.POINTER(vp)]
         L.sitkhip_cnn_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
@@ -817,3 +823,51 @@ def synth_blobs(size, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), centres=(
     _check(lib().sitkhip_synth_blobs(C.byref(g), nb, _dp(c) if nb else None, _dp(a) if nb else None,
                                      _dp(s) if nb else None, float(noise), int(seed), C.byref(h)))
     return DeviceImage(h)
+
+
+# ------------------------------------------------------------------------ B-spline
+class BSplineMattes:
+    """Mattes MI through a cubic B-spline transform with a fixed control grid
+    (grid_size = mesh + 3, origin, spacing, direction). Uses `metric` (a
+    ResidentMattes with bins set), which must outlive it."""
+
+    def __init__(self, metric, grid_size, grid_origin, grid_spacing, grid_direction=None):
+        self._metric = metric
+        gs = (C.c_size_t * 3)(*[int(v) for v in grid_size])
+        o = np.ascontiguousarray(grid_origin, dtype=np.float64)
+        sp = np.ascontiguousarray(grid_spacing, dtype=np.float64)
+        d = np.ascontiguousarray(np.eye(3) if grid_direction is None else grid_direction,
+                                 dtype=np.float64).reshape(9)
+        h = C.c_void_p()
+        _check(lib().sitkhip_bspline_create(metric._h, gs, _dp(o), _dp(sp), _dp(d), C.byref(h)))
+        self._h = h
+        n = C.c_size_t()
+        _check(lib().sitkhip_bspline_parameter_count(self._h, C.byref(n)))
+        self.nparams = n.value
+
+    def value(self, coeffs):
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        v, n = C.c_double(), C.c_size_t()
+        _check(lib().sitkhip_bspline_mattes_value(self._h, _dp(c), C.byref(v), C.byref(n)))
+        return v.value, n.value
+
+    def value_and_derivative(self, coeffs):
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        if c.size != self.nparams:
+            raise ValueError("3 x control-point coefficients expected")
+        d = np.empty(self.nparams)
+        v, n = C.c_double(), C.c_size_t()
+        _check(lib().sitkhip_bspline_mattes_value_and_derivative(self._h, _dp(c), C.byref(v), _dp(d),
+                                                                 C.byref(n)))
+        return v.value, d, n.value
+
+    def free(self) -> None:
+        if self._h:
+            lib().sitkhip_bspline_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

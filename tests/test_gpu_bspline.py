@@ -1,0 +1,76 @@
+"""Mattes MI through a cubic B-spline transform on the device against the
+oracle's sparse-support evaluation (crates/sitk/src/registration/mattes.rs:
+722-796, FD-pinned in tests/test_oracle_golden.py). Tolerances as for the
+affine path: F64 value 1e-9, derivative 1e-7 (relative, vector norm);
+F32 value 1e-6, derivative 1e-4; the valid count exact."""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import blob_volume, rel_vec
+
+pytestmark = pytest.mark.gpu
+
+SP = [1.0, 1.1, 0.9]
+ORG = [-2.0, 1.0, 0.5]
+
+
+def _case(hip, precision, domain_size, mesh, sampled, seed):
+    f = blob_volume((18, 20, 22), 30 + seed, spacing=SP)
+    m = blob_volume((18, 20, 22), 60 + seed, spacing=SP) * 0.4 + f * 0.6
+    idx = None
+    if sampled:
+        idx = hip.draw_samples(f.size, hip.SAMPLING_RANDOM, 0.5, 3 + seed)
+    df = hip.DeviceImage.upload(f, spacing=SP, origin=ORG)
+    dm = hip.DeviceImage.upload(m, spacing=SP, origin=ORG)
+    met = hip.ResidentMattes(df, dm, bins=24, indices=idx, precision=precision)
+    rng = np.random.default_rng(seed)
+    t = O.make_bspline(domain_size, SP, ORG, np.eye(3), mesh, np.zeros(1))
+    ncp = int(np.prod(list(t.grid_size)))
+    coeffs = rng.normal(0.0, 0.6, 3 * ncp)
+    t = O.make_bspline(domain_size, SP, ORG, np.eye(3), mesh, coeffs)
+    bs = hip.BSplineMattes(met, list(t.grid_size), list(t.grid_origin), list(t.grid_spacing),
+                           np.array(list(t.grid_direction)).reshape(3, 3))
+    pb = O.MetricProblem(f, SP, ORG, m, SP, ORG, indices=idx)
+    g = pb.mattes_geometry(24)
+    return bs, pb, g, t, coeffs, met
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+@pytest.mark.parametrize("domain,mesh,sampled", [((22, 20, 18), (3, 3, 3), False),
+                                                 ((22, 20, 18), (2, 4, 3), True),
+                                                 ((12, 10, 14), (2, 2, 2), False)])
+def test_bspline_value_and_derivative(hip, precision, domain, mesh, sampled):
+    bs, pb, g, t, coeffs, met = _case(hip, precision, domain, mesh, sampled, seed=len(domain) + mesh[1])
+    want_v, want_d, want_n = pb.mattes_evaluate(g, t)
+    v, d, n = bs.value_and_derivative(coeffs)
+    assert n == want_n
+    tv, td = (1e-9, 1e-7) if precision == 1 else (1e-6, 1e-4)
+    assert abs(v - want_v) <= tv * abs(want_v), (v, want_v)
+    assert rel_vec(d, want_d) < td, rel_vec(d, want_d)
+    v2, n2 = bs.value(coeffs)
+    assert v2 == v and n2 == n
+    # deterministic
+    v3, d3, _ = bs.value_and_derivative(coeffs)
+    assert v3 == v and np.array_equal(d3, d)
+
+
+def test_zero_coefficients_are_the_identity(hip):
+    bs, pb, g, t, coeffs, met = _case(hip, 1, (22, 20, 18), (3, 3, 3), False, seed=9)
+    zero = np.zeros_like(coeffs)
+    v, d, n = bs.value_and_derivative(zero)
+    ident = O.make_transform(O.TRANSLATION, [0.0, 0.0, 0.0])
+    want_v, _, want_n = pb.mattes_evaluate(g, ident)
+    assert n == want_n and abs(v - want_v) <= 1e-12 * abs(want_v)
+
+
+def test_bspline_refusals(hip):
+    f = blob_volume((12, 12, 12), 1)
+    df = hip.DeviceImage.upload(f)
+    ms = hip.ResidentMetric(df, df)
+    with pytest.raises(hip.CudaError, match="HistogramShape"):
+        hip.BSplineMattes(ms, (6, 6, 6), (0, 0, 0), (3, 3, 3))
+    mt = hip.ResidentMattes(df, df, bins=20)
+    with pytest.raises(hip.CudaError, match="InvalidArgument"):
+        hip.BSplineMattes(mt, (3, 6, 6), (0, 0, 0), (3, 3, 3))
