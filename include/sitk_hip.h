@@ -443,6 +443,13 @@ int sitkhip_cnn_train_step(sitkhip_cnn net, int batch, const float *targets, flo
 int sitkhip_cnn_activation(sitkhip_cnn net, int index, int gradient, int batch, float *out,
                            size_t n);
 
+/* ----------------------------------------------------------------- memory */
+/* Device memory is cached by size class after a free (hipMalloc / hipFree
+ * cost tens of microseconds and hipFree waits for the device); this returns
+ * every cached block to the runtime. No counterpart in the reference, whose
+ * buffers free on drop (src/cuda/buffer.rs:12-14). */
+int sitkhip_empty_cache(void);
+
 /* -------------------------------------------------------------- profiling */
 /* When enabled, every kernel launch of this library is bracketed by HIP events
  * on the library stream and its duration accumulated per kernel name. */

@@ -507,9 +507,8 @@ void batch_evaluate(BatchEngine &b, const double *stages, const uint8_t *active,
 BatchTiling batch_autotune(BatchEngine &b, const double *stages, double *best_ms) {
     set_control(b, stages, nullptr);
     const int cu = std::max(8, backend().cu_count);
-    const size_t tiles[] = {1024, 2048, 4096, 8192, 16384};
-    const unsigned grids[] = {static_cast<unsigned>(cu * 2), static_cast<unsigned>(cu * 4),
-                              static_cast<unsigned>(cu * 8)};
+    const size_t tiles[] = {2048, 8192};
+    const unsigned grids[] = {static_cast<unsigned>(cu * 4), static_cast<unsigned>(cu * 8)};
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
@@ -523,8 +522,8 @@ BatchTiling batch_autotune(BatchEngine &b, const double *stages, double *best_ms
                 t.grid = g - g % 8u;
                 batch_set_tiling(b, t);
                 ensure_graph(b);
-                for (int w = 0; w < 2; ++w) HIP_OK(hipGraphLaunch(b.exec, backend().stream));
-                const int reps = 5;
+                HIP_OK(hipGraphLaunch(b.exec, backend().stream));
+                const int reps = 3;
                 HIP_OK(hipEventRecord(e0, backend().stream));
                 for (int r = 0; r < reps; ++r) HIP_OK(hipGraphLaunch(b.exec, backend().stream));
                 HIP_OK(hipEventRecord(e1, backend().stream));

@@ -237,9 +237,7 @@ struct BSplineMetric {
     DeviceBuffer<int> cell_begin;
     DeviceBuffer<double> coef, cellT, deriv;
     HostResult *res = nullptr, *res_dev = nullptr;
-    ~BSplineMetric() {
-        if (res) (void)hipHostFree(res);
-    }
+    ~BSplineMetric() { pinned_slot_free(res); }
 };
 
 BSplineMetric *bspline_create(Metric &m, const size_t gs[3], const double origin[3],
@@ -322,10 +320,9 @@ BSplineMetric *bspline_create(Metric &m, const size_t gs[3], const double origin
         b->B.coef = b->coef.ptr;
         b->cellT.alloc(static_cast<size_t>(b->ncells) * 192);
         b->deriv.alloc(3 * ncp);
-        HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&b->res), sizeof(HostResult),
-                             hipHostMallocMapped | hipHostMallocCoherent));
-        std::memset(b->res, 0, sizeof(HostResult));
-        HIP_OK(hipHostGetDevicePointer(reinterpret_cast<void **>(&b->res_dev), b->res, 0));
+        void *dev = nullptr;
+        b->res = static_cast<HostResult *>(pinned_slot_alloc(sizeof(HostResult), &dev));
+        b->res_dev = static_cast<HostResult *>(dev);
         sync();
     } catch (...) {
         delete b;

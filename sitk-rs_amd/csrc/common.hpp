@@ -39,6 +39,23 @@ struct Backend {
 };
 Backend &backend();
 
+// Device memory through a caching allocator: a freed block is kept in a
+// per-size-class free list and handed to the next request of that class
+// (hipMalloc and hipFree cost tens of microseconds and hipFree waits for the
+// device). Every operation of this library is ordered on its one stream, so a
+// block freed after an enqueued use and reused by a later enqueued operation
+// is never touched by both at once. sitkhip_empty_cache() returns the cached
+// blocks to the runtime.
+void *device_alloc(size_t bytes);
+void device_free(void *ptr, size_t bytes);
+void device_empty_cache();
+
+// Pinned, device-mapped host slots of `bytes` (<= 256) for small per-metric
+// result blocks, from a pool of pinned pages: returns the host view and sets
+// *dev to the device view.
+void *pinned_slot_alloc(size_t bytes, void **dev);
+void pinned_slot_free(void *host);
+
 // RAII device allocation (crates/sitk/src/cuda/buffer.rs:12-149). A zero-length
 // request allocates one element, so a handle is never null
 // (crates/sitk/src/cuda/ops/resident.rs:544-546).
@@ -58,12 +75,11 @@ struct DeviceBuffer {
     ~DeviceBuffer() { release(); }
     void alloc(size_t n) {
         release();
-        size_t bytes = (n == 0 ? 1 : n) * sizeof(T);
-        HIP_OK(hipMalloc(reinterpret_cast<void **>(&ptr), bytes));
+        ptr = static_cast<T *>(device_alloc((n == 0 ? 1 : n) * sizeof(T)));
         len = n;
     }
     void release() {
-        if (ptr) { (void)hipFree(ptr); ptr = nullptr; }
+        if (ptr) { device_free(ptr, (len == 0 ? 1 : len) * sizeof(T)); ptr = nullptr; }
         len = 0;
     }
     size_t bytes() const { return len * sizeof(T); }

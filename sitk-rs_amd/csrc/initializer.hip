@@ -332,21 +332,27 @@ __global__ void relu_mask(uint16_t *__restrict__ dy, const uint16_t *__restrict_
         if (!(bf2f(y[i]) > 0.0f)) dy[i] = 0;
 }
 
-// Bias gradient: db[c] = sum over positions of dy[m][c]; one workgroup per
-// channel, fixed tree.
-__global__ __launch_bounds__(256) void bias_grad(const uint16_t *__restrict__ dy, int M, int C,
-                                                 float *__restrict__ db) {
+// Bias gradient, first stage: partial[s][c] = sum of dy[m][c] over the
+// positions of split s (workgroup (s), lanes over channels then positions,
+// fixed tree). The second stage is sum_partials over the splits.
+constexpr int kBiasSplits = 256;
+__global__ __launch_bounds__(256) void bias_grad_partials(const uint16_t *__restrict__ dy, int M,
+                                                          int C, float *__restrict__ partial) {
     __shared__ float red[256];
-    const int c = blockIdx.x;
-    float s = 0.0f;
-    for (int m = threadIdx.x; m < M; m += 256) s += bf2f(dy[static_cast<size_t>(m) * C + c]);
-    red[threadIdx.x] = s;
+    const int s = blockIdx.x;
+    const int per = (M + kBiasSplits - 1) / kBiasSplits;
+    const int m0 = s * per, m1 = min(M, m0 + per);
+    const int lanes_per_c = 256 / C;  // C divides 256 (16..128)
+    const int c = threadIdx.x % C, part = threadIdx.x / C;
+    float acc = 0.0f;
+    for (int m = m0 + part; m < m1; m += lanes_per_c) acc += bf2f(dy[static_cast<size_t>(m) * C + c]);
+    red[threadIdx.x] = acc;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
+    if (threadIdx.x < C) {
+        float t = 0.0f;
+        for (int p = 0; p < lanes_per_c; ++p) t += red[p * C + threadIdx.x];
+        partial[static_cast<size_t>(s) * C + threadIdx.x] = t;
     }
-    if (threadIdx.x == 0) db[c] = red[0];
 }
 
 // Global average pool of the last conv output [n][64][128] -> feat [n][128].
@@ -630,7 +636,7 @@ Net *create(int max_batch, uint64_t seed) {
             const int splits = static_cast<int>(std::min<size_t>(256, std::max<size_t>(1, M / 4096)));
             most = std::max(most, static_cast<size_t>(splits) * g.co * 27 * g.ci);
         }
-        net->partials.alloc(most);
+        net->partials.alloc(std::max<size_t>(most, static_cast<size_t>(kBiasSplits) * 128));
         net->feat.alloc(static_cast<size_t>(max_batch) * kFeat);
         net->hid.alloc(static_cast<size_t>(max_batch) * kHidden);
         net->out.alloc(static_cast<size_t>(max_batch) * kOut);
@@ -713,9 +719,15 @@ void backward(Net &net, int n, const float *dout_host) {
             LaunchScope scope("cnn_relu_mask");
             hipLaunchKernelGGL(relu_mask, dim3(grid_for(ny, 256, 8192)), dim3(256), 0, st,
                                net.dact[l + 1].ptr, net.act[l + 1].ptr, ny);
-            hipLaunchKernelGGL(bias_grad, dim3(g.co), dim3(256), 0, st, net.dact[l + 1].ptr, M, g.co,
-                               net.grads.ptr + net.off_b[l]);
             check_launch("cnn_relu_mask");
+        }
+        {
+            LaunchScope scope("cnn_bias_grad");
+            hipLaunchKernelGGL(bias_grad_partials, dim3(kBiasSplits), dim3(256), 0, st,
+                               net.dact[l + 1].ptr, M, g.co, net.partials.ptr);
+            hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, net.partials.ptr, kBiasSplits,
+                               static_cast<size_t>(g.co), net.grads.ptr + net.off_b[l]);
+            check_launch("cnn_bias_grad");
         }
         // weight gradient: [co] x [27 ci] over K = M output positions, split
         const int R = 27 * g.ci;

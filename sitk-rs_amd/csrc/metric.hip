@@ -190,26 +190,35 @@ __global__ __launch_bounds__(kBlock) void range_partials(const float *__restrict
 }
 
 
-static void range_of(const float *v, const uint8_t *mask, long long n, double out[2]) {
+// (min, max) of two value sets (either masked) with one synchronisation:
+// both partial reductions are enqueued, then read back together.
+static void ranges_of(const float *v1, const uint8_t *mask1, long long n1, double out1[2],
+                      const float *v2, const uint8_t *mask2, long long n2, double out2[2]) {
     const unsigned G = 512;
-    DeviceBuffer<double> part(G * 3);
+    DeviceBuffer<double> part(2 * G * 3);
     {
         LaunchScope scope("range_partials");
-        hipLaunchKernelGGL(range_partials, dim3(G), dim3(kBlock), 0, backend().stream, v, mask, n,
+        hipLaunchKernelGGL(range_partials, dim3(G), dim3(kBlock), 0, backend().stream, v1, mask1, n1,
                            part.ptr);
+        hipLaunchKernelGGL(range_partials, dim3(G), dim3(kBlock), 0, backend().stream, v2, mask2, n2,
+                           part.ptr + G * 3);
         check_launch("range_partials");
     }
-    std::vector<double> h(G * 3);
+    std::vector<double> h(2 * G * 3);
     part.download(h.data(), h.size());
     sync();
-    double lo = INFINITY, hi = -INFINITY, c = 0.0;
-    for (unsigned b = 0; b < G; ++b) {
-        lo = std::fmin(lo, h[b * 3]);
-        hi = std::fmax(hi, h[b * 3 + 1]);
-        c += h[b * 3 + 2];
+    for (int k = 0; k < 2; ++k) {
+        double lo = INFINITY, hi = -INFINITY, c = 0.0;
+        for (unsigned b = 0; b < G; ++b) {
+            const double *r = h.data() + (k * G + b) * 3;
+            lo = std::fmin(lo, r[0]);
+            hi = std::fmax(hi, r[1]);
+            c += r[2];
+        }
+        double *out = k == 0 ? out1 : out2;
+        if (c == 0.0) { out[0] = 0.0; out[1] = 0.0; }
+        else { out[0] = lo; out[1] = hi; }
     }
-    if (c == 0.0) { out[0] = 0.0; out[1] = 0.0; }
-    else { out[0] = lo; out[1] = hi; }
 }
 
 void invert3_host(const double m[9], double out[9]) {
@@ -304,7 +313,16 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
             // Ascending order: the metric is a sum over the sample multiset, so
             // the order only changes rounding of the FP64 moments (the histogram
             // is integer) and it turns the gathers into near-streaming reads.
-            std::sort(list.begin(), list.end());
+            // Dense draws are counted per voxel (linear time), sparse ones sorted.
+            if (list.size() >= nf / 16) {
+                std::vector<uint32_t> count(nf, 0);
+                for (uint32_t v : list) ++count[v];
+                size_t o = 0;
+                for (size_t v = 0; v < nf; ++v)
+                    for (uint32_t k = count[v]; k > 0; --k) list[o++] = static_cast<uint32_t>(v);
+            } else {
+                std::sort(list.begin(), list.end());
+            }
         } else if (fmask) {
             for (size_t s = 0; s < nf; ++s)
                 if (hmask[s]) list.push_back(static_cast<uint32_t>(s));
@@ -356,16 +374,14 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
         invert3_host(mi2p, m->Mv.p2i);
 
         m->partials.alloc(kGrid * 14);
-        HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&m->res), sizeof(HostResult),
-                             hipHostMallocMapped | hipHostMallocCoherent));
-        std::memset(m->res, 0, sizeof(HostResult));
-        HIP_OK(hipHostGetDevicePointer(reinterpret_cast<void **>(&m->res_dev), m->res, 0));
-        sync();
+        void *dev = nullptr;
+        m->res = static_cast<HostResult *>(pinned_slot_alloc(sizeof(HostResult), &dev));
+        m->res_dev = static_cast<HostResult *>(dev);
 
         // Value ranges (ops/mattes.rs:587-589): fixed over the sample set, moving
         // over its mask.
-        range_of(m->fval.ptr, nullptr, m->SS.n, m->fixed_range);
-        range_of(m->moving.ptr, m->Mv.mask, static_cast<long long>(nm), m->moving_range);
+        ranges_of(m->fval.ptr, nullptr, m->SS.n, m->fixed_range, m->moving.ptr, m->Mv.mask,
+                  static_cast<long long>(nm), m->moving_range);
     } catch (...) {
         delete m;
         throw;
@@ -457,7 +473,6 @@ void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b) {
     HIP_OK(hipMemsetAsync(m.joint.ptr, 0, m.joint.bytes(), backend().stream));
     HIP_OK(hipMemsetAsync(m.marg.ptr, 0, m.marg.bytes(), backend().stream));
     m.bins_set = true;
-    sync();
 }
 
 static void require_bins(const Metric &m) {

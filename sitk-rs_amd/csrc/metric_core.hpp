@@ -21,7 +21,10 @@
 namespace sitkhip {
 
 constexpr int kBlock = 256;
-constexpr unsigned kGrid = 1024;      // fixed: the reduction tree must not depend on n
+#ifndef SITKHIP_GRID
+#define SITKHIP_GRID 1024
+#endif
+constexpr unsigned kGrid = SITKHIP_GRID;  // fixed: the reduction tree must not depend on n
 constexpr int kLdsMaxBins = 90;       // bins^2 * 8 B of LDS per workgroup (<= 64 KB)
 constexpr double kFix = 4294967296.0; // 2^32 fixed-point units per histogram unit
 constexpr double kInvFix = 1.0 / 4294967296.0;
@@ -601,9 +604,7 @@ struct Metric {
     DeviceBuffer<unsigned long long> joint, marg;
     DeviceBuffer<double> pratio;
     DeviceBuffer<float> pratio_f;
-    ~Metric() {
-        if (res) (void)hipHostFree(res);
-    }
+    ~Metric() { pinned_slot_free(res); }
     double fixed_range[2] = {0, 0};
     double moving_range[2] = {0, 0};
     size_t nsamples = 0;

@@ -17,6 +17,8 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -568,6 +570,12 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
                     const sitkhip_registration_config *cfg, sitkhip_transform *tr,
                     sitkhip_registration_result *res, bool autotune) {
     const auto t0 = std::chrono::steady_clock::now();
+    // SITKHIP_TRACE=1: per-level phase times on stderr
+    const bool trace = std::getenv("SITKHIP_TRACE") != nullptr;
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
     if (n == 0) fail(SITKHIP_ERR_INVALID_ARGUMENT, "an empty batch");
     int max_levels = 0;
     for (size_t i = 0; i < n; ++i) {
@@ -592,6 +600,7 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
         for (size_t i = 0; i < n; ++i)
             if (level < cfg[i].levels) who.push_back(i);
         const size_t m = who.size();
+        const auto tp0 = now();
         std::vector<LevelInputs> inputs;
         inputs.reserve(m);
         std::vector<Metric *> metrics(m);
@@ -600,8 +609,10 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
             inputs.push_back(prepare_level(*fixed[i], *moving[i], cfg[i], level));
             metrics[k] = inputs.back().metric.get();
         }
+        const auto tp1 = now();
         std::unique_ptr<BatchEngine, void (*)(BatchEngine *)> eng(batch_create(metrics.data(), m),
                                                                    batch_destroy);
+        const auto tp2 = now();
         std::vector<RsgdRun> runs(m);
         std::vector<double> stages(12 * m);
         std::vector<uint8_t> active(m, 1);
@@ -619,9 +630,16 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
             r.T.set_parameters(r.p.data());
             r.T.stage(&stages[12 * k]);
         }
-        if (autotune) batch_autotune(*eng, stages.data(), nullptr);
+        // tuning pays only when one evaluation is long enough to time
+        size_t total_samples = 0;
+        for (size_t k = 0; k < m; ++k) total_samples += metric_sample_count(*metrics[k]);
+        if (autotune && total_samples >= (static_cast<size_t>(1) << 22))
+            batch_autotune(*eng, stages.data(), nullptr);
+        const auto tp3 = now();
+        int rounds = 0;
         std::vector<BatchOutput> out(m);
         for (;;) {
+            ++rounds;
             batch_evaluate(*eng, stages.data(), active.data(), out);
             bool any = false;
             for (size_t k = 0; k < m; ++k) {
@@ -641,6 +659,14 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
                 }
             }
             if (!any) break;
+        }
+        if (trace) {
+            const auto tp4 = now();
+            std::fprintf(stderr,
+                         "register_batch level %d: %zu pairs, prepare %.1f ms, batch %.1f ms, "
+                         "tune %.1f ms, %d evaluations %.1f ms (%.3f ms each)\n",
+                         level, m, ms(tp0, tp1), ms(tp1, tp2), ms(tp2, tp3), rounds, ms(tp3, tp4),
+                         ms(tp3, tp4) / rounds);
         }
         for (size_t k = 0; k < m; ++k) {
             const size_t i = who[k];

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "common.hpp"
@@ -42,6 +43,94 @@ void check_launch(const char *name) {
         fail(SITKHIP_ERR_DRIVER,
              std::string("launch of ") + name + " failed: " + hipGetErrorString(e));
     }
+}
+
+// ------------------------------------------------------ caching allocator
+namespace {
+std::mutex g_alloc_mu;
+std::unordered_map<size_t, std::vector<void *>> g_free_blocks;
+
+// Size classes: 8 steps per power of two (at most 12.5% rounding), >= 256 B.
+size_t size_class(size_t bytes) {
+    if (bytes <= 256) return 256;
+    int lg = 63 - __builtin_clzll(static_cast<unsigned long long>(bytes));
+    const size_t step = static_cast<size_t>(1) << (lg >= 3 ? lg - 3 : 0);
+    return (bytes + step - 1) / step * step;
+}
+}  // namespace
+
+void *device_alloc(size_t bytes) {
+    const size_t c = size_class(bytes);
+    {
+        std::lock_guard<std::mutex> lk(g_alloc_mu);
+        auto it = g_free_blocks.find(c);
+        if (it != g_free_blocks.end() && !it->second.empty()) {
+            void *p = it->second.back();
+            it->second.pop_back();
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, c);
+    if (e != hipSuccess) {
+        // out of memory with blocks cached: give them back and try once more
+        (void)hipGetLastError();
+        device_empty_cache();
+        HIP_OK(hipMalloc(&p, c));
+    }
+    return p;
+}
+
+void device_free(void *ptr, size_t bytes) {
+    if (!ptr) return;
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    g_free_blocks[size_class(bytes)].push_back(ptr);
+}
+
+void device_empty_cache() {
+    std::unordered_map<size_t, std::vector<void *>> blocks;
+    {
+        std::lock_guard<std::mutex> lk(g_alloc_mu);
+        blocks.swap(g_free_blocks);
+    }
+    if (blocks.empty()) return;
+    (void)hipStreamSynchronize(backend().stream);
+    for (auto &kv : blocks)
+        for (void *p : kv.second) (void)hipFree(p);
+}
+
+namespace {
+constexpr size_t kSlotBytes = 256;
+constexpr size_t kSlotsPerPage = 4096;
+std::mutex g_pin_mu;
+std::vector<char *> g_free_slots;                       // host addresses
+std::unordered_map<char *, char *> g_slot_dev;          // host slot -> device view
+}  // namespace
+
+void *pinned_slot_alloc(size_t bytes, void **dev) {
+    if (bytes > kSlotBytes) fail(SITKHIP_ERR_INVALID_ARGUMENT, "pinned slot too large");
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    if (g_free_slots.empty()) {
+        char *page = nullptr, *dpage = nullptr;
+        HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&page), kSlotBytes * kSlotsPerPage,
+                             hipHostMallocMapped | hipHostMallocCoherent));
+        HIP_OK(hipHostGetDevicePointer(reinterpret_cast<void **>(&dpage), page, 0));
+        for (size_t i = kSlotsPerPage; i-- > 0;) {
+            g_free_slots.push_back(page + i * kSlotBytes);
+            g_slot_dev[page + i * kSlotBytes] = dpage + i * kSlotBytes;
+        }
+    }
+    char *h = g_free_slots.back();
+    g_free_slots.pop_back();
+    std::memset(h, 0, kSlotBytes);
+    *dev = g_slot_dev[h];
+    return h;
+}
+
+void pinned_slot_free(void *host) {
+    if (!host) return;
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g_free_slots.push_back(static_cast<char *>(host));
 }
 
 // ----------------------------------------------------------------- backend
@@ -513,6 +602,10 @@ int sitkhip_profile_query(const char *prefix, uint64_t *count, double *total_ms)
         if (total_ms) *total_ms = ms;
     });
     return rc == SITKHIP_OK ? matched : -rc;
+}
+
+int sitkhip_empty_cache(void) {
+    return run_guarded([&] { device_empty_cache(); });
 }
 
 int sitkhip_profile_dump(char *buf, size_t len) {

@@ -110,3 +110,125 @@ def tre(T_est: W.AffineMap, pp) -> float:
     g = np.stack(np.meshgrid(*(np.arange(0, e, 8),) * 3, indexing="ij"), -1).reshape(-1, 3)
     g = g.astype(np.float64)
     return float(np.linalg.norm(T_est.apply(g) - pp["T_gt"].inverse().apply(g), axis=1).mean())
+
+
+def save(net: S.CNN, path: str) -> None:
+    np.save(path, net.params())
+
+
+def load(net: S.CNN, path: str) -> None:
+    net.set_params(np.load(path, allow_pickle=False))
+
+
+def train_schedule(net: S.CNN, steps: int, batch: int, lr: float = 1e-3, final_lr: float = 1e-4,
+                   seed: int = 0, log=None, log_every: int = 250):
+    """Online training with a cosine learning-rate decay from lr to final_lr."""
+    rng = np.random.default_rng(seed)
+    losses = []
+    t0 = time.perf_counter()
+    for step in range(steps):
+        a = final_lr + 0.5 * (lr - final_lr) * (1.0 + math.cos(math.pi * step / max(steps - 1, 1)))
+        targets = np.empty((batch, 12), np.float32)
+        for b in range(batch):
+            pp = pair_params(rng)
+            f, m = make_pair(pp)
+            net.set_input(b, f, m)
+            targets[b] = label_of(pp["T_gt"].inverse())
+            f.free()
+            m.free()
+        losses.append(net.train_step(batch, targets, lr=a))
+        if log and (step + 1) % log_every == 0:
+            log(f"step {step + 1}/{steps} lr {a:.2e} loss {np.mean(losses[-log_every:]):.5f} "
+                f"({time.perf_counter() - t0:.1f}s)")
+    return losses
+
+
+def plain_config(iterations=100):
+    """The plain method for these pairs: three-level pyramid (shrink 4/2/1,
+    sigma 2/1/0 mm), Mattes 50 bins at 25% sampling, RegularStepGradientDescent
+    with physical-shift scales from the centered initializer."""
+    return S.registration_config(metric=S.MATTES, bins=50, sampling=S.SAMPLING_RANDOM,
+                                 sampling_percentage=0.25, seed=1, optimizer=S.REGULAR_STEP,
+                                 minimum_step=1e-4, iterations=iterations, gradient_tolerance=1e-8,
+                                 learning_rate_mode=S.LR_ONCE,
+                                 shrink_factors=((4, 4, 4), (2, 2, 2), (1, 1, 1)),
+                                 sigmas=(2.0, 1.0, 0.0))
+
+
+def refine_config_fast(iterations=30):
+    """The refinement after the prediction: Mattes 50 bins at 25% sampling, one
+    level at full resolution, `iterations` RegularStepGradientDescent steps."""
+    return S.registration_config(metric=S.MATTES, bins=50, sampling=S.SAMPLING_RANDOM,
+                                 sampling_percentage=0.25, seed=1, optimizer=S.REGULAR_STEP,
+                                 minimum_step=1e-4, iterations=iterations, gradient_tolerance=1e-8,
+                                 learning_rate_mode=S.LR_ONCE)
+
+
+def evaluate(net: S.CNN, n_pairs: int, seed: int = 12345, refine_iterations: int = 30,
+             plain_pairs: int = 16, log=None):
+    """Held-out evaluation of learned initialisation + batched refinement
+    (with the centered-initializer fallback) against the plain method one
+    pair at a time. Returns a dict of rates and success fractions."""
+    rng = np.random.default_rng(seed)
+    pps = [pair_params(rng) for _ in range(n_pairs)]
+    pairs = [make_pair(pp) for pp in pps]
+    centres = [pp["centre"] for pp in pps]
+    cfg = refine_config_fast(refine_iterations)
+    S.backend_init(0)
+    t0 = time.perf_counter()
+    pred = predict(net, [(f, m, c) for (f, m), c in zip(pairs, centres)])
+    t_pred = time.perf_counter() - t0
+    p0 = [affine_params(T) for T in pred]
+    got, res = S.register_batch([f for f, _ in pairs], [m for _, m in pairs], [cfg] * n_pairs,
+                                [S.AFFINE] * n_pairs, p0, centres)
+    # the centered initializer's value (identity about the centre, same geometry)
+    ident = [list(np.eye(3).reshape(9)) + [0.0, 0.0, 0.0]] * n_pairs
+    zero_cfg = refine_config_fast(0)
+    _, res0 = S.register_batch([f for f, _ in pairs], [m for _, m in pairs], [zero_cfg] * n_pairs,
+                               [S.AFFINE] * n_pairs, ident, centres)
+    worse = [i for i in range(n_pairs) if res[i].value > res0[i].value]
+    if worse:
+        got_f, res_f = S.register_batch([pairs[i][0] for i in worse], [pairs[i][1] for i in worse],
+                                        [cfg] * len(worse), [S.AFFINE] * len(worse),
+                                        [ident[i] for i in worse], [centres[i] for i in worse])
+        for k, i in enumerate(worse):
+            got[i] = got_f[k]
+    t_learned = time.perf_counter() - t0
+
+    def tre_params(p, pp):
+        return tre(W.AffineMap(np.asarray(p[:9]).reshape(3, 3), np.asarray(p[9:]), pp["centre"]), pp)
+
+    tre_l = np.array([tre_params(g, pp) for g, pp in zip(got, pps)])
+    tre_pred = np.array([tre(T, pp) for T, pp in zip(pred, pps)])
+    k = min(plain_pairs, n_pairs)
+    pc = plain_config()
+    t1 = time.perf_counter()
+    plain = [S.register(pairs[i][0], pairs[i][1], pc, S.AFFINE, ident[i], center=centres[i])
+             for i in range(k)]
+    t_plain = time.perf_counter() - t1
+    tre_p = np.array([tre_params(g, pp) for (g, _), pp in zip(plain, pps[:k])])
+    # the plain method's fixed-budget refinement alone, batched (what the network adds)
+    got_c, _ = S.register_batch([f for f, _ in pairs], [m for _, m in pairs], [cfg] * n_pairs,
+                                [S.AFFINE] * n_pairs, ident, centres)
+    tre_c = np.array([tre_params(g, pp) for g, pp in zip(got_c, pps)])
+    out = {
+        "pairs": n_pairs,
+        "prediction_tre_median_mm": float(np.median(tre_pred)),
+        "learned": {"seconds": t_learned, "predict_seconds": t_pred,
+                    "registrations_per_s": n_pairs / t_learned,
+                    "success_fraction": float(np.mean(tre_l <= 0.5)),
+                    "fallbacks": len(worse)},
+        "centered_refine_only": {"success_fraction": float(np.mean(tre_c <= 0.5))},
+        "plain": {"pairs": k, "seconds": t_plain, "registrations_per_s": k / t_plain,
+                  "success_fraction": float(np.mean(tre_p <= 0.5))},
+    }
+    out["learned"]["successful_per_s"] = out["learned"]["success_fraction"] * n_pairs / t_learned
+    out["plain"]["successful_per_s"] = out["plain"]["success_fraction"] * k / t_plain
+    out["speedup_successful"] = (out["learned"]["successful_per_s"] /
+                                 max(out["plain"]["successful_per_s"], 1e-12))
+    for f, m in pairs:
+        f.free()
+        m.free()
+    if log:
+        log(str(out))
+    return out

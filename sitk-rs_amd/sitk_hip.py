@@ -223,6 +223,7 @@ def lib():
         L.sitkhip_cnn_adam.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
         L.sitkhip_cnn_train_step.argtypes = [vp, C.c_int, fp, C.c_float, fp]
         L.sitkhip_cnn_activation.argtypes = [vp, C.c_int, C.c_int, C.c_int, fp, C.c_size_t]
+        L.sitkhip_empty_cache.argtypes = []
         L.sitkhip_profile_enable.argtypes = [C.c_int]
         L.sitkhip_profile_query.argtypes = [C.c_char_p, C.POINTER(C.c_uint64), dp]
         L.sitkhip_profile_dump.argtypes = [C.c_char_p, C.c_size_t]
@@ -871,3 +872,8 @@ class BSplineMattes:
             self.free()
         except Exception:
             pass
+
+
+def empty_cache() -> None:
+    """Return the library's cached device blocks to the runtime."""
+    _check(lib().sitkhip_empty_cache())
