@@ -185,6 +185,12 @@ typedef struct sitkhip_metric_s *sitkhip_metric;
 int sitkhip_metric_create(int kind, sitkhip_image fixed, const int64_t *fixed_indices,
                           size_t n_indices, sitkhip_mask fixed_mask, sitkhip_image moving,
                           sitkhip_mask moving_mask, int precision, sitkhip_metric *out);
+/* The same with FixedPoints::Indices drawn on the device: `n_draws` random
+ * draws with replacement, SplitMix64 from `seed` (metric.rs:101-122, 419-437;
+ * the multiset sitkhip_draw_samples(voxels, 2, ...) returns), laid out in
+ * ascending voxel order by a device counting sort. No fixed mask. */
+int sitkhip_metric_create_drawn(int kind, sitkhip_image fixed, size_t n_draws, uint64_t seed,
+                                sitkhip_image moving, int precision, sitkhip_metric *out);
 void sitkhip_metric_free(sitkhip_metric m);
 /* sample_count (mean_squares.rs:313-318): samples the metric walks (after the
  * fixed mask). */

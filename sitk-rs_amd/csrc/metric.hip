@@ -33,6 +33,7 @@
 //     assigned so that the workgroups one XCD runs (b mod 8) hold adjacent
 //     slices and share that XCD's L2 for the moving-volume gathers.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cfloat>
@@ -255,8 +256,59 @@ static void index_to_physical(const sitkhip_geometry &g, double out[9]) {
         for (int c = 0; c < 3; ++c) out[r * 3 + c] = g.direction[r * 3 + c] * g.spacing[c];
 }
 
-Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
-                      const Mask *fmask, const Image &moving, const Mask *mmask, int precision) {
+// Random sampling on the device: draw i is SplitMix64(seed + (i + 1) gamma)
+// mod voxels (metric.rs:101-122, 419-437: the same draws as the host's), and
+// the multiset is laid out in ascending voxel order by a counting sort
+// (counts, exclusive scan, fill) -- the order the host path sorts into.
+__global__ void draw_counts(uint64_t seed, uint64_t voxels, size_t n, uint32_t *__restrict__ counts) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        uint64_t z = seed + static_cast<uint64_t>(i + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        atomicAdd(&counts[z % voxels], 1u);
+    }
+}
+
+__global__ void fill_sorted(const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
+                            size_t voxels, uint32_t *__restrict__ out) {
+    for (size_t v = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < voxels;
+         v += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const uint32_t c = counts[v], o = offsets[v];
+        for (uint32_t k = 0; k < c; ++k) out[o + k] = static_cast<uint32_t>(v);
+    }
+}
+
+static void device_draws(Metric &m, size_t voxels, size_t n, uint64_t seed) {
+    hipStream_t st = backend().stream;
+    DeviceBuffer<uint32_t> counts(voxels), offsets(voxels);
+    HIP_OK(hipMemsetAsync(counts.ptr, 0, voxels * sizeof(uint32_t), st));
+    {
+        LaunchScope scope("draw_counts");
+        hipLaunchKernelGGL(draw_counts, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, seed,
+                           static_cast<uint64_t>(voxels), n, counts.ptr);
+        check_launch("draw_counts");
+    }
+    size_t temp = 0;
+    HIP_OK(hipcub::DeviceScan::ExclusiveSum(nullptr, temp, counts.ptr, offsets.ptr,
+                                            static_cast<int>(voxels), st));
+    DeviceBuffer<unsigned char> scratch(temp);
+    HIP_OK(hipcub::DeviceScan::ExclusiveSum(scratch.ptr, temp, counts.ptr, offsets.ptr,
+                                            static_cast<int>(voxels), st));
+    m.idx.alloc(n);
+    {
+        LaunchScope scope("fill_sorted");
+        hipLaunchKernelGGL(fill_sorted, dim3(grid_for(voxels, 256, 8192)), dim3(256), 0, st, counts.ptr,
+                           offsets.ptr, voxels, m.idx.ptr);
+        check_launch("fill_sorted");
+    }
+}
+
+static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *indices,
+                                  size_t n_indices, const Mask *fmask, const Image &moving,
+                                  const Mask *mmask, int precision, bool draw_on_device,
+                                  size_t n_draws, uint64_t draw_seed) {
     if (kind != SITKHIP_METRIC_MEAN_SQUARES && kind != SITKHIP_METRIC_MATTES)
         fail(SITKHIP_ERR_INVALID_ARGUMENT, "unknown metric kind");
     if (precision != SITKHIP_PRECISION_F32 && precision != SITKHIP_PRECISION_F64)
@@ -298,8 +350,12 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
             sync();
         }
         std::vector<uint32_t> list;
-        bool grid = indices == nullptr;
-        if (!grid) {
+        bool grid = indices == nullptr && !draw_on_device;
+        if (draw_on_device) {
+            if (fmask) fail(SITKHIP_ERR_INVALID_ARGUMENT, "device draws take no fixed mask");
+            if (n_draws >= (1ull << 31)) fail(SITKHIP_ERR_INVALID_ARGUMENT, "too many draws");
+            device_draws(*m, nf, n_draws, draw_seed);
+        } else if (!grid) {
             list.reserve(n_indices);
             for (size_t s = 0; s < n_indices; ++s) {
                 const int64_t v = indices[s];
@@ -331,6 +387,9 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
         if (grid) {
             m->SS.idx = nullptr;
             m->SS.n = static_cast<long long>(nf);
+        } else if (draw_on_device) {
+            m->SS.idx = m->idx.ptr;
+            m->SS.n = static_cast<long long>(n_draws);
         } else {
             m->idx.alloc(list.size());
             m->idx.upload(list.data(), list.size());
@@ -387,6 +446,18 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
         throw;
     }
     return m;
+}
+
+Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
+                      const Mask *fmask, const Image &moving, const Mask *mmask, int precision) {
+    return metric_create_impl(kind, fixed, indices, n_indices, fmask, moving, mmask, precision,
+                              false, 0, 0);
+}
+
+Metric *metric_create_drawn(int kind, const Image &fixed, size_t n_draws, uint64_t seed,
+                            const Image &moving, int precision) {
+    return metric_create_impl(kind, fixed, nullptr, 0, nullptr, moving, nullptr, precision, true,
+                              n_draws, seed);
 }
 
 void metric_destroy(Metric *m) { delete m; }
@@ -664,6 +735,17 @@ int sitkhip_metric_create(int kind, sitkhip_image fixed, const int64_t *fixed_in
         *out = reinterpret_cast<sitkhip_metric>(
             metric_create(kind, *f, fixed_indices, n_indices, reinterpret_cast<Mask *>(fixed_mask),
                           *mv, reinterpret_cast<Mask *>(moving_mask), precision));
+    });
+}
+
+int sitkhip_metric_create_drawn(int kind, sitkhip_image fixed, size_t n_draws, uint64_t seed,
+                                sitkhip_image moving, int precision, sitkhip_metric *out) {
+    return run_guarded([&] {
+        auto f = reinterpret_cast<Image *>(fixed);
+        auto mv = reinterpret_cast<Image *>(moving);
+        if (!f || !mv || !out) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null image or out-pointer");
+        *out = reinterpret_cast<sitkhip_metric>(
+            metric_create_drawn(kind, *f, n_draws, seed, *mv, precision));
     });
 }
 

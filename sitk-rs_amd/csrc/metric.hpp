@@ -11,6 +11,10 @@ struct Metric;
 
 Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
                       const Mask *fmask, const Image &moving, const Mask *mmask, int precision);
+// A metric whose samples are `n_draws` random draws (SplitMix64 from `seed`,
+// with replacement) over the fixed grid, drawn and sorted on the device.
+Metric *metric_create_drawn(int kind, const Image &fixed, size_t n_draws, uint64_t seed,
+                            const Image &moving, int precision);
 void metric_destroy(Metric *m);
 size_t metric_sample_count(const Metric &m);
 void metric_value_ranges(const Metric &m, double fixed_range[2], double moving_range[2]);

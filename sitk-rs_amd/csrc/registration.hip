@@ -399,34 +399,27 @@ static LevelInputs prepare_level(const Image &fixed, const Image &moving,
     }
     const Image &fixed_level = *fl.img;
     const size_t nvox = fixed_level.len();
-    std::vector<int64_t> draws;
-    const int64_t *idx = nullptr;
-    size_t nidx = 0;
-    if (cfg.sampling == 1 || cfg.sampling == 2) {
+    LevelInputs out;
+    out.grid = fixed_level.geom;
+    if (cfg.sampling == 2) {
+        // random draws (metric.rs:419-437), made and sorted on the device
+        const size_t cnt = static_cast<size_t>(static_cast<double>(nvox) * cfg.sampling_percentage);
+        out.metric.reset(metric_create_drawn(cfg.metric, fixed_level, cnt, cfg.seed, *sm.img,
+                                             cfg.precision));
+    } else {
+        std::vector<int64_t> draws;
+        const int64_t *idx = nullptr;
+        size_t nidx = 0;
         if (cfg.sampling == 1) {
             size_t stride = static_cast<size_t>(std::ceil(1.0 / cfg.sampling_percentage));
             if (stride < 1) stride = 1;
             for (size_t v = 0; v < nvox; v += stride) draws.push_back(static_cast<int64_t>(v));
-        } else {
-            const size_t cnt = static_cast<size_t>(static_cast<double>(nvox) * cfg.sampling_percentage);
-            draws.resize(cnt);
-            uint64_t state = cfg.seed;
-            for (size_t i = 0; i < cnt; ++i) {
-                state += 0x9E3779B97F4A7C15ull;
-                uint64_t z = state;
-                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-                z ^= z >> 31;
-                draws[i] = static_cast<int64_t>(z % static_cast<uint64_t>(nvox));
-            }
+            idx = draws.data();
+            nidx = draws.size();
         }
-        idx = draws.data();
-        nidx = draws.size();
+        out.metric.reset(
+            metric_create(cfg.metric, fixed_level, idx, nidx, nullptr, *sm.img, nullptr, cfg.precision));
     }
-    LevelInputs out;
-    out.grid = fixed_level.geom;
-    out.metric.reset(
-        metric_create(cfg.metric, fixed_level, idx, nidx, nullptr, *sm.img, nullptr, cfg.precision));
     if (cfg.metric == SITKHIP_METRIC_MATTES) {
         double fr[2], mr[2];
         metric_value_ranges(*out.metric, fr, mr);

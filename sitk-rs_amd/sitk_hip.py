@@ -165,6 +165,8 @@ def lib():
         L.sitkhip_recursive_gaussian.argtypes = [vp, dp, C.POINTER(vp)]
         L.sitkhip_metric_create.argtypes = [C.c_int, vp, C.POINTER(C.c_int64), C.c_size_t, vp, vp,
                                             vp, C.c_int, C.POINTER(vp)]
+        L.sitkhip_metric_create_drawn.argtypes = [C.c_int, vp, C.c_size_t, C.c_uint64, vp, C.c_int,
+                                                  C.POINTER(vp)]
         L.sitkhip_metric_free.argtypes = [vp]
         L.sitkhip_metric_free.restype = None
         L.sitkhip_metric_sample_count.argtypes = [vp, szp]
@@ -384,8 +386,14 @@ def recursive_gaussian(src: DeviceImage, sigma) -> DeviceImage:
 class _Resident:
     def __init__(self, kind, fixed: DeviceImage, moving: DeviceImage, indices=None,
                  fixed_mask: DeviceMask | None = None, moving_mask: DeviceMask | None = None,
-                 precision: int = PRECISION_F32):
+                 precision: int = PRECISION_F32, device_draws=None):
         h = C.c_void_p()
+        if device_draws is not None:
+            n, seed = device_draws
+            _check(lib().sitkhip_metric_create_drawn(kind, fixed._h, int(n), int(seed), moving._h,
+                                                     precision, C.byref(h)))
+            self._h = h
+            return
         if indices is None:
             ip, n = None, 0
         else:

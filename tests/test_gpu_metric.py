@@ -304,3 +304,27 @@ def test_mean_squares_sampled_and_masked(hip):
     v, deriv, valid = hip.contract_ms(mom, j0, c, 6)
     wv, wd, wn = pb.ms_evaluate(t)
     assert valid == wn and abs(v - wv) <= 1e-12 * abs(wv) and rel_vec(deriv, wd) < 1e-9
+
+
+@pytest.mark.parametrize("kind", ["mattes", "ms"])
+def test_device_draws_equal_host_draws(hip, kind):
+    """Samples drawn on the device (SplitMix64, counting sort) are the host's
+    draws (metric.rs:419-437) in the same ascending order: the metrics agree
+    bit for bit."""
+    f, m = ledger_pair()
+    df = hip.DeviceImage.upload(f, spacing=SP, origin=ORG)
+    dm = hip.DeviceImage.upload(m, spacing=SP, origin=ORG)
+    idx = hip.draw_samples(f.size, hip.SAMPLING_RANDOM, 0.3, 99)
+    t = O.make_transform(O.EULER3D, [0.05, -0.03, 0.08, 1.2, -0.7, 0.4], center=[9.0, 16.0, 18.0])
+    st = stages_of(t)
+    if kind == "mattes":
+        a = hip.ResidentMattes(df, dm, bins=40, indices=idx)
+        b = hip.ResidentMattes(df, dm, bins=40, device_draws=(idx.size, 99))
+        assert a.sample_count() == b.sample_count() == idx.size
+        va, ma, na = a.value_and_moments(st)
+        vb, mb, nb = b.value_and_moments(st)
+        assert va == vb and na == nb and np.array_equal(ma, mb)
+    else:
+        a = hip.ResidentMetric(df, dm, indices=idx)
+        b = hip.ResidentMetric(df, dm, device_draws=(idx.size, 99))
+        assert np.array_equal(a.evaluate(st), b.evaluate(st))
