@@ -379,7 +379,7 @@ __global__ void copy_f32(const float *__restrict__ s, float *__restrict__ d, siz
 }
 
 // recursive_gaussian (pyramid.rs:217-293 / recursive_gaussian.rs:171-204).
-Image *recursive_gaussian(const Image &in, const double sigma[3]) {
+Image *recursive_gaussian(const Image &in, const double sigma[3], bool synchronous) {
     for (int d = 0; d < 3; ++d) {
         if (!(sigma[d] >= 0.0))
             fail(SITKHIP_ERR_INVALID_ARGUMENT, "recursive_gaussian: sigma must be >= 0");
@@ -403,7 +403,7 @@ Image *recursive_gaussian(const Image &in, const double sigma[3]) {
             hipLaunchKernelGGL(copy_f32, dim3(grid_for(n, 256, 4096)), dim3(256), 0,
                                backend().stream, in.buf.ptr, out->buf.ptr, n);
             check_launch("copy_f32");
-            sync();
+            if (synchronous) sync();
             return out;
         }
         // FP64 intermediates, kept between calls (every call is synchronous and
@@ -430,7 +430,7 @@ Image *recursive_gaussian(const Image &in, const double sigma[3]) {
             IirCoef kl = zero_order_coefficients(sigma[axes[na - 1]] / g.spacing[axes[na - 1]]);
             launch_axis<double, float>(cur, other, out->buf.ptr, g, axes[na - 1], kl);
         }
-        sync();
+        if (synchronous) sync();
     } catch (...) {
         delete out;
         throw;
@@ -446,6 +446,6 @@ extern "C" int sitkhip_recursive_gaussian(sitkhip_image in, const double sigma[3
         auto img = reinterpret_cast<sitkhip::Image *>(in);
         if (!img || !sigma || !out)
             sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "null image, sigma or out-pointer");
-        *out = reinterpret_cast<sitkhip_image>(sitkhip::recursive_gaussian(*img, sigma));
+        *out = reinterpret_cast<sitkhip_image>(sitkhip::recursive_gaussian(*img, sigma, true));
     });
 }

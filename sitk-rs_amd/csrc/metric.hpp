@@ -31,8 +31,10 @@ void invert3_host(const double m[9], double out[9]);
 // Pyramid placement (pyramid.hip).
 void shrunk_geometry(const sitkhip_geometry &g, const size_t f[3], sitkhip_geometry &out,
                      long long offset[3]);
+// `synchronous` = false leaves the work enqueued on the library stream (the
+// C ABI entry points wait; internal callers that go on enqueueing need not).
 Image *resample_linear(const Image &in, const sitkhip_geometry &og, const double *stages,
-                       int nstage, double dflt);
-Image *recursive_gaussian(const Image &in, const double sigma[3]);
+                       int nstage, double dflt, bool synchronous = true);
+Image *recursive_gaussian(const Image &in, const double sigma[3], bool synchronous = true);
 
 }  // namespace sitkhip

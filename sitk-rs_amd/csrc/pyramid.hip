@@ -188,7 +188,7 @@ static Stages stages_or_identity(const double *stages, int nstage) {
 }
 
 Image *resample_linear(const Image &in, const sitkhip_geometry &og, const double *stages,
-                       int nstage, double dflt) {
+                       int nstage, double dflt, bool synchronous) {
     validate_geometry(og, "resample output");
     const size_t n = geometry_len(og);
     if (n >= (1ull << 32)) fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "output above 2^32 voxels");
@@ -202,7 +202,7 @@ Image *resample_linear(const Image &in, const sitkhip_geometry &og, const double
                            static_cast<uint32_t>(n));
         check_launch("resample_linear3");
     }
-    sync();
+    if (synchronous) sync();
     return out;
 }
 

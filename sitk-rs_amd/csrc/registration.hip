@@ -384,9 +384,9 @@ static LevelInputs prepare_level(const Image &fixed, const Image &moving,
     const bool smooth = s > 0.0;
     const bool native = f[0] == 1 && f[1] == 1 && f[2] == 1;
     ImageHolder sf, sm, fl;
-    sf.img = smooth ? recursive_gaussian(fixed, sig) : const_cast<Image *>(&fixed);
+    sf.img = smooth ? recursive_gaussian(fixed, sig, false) : const_cast<Image *>(&fixed);
     sf.owned = smooth;
-    sm.img = smooth ? recursive_gaussian(moving, sig) : const_cast<Image *>(&moving);
+    sm.img = smooth ? recursive_gaussian(moving, sig, false) : const_cast<Image *>(&moving);
     sm.owned = smooth;
     if (native) {
         fl.img = sf.img;
@@ -394,7 +394,7 @@ static LevelInputs prepare_level(const Image &fixed, const Image &moving,
         sitkhip_geometry grid;
         long long off[3];
         shrunk_geometry(fixed.geom, f, grid, off);
-        fl.img = resample_linear(*sf.img, grid, nullptr, 0, 0.0);
+        fl.img = resample_linear(*sf.img, grid, nullptr, 0, 0.0, false);
         fl.owned = true;
     }
     const Image &fixed_level = *fl.img;
