@@ -39,6 +39,7 @@ namespace sitkhip {
 constexpr int kBatchMaxBins = 64;  // LDS histogram (B^2 + B) * 8 B <= 33 KB
 
 struct PairDesc {
+    const ChainConst *C;
     FixedGrid F;
     MovingVol M;
     SampleSet SS;
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void batch_pass1(const PairDesc *__restrict
             const int nb = B * B + B;
             for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
             __syncthreads();
-            hist_range<R>(d.F, d.M, S, c.X, d.SS, d.fbin, d.P, s_hist, s_hist + B * B, begin, end);
+            hist_range<R>(d.F, d.M, S, c.X, d.C, d.SS, d.fbin, d.P, s_hist, s_hist + B * B, begin, end);
             __syncthreads();
             for (int i = threadIdx.x; i < nb; i += kBlock) {
                 const unsigned long long v = s_hist[i];
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void batch_pass1(const PairDesc *__restrict
             double acc[14];
 #pragma unroll
             for (int k = 0; k < 14; ++k) acc[k] = 0.0;
-            ms_range<R, true>(d.F, d.M, S, c.X, d.SS, d.fval, begin, end, acc);
+            ms_range<R, true>(d.F, d.M, S, c.X, d.C, d.SS, d.fval, begin, end, acc);
             block_reduce_store<14>(acc, d.partials, static_cast<unsigned>(tl.index));
         }
         __syncthreads();  // the next tile reuses the LDS
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void batch_pass2(const PairDesc *__restrict
         double acc[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) acc[k] = 0.0;
-        deriv_range<R>(d.F, d.M, S, c.X, d.SS, d.fbin, d.P, s_pr, begin, end, acc);
+        deriv_range<R>(d.F, d.M, S, c.X, d.C, d.SS, d.fbin, d.P, s_pr, begin, end, acc);
         block_reduce_store<12>(acc, d.partials, static_cast<unsigned>(tl.index));
         __syncthreads();
     }
@@ -318,6 +319,7 @@ BatchEngine *batch_create(Metric *const *metrics, size_t n) {
                 fail(SITKHIP_ERR_INVALID_ARGUMENT, "a batched pair holds fewer than 2^31 samples");
             PairDesc &d = b->hdesc[i];
             std::memset(&d, 0, sizeof(d));
+            d.C = m->chain.ptr;
             d.F = m->F;
             d.M = m->Mv;
             d.SS = m->SS;
@@ -492,11 +494,14 @@ void batch_evaluate(BatchEngine &b, const double *stages, const uint8_t *active,
         const HostResult &r = b.res[i];
         BatchOutput &o = out[i];
         std::memcpy(o.moments, r.moments, sizeof(o.moments));
+        const bool f32 = b.precision == SITKHIP_PRECISION_F32;
         if (b.metrics[i]->kind == SITKHIP_METRIC_MATTES) {
             o.value = r.tail.value;
             o.valid = static_cast<size_t>(r.tail.valid);
             o.moments[12] = o.moments[13] = 0.0;
+            if (f32) moments_to_physical(b.metrics[i]->F, o.moments, o.moments + 3);
         } else {
+            if (f32) moments_to_physical(b.metrics[i]->F, o.moments + 1, o.moments + 4);
             const double count = r.moments[13];
             o.valid = static_cast<size_t>(count);
             o.value = count == 0.0 ? DBL_MAX : r.moments[0] / count;

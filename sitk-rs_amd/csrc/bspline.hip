@@ -120,7 +120,7 @@ __device__ __forceinline__ void map_point(const BSplineMap &B, const double x[3]
 // integer histogram).
 template <typename R>
 __global__ __launch_bounds__(kBlock) void bs_hist(FixedGrid F, MovingVol M, BSplineMap B,
-                                                  SampleSet SS, const uint8_t *__restrict__ fbin,
+                                                  const ChainConst *__restrict__ C, SampleSet SS, const uint8_t *__restrict__ fbin,
                                                   MattesParams P, unsigned long long *__restrict__ g_joint,
                                                   unsigned long long *__restrict__ g_marg) {
     extern __shared__ unsigned long long s_hist[];
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kBlock) void bs_hist(FixedGrid F, MovingVol M, BSpl
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
     const IndexMap none{};
-    hist_range<R>(F, M, B, none, SS, fbin, P, s_hist, s_hist + P.bins * P.bins, begin, end);
+    hist_range<R>(F, M, B, none, C, SS, fbin, P, s_hist, s_hist + P.bins * P.bins, begin, end);
     __syncthreads();
     for (int i = threadIdx.x; i < nb; i += kBlock) {
         const unsigned long long v = s_hist[i];
@@ -340,7 +340,7 @@ static void bs_value_pass(BSplineMetric &b) {
     {
         LaunchScope scope("bs_hist");
         hipLaunchKernelGGL(bs_hist<R>, dim3(kGrid), dim3(kBlock), shm, backend().stream, m.F, m.Mv,
-                           b.B, b.SS, b.fbin.ptr, m.P, m.joint.ptr, m.marg.ptr);
+                           b.B, m.chain.ptr, b.SS, b.fbin.ptr, m.P, m.joint.ptr, m.marg.ptr);
         check_launch("bs_hist");
     }
     launch_tail(m, &b.res_dev->tail);

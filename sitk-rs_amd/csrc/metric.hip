@@ -39,6 +39,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -50,8 +51,9 @@
 namespace sitkhip {
 
 // -------------------------------------------------------------- mean squares
-template <typename R, bool MOMENTS>
-__global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Stages S, IndexMap X, SampleSet SS,
+template <typename R, bool MOMENTS, typename Map>
+__global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Map S, IndexMap X,
+                                                  const ChainConst *__restrict__ C, SampleSet SS,
                                                   const float *__restrict__ fval,
                                                   double *__restrict__ partials) {
     constexpr int NS = MOMENTS ? 14 : 2;
@@ -61,14 +63,15 @@ __global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Stag
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    ms_range<R, MOMENTS>(F, M, S, X, SS, fval, begin, end, acc);
+    ms_range<R, MOMENTS>(F, M, S, X, C, SS, fval, begin, end, acc);
     block_reduce_store<NS>(acc, partials, logical);
 }
 
 // Pass 1: joint histogram (fixed point) + fixed-marginal counts.
-template <typename R, bool LDS>
-__global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, Stages S,
-                                                      IndexMap X, SampleSet SS, const uint8_t *__restrict__ fbin,
+template <typename R, bool LDS, typename Map>
+__global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, Map S,
+                                                      IndexMap X, const ChainConst *__restrict__ C,
+                                                      SampleSet SS, const uint8_t *__restrict__ fbin,
                                                       MattesParams P,
                                                       unsigned long long *__restrict__ g_joint,
                                                       unsigned long long *__restrict__ g_marg) {
@@ -84,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, 
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    hist_range<R>(F, M, S, X, SS, fbin, P, joint, marg, begin, end);
+    hist_range<R>(F, M, S, X, C, SS, fbin, P, joint, marg, begin, end);
     if (LDS) {
         __syncthreads();
         for (int i = threadIdx.x; i < nb; i += kBlock) {
@@ -107,9 +110,10 @@ __global__ __launch_bounds__(kTail) void mattes_tail_kernel(unsigned long long *
 }
 
 // Pass 2: the 12 pRatio-weighted moments, folded by the last workgroup.
-template <typename R, bool LDS>
-__global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M, Stages S,
-                                                       IndexMap X, SampleSet SS, const uint8_t *__restrict__ fbin,
+template <typename R, bool LDS, typename Map>
+__global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M, Map S,
+                                                       IndexMap X, const ChainConst *__restrict__ C,
+                                                       SampleSet SS, const uint8_t *__restrict__ fbin,
                                                        MattesParams P,
                                                        const R *__restrict__ g_pratio,
                                                        double *__restrict__ partials) {
@@ -127,7 +131,7 @@ __global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M,
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    deriv_range<R>(F, M, S, X, SS, fbin, P, pr, begin, end, acc);
+    deriv_range<R>(F, M, S, X, C, SS, fbin, P, pr, begin, end, acc);
     block_reduce_store<12>(acc, partials, logical);
 }
 
@@ -431,6 +435,20 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
         double mi2p[9];
         index_to_physical(mg, mi2p);
         invert3_host(mi2p, m->Mv.p2i);
+        for (int i = 0; i < 9; ++i) m->Mv.p2i_f[i] = static_cast<float>(m->Mv.p2i[i]);
+        {
+            ChainConst cc;
+            for (int d = 0; d < 3; ++d) {
+                cc.f_origin[d] = m->F.origin[d];
+                cc.m_origin[d] = m->Mv.origin[d];
+            }
+            for (int i = 0; i < 9; ++i) {
+                cc.f_i2p[i] = m->F.i2p[i];
+                cc.m_p2i[i] = m->Mv.p2i[i];
+            }
+            m->chain.alloc(1);
+            m->chain.upload(&cc, 1);
+        }
 
         m->partials.alloc(kGrid * 14);
         void *dev = nullptr;
@@ -552,9 +570,23 @@ static void require_bins(const Metric &m) {
         fail(SITKHIP_ERR_HISTOGRAM_SHAPE, "the Mattes histogram geometry has not been set");
 }
 
+// Calls f with the point map as the kernels take it: one stage in registers
+// (the common case, 12 doubles of kernel argument) or the general replay.
+template <typename Fn>
+static void with_map(const Stages &S, Fn &&f) {
+    if (S.n == 1) {
+        OneStage one;
+        std::memcpy(one.m, S.s, sizeof(one.m));
+        f(one);
+    } else {
+        f(S);
+    }
+}
+
 void launch_tail(Metric &m, TailOut *out) {
     const int B = m.P.bins;
-    const bool lds = B <= kLdsMaxBins;
+    // staged histogram plus the tail's own ~14 KB of LDS within 64 KB
+    const bool lds = B <= 76;
     const size_t shm = lds ? static_cast<size_t>(B * B + B) * sizeof(unsigned long long) : 0;
     LaunchScope scope("mattes_tail");
     if (lds)
@@ -573,14 +605,18 @@ static void launch_hist(Metric &m, const Stages &S, bool tail) {
     const size_t shm = lds ? static_cast<size_t>(B * B + B) * sizeof(unsigned long long) : 0;
     {
         LaunchScope scope("mattes_hist");
-        if (lds)
-            hipLaunchKernelGGL((mattes_hist<R, true>), dim3(kGrid), dim3(kBlock), shm,
-                               backend().stream, m.F, m.Mv, S, make_index_map(m.F, m.Mv, S.s, S.n), m.SS, m.fbin.ptr, m.P, m.joint.ptr,
-                               m.marg.ptr);
-        else
-            hipLaunchKernelGGL((mattes_hist<R, false>), dim3(kGrid), dim3(kBlock), 0,
-                               backend().stream, m.F, m.Mv, S, make_index_map(m.F, m.Mv, S.s, S.n), m.SS, m.fbin.ptr, m.P, m.joint.ptr,
-                               m.marg.ptr);
+        const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
+        with_map(S, [&](const auto &map) {
+            using Map = std::decay_t<decltype(map)>;
+            if (lds)
+                hipLaunchKernelGGL((mattes_hist<R, true, Map>), dim3(kGrid), dim3(kBlock), shm,
+                                   backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P,
+                                   m.joint.ptr, m.marg.ptr);
+            else
+                hipLaunchKernelGGL((mattes_hist<R, false, Map>), dim3(kGrid), dim3(kBlock), 0,
+                                   backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P,
+                                   m.joint.ptr, m.marg.ptr);
+        });
         check_launch("mattes_hist");
     }
     if (!tail) return;
@@ -602,14 +638,18 @@ static void launch_deriv(Metric &m, const Stages &S, const R *pratio) {
     const bool lds = shm + 2048 <= 64 * 1024;  // plus the reduction's static LDS
     {
         LaunchScope scope("mattes_deriv");
-        if (lds)
-            hipLaunchKernelGGL((mattes_deriv<R, true>), dim3(kGrid), dim3(kBlock), shm,
-                               backend().stream, m.F, m.Mv, S, make_index_map(m.F, m.Mv, S.s, S.n), m.SS, m.fbin.ptr, m.P, pratio,
-                               m.partials.ptr);
-        else
-            hipLaunchKernelGGL((mattes_deriv<R, false>), dim3(kGrid), dim3(kBlock), 0,
-                               backend().stream, m.F, m.Mv, S, make_index_map(m.F, m.Mv, S.s, S.n), m.SS, m.fbin.ptr, m.P, pratio,
-                               m.partials.ptr);
+        const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
+        with_map(S, [&](const auto &map) {
+            using Map = std::decay_t<decltype(map)>;
+            if (lds)
+                hipLaunchKernelGGL((mattes_deriv<R, true, Map>), dim3(kGrid), dim3(kBlock), shm,
+                                   backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio,
+                                   m.partials.ptr);
+            else
+                hipLaunchKernelGGL((mattes_deriv<R, false, Map>), dim3(kGrid), dim3(kBlock), 0,
+                                   backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio,
+                                   m.partials.ptr);
+        });
         check_launch("mattes_deriv");
     }
     launch_fold<12>(m);
@@ -655,6 +695,7 @@ void mattes_derivative_moments(Metric &m, const double *stages, int nstage, cons
     }
     sync();
     std::memcpy(moments, m.res->moments, 12 * sizeof(double));
+    if (m.precision == SITKHIP_PRECISION_F32) moments_to_physical(m.F, moments, moments + 3);
 }
 
 static void value_pass(Metric &m, const Stages &S) {
@@ -681,7 +722,10 @@ void mattes_value_and_moments(Metric &m, const double *stages, int nstage, doubl
     else launch_deriv<float>(m, S, m.pratio_f.ptr);
     sync();
     const TailOut &t = m.res->tail;
-    if (moments) std::memcpy(moments, m.res->moments, 12 * sizeof(double));
+    if (moments) {
+        std::memcpy(moments, m.res->moments, 12 * sizeof(double));
+        if (m.precision == SITKHIP_PRECISION_F32) moments_to_physical(m.F, moments, moments + 3);
+    }
     if (value) *value = t.value;
     if (valid) *valid = static_cast<size_t>(t.valid);
 }
@@ -690,8 +734,12 @@ template <typename R, bool MOM>
 static void launch_ms(Metric &m, const Stages &S) {
     {
         LaunchScope scope(MOM ? "ms_moments" : "ms_value");
-        hipLaunchKernelGGL((ms_pass<R, MOM>), dim3(kGrid), dim3(kBlock), 0, backend().stream, m.F,
-                           m.Mv, S, make_index_map(m.F, m.Mv, S.s, S.n), m.SS, m.fval.ptr, m.partials.ptr);
+        const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
+        with_map(S, [&](const auto &map) {
+            using Map = std::decay_t<decltype(map)>;
+            hipLaunchKernelGGL((ms_pass<R, MOM, Map>), dim3(kGrid), dim3(kBlock), 0, backend().stream,
+                               m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fval.ptr, m.partials.ptr);
+        });
         check_launch("ms_pass");
     }
     launch_fold<MOM ? 14 : 2>(m);
@@ -705,6 +753,7 @@ void ms_moments(Metric &m, const double *stages, int nstage, double out[14]) {
     else launch_ms<float, true>(m, S);
     sync();
     std::memcpy(out, m.res->moments, 14 * sizeof(double));
+    if (m.precision == SITKHIP_PRECISION_F32) moments_to_physical(m.F, out + 1, out + 4);
 }
 
 void ms_value(Metric &m, const double *stages, int nstage, double *sq, size_t *valid) {

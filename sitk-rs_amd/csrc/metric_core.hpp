@@ -283,17 +283,36 @@ inline IndexMap make_index_map(const FixedGrid &F, const MovingVol &M, const dou
     return X;
 }
 
-// The fixed physical point for the moment sums of the FP32 path: the same
-// affine map as the sampler's point, with FMAs (the moments only accumulate
-// it; no decision depends on it).
-__device__ __forceinline__ void fixed_point_fma(const FixedGrid &F, uint32_t flat, double x[3]) {
-    uint32_t i0, i1, i2;
-    split_index(F, flat, i0, i1, i2);
-    const double d0 = static_cast<double>(i0), d1 = static_cast<double>(i1),
-                 d2 = static_cast<double>(i2);
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-        x[r] = fma(F.i2p[r * 3 + 2], d2, fma(F.i2p[r * 3 + 1], d1, fma(F.i2p[r * 3 + 0], d0, F.origin[r])));
+// The point the moment sums weight by: FP64 path, the fixed physical point
+// (the reference's); FP32 path, the fixed index itself (exact small integers,
+// no geometry in registers) -- its moments B'[d][c] become the physical ones
+// on the host, B[d][e] = origin_e A_d + sum_c I2P[e][c] B'[d][c]
+// (moments_to_physical).
+template <typename R>
+__device__ __forceinline__ void moment_point(const FixedGrid &F, const ChainConst *__restrict__ C,
+                                             uint32_t flat, double x[3]) {
+    if constexpr (sizeof(R) == 4) {
+        uint32_t i0, i1, i2;
+        split_index(F, flat, i0, i1, i2);
+        x[0] = static_cast<double>(i0);
+        x[1] = static_cast<double>(i1);
+        x[2] = static_cast<double>(i2);
+    } else {
+        fixed_point(F, *C, flat, x);
+    }
+}
+
+// Index-space first/second moments -> physical: `a` the 3 first moments, `b`
+// the 3 x 3 second moments [d][c] (rewritten in place as [d][e]).
+inline void moments_to_physical(const FixedGrid &F, const double *a, double *b) {
+    double out[9];
+    for (int d = 0; d < 3; ++d)
+        for (int e = 0; e < 3; ++e) {
+            double acc = F.origin[e] * a[d];
+            for (int c = 0; c < 3; ++c) acc += F.i2p[e * 3 + c] * b[d * 3 + c];
+            out[d * 3 + e] = acc;
+        }
+    for (int i = 0; i < 9; ++i) b[i] = out[i];
 }
 
 // Whether a point map is affine (the composed index map applies to it).
@@ -307,7 +326,8 @@ struct MapTraits {
 // chain always (the F64 path follows the reference term for term).
 template <typename R, typename Map>
 __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol &M,
-                                             const Map &S, const IndexMap &X, const Batch &b,
+                                             const Map &S, const IndexMap &X,
+                                             const ChainConst *__restrict__ C, const Batch &b,
                                              Located<R> &L) {
     Cell cell[kU];
 #pragma unroll
@@ -324,9 +344,9 @@ __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol
         }
         if (!done) {
             double x[3], p[3];
-            fixed_point(F, b.flat[u], x);
+            fixed_point(F, *C, b.flat[u], x);
             map_point(S, x, p);
-            L.ok[u] = b.live[u] && locate(M, p, cell[u]);
+            L.ok[u] = b.live[u] && locate(M, *C, p, cell[u]);
         }
         if (!L.ok[u]) {
             cell[u].base[0] = cell[u].base[1] = cell[u].base[2] = 0;
@@ -350,7 +370,8 @@ __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol
 // [sq, count] without MOMENTS.
 template <typename R, bool MOMENTS, typename Map>
 __device__ __forceinline__ void ms_range(const FixedGrid &F, const MovingVol &M, const Map &S,
-                                         const IndexMap &X, const SampleSet &SS, const float *__restrict__ fval,
+                                         const IndexMap &X, const ChainConst *__restrict__ C,
+                                         const SampleSet &SS, const float *__restrict__ fval,
                                          long long begin, long long end,
                                          double (&acc)[MOMENTS ? 14 : 2]) {
     for (long long s0 = begin + threadIdx.x; s0 < end; s0 += kBlock * kU) {
@@ -365,7 +386,7 @@ __device__ __forceinline__ void ms_range(const FixedGrid &F, const MovingVol &M,
             fv[u] = fval[sc];
         }
         Located<R> L;
-        locate_batch<R>(F, M, S, X, b, L);
+        locate_batch<R>(F, M, S, X, C, b, L);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!L.ok[u]) continue;
@@ -375,8 +396,7 @@ __device__ __forceinline__ void ms_range(const FixedGrid &F, const MovingVol &M,
             acc[0] = fma(diff, diff, acc[0]);
             if (MOMENTS) {
                 double x[3];
-                if constexpr (sizeof(R) == 4) fixed_point_fma(F, b.flat[u], x);
-                else fixed_point(F, b.flat[u], x);
+                moment_point<R>(F, C, b.flat[u], x);
                 physical_gradient<R>(M, gi, g);
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
@@ -399,7 +419,8 @@ __device__ __forceinline__ void ms_range(const FixedGrid &F, const MovingVol &M,
 // (64-bit fixed point) and one count into marg[fixed bin].
 template <typename R, typename Map>
 __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &M, const Map &S,
-                                           const IndexMap &X, const SampleSet &SS, const uint8_t *__restrict__ fbin,
+                                           const IndexMap &X, const ChainConst *__restrict__ C,
+                                           const SampleSet &SS, const uint8_t *__restrict__ fbin,
                                            const MattesParams &P, unsigned long long *joint,
                                            unsigned long long *marg, long long begin,
                                            long long end) {
@@ -416,7 +437,7 @@ __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &
             b.live[u] = in && b.fb[u] != kNoBin;
         }
         Located<R> L;
-        locate_batch<R>(F, M, S, X, b, L);
+        locate_batch<R>(F, M, S, X, C, b, L);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!L.ok[u]) continue;
@@ -441,7 +462,8 @@ __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &
 // (src/cuda/ops/mattes.rs:75-91).
 template <typename R, typename Map>
 __device__ __forceinline__ void deriv_range(const FixedGrid &F, const MovingVol &M, const Map &S,
-                                            const IndexMap &X, const SampleSet &SS, const uint8_t *__restrict__ fbin,
+                                            const IndexMap &X, const ChainConst *__restrict__ C,
+                                           const SampleSet &SS, const uint8_t *__restrict__ fbin,
                                             const MattesParams &P, const R *pr, long long begin,
                                             long long end, double (&acc)[12]) {
     const int B = P.bins;
@@ -457,7 +479,7 @@ __device__ __forceinline__ void deriv_range(const FixedGrid &F, const MovingVol 
             b.live[u] = in && b.fb[u] != kNoBin;
         }
         Located<R> L;
-        locate_batch<R>(F, M, S, X, b, L);
+        locate_batch<R>(F, M, S, X, C, b, L);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!L.ok[u]) continue;
@@ -473,8 +495,7 @@ __device__ __forceinline__ void deriv_range(const FixedGrid &F, const MovingVol 
             for (int k = 0; k < 4; ++k) w += row[k] * bspline3_deriv<R>(arg[k]);
             const double wd = static_cast<double>(w);
             double x[3];
-            if constexpr (sizeof(R) == 4) fixed_point_fma(F, b.flat[u], x);
-            else fixed_point(F, b.flat[u], x);
+            moment_point<R>(F, C, b.flat[u], x);
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const double wg = wd * static_cast<double>(g[d]);
@@ -500,49 +521,73 @@ struct TailOut {
 // evaluation. The joint mass is summed exactly in integers and rounded once.
 constexpr int kTail = 1024;
 
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 __device__ void mattes_tail_block(unsigned long long *g_joint, unsigned long long *g_marg, int B,
                                   double mbs, double *pratio, float *pratio_f, TailOut *out,
                                   unsigned long long *s_hist) {
-    __shared__ unsigned long long s_u[kTail];
-    __shared__ double s_d[kTail];
+    constexpr int kWaves = kTail / 64;
+    __shared__ unsigned long long s_wu[kWaves][2];
+    __shared__ double s_wd[kWaves];
+    __shared__ unsigned long long s_colpart[kTail];  // column partial sums
     __shared__ double s_mm[256], s_fm[256], s_logfm[256];
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int nbin = B * B;
     const unsigned long long *hj = g_joint;
     const unsigned long long *hm = g_marg;
     if (s_hist) {
-        // stage the histogram in LDS
         for (int i = t; i < nbin; i += kTail) s_hist[i] = g_joint[i];
         for (int i = t; i < B; i += kTail) s_hist[nbin + i] = g_marg[i];
         __syncthreads();
         hj = s_hist;
         hm = s_hist + nbin;
     }
+    // Exact integer sums: the joint mass, the valid count (fixed marginal),
+    // and per moving bin m the column sum over the fixed bins, each lane of
+    // column m (R lanes per column) summing rows grp, grp + R, ...
     unsigned long long mass = 0;
     for (int i = t; i < nbin; i += kTail) mass += hj[i];
-    s_u[t] = mass;
-    __syncthreads();
-    for (int w = kTail / 2; w > 0; w >>= 1) {
-        if (t < w) s_u[t] += s_u[t + w];
-        __syncthreads();
+    unsigned long long cnt = t < B ? hm[t] : 0ull;
+    const int R = kTail / B;  // B <= 255 -> R >= 4
+    if (t < R * B) {
+        const int col = t % B, grp = t / B;
+        unsigned long long c = 0;
+        for (int f = grp; f < B; f += R) c += hj[f * B + col];
+        s_colpart[t] = c;
     }
-    const double joint_sum = static_cast<double>(s_u[0]) * kInvFix;
-    unsigned long long valid = 0;
-    for (int f = 0; f < B; ++f) valid += hm[f];  // every lane, same order
+    mass = wave_sum_u64(mass);
+    cnt = wave_sum_u64(cnt);
+    if (lane == 0) {
+        s_wu[wave][0] = mass;
+        s_wu[wave][1] = cnt;
+    }
+    __syncthreads();
+    unsigned long long tot = 0, valid = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        tot += s_wu[w][0];
+        valid += s_wu[w][1];
+    }
+    const double joint_sum = static_cast<double>(tot) * kInvFix;
     const bool degenerate = valid == 0 || joint_sum < DBL_EPSILON;
     const double inv_sum = degenerate ? 0.0 : 1.0 / joint_sum;
     const double n_factor = degenerate ? 0.0 : 1.0 / (mbs * static_cast<double>(valid));
-    // marginals: fixed = counts * inv_sum; moving = column sums of the normalised
-    // joint, accumulated over f in order by 4 lanes per column then combined.
     if (t < B) {
         const double fm = static_cast<double>(hm[t]) * inv_sum;
         s_fm[t] = fm;
         s_logfm[t] = fm > DBL_EPSILON ? log(fm) : 0.0;
-    }
-    if (t < B) {
-        double mm = 0.0;
-        for (int f = 0; f < B; ++f) mm += static_cast<double>(hj[f * B + t]) * kInvFix * inv_sum;
-        s_mm[t] = mm;
+        unsigned long long col = 0;
+        for (int g = 0; g < R; ++g) col += s_colpart[g * B + t];
+        s_mm[t] = static_cast<double>(col) * kInvFix * inv_sum;
     }
     __syncthreads();
     double part = 0.0;
@@ -561,17 +606,18 @@ __device__ void mattes_tail_block(unsigned long long *g_joint, unsigned long lon
         pratio[i] = pr_out;
         pratio_f[i] = static_cast<float>(pr_out);
     }
-    s_d[t] = part;
+    part = wave_sum_f64(part);
+    if (lane == 0) s_wd[wave] = part;
+    // every read of the histogram is done once all lanes pass this barrier:
+    // clear it for the next evaluation
     __syncthreads();
-    for (int w = kTail / 2; w > 0; w >>= 1) {
-        if (t < w) s_d[t] += s_d[t + w];
-        __syncthreads();
-    }
-    // every read of the histogram is done: clear it for the next evaluation
     for (int i = t; i < nbin; i += kTail) g_joint[i] = 0ull;
     for (int i = t; i < B; i += kTail) g_marg[i] = 0ull;
     if (t == 0) {
-        out->value = degenerate ? DBL_MAX : -s_d[0];
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) sum += s_wd[w];
+        out->value = degenerate ? DBL_MAX : -sum;
         out->valid = static_cast<double>(valid);
         out->joint_sum = joint_sum;
     }
@@ -588,6 +634,7 @@ struct Metric {
     FixedGrid F{};
     MovingVol Mv{};
     SampleSet SS{};
+    DeviceBuffer<ChainConst> chain;  // the exact chain's geometry (device copy)
     DeviceBuffer<float> moving;
     DeviceBuffer<uint8_t> mmask;
     DeviceBuffer<uint32_t> idx;

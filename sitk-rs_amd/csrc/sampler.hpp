@@ -67,6 +67,17 @@ struct MovingVol {
     long long stride1, stride2;
     double origin[3];
     double p2i[9];
+    float p2i_f[9];       // p2i rounded to float (the FP32 path's gradient)
+};
+
+// The FP64 geometry of the exact chain in device memory, for kernels whose
+// fast path does not need it: read there only when a sample falls back to the
+// chain, instead of occupying scalar registers for the whole kernel.
+struct ChainConst {
+    double f_origin[3];
+    double f_i2p[9];
+    double m_origin[3];
+    double m_p2i[9];
 };
 
 // VirtualGrid::write_point for flat voxel `flat` (origin first, then the three
@@ -84,6 +95,25 @@ __device__ __forceinline__ void fixed_point(const FixedGrid &F, uint32_t flat, d
         acc += F.i2p[r * 3 + 0] * id0;
         acc += F.i2p[r * 3 + 1] * id1;
         acc += F.i2p[r * 3 + 2] * id2;
+        x[r] = acc;
+    }
+}
+
+// The same point with the geometry read from a ChainConst.
+__device__ __forceinline__ void fixed_point(const FixedGrid &F, const ChainConst &C, uint32_t flat,
+                                            double x[3]) {
+    uint32_t q0 = fdiv(flat, F.div0);
+    uint32_t i0 = flat - q0 * F.size[0];
+    uint32_t i2 = fdiv(q0, F.div1);
+    uint32_t i1 = q0 - i2 * F.size[1];
+    const double id0 = static_cast<double>(i0), id1 = static_cast<double>(i1),
+                 id2 = static_cast<double>(i2);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double acc = C.f_origin[r];
+        acc += C.f_i2p[r * 3 + 0] * id0;
+        acc += C.f_i2p[r * 3 + 1] * id1;
+        acc += C.f_i2p[r * 3 + 2] * id2;
         x[r] = acc;
     }
 }
@@ -152,6 +182,8 @@ __device__ __forceinline__ void split_index(const FixedGrid &F, uint32_t flat, u
 
 // continuous_index + mask_allows + is_inside + floor. Returns false for a
 // sample the reference drops (outside the buffer or on a zero mask voxel).
+__device__ __forceinline__ bool locate_index(const MovingVol &M, const double c[3], Cell &cell);
+
 __device__ __forceinline__ bool locate(const MovingVol &M, const double p[3], Cell &cell) {
     double c[3];
 #pragma unroll
@@ -162,6 +194,28 @@ __device__ __forceinline__ bool locate(const MovingVol &M, const double p[3], Ce
         acc += M.p2i[r * 3 + 2] * (p[2] - M.origin[2]);
         c[r] = acc;
     }
+    return locate_index(M, c, cell);
+}
+
+// The same with the moving geometry read from a ChainConst.
+__device__ __forceinline__ bool locate(const MovingVol &M, const ChainConst &C, const double p[3],
+                                       Cell &cell) {
+    double c[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double acc = -0.0;
+        acc += C.m_p2i[r * 3 + 0] * (p[0] - C.m_origin[0]);
+        acc += C.m_p2i[r * 3 + 1] * (p[1] - C.m_origin[1]);
+        acc += C.m_p2i[r * 3 + 2] * (p[2] - C.m_origin[2]);
+        c[r] = acc;
+    }
+    return locate_index(M, c, cell);
+}
+
+// mask_allows + is_inside + floor on the continuous index c (which may be
+// modified: it is a copy).
+__device__ __forceinline__ bool locate_index(const MovingVol &M, const double c_in[3], Cell &cell) {
+    double c[3] = {c_in[0], c_in[1], c_in[2]};
     if (M.mask) {
         long long flat = 0;
 #pragma unroll
@@ -322,6 +376,19 @@ __device__ __forceinline__ void physical_gradient(const MovingVol &M, const R gi
         acc += gi[0] * static_cast<R>(M.p2i[0 * 3 + d]);
         acc += gi[1] * static_cast<R>(M.p2i[1 * 3 + d]);
         acc += gi[2] * static_cast<R>(M.p2i[2 * 3 + d]);
+        gp[d] = acc;
+    }
+}
+
+template <>
+__device__ __forceinline__ void physical_gradient<float>(const MovingVol &M, const float gi[3],
+                                                         float gp[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        float acc = -0.0f;
+        acc += gi[0] * M.p2i_f[0 * 3 + d];
+        acc += gi[1] * M.p2i_f[1 * 3 + d];
+        acc += gi[2] * M.p2i_f[2 * 3 + d];
         gp[d] = acc;
     }
 }

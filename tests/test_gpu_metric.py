@@ -328,3 +328,21 @@ def test_device_draws_equal_host_draws(hip, kind):
         a = hip.ResidentMetric(df, dm, indices=idx)
         b = hip.ResidentMetric(df, dm, device_draws=(idx.size, 99))
         assert np.array_equal(a.evaluate(st), b.evaluate(st))
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+@pytest.mark.parametrize("bins", [76, 77, 90, 91, 128, 255])
+def test_bin_counts_across_the_lds_limits(hip, precision, bins):
+    """Every histogram placement: LDS value pass up to 90 bins (global atomics
+    above), staged tail up to 76 bins (global reads above), LDS pRatio in the
+    derivative pass while it fits."""
+    f, m = ledger_pair()
+    d, pb = mattes_pair(hip, f, m, precision, bins=bins)
+    g = pb.mattes_geometry(bins)
+    t = O.make_transform(O.EULER3D, [0.05, -0.03, 0.08, 1.2, -0.7, 0.4], center=[9.0, 16.0, 18.0])
+    want_v, want_d, want_n = pb.mattes_evaluate(g, t)
+    v, mom, n = d.value_and_moments(stages_of(t))
+    assert n == want_n
+    tv, td = (1e-9, 1e-7) if precision == 1 else (1e-6, 1e-4)
+    assert abs(v - want_v) <= tv * abs(want_v), (v, want_v)
+    assert rel_vec(contract(mom, t), want_d) < td
