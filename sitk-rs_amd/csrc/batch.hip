@@ -114,9 +114,12 @@ __global__ __launch_bounds__(kBlock) void batch_pass1(const PairDesc *__restrict
             __syncthreads();
             hist_range<R>(d.F, d.M, S, c.X, d.C, d.SS, d.fbin, d.P, s_hist, s_hist + B * B, begin, end);
             __syncthreads();
+            const unsigned copy = blockIdx.x % kHistCopies;
+            unsigned long long *gj = d.joint + static_cast<size_t>(copy) * B * B;
+            unsigned long long *gm = d.marg + copy * B;
             for (int i = threadIdx.x; i < nb; i += kBlock) {
                 const unsigned long long v = s_hist[i];
-                if (v) atomicAdd(i < B * B ? d.joint + i : d.marg + (i - B * B), v);
+                if (v) atomicAdd(i < B * B ? gj + i : gm + (i - B * B), v);
             }
         } else {
             double acc[14];

@@ -125,6 +125,9 @@ __global__ __launch_bounds__(kBlock) void bs_hist(FixedGrid F, MovingVol M, BSpl
                                                   unsigned long long *__restrict__ g_marg) {
     extern __shared__ unsigned long long s_hist[];
     const int nb = P.bins * P.bins + P.bins;
+    const unsigned copy = blockIdx.x % kHistCopies;
+    g_joint += static_cast<size_t>(copy) * P.bins * P.bins;
+    g_marg += copy * P.bins;
     for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
     __syncthreads();
     long long begin, end;

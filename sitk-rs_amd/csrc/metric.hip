@@ -60,10 +60,12 @@ __global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Map 
     double acc[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+    __shared__ double s_stage[kMaxStages * 12];
+    const StageView V = stage_to_lds(S, s_stage);
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    ms_range<R, MOMENTS>(F, M, S, X, C, SS, fval, begin, end, acc);
+    ms_range<R, MOMENTS>(F, M, V, X, C, SS, fval, begin, end, acc);
     block_reduce_store<NS>(acc, partials, logical);
 }
 
@@ -78,16 +80,21 @@ __global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, 
     extern __shared__ unsigned long long s_hist[];  // bins^2 joint, then bins marginal
     const int B = P.bins;
     const int nb = B * B + B;
+    const unsigned copy = blockIdx.x % kHistCopies;
+    g_joint += static_cast<size_t>(copy) * B * B;
+    g_marg += copy * B;
     unsigned long long *joint = LDS ? s_hist : g_joint;
     unsigned long long *marg = LDS ? s_hist + B * B : g_marg;
     if (LDS) {
         for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
         __syncthreads();
     }
+    __shared__ double s_stage[kMaxStages * 12];
+    const StageView V = stage_to_lds(S, s_stage);
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    hist_range<R>(F, M, S, X, C, SS, fbin, P, joint, marg, begin, end);
+    hist_range<R>(F, M, V, X, C, SS, fbin, P, joint, marg, begin, end);
     if (LDS) {
         __syncthreads();
         for (int i = threadIdx.x; i < nb; i += kBlock) {
@@ -104,9 +111,11 @@ template <bool STAGE>
 __global__ __launch_bounds__(kTail) void mattes_tail_kernel(unsigned long long *g_joint,
                                                              unsigned long long *g_marg, int B,
                                                              double mbs, double *pratio,
-                                                             float *pratio_f, TailOut *out) {
+                                                             float *pratio_f, TailOut *out,
+                                                             int clear) {
     extern __shared__ unsigned long long s_stage[];
-    mattes_tail_block(g_joint, g_marg, B, mbs, pratio, pratio_f, out, STAGE ? s_stage : nullptr);
+    mattes_tail_block(g_joint, g_marg, B, mbs, pratio, pratio_f, out, STAGE ? s_stage : nullptr,
+                      clear != 0);
 }
 
 // Pass 2: the 12 pRatio-weighted moments, folded by the last workgroup.
@@ -116,8 +125,18 @@ __global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M,
                                                        SampleSet SS, const uint8_t *__restrict__ fbin,
                                                        MattesParams P,
                                                        const R *__restrict__ g_pratio,
-                                                       double *__restrict__ partials) {
+                                                       double *__restrict__ partials,
+                                                       unsigned long long *__restrict__ clr,
+                                                       unsigned clr_n,
+                                                       unsigned long long *__restrict__ clr2,
+                                                       unsigned clr2_n) {
     extern __shared__ unsigned char s_raw[];
+    // the value pass's histogram copies, read by the tail, are cleared here
+    // for the next evaluation (every workgroup a slice)
+    for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < clr_n; i += gridDim.x * kBlock)
+        clr[i] = 0ull;
+    for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < clr2_n; i += gridDim.x * kBlock)
+        clr2[i] = 0ull;
     R *s_pr = reinterpret_cast<R *>(s_raw);
     const int B = P.bins;
     const R *pr = LDS ? s_pr : g_pratio;
@@ -128,10 +147,12 @@ __global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M,
     double acc[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+    __shared__ double s_stage[kMaxStages * 12];
+    const StageView V = stage_to_lds(S, s_stage);
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    deriv_range<R>(F, M, S, X, C, SS, fbin, P, pr, begin, end, acc);
+    deriv_range<R>(F, M, V, X, C, SS, fbin, P, pr, begin, end, acc);
     block_reduce_store<12>(acc, partials, logical);
 }
 
@@ -555,8 +576,8 @@ void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b) {
                            b.fixed_normalized_min, m.fbin.ptr);
         check_launch("fixed_bins");
     }
-    m.joint.alloc(static_cast<size_t>(B) * B);
-    m.marg.alloc(static_cast<size_t>(B));
+    m.joint.alloc(static_cast<size_t>(kHistCopies) * B * B);
+    m.marg.alloc(static_cast<size_t>(kHistCopies) * B);
     m.pratio.alloc(static_cast<size_t>(B) * B);
     m.pratio_f.alloc(static_cast<size_t>(B) * B);
     HIP_OK(hipMemsetAsync(m.joint.ptr, 0, m.joint.bytes(), backend().stream));
@@ -583,7 +604,7 @@ static void with_map(const Stages &S, Fn &&f) {
     }
 }
 
-void launch_tail(Metric &m, TailOut *out) {
+void launch_tail(Metric &m, TailOut *out, bool clear) {
     const int B = m.P.bins;
     // staged histogram plus the tail's own ~14 KB of LDS within 64 KB
     const bool lds = B <= 76;
@@ -591,15 +612,19 @@ void launch_tail(Metric &m, TailOut *out) {
     LaunchScope scope("mattes_tail");
     if (lds)
         hipLaunchKernelGGL((mattes_tail_kernel<true>), dim3(1), dim3(kTail), shm, backend().stream,
-                           m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out);
+                           m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out,
+                           clear ? 1 : 0);
     else
         hipLaunchKernelGGL((mattes_tail_kernel<false>), dim3(1), dim3(kTail), 0, backend().stream,
-                           m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out);
+                           m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out,
+                           clear ? 1 : 0);
     check_launch("mattes_tail");
 }
 
+// tail: run the tail after the value pass; clear: the tail clears the
+// histogram (otherwise the derivative pass that follows does).
 template <typename R>
-static void launch_hist(Metric &m, const Stages &S, bool tail) {
+static void launch_hist(Metric &m, const Stages &S, bool tail, bool clear = true) {
     const int B = m.P.bins;
     const bool lds = B <= kLdsMaxBins;
     const size_t shm = lds ? static_cast<size_t>(B * B + B) * sizeof(unsigned long long) : 0;
@@ -620,7 +645,7 @@ static void launch_hist(Metric &m, const Stages &S, bool tail) {
         check_launch("mattes_hist");
     }
     if (!tail) return;
-    launch_tail(m, &m.res_dev->tail);
+    launch_tail(m, &m.res_dev->tail, clear);
 }
 
 template <int NS>
@@ -631,9 +656,15 @@ static void launch_fold(Metric &m) {
     check_launch("fold_partials");
 }
 
+// clear_hist: the derivative pass also clears the histogram copies (after a
+// tail that left them).
 template <typename R>
-static void launch_deriv(Metric &m, const Stages &S, const R *pratio) {
+static void launch_deriv(Metric &m, const Stages &S, const R *pratio, bool clear_hist = false) {
     const int B = m.P.bins;
+    unsigned long long *clr = clear_hist ? m.joint.ptr : nullptr;
+    const unsigned clr_n = clear_hist ? static_cast<unsigned>(m.joint.len) : 0u;
+    unsigned long long *clr2 = clear_hist ? m.marg.ptr : nullptr;
+    const unsigned clr2_n = clear_hist ? static_cast<unsigned>(m.marg.len) : 0u;
     const size_t shm = static_cast<size_t>(B) * B * sizeof(R);
     const bool lds = shm + 2048 <= 64 * 1024;  // plus the reduction's static LDS
     {
@@ -644,11 +675,11 @@ static void launch_deriv(Metric &m, const Stages &S, const R *pratio) {
             if (lds)
                 hipLaunchKernelGGL((mattes_deriv<R, true, Map>), dim3(kGrid), dim3(kBlock), shm,
                                    backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio,
-                                   m.partials.ptr);
+                                   m.partials.ptr, clr, clr_n, clr2, clr2_n);
             else
                 hipLaunchKernelGGL((mattes_deriv<R, false, Map>), dim3(kGrid), dim3(kBlock), 0,
                                    backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio,
-                                   m.partials.ptr);
+                                   m.partials.ptr, clr, clr_n, clr2, clr2_n);
         });
         check_launch("mattes_deriv");
     }
@@ -662,12 +693,18 @@ void mattes_joint_histogram(Metric &m, const double *stages, int nstage, double 
     const int B = m.P.bins;
     if (m.precision == SITKHIP_PRECISION_F64) launch_hist<double>(m, S, false);
     else launch_hist<float>(m, S, false);
-    std::vector<unsigned long long> hj(static_cast<size_t>(B) * B), hm(B);
-    m.joint.download(hj.data(), hj.size());
-    m.marg.download(hm.data(), hm.size());
+    std::vector<unsigned long long> cj(static_cast<size_t>(kHistCopies) * B * B),
+        cm(static_cast<size_t>(kHistCopies) * B);
+    m.joint.download(cj.data(), cj.size());
+    m.marg.download(cm.data(), cm.size());
     HIP_OK(hipMemsetAsync(m.joint.ptr, 0, m.joint.bytes(), backend().stream));
     HIP_OK(hipMemsetAsync(m.marg.ptr, 0, m.marg.bytes(), backend().stream));
     sync();
+    std::vector<unsigned long long> hj(static_cast<size_t>(B) * B, 0), hm(B, 0);
+    for (int x = 0; x < kHistCopies; ++x) {
+        for (int i = 0; i < B * B; ++i) hj[i] += cj[static_cast<size_t>(x) * B * B + i];
+        for (int i = 0; i < B; ++i) hm[i] += cm[static_cast<size_t>(x) * B + i];
+    }
     unsigned long long v = 0;
     for (int i = 0; i < B * B; ++i)
         if (joint) joint[i] = static_cast<double>(hj[i]) * kInvFix;
@@ -717,9 +754,13 @@ void mattes_value_and_moments(Metric &m, const double *stages, int nstage, doubl
                               double moments[12], size_t *valid) {
     require_bins(m);
     Stages S = make_stages(stages, nstage);
-    value_pass(m, S);
-    if (m.precision == SITKHIP_PRECISION_F64) launch_deriv<double>(m, S, m.pratio.ptr);
-    else launch_deriv<float>(m, S, m.pratio_f.ptr);
+    if (m.precision == SITKHIP_PRECISION_F64) {
+        launch_hist<double>(m, S, true, false);
+        launch_deriv<double>(m, S, m.pratio.ptr, true);
+    } else {
+        launch_hist<float>(m, S, true, false);
+        launch_deriv<float>(m, S, m.pratio_f.ptr, true);
+    }
     sync();
     const TailOut &t = m.res->tail;
     if (moments) {

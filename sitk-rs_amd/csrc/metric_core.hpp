@@ -29,6 +29,11 @@ constexpr int kLdsMaxBins = 90;       // bins^2 * 8 B of LDS per workgroup (<= 6
 constexpr double kFix = 4294967296.0; // 2^32 fixed-point units per histogram unit
 constexpr double kInvFix = 1.0 / 4294967296.0;
 constexpr uint8_t kNoBin = 255;
+// The global joint histogram is kept as one copy per XCD (a workgroup folds
+// its LDS histogram into copy blockIdx mod 8, the XCD it runs on), so the
+// 64-bit atomics of different XCDs do not contend on one set of addresses; the
+// tail adds the copies (exact integers).
+constexpr int kHistCopies = 8;
 
 struct SampleSet {
     const uint32_t *idx;  // nullptr: the whole fixed grid in order
@@ -215,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void fold_partials(const double *__restrict
 // Samples per lane per iteration: all U samples' loads are issued before any of
 // their values is used, so each lane keeps U independent gathers in flight.
 #ifndef SITKHIP_BATCH
-#define SITKHIP_BATCH 4
+#define SITKHIP_BATCH 2
 #endif
 constexpr int kU = SITKHIP_BATCH;
 
@@ -534,7 +539,7 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 
 __device__ void mattes_tail_block(unsigned long long *g_joint, unsigned long long *g_marg, int B,
                                   double mbs, double *pratio, float *pratio_f, TailOut *out,
-                                  unsigned long long *s_hist) {
+                                  unsigned long long *s_hist, bool clear = true) {
     constexpr int kWaves = kTail / 64;
     __shared__ unsigned long long s_wu[kWaves][2];
     __shared__ double s_wd[kWaves];
@@ -542,15 +547,32 @@ __device__ void mattes_tail_block(unsigned long long *g_joint, unsigned long lon
     __shared__ double s_mm[256], s_fm[256], s_logfm[256];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int nbin = B * B;
-    const unsigned long long *hj = g_joint;
-    const unsigned long long *hm = g_marg;
-    if (s_hist) {
-        for (int i = t; i < nbin; i += kTail) s_hist[i] = g_joint[i];
-        for (int i = t; i < B; i += kTail) s_hist[nbin + i] = g_marg[i];
-        __syncthreads();
-        hj = s_hist;
-        hm = s_hist + nbin;
+    // Fold the per-XCD copies into copy 0 (exact), staged in LDS when given:
+    // a lane's bins of all copies are loaded before any is summed. Each bin is
+    // read and written by one lane only, so the loop needs no barrier.
+    constexpr int kPer = 4;
+    for (int i0 = t; i0 < nbin + B; i0 += kTail * kPer) {
+        unsigned long long v[kPer] = {0, 0, 0, 0};
+#pragma unroll
+        for (int x = 0; x < kHistCopies; ++x)
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int i = i0 + j * kTail;
+                if (i < nbin) v[j] += g_joint[static_cast<size_t>(x) * nbin + i];
+                else if (i < nbin + B) v[j] += g_marg[x * B + (i - nbin)];
+            }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int i = i0 + j * kTail;
+            if (i >= nbin + B) continue;
+            if (s_hist) s_hist[i] = v[j];
+            else if (i < nbin) g_joint[i] = v[j];
+            else g_marg[i - nbin] = v[j];
+        }
     }
+    __syncthreads();
+    const unsigned long long *hj = s_hist ? s_hist : g_joint;
+    const unsigned long long *hm = s_hist ? s_hist + nbin : g_marg;
     // Exact integer sums: the joint mass, the valid count (fixed marginal),
     // and per moving bin m the column sum over the fixed bins, each lane of
     // column m (R lanes per column) summing rows grp, grp + R, ...
@@ -611,8 +633,10 @@ __device__ void mattes_tail_block(unsigned long long *g_joint, unsigned long lon
     // every read of the histogram is done once all lanes pass this barrier:
     // clear it for the next evaluation
     __syncthreads();
-    for (int i = t; i < nbin; i += kTail) g_joint[i] = 0ull;
-    for (int i = t; i < B; i += kTail) g_marg[i] = 0ull;
+    if (clear) {
+        for (int i = t; i < kHistCopies * nbin; i += kTail) g_joint[i] = 0ull;
+        for (int i = t; i < kHistCopies * B; i += kTail) g_marg[i] = 0ull;
+    }
     if (t == 0) {
         double sum = 0.0;
 #pragma unroll
@@ -659,6 +683,6 @@ struct Metric {
 };
 
 // The Mattes tail of metric m into `out` (one workgroup; metric.hip).
-void launch_tail(Metric &m, TailOut *out);
+void launch_tail(Metric &m, TailOut *out, bool clear = true);
 
 }  // namespace sitkhip

@@ -151,6 +151,42 @@ __device__ __forceinline__ void map_point(const OneStage &S, const double x[3], 
     p[2] = a2 + m[11];
 }
 
+// Stages staged in LDS by the calling workgroup (read only where a sample
+// takes the exact chain, so the stage does not pin scalar registers for the
+// whole kernel): the same replay as map_point(Stages).
+struct StageView {
+    const double *m;  // n x 12 doubles in LDS
+    int n;
+};
+
+__device__ __forceinline__ void map_point(const StageView &S, const double x[3], double p[3]) {
+    double q0 = x[0], q1 = x[1], q2 = x[2];
+    for (int s = 0; s < S.n; ++s) {
+        const double *m = S.m + 12 * s;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        a0 += m[0] * q0; a0 += m[1] * q1; a0 += m[2] * q2;
+        a1 += m[3] * q0; a1 += m[4] * q1; a1 += m[5] * q2;
+        a2 += m[6] * q0; a2 += m[7] * q1; a2 += m[8] * q2;
+        q0 = a0 + m[9];
+        q1 = a1 + m[10];
+        q2 = a2 + m[11];
+    }
+    p[0] = q0; p[1] = q1; p[2] = q2;
+}
+
+// Copies the stages into the workgroup's LDS (s_stage: kMaxStages * 12
+// doubles); all lanes must call it, and it ends with a barrier.
+__device__ __forceinline__ StageView stage_to_lds(const Stages &S, double *s_stage) {
+    for (int i = threadIdx.x; i < S.n * 12; i += blockDim.x) s_stage[i] = S.s[i];
+    __syncthreads();
+    return StageView{s_stage, S.n};
+}
+__device__ __forceinline__ StageView stage_to_lds(const OneStage &S, double *s_stage) {
+    for (int i = threadIdx.x; i < 12; i += blockDim.x) s_stage[i] = S.m[i];
+    __syncthreads();
+    return StageView{s_stage, 1};
+}
+
 // The located sample: which cell of the moving volume and where inside it.
 struct Cell {
     int base[3];
