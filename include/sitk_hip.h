@@ -393,6 +393,39 @@ int sitkhip_bspline_mattes_value(sitkhip_bspline b, const double *coeffs, double
 int sitkhip_bspline_mattes_value_and_derivative(sitkhip_bspline b, const double *coeffs,
                                                 double *value, double *deriv, size_t *valid);
 
+/* A deformable registration: Mattes MI through a cubic B-spline transform
+ * whose domain is the fixed image's extent (bspline.rs:324-339) with `mesh`
+ * cells per axis, optimised by limited-memory BFGS on the host
+ * (LBFGSBOptimizer with every variable unbounded, lbfgsb.rs:1590-1674;
+ * ITK's stopping rules; backtracking Armijo steps). coeffs: the initial
+ * coefficients in (3 x (mesh+3)^3, [d][cp]), the best ones out. */
+typedef struct {
+    int bins;
+    int sampling;               /* 0 every voxel, 2 random (drawn on the device) */
+    double sampling_percentage;
+    uint64_t seed;
+    int precision;
+    int iterations;             /* maximum iterations */
+    int max_evaluations;
+    int corrections;            /* stored corrections m (5) */
+    double factr;               /* relative-reduction factor (1e7) */
+    double pgtol;               /* gradient infinity-norm tolerance (1e-5) */
+    size_t mesh[3];
+} sitkhip_bspline_config;
+
+typedef struct {
+    double value;
+    size_t valid_points;
+    int iterations;
+    int evaluations;
+    int stop;                   /* 0 iterations, 1 gradient, 2 reduction, 3 evaluations, 4 no descent */
+    double seconds;
+} sitkhip_bspline_result;
+
+int sitkhip_register_bspline(sitkhip_image fixed, sitkhip_image moving,
+                             const sitkhip_bspline_config *config, double *coeffs, size_t ncoeffs,
+                             sitkhip_bspline_result *result);
+
 /* ------------------------------------------------------------- generators */
 /* Seeded synthetic volume (no counterpart in the reference; the benchmark
  * pairs' generator): the sum over `nblobs` Gaussian blobs (physical centres

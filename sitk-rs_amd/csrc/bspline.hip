@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -433,3 +434,160 @@ int sitkhip_bspline_mattes_value_and_derivative(sitkhip_bspline h, const double 
 }
 
 }  // extern "C"
+
+// ============================================================ registration
+// A B-spline registration on the device metric, driven by a limited-memory
+// BFGS optimizer on the host (LBFGSBOptimizer with every variable unbounded,
+// src/registration/lbfgsb.rs:1590-1674: the case the reference documents as
+// reducing to unconstrained L-BFGS). Its stopping rules are ITK's: relative
+// reduction (f_k - f_{k+1}) / max(|f_k|, |f_{k+1}|, 1) <= factr * eps, the
+// gradient's infinity norm <= pgtol, the iteration and evaluation caps; the
+// result is the best point evaluated. The step search is a backtracking
+// Armijo search from the unit step (the reference uses the netlib Moré-Thuente
+// search; recorded as a deviation).
+namespace sitkhip {
+
+struct LbfgsResult {
+    double value;
+    size_t valid;
+    int iterations;
+    int evaluations;
+    int stop;  // 0 max iterations, 1 gradient, 2 relative reduction, 3 evaluations, 4 no descent
+};
+
+static double dot(const std::vector<double> &a, const std::vector<double> &b) {
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+static LbfgsResult lbfgs(BSplineMetric &b, std::vector<double> &x, int max_it, int max_eval, int m,
+                         double factr, double pgtol) {
+    const size_t n = x.size();
+    std::vector<double> g(n), gn(n), xn(n), d(n), q(n);
+    std::vector<std::vector<double>> S, Y;
+    std::vector<double> rho;
+    LbfgsResult r{DBL_MAX, 0, 0, 0, 0};
+    size_t valid = 0;
+    double f = 0.0;
+    bspline_evaluate(b, x.data(), true, &f, g.data(), &valid);
+    r.evaluations = 1;
+    std::vector<double> best = x;
+    double fbest = f;
+    size_t vbest = valid;
+    for (;;) {
+        double ginf = 0.0;
+        for (double v : g) ginf = std::max(ginf, std::fabs(v));
+        if (ginf <= pgtol) { r.stop = 1; break; }
+        if (r.iterations >= max_it) { r.stop = 0; break; }
+        // two-loop recursion (Nocedal 1980) for d = -H g
+        q = g;
+        const int k = static_cast<int>(S.size());
+        std::vector<double> alpha(k);
+        for (int i = k - 1; i >= 0; --i) {
+            alpha[i] = rho[i] * dot(S[i], q);
+            for (size_t j = 0; j < n; ++j) q[j] -= alpha[i] * Y[i][j];
+        }
+        double gamma = 1.0 / std::max(ginf, 1e-300);  // first step: unit infinity-norm move
+        if (k > 0) gamma = dot(S[k - 1], Y[k - 1]) / dot(Y[k - 1], Y[k - 1]);
+        for (size_t j = 0; j < n; ++j) q[j] *= gamma;
+        for (int i = 0; i < k; ++i) {
+            const double beta = rho[i] * dot(Y[i], q);
+            for (size_t j = 0; j < n; ++j) q[j] += S[i][j] * (alpha[i] - beta);
+        }
+        for (size_t j = 0; j < n; ++j) d[j] = -q[j];
+        double gd = dot(g, d);
+        if (!(gd < 0.0)) {  // not a descent direction: restart from steepest descent
+            S.clear(); Y.clear(); rho.clear();
+            for (size_t j = 0; j < n; ++j) d[j] = -g[j] / std::max(ginf, 1e-300);
+            gd = dot(g, d);
+        }
+        double t = 1.0, fn = 0.0;
+        size_t vn = 0;
+        bool accepted = false;
+        for (int tries = 0; tries < 30 && r.evaluations < max_eval; ++tries) {
+            for (size_t j = 0; j < n; ++j) xn[j] = x[j] + t * d[j];
+            bspline_evaluate(b, xn.data(), true, &fn, gn.data(), &vn);
+            ++r.evaluations;
+            if (fn <= f + 1e-4 * t * gd) { accepted = true; break; }
+            t *= 0.5;
+        }
+        if (!accepted) { r.stop = r.evaluations >= max_eval ? 3 : 4; break; }
+        ++r.iterations;
+        std::vector<double> s(n), y(n);
+        for (size_t j = 0; j < n; ++j) { s[j] = xn[j] - x[j]; y[j] = gn[j] - g[j]; }
+        const double sy = dot(s, y);
+        if (sy > DBL_EPSILON * dot(y, y)) {
+            if (static_cast<int>(S.size()) == m) {
+                S.erase(S.begin()); Y.erase(Y.begin()); rho.erase(rho.begin());
+            }
+            S.push_back(s); Y.push_back(y); rho.push_back(1.0 / sy);
+        }
+        const double fprev = f;
+        x = xn; g = gn; f = fn; valid = vn;
+        if (f < fbest) { fbest = f; best = x; vbest = valid; }
+        if ((fprev - f) / std::max({std::fabs(fprev), std::fabs(f), 1.0}) <= factr * DBL_EPSILON) {
+            r.stop = 2;
+            break;
+        }
+        if (r.evaluations >= max_eval) { r.stop = 3; break; }
+    }
+    x = best;
+    r.value = fbest;
+    r.valid = vbest;
+    return r;
+}
+
+}  // namespace sitkhip
+
+extern "C" int sitkhip_register_bspline(sitkhip_image fixed, sitkhip_image moving,
+                                        const sitkhip_bspline_config *cfg, double *coeffs,
+                                        size_t ncoeffs, sitkhip_bspline_result *result) {
+    return run_guarded([&] {
+        auto f = reinterpret_cast<Image *>(fixed);
+        auto mv = reinterpret_cast<Image *>(moving);
+        if (!f || !mv || !cfg || !coeffs || !result) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null argument");
+        const auto t0 = std::chrono::steady_clock::now();
+        const sitkhip_geometry &g = f->geom;
+        size_t gs[3];
+        double origin[3], spacing[3], shift[3];
+        for (int d = 0; d < 3; ++d) {
+            if (cfg->mesh[d] < 1) fail(SITKHIP_ERR_INVALID_ARGUMENT, "mesh must be >= 1 per axis");
+            gs[d] = cfg->mesh[d] + 3;
+            // the transform domain is the fixed image's extent (bspline.rs:324-339)
+            spacing[d] = static_cast<double>(g.size[d]) * g.spacing[d] / static_cast<double>(cfg->mesh[d]);
+            shift[d] = -0.5 * spacing[d] * (3.0 - 1.0);
+        }
+        for (int r = 0; r < 3; ++r) {
+            double acc = 0.0;
+            for (int c = 0; c < 3; ++c) acc += g.direction[r * 3 + c] * shift[c];
+            origin[r] = g.origin[r] + acc;
+        }
+        const size_t ncp = gs[0] * gs[1] * gs[2];
+        if (ncoeffs != 3 * ncp) fail(SITKHIP_ERR_INVALID_ARGUMENT, "coefficient count must be 3 x (mesh+3)^3");
+        Metric *m = nullptr;
+        if (cfg->sampling == 2) {
+            const size_t cnt = static_cast<size_t>(static_cast<double>(f->len()) * cfg->sampling_percentage);
+            m = metric_create_drawn(SITKHIP_METRIC_MATTES, *f, cnt, cfg->seed, *mv, cfg->precision);
+        } else {
+            m = metric_create(SITKHIP_METRIC_MATTES, *f, nullptr, 0, nullptr, *mv, nullptr, cfg->precision);
+        }
+        std::unique_ptr<Metric, void (*)(Metric *)> mh(m, metric_destroy);
+        double fr[2], mr[2];
+        metric_value_ranges(*m, fr, mr);
+        sitkhip_mattes_bins bins;
+        mattes_bins_from_ranges(fr, mr, static_cast<size_t>(cfg->bins), &bins);
+        mattes_set_bins(*m, bins);
+        std::unique_ptr<BSplineMetric> b(bspline_create(*m, gs, origin, spacing, g.direction));
+        std::vector<double> x(coeffs, coeffs + ncoeffs);
+        LbfgsResult r = lbfgs(*b, x, cfg->iterations, cfg->max_evaluations, cfg->corrections,
+                              cfg->factr, cfg->pgtol);
+        std::memcpy(coeffs, x.data(), ncoeffs * sizeof(double));
+        result->value = r.value;
+        result->valid_points = r.valid;
+        result->iterations = r.iterations;
+        result->evaluations = r.evaluations;
+        result->stop = r.stop;
+        result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    });
+}

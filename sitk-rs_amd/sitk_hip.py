@@ -112,6 +112,18 @@ class RegistrationConfig(C.Structure):
     ]
 
 
+class BSplineConfig(C.Structure):
+    _fields_ = [("bins", C.c_int), ("sampling", C.c_int), ("sampling_percentage", C.c_double),
+                ("seed", C.c_uint64), ("precision", C.c_int), ("iterations", C.c_int),
+                ("max_evaluations", C.c_int), ("corrections", C.c_int), ("factr", C.c_double),
+                ("pgtol", C.c_double), ("mesh", C.c_size_t * 3)]
+
+
+class BSplineResult(C.Structure):
+    _fields_ = [("value", C.c_double), ("valid_points", C.c_size_t), ("iterations", C.c_int),
+                ("evaluations", C.c_int), ("stop", C.c_int), ("seconds", C.c_double)]
+
+
 class RegistrationResult(C.Structure):
     _fields_ = [
         ("value", C.c_double),
@@ -209,6 +221,8 @@ def lib():
         L.sitkhip_bspline_parameter_count.argtypes = [vp, szp]
         L.sitkhip_bspline_mattes_value.argtypes = [vp, dp, dp, szp]
         L.sitkhip_bspline_mattes_value_and_derivative.argtypes = [vp, dp, dp, dp, szp]
+        L.sitkhip_register_bspline.argtypes = [vp, vp, C.POINTER(BSplineConfig), dp, C.c_size_t,
+                                               C.POINTER(BSplineResult)]
         L.sitkhip_synth_blobs.argtypes = [gp, C.c_int, dp, dp, dp, C.c_double, C.c_uint64,
                                           C.POINTER(vp)]
         L.sitkhip_cnn_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
@@ -885,3 +899,22 @@ class BSplineMattes:
 def empty_cache() -> None:
     """Return the library's cached device blocks to the runtime."""
     _check(lib().sitkhip_empty_cache())
+
+
+def register_bspline(fixed: DeviceImage, moving: DeviceImage, mesh, coeffs=None, bins=32,
+                     sampling=SAMPLING_NONE, sampling_percentage=1.0, seed=0,
+                     precision=PRECISION_F32, iterations=100, max_evaluations=2000, corrections=5,
+                     factr=1e7, pgtol=1e-5):
+    """Deformable Mattes registration through a cubic B-spline transform over
+    the fixed image's extent. Returns (coefficients [3 x ncp], BSplineResult)."""
+    c = BSplineConfig()
+    c.bins, c.sampling, c.sampling_percentage, c.seed = bins, sampling, sampling_percentage, seed
+    c.precision, c.iterations, c.max_evaluations = precision, iterations, max_evaluations
+    c.corrections, c.factr, c.pgtol = corrections, factr, pgtol
+    c.mesh[:] = [int(v) for v in mesh]
+    ncp = int(np.prod([m + 3 for m in mesh]))
+    x = np.zeros(3 * ncp) if coeffs is None else np.ascontiguousarray(coeffs, dtype=np.float64).copy()
+    r = BSplineResult()
+    _check(lib().sitkhip_register_bspline(fixed._h, moving._h, C.byref(c), _dp(x), x.size,
+                                          C.byref(r)))
+    return x, r
