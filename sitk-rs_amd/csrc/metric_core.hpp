@@ -537,7 +537,7 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
     return v;
 }
 
-__device__ void mattes_tail_block(unsigned long long *g_joint, unsigned long long *g_marg, int B,
+__device__ __forceinline__ void mattes_tail_block(unsigned long long *g_joint, unsigned long long *g_marg, int B,
                                   double mbs, double *pratio, float *pratio_f, TailOut *out,
                                   unsigned long long *s_hist, bool clear = true) {
     constexpr int kWaves = kTail / 64;
