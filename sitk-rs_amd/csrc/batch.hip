@@ -112,7 +112,8 @@ __global__ __launch_bounds__(kBlock) void batch_pass1(const PairDesc *__restrict
             const int nb = B * B + B;
             for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
             __syncthreads();
-            hist_range<R>(d.F, d.M, S, c.X, d.C, d.SS, d.fbin, d.P, s_hist, s_hist + B * B, begin, end);
+            hist_range<R, false>(d.F, d.M, S, c.X, d.C, d.SS, d.fbin, d.P, s_hist, s_hist + B * B,
+                                 static_cast<SampleRec<R> *>(nullptr), begin, end);
             __syncthreads();
             const unsigned copy = blockIdx.x % kHistCopies;
             unsigned long long *gj = d.joint + static_cast<size_t>(copy) * B * B;

@@ -135,7 +135,8 @@ __global__ __launch_bounds__(kBlock) void bs_hist(FixedGrid F, MovingVol M, BSpl
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
     const IndexMap none{};
-    hist_range<R>(F, M, B, none, C, SS, fbin, P, s_hist, s_hist + P.bins * P.bins, begin, end);
+    hist_range<R, false>(F, M, B, none, C, SS, fbin, P, s_hist, s_hist + P.bins * P.bins,
+                         static_cast<SampleRec<R> *>(nullptr), begin, end);
     __syncthreads();
     for (int i = threadIdx.x; i < nb; i += kBlock) {
         const unsigned long long v = s_hist[i];
@@ -181,12 +182,14 @@ __global__ __launch_bounds__(kCellChunk) void bs_cells(FixedGrid F, MovingVol M,
                     Trilinear<R>::template eval<true>(fr, v, mv, gi);
                     if (Parzen<R>::in_range(P, mv)) {
                         physical_gradient<R>(M, gi, gp);
-                        R arg[4];
-                        const int m0 = Parzen<R>::taps(P, mv, arg);
+                        R arg[4], dv[4];
+                        bool mid;
+                        const int m0 = Parzen<R>::taps(P, mv, arg, mid);
+                        parzen_derivs<R>(arg, mid, dv);
                         const R *row = pratio + fbin[s] * P.bins + m0;
                         R a = R(0);
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) a += row[k] * bspline3_deriv<R>(arg[k]);
+                        for (int k = 0; k < 4; ++k) a += row[k] * dv[k];
 #pragma unroll
                         for (int d = 0; d < 3; ++d)
                             g[d] = static_cast<double>(a) * static_cast<double>(gp[d]);

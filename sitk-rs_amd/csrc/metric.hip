@@ -69,14 +69,16 @@ __global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Map 
     block_reduce_store<NS>(acc, partials, logical);
 }
 
-// Pass 1: joint histogram (fixed point) + fixed-marginal counts.
-template <typename R, bool LDS, typename Map>
+// Pass 1: joint histogram (fixed point) + fixed-marginal counts; with REC
+// also the per-sample records for the derivative pass that follows.
+template <typename R, bool LDS, bool REC, typename Map>
 __global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, Map S,
                                                       IndexMap X, const ChainConst *__restrict__ C,
                                                       SampleSet SS, const uint8_t *__restrict__ fbin,
                                                       MattesParams P,
                                                       unsigned long long *__restrict__ g_joint,
-                                                      unsigned long long *__restrict__ g_marg) {
+                                                      unsigned long long *__restrict__ g_marg,
+                                                      SampleRec<R> *__restrict__ rec) {
     extern __shared__ unsigned long long s_hist[];  // bins^2 joint, then bins marginal
     const int B = P.bins;
     const int nb = B * B + B;
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, 
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    hist_range<R>(F, M, V, X, C, SS, fbin, P, joint, marg, begin, end);
+    hist_range<R, REC>(F, M, V, X, C, SS, fbin, P, joint, marg, rec, begin, end);
     if (LDS) {
         __syncthreads();
         for (int i = threadIdx.x; i < nb; i += kBlock) {
@@ -104,18 +106,60 @@ __global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, 
     }
 }
 
-// The tail as its own one-workgroup launch. STAGE: the histogram is staged in
-// (dynamic) LDS first; otherwise (bin counts whose histogram does not fit) it is
-// read from global memory.
+// The tail split over T = gridDim.x workgroups (T <= kMaxTailSplit): every
+// workgroup folds the whole histogram into its LDS and takes the statistics,
+// then computes the pRatio of its share of the bins and its exact partial
+// value sum. The workgroup that finishes last (a device-scope ticket, which it
+// resets) adds the T partials, writes the result and clears the copies. Without
+// staging (bin counts whose histogram does not fit in LDS) there is one
+// workgroup, reading global memory, and the fold goes into copy 0.
 template <bool STAGE>
 __global__ __launch_bounds__(kTail) void mattes_tail_kernel(unsigned long long *g_joint,
                                                              unsigned long long *g_marg, int B,
                                                              double mbs, double *pratio,
                                                              float *pratio_f, TailOut *out,
-                                                             int clear) {
+                                                             unsigned long long *g_part,
+                                                             unsigned *ticket, int clear) {
     extern __shared__ unsigned long long s_stage[];
-    mattes_tail_block(g_joint, g_marg, B, mbs, pratio, pratio_f, out, STAGE ? s_stage : nullptr,
-                      clear != 0);
+    __shared__ TailShared sh;
+    __shared__ int s_last;
+    const int t = threadIdx.x, nbin = B * B, T = static_cast<int>(gridDim.x);
+    unsigned long long *s_hist = STAGE ? s_stage : nullptr;
+    tail_fold(g_joint, g_marg, B, s_hist);
+    __syncthreads();
+    const unsigned long long *hj = STAGE ? s_hist : g_joint;
+    const unsigned long long *hm = STAGE ? s_hist + nbin : g_marg;
+    const TailStats st = tail_stats(hj, hm, B, mbs, sh);
+    const int per = (nbin + T - 1) / T;
+    const int i0 = static_cast<int>(blockIdx.x) * per;
+    const int i1 = i0 + per < nbin ? i0 + per : nbin;
+    const __int128 sum = tail_bins(hj, B, i0, i1, st, pratio, pratio_f, sh);
+    if (t == 0) {
+        g_part[2 * blockIdx.x] = static_cast<unsigned long long>(sum);
+        g_part[2 * blockIdx.x + 1] = static_cast<unsigned long long>(sum >> 64);
+        __threadfence();
+        s_last = atomicAdd(ticket, 1u) == static_cast<unsigned>(T - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // every workgroup's reads of the copies ended before its ticket
+    if (clear) {
+        for (int i = t; i < kHistCopies * nbin; i += kTail) g_joint[i] = 0ull;
+        for (int i = t; i < kHistCopies * B; i += kTail) g_marg[i] = 0ull;
+    }
+    if (t == 0) {
+        __int128 total = 0;
+        for (int b = 0; b < T; ++b) {
+            const unsigned long long lo = __hip_atomic_load(g_part + 2 * b, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long hi = __hip_atomic_load(g_part + 2 * b + 1, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT);
+            total += static_cast<__int128>((static_cast<unsigned __int128>(hi) << 64) | lo);
+        }
+        tail_finish(total, st, out);
+        *ticket = 0u;
+    }
 }
 
 // Pass 2: the 12 pRatio-weighted moments, folded by the last workgroup.
@@ -153,6 +197,45 @@ __global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M,
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
     deriv_range<R>(F, M, V, X, C, SS, fbin, P, pr, begin, end, acc);
+    block_reduce_store<12>(acc, partials, logical);
+}
+
+// Pass 2 from the value pass's records (same transform): the same moments
+// as mattes_deriv, in the same order.
+template <typename R, bool LDS>
+__global__ __launch_bounds__(kBlock) void mattes_deriv_rec(FixedGrid F, MovingVol M,
+                                                           const ChainConst *__restrict__ C,
+                                                           SampleSet SS, const uint8_t *__restrict__ fbin,
+                                                           MattesParams P,
+                                                           const R *__restrict__ g_pratio,
+                                                           const SampleRec<R> *__restrict__ rec,
+                                                           double *__restrict__ partials,
+                                                           unsigned long long *__restrict__ clr,
+                                                           unsigned clr_n,
+                                                           unsigned long long *__restrict__ clr2,
+                                                           unsigned clr2_n) {
+    extern __shared__ unsigned char s_raw[];
+    long long begin, end;
+    unsigned logical;
+    block_slice(SS.n, begin, end, logical);
+    // the first round's loads go out before the pRatio staging waits
+    RecRound<R> first;
+    load_rec_round<R>(SS, fbin, rec, begin + threadIdx.x, begin, end, first);
+    for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < clr_n; i += gridDim.x * kBlock)
+        clr[i] = 0ull;
+    for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < clr2_n; i += gridDim.x * kBlock)
+        clr2[i] = 0ull;
+    R *s_pr = reinterpret_cast<R *>(s_raw);
+    const int B = P.bins;
+    const R *pr = LDS ? s_pr : g_pratio;
+    if (LDS) {
+        for (int i = threadIdx.x; i < B * B; i += kBlock) s_pr[i] = g_pratio[i];
+        __syncthreads();
+    }
+    double acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+    deriv_records<R>(F, M, C, SS, fbin, P, pr, rec, begin, end, first, acc);
     block_reduce_store<12>(acc, partials, logical);
 }
 
@@ -580,6 +663,9 @@ void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b) {
     m.marg.alloc(static_cast<size_t>(kHistCopies) * B);
     m.pratio.alloc(static_cast<size_t>(B) * B);
     m.pratio_f.alloc(static_cast<size_t>(B) * B);
+    m.tail_part.alloc(2 * kMaxTailSplit);
+    m.tail_ticket.alloc(1);
+    HIP_OK(hipMemsetAsync(m.tail_ticket.ptr, 0, m.tail_ticket.bytes(), backend().stream));
     HIP_OK(hipMemsetAsync(m.joint.ptr, 0, m.joint.bytes(), backend().stream));
     HIP_OK(hipMemsetAsync(m.marg.ptr, 0, m.marg.bytes(), backend().stream));
     m.bins_set = true;
@@ -606,25 +692,29 @@ static void with_map(const Stages &S, Fn &&f) {
 
 void launch_tail(Metric &m, TailOut *out, bool clear) {
     const int B = m.P.bins;
-    // staged histogram plus the tail's own ~14 KB of LDS within 64 KB
+    // staged histogram plus the tail's own ~15 KB of LDS within 64 KB
     const bool lds = B <= 76;
     const size_t shm = lds ? static_cast<size_t>(B * B + B) * sizeof(unsigned long long) : 0;
+    // about one bin per lane; one workgroup without staging
+    int T = lds ? (B * B + kTail - 1) / kTail : 1;
+    if (T > kMaxTailSplit) T = kMaxTailSplit;
     LaunchScope scope("mattes_tail");
     if (lds)
-        hipLaunchKernelGGL((mattes_tail_kernel<true>), dim3(1), dim3(kTail), shm, backend().stream,
+        hipLaunchKernelGGL((mattes_tail_kernel<true>), dim3(T), dim3(kTail), shm, backend().stream,
                            m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out,
-                           clear ? 1 : 0);
+                           m.tail_part.ptr, m.tail_ticket.ptr, clear ? 1 : 0);
     else
         hipLaunchKernelGGL((mattes_tail_kernel<false>), dim3(1), dim3(kTail), 0, backend().stream,
                            m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out,
-                           clear ? 1 : 0);
+                           m.tail_part.ptr, m.tail_ticket.ptr, clear ? 1 : 0);
     check_launch("mattes_tail");
 }
 
 // tail: run the tail after the value pass; clear: the tail clears the
 // histogram (otherwise the derivative pass that follows does).
 template <typename R>
-static void launch_hist(Metric &m, const Stages &S, bool tail, bool clear = true) {
+static void launch_hist(Metric &m, const Stages &S, bool tail, bool clear = true,
+                        SampleRec<R> *rec = nullptr) {
     const int B = m.P.bins;
     const bool lds = B <= kLdsMaxBins;
     const size_t shm = lds ? static_cast<size_t>(B * B + B) * sizeof(unsigned long long) : 0;
@@ -633,14 +723,18 @@ static void launch_hist(Metric &m, const Stages &S, bool tail, bool clear = true
         const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
         with_map(S, [&](const auto &map) {
             using Map = std::decay_t<decltype(map)>;
-            if (lds)
-                hipLaunchKernelGGL((mattes_hist<R, true, Map>), dim3(kGrid), dim3(kBlock), shm,
-                                   backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P,
-                                   m.joint.ptr, m.marg.ptr);
-            else
-                hipLaunchKernelGGL((mattes_hist<R, false, Map>), dim3(kGrid), dim3(kBlock), 0,
-                                   backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P,
-                                   m.joint.ptr, m.marg.ptr);
+            auto go = [&](auto kernel, size_t bytes) {
+                hipLaunchKernelGGL(kernel, dim3(kGrid), dim3(kBlock), bytes, backend().stream, m.F, m.Mv,
+                                   map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, m.joint.ptr, m.marg.ptr,
+                                   rec);
+            };
+            if (lds) {
+                if (rec) go(mattes_hist<R, true, true, Map>, shm);
+                else go(mattes_hist<R, true, false, Map>, shm);
+            } else {
+                if (rec) go(mattes_hist<R, false, true, Map>, 0);
+                else go(mattes_hist<R, false, false, Map>, 0);
+            }
         });
         check_launch("mattes_hist");
     }
@@ -684,6 +778,35 @@ static void launch_deriv(Metric &m, const Stages &S, const R *pratio, bool clear
         check_launch("mattes_deriv");
     }
     launch_fold<12>(m);
+}
+
+// The derivative pass from the records of the value pass just run; it also
+// clears the histogram copies the tail left.
+template <typename R>
+static void launch_deriv_rec(Metric &m, const R *pratio, const SampleRec<R> *rec) {
+    const int B = m.P.bins;
+    const size_t shm = static_cast<size_t>(B) * B * sizeof(R);
+    const bool lds = shm + 2048 <= 64 * 1024;  // plus the reduction's static LDS
+    {
+        LaunchScope scope("mattes_deriv");
+        auto go = [&](auto kernel, size_t bytes) {
+            hipLaunchKernelGGL(kernel, dim3(kGrid), dim3(kBlock), bytes, backend().stream, m.F, m.Mv,
+                               m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio, rec, m.partials.ptr,
+                               m.joint.ptr, static_cast<unsigned>(m.joint.len), m.marg.ptr,
+                               static_cast<unsigned>(m.marg.len));
+        };
+        if (lds) go(mattes_deriv_rec<R, true>, shm);
+        else go(mattes_deriv_rec<R, false>, 0);
+        check_launch("mattes_deriv");
+    }
+    launch_fold<12>(m);
+}
+
+template <typename R>
+static SampleRec<R> *sample_records(Metric &m) {
+    const size_t bytes = static_cast<size_t>(m.SS.n) * sizeof(SampleRec<R>);
+    if (m.recs.len < bytes) m.recs.alloc(bytes);
+    return reinterpret_cast<SampleRec<R> *>(m.recs.ptr);
 }
 
 void mattes_joint_histogram(Metric &m, const double *stages, int nstage, double *joint,
@@ -755,11 +878,13 @@ void mattes_value_and_moments(Metric &m, const double *stages, int nstage, doubl
     require_bins(m);
     Stages S = make_stages(stages, nstage);
     if (m.precision == SITKHIP_PRECISION_F64) {
-        launch_hist<double>(m, S, true, false);
-        launch_deriv<double>(m, S, m.pratio.ptr, true);
+        SampleRec<double> *rec = sample_records<double>(m);
+        launch_hist<double>(m, S, true, false, rec);
+        launch_deriv_rec<double>(m, m.pratio.ptr, rec);
     } else {
-        launch_hist<float>(m, S, true, false);
-        launch_deriv<float>(m, S, m.pratio_f.ptr, true);
+        SampleRec<float> *rec = sample_records<float>(m);
+        launch_hist<float>(m, S, true, false, rec);
+        launch_deriv_rec<float>(m, m.pratio_f.ptr, rec);
     }
     sync();
     const TailOut &t = m.res->tail;

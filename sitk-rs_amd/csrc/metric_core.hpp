@@ -20,7 +20,10 @@
 
 namespace sitkhip {
 
-constexpr int kBlock = 256;
+#ifndef SITKHIP_BLOCK
+#define SITKHIP_BLOCK 256
+#endif
+constexpr int kBlock = SITKHIP_BLOCK;  // lanes per workgroup of the sample passes
 #ifndef SITKHIP_GRID
 #define SITKHIP_GRID 1024
 #endif
@@ -49,41 +52,82 @@ struct MattesParams {
 
 // ------------------------------------------------------------- cubic B-spline
 // itk::BSplineKernelFunction<3> and its derivative in the reference's
-// expression order (registration/mattes.rs:96-124).
+// expression order (registration/mattes.rs:96-124), piece by piece: the value
+// on |u| < 1 (inner) and 1 <= |u| < 2 (outer); the derivative on [0, 1) (d1),
+// (-1, 0) (d2), [1, 2) (d3) and (-2, -1] (d4).
 template <typename R>
-__device__ __forceinline__ R bspline3(R u) {
-    const R a = fabs(u);
-    if (a < R(1)) {
+struct B3 {
+    __device__ __forceinline__ static R inner(R a) {
         const R sq = a * a;
         return (R(4) - R(6) * sq + R(3) * sq * a) / R(6);
-    } else if (a < R(2)) {
+    }
+    __device__ __forceinline__ static R outer(R a) {
         const R sq = a * a;
         return (R(8) - R(12) * a + R(6) * sq - sq * a) / R(6);
     }
-    return R(0);
-}
-
+    __device__ __forceinline__ static R d1(R u) { return R(-2) * u + R(1.5) * u * u; }
+    __device__ __forceinline__ static R d2(R u) { return R(-2) * u - R(1.5) * u * u; }
+    __device__ __forceinline__ static R d3(R u) { return R(-2) + R(2) * u - R(0.5) * u * u; }
+    __device__ __forceinline__ static R d4(R u) { return R(2) + R(2) * u + R(0.5) * u * u; }
+};
 // FP32: the same polynomials, (2-a)^3/6 form and a multiply by 1/6 (no
 // correctly rounded division on the FP32 path).
 template <>
-__device__ __forceinline__ float bspline3<float>(float u) {
-    const float a = fabsf(u);
+__device__ __forceinline__ float B3<float>::inner(float a) {
     const float sq = a * a;
-    if (a < 1.0f) return fmaf(fmaf(3.0f, a, -6.0f), sq, 4.0f) * (1.0f / 6.0f);
-    if (a < 2.0f) {
-        const float t = 2.0f - a;
-        return t * t * t * (1.0f / 6.0f);
-    }
-    return 0.0f;
+    return fmaf(fmaf(3.0f, a, -6.0f), sq, 4.0f) * (1.0f / 6.0f);
+}
+template <>
+__device__ __forceinline__ float B3<float>::outer(float a) {
+    const float t = 2.0f - a;
+    return t * t * t * (1.0f / 6.0f);
+}
+
+template <typename R>
+__device__ __forceinline__ R bspline3(R u) {
+    const R a = fabs(u);
+    if (a < R(1)) return B3<R>::inner(a);
+    if (a < R(2)) return B3<R>::outer(a);
+    return R(0);
 }
 
 template <typename R>
 __device__ __forceinline__ R bspline3_deriv(R u) {
-    if (u >= R(0) && u < R(1)) return R(-2) * u + R(1.5) * u * u;
-    if (u > R(-1) && u < R(0)) return R(-2) * u - R(1.5) * u * u;
-    if (u >= R(1) && u < R(2)) return R(-2) + R(2) * u - R(0.5) * u * u;
-    if (u > R(-2) && u <= R(-1)) return R(2) + R(2) * u + R(0.5) * u * u;
+    if (u >= R(0) && u < R(1)) return B3<R>::d1(u);
+    if (u > R(-1) && u < R(0)) return B3<R>::d2(u);
+    if (u >= R(1) && u < R(2)) return B3<R>::d3(u);
+    if (u > R(-2) && u <= R(-1)) return B3<R>::d4(u);
     return R(0);
+}
+
+// The four taps of a Parzen window. When the bin index was not clamped
+// (`mid`), arg = (-1-f, -f, 1-f, 2-f) with f in [0, 1): each tap lies in a
+// known piece, and where two pieces meet (f = 0) they give the same bits, so
+// these equal bspline3 / bspline3_deriv tap by tap. Otherwise the generic
+// forms.
+template <typename R>
+__device__ __forceinline__ void parzen_values(const R arg[4], bool mid, R v[4]) {
+    if (mid) {
+        v[0] = B3<R>::outer(fabs(arg[0]));
+        v[1] = B3<R>::inner(fabs(arg[1]));
+        v[2] = B3<R>::inner(fabs(arg[2]));
+        v[3] = arg[3] < R(2) ? B3<R>::outer(fabs(arg[3])) : R(0);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = bspline3<R>(arg[k]);
+    }
+}
+template <typename R>
+__device__ __forceinline__ void parzen_derivs(const R arg[4], bool mid, R d[4]) {
+    if (mid) {
+        d[0] = B3<R>::d4(arg[0]);
+        d[1] = arg[1] >= R(0) ? B3<R>::d1(arg[1]) : B3<R>::d2(arg[1]);
+        d[2] = arg[2] < R(1) ? B3<R>::d1(arg[2]) : B3<R>::d3(arg[2]);
+        d[3] = arg[3] < R(2) ? B3<R>::d3(arg[3]) : R(0);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) d[k] = bspline3_deriv<R>(arg[k]);
+    }
 }
 
 // Workgroup slice of the samples: contiguous, XCD-grouped (see file comment).
@@ -108,9 +152,11 @@ struct Parzen;
 
 template <>
 struct Parzen<double> {
-    __device__ __forceinline__ static int taps(const MattesParams &P, double mv, double arg[4]) {
+    __device__ __forceinline__ static int taps(const MattesParams &P, double mv, double arg[4],
+                                               bool &mid) {
         const double term = mv / P.mbs - P.mnmin;
         long long idx = static_cast<long long>(term);
+        mid = idx >= 2 && idx <= P.bins - 3;
         if (idx < 2) idx = 2;
         else if (idx > P.bins - 3) idx = P.bins - 3;
         const int start = static_cast<int>(idx) - 1;
@@ -127,9 +173,11 @@ template <>
 struct Parzen<float> {
     // term = mv/binSize - normalizedMin = (mv - trueMin)/binSize + padding: the
     // second form has no cancellation, so it is accurate in FP32.
-    __device__ __forceinline__ static int taps(const MattesParams &P, float mv, float arg[4]) {
+    __device__ __forceinline__ static int taps(const MattesParams &P, float mv, float arg[4],
+                                               bool &mid) {
         const float term = fmaf(mv - P.mmin_f, P.inv_mbs_f, 2.0f);
         int idx = static_cast<int>(term);
+        mid = idx >= 2 && idx <= P.bins - 3;
         idx = idx < 2 ? 2 : (idx > P.bins - 3 ? P.bins - 3 : idx);
         const int start = idx - 1;
 #pragma unroll
@@ -193,13 +241,24 @@ __global__ __launch_bounds__(kBlock) void fold_partials(const double *__restrict
     double t[NSLOT];
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) t[k] = 0.0;
-    for (unsigned b0 = 0; b0 < G; b0 += kBlock) {
-        const double *row = partials + static_cast<size_t>(b0 + threadIdx.x) * NSLOT;
-        double v[NSLOT];
+    // four rows per lane per round, all loaded before any is added
+    constexpr int kRows = 4;
+    for (unsigned b0 = 0; b0 < G; b0 += kBlock * kRows) {
+        double v[kRows][NSLOT];
 #pragma unroll
-        for (int k = 0; k < NSLOT; ++k) v[k] = row[k];
+        for (int r = 0; r < kRows; ++r) {
+            const unsigned row = b0 + r * kBlock + threadIdx.x;
+            const double *p = partials + static_cast<size_t>(row < G ? row : 0u) * NSLOT;
 #pragma unroll
-        for (int k = 0; k < NSLOT; ++k) t[k] += v[k];
+            for (int k = 0; k < NSLOT; ++k) {
+                const double x = p[k];
+                v[r][k] = row < G ? x : 0.0;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+#pragma unroll
+            for (int k = 0; k < NSLOT; ++k) t[k] += v[r][k];
     }
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) {
@@ -348,10 +407,16 @@ __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol
             }
         }
         if (!done) {
+            // The exact chain's geometry is read through a pointer held in
+            // vector registers: vector loads, so the rarely taken chain does
+            // not claim the scalar registers the fast path keeps its
+            // constants in.
+            const ChainConst *Cv = C;
+            asm volatile("" : "+v"(Cv));
             double x[3], p[3];
-            fixed_point(F, *C, b.flat[u], x);
+            fixed_point(F, *Cv, b.flat[u], x);
             map_point(S, x, p);
-            L.ok[u] = b.live[u] && locate(M, *C, p, cell[u]);
+            L.ok[u] = b.live[u] && locate(M, *Cv, p, cell[u]);
         }
         if (!L.ok[u]) {
             cell[u].base[0] = cell[u].base[1] = cell[u].base[2] = 0;
@@ -419,16 +484,28 @@ __device__ __forceinline__ void ms_range(const FixedGrid &F, const MovingVol &M,
     }
 }
 
+// What the value pass leaves per sample for a derivative pass that follows it
+// at the same transform: the moving value and its index-space gradient, or a
+// NaN value where the sample is dropped (outside the moving buffer or mask,
+// or outside the Parzen range). Written for every sample of the set.
+template <typename R>
+struct alignas(4 * sizeof(R)) SampleRec {
+    R mv;
+    R gi[3];
+};
+
 // Mattes value pass (mattes.rs:442-486): per valid sample, the four cubic
 // B-spline taps of the moving Parzen window into row `fixed bin` of `joint`
-// (64-bit fixed point) and one count into marg[fixed bin].
-template <typename R, typename Map>
+// (64-bit fixed point) and one count into marg[fixed bin]. With REC, also the
+// per-sample records (the value is the same: the interpolation computes it
+// the same way with or without the gradient).
+template <typename R, bool REC, typename Map>
 __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &M, const Map &S,
                                            const IndexMap &X, const ChainConst *__restrict__ C,
                                            const SampleSet &SS, const uint8_t *__restrict__ fbin,
                                            const MattesParams &P, unsigned long long *joint,
-                                           unsigned long long *marg, long long begin,
-                                           long long end) {
+                                           unsigned long long *marg, SampleRec<R> *__restrict__ rec,
+                                           long long begin, long long end) {
     const int B = P.bins;
     for (long long s0 = begin + threadIdx.x; s0 < end; s0 += kBlock * kU) {
         Batch b;
@@ -445,16 +522,32 @@ __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &
         locate_batch<R>(F, M, S, X, C, b, L);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            if (!L.ok[u]) continue;
-            R mv, gi[3];
-            Trilinear<R>::template eval<false>(L.fr[u], L.v[u], mv, gi);
-            if (!Parzen<R>::in_range(P, mv)) continue;
-            R arg[4];
-            const int m0 = Parzen<R>::taps(P, mv, arg);
+            R mv = R(0), gi[3] = {R(0), R(0), R(0)};
+            bool use = L.ok[u];
+            if (use) {
+                Trilinear<R>::template eval<REC>(L.fr[u], L.v[u], mv, gi);
+                use = Parzen<R>::in_range(P, mv);
+            }
+            if (REC) {
+                const long long s = s0 + static_cast<long long>(u) * kBlock;
+                if (s < end) {
+                    SampleRec<R> r;
+                    r.mv = use ? mv : static_cast<R>(NAN);
+                    r.gi[0] = gi[0];
+                    r.gi[1] = gi[1];
+                    r.gi[2] = gi[2];
+                    rec[s] = r;
+                }
+            }
+            if (!use) continue;
+            R arg[4], wv[4];
+            bool mid;
+            const int m0 = Parzen<R>::taps(P, mv, arg, mid);
+            parzen_values<R>(arg, mid, wv);
             unsigned long long *row = joint + b.fb[u] * B + m0;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const unsigned long long q = to_fixed(bspline3<R>(arg[k]));
+                const unsigned long long q = to_fixed(wv[k]);
                 if (q) atomicAdd(row + k, q);
             }
             atomicAdd(marg + b.fb[u], 1ull);
@@ -492,15 +585,85 @@ __device__ __forceinline__ void deriv_range(const FixedGrid &F, const MovingVol 
             Trilinear<R>::template eval<true>(L.fr[u], L.v[u], mv, gi);
             if (!Parzen<R>::in_range(P, mv)) continue;
             physical_gradient<R>(M, gi, g);
-            R arg[4];
-            const int m0 = Parzen<R>::taps(P, mv, arg);
+            R arg[4], dv[4];
+            bool mid;
+            const int m0 = Parzen<R>::taps(P, mv, arg, mid);
+            parzen_derivs<R>(arg, mid, dv);
             const R *row = pr + b.fb[u] * B + m0;
             R w = R(0);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) w += row[k] * bspline3_deriv<R>(arg[k]);
+            for (int k = 0; k < 4; ++k) w += row[k] * dv[k];
             const double wd = static_cast<double>(w);
             double x[3];
             moment_point<R>(F, C, b.flat[u], x);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double wg = wd * static_cast<double>(g[d]);
+                acc[d] += wg;
+#pragma unroll
+                for (int e = 0; e < 3; ++e) acc[3 + d * 3 + e] = fma(wg, x[e], acc[3 + d * 3 + e]);
+            }
+        }
+    }
+}
+
+// The derivative pass from the value pass's records: the same samples, the
+// same per-lane order and the same arithmetic as deriv_range, without the
+// gather and interpolation (mattes.rs:75-91 as above). A lane visits samples
+// begin + lane, + kBlock, + 2 kBlock, ... whatever the unroll, so kRecU
+// samples' loads are issued at a time without changing the sums. `first`
+// holds the first round's loads, issued by the caller.
+constexpr int kRecU = 8;
+
+template <typename R>
+struct RecRound {
+    SampleRec<R> r[kRecU];
+    uint32_t flat[kRecU];
+    int fb[kRecU];
+};
+
+template <typename R>
+__device__ __forceinline__ void load_rec_round(const SampleSet &SS, const uint8_t *__restrict__ fbin,
+                                               const SampleRec<R> *__restrict__ rec, long long s0,
+                                               long long begin, long long end, RecRound<R> &q) {
+#pragma unroll
+    for (int u = 0; u < kRecU; ++u) {
+        const long long s = s0 + static_cast<long long>(u) * kBlock;
+        const long long sc = s < end ? s : begin;
+        q.fb[u] = s < end ? static_cast<int>(fbin[sc]) : static_cast<int>(kNoBin);
+        q.r[u] = rec[sc];
+        q.flat[u] = sample_flat(SS, sc);
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void deriv_records(const FixedGrid &F, const MovingVol &M,
+                                              const ChainConst *__restrict__ C,
+                                              const SampleSet &SS, const uint8_t *__restrict__ fbin,
+                                              const MattesParams &P, const R *pr,
+                                              const SampleRec<R> *__restrict__ rec,
+                                              long long begin, long long end,
+                                              const RecRound<R> &first, double (&acc)[12]) {
+    const int B = P.bins;
+    RecRound<R> q = first;
+    for (long long s0 = begin + threadIdx.x; s0 < end; s0 += kBlock * kRecU) {
+        if (s0 != begin + threadIdx.x) load_rec_round<R>(SS, fbin, rec, s0, begin, end, q);
+#pragma unroll
+        for (int u = 0; u < kRecU; ++u) {
+            if (q.fb[u] == kNoBin || isnan(q.r[u].mv)) continue;
+            R g[3];
+            physical_gradient<R>(M, q.r[u].gi, g);
+            R arg[4], dv[4];
+            bool mid;
+            const int m0 = Parzen<R>::taps(P, q.r[u].mv, arg, mid);
+            parzen_derivs<R>(arg, mid, dv);
+            const R *row = pr + q.fb[u] * B + m0;
+            R w = R(0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w += row[k] * dv[k];
+            const double wd = static_cast<double>(w);
+            double x[3];
+            moment_point<R>(F, C, q.flat[u], x);
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const double wg = wd * static_cast<double>(g[d]);
@@ -520,131 +683,207 @@ struct TailOut {
     double pad;
 };
 
-// The Mattes tail (mattes.rs:273-328) on the finished integer histogram, one
-// workgroup of kTail lanes: -MI, valid, and the pRatio table (pRatio * n_factor;
-// zero where the value sum skips the bin). Clears the histogram for the next
-// evaluation. The joint mass is summed exactly in integers and rounded once.
+// The Mattes tail (mattes.rs:273-328) on the finished integer histogram: -MI,
+// valid, and the pRatio table (pRatio * n_factor; zero where the value sum
+// skips the bin). The joint mass is summed exactly in integers and rounded
+// once; the value terms are summed exactly as 128-bit fixed-point integers
+// (2^-kValShift units) and rounded once, so the value does not depend on how
+// the bins are split over lanes and workgroups: the one-workgroup tail of the
+// batched engine and the split tail below give the same bits.
 constexpr int kTail = 1024;
+constexpr int kValShift = 100;
+constexpr int kMaxTailSplit = 16;
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
-__device__ __forceinline__ double wave_sum_f64(double v) {
+__device__ __forceinline__ __int128 wave_sum_i128(__int128 v) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long lo = __shfl_xor(static_cast<unsigned long long>(v), off, 64);
+        const long long hi = __shfl_xor(static_cast<long long>(v >> 64), off, 64);
+        v += static_cast<__int128>((static_cast<unsigned __int128>(static_cast<unsigned long long>(hi)) << 64) | lo);
+    }
     return v;
 }
+// t * 2^kValShift truncated toward minus infinity; exact for |t| >= 2^-47.
+// |t| < 2^8 here (a value term is jp * log(ratio) with jp <= 1 and the ratio
+// within [eps, 1 / eps^2]), so the product fits in 108 bits.
+__device__ __forceinline__ __int128 value_to_fixed(double t) {
+    if (t == 0.0) return 0;
+    int ex;
+    const double f = frexp(t, &ex);  // t = f * 2^ex, 0.5 <= |f| < 1
+    const long long mant = static_cast<long long>(ldexp(f, 53));  // exact
+    const int sh = ex - 53 + kValShift;
+    const __int128 q = static_cast<__int128>(mant);
+    return sh >= 0 ? (q << sh) : (sh > -127 ? (q >> (-sh)) : (mant < 0 ? -1 : 0));
+}
 
-__device__ __forceinline__ void mattes_tail_block(unsigned long long *g_joint, unsigned long long *g_marg, int B,
-                                  double mbs, double *pratio, float *pratio_f, TailOut *out,
-                                  unsigned long long *s_hist, bool clear = true) {
-    constexpr int kWaves = kTail / 64;
-    __shared__ unsigned long long s_wu[kWaves][2];
-    __shared__ double s_wd[kWaves];
-    __shared__ unsigned long long s_colpart[kTail];  // column partial sums
-    __shared__ double s_mm[256], s_fm[256], s_logfm[256];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int nbin = B * B;
-    // Fold the per-XCD copies into copy 0 (exact), staged in LDS when given:
-    // a lane's bins of all copies are loaded before any is summed. Each bin is
-    // read and written by one lane only, so the loop needs no barrier.
+// Folds the per-XCD copies into s_hist (LDS) when given, else into copy 0 (one
+// workgroup only). A lane's bins of all copies are loaded before any is summed;
+// every load is unconditional (an index past the end reads element 0 and is
+// dropped), so all of a lane's loads are in flight together. Each bin is read
+// and written by one lane only, so the loop needs no barrier.
+__device__ __forceinline__ void tail_fold(unsigned long long *g_joint, unsigned long long *g_marg,
+                                          int B, unsigned long long *s_hist) {
+    const int t = threadIdx.x, nbin = B * B, ntot = nbin + B;
     constexpr int kPer = 4;
-    for (int i0 = t; i0 < nbin + B; i0 += kTail * kPer) {
+    for (int i0 = t; i0 < ntot; i0 += kTail * kPer) {
+        const unsigned long long *src[kPer];
+        int stride[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int i = i0 + j * kTail;
+            const bool marg = i >= nbin && i < ntot;
+            src[j] = marg ? g_marg + (i - nbin) : g_joint + (i < nbin ? i : 0);
+            stride[j] = marg ? B : nbin;
+        }
         unsigned long long v[kPer] = {0, 0, 0, 0};
 #pragma unroll
         for (int x = 0; x < kHistCopies; ++x)
 #pragma unroll
-            for (int j = 0; j < kPer; ++j) {
-                const int i = i0 + j * kTail;
-                if (i < nbin) v[j] += g_joint[static_cast<size_t>(x) * nbin + i];
-                else if (i < nbin + B) v[j] += g_marg[x * B + (i - nbin)];
-            }
+            for (int j = 0; j < kPer; ++j) v[j] += src[j][x * stride[j]];
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
             const int i = i0 + j * kTail;
-            if (i >= nbin + B) continue;
+            if (i >= ntot) continue;
             if (s_hist) s_hist[i] = v[j];
             else if (i < nbin) g_joint[i] = v[j];
             else g_marg[i - nbin] = v[j];
         }
     }
-    __syncthreads();
-    const unsigned long long *hj = s_hist ? s_hist : g_joint;
-    const unsigned long long *hm = s_hist ? s_hist + nbin : g_marg;
-    // Exact integer sums: the joint mass, the valid count (fixed marginal),
-    // and per moving bin m the column sum over the fixed bins, each lane of
-    // column m (R lanes per column) summing rows grp, grp + R, ...
+}
+
+// Whole-histogram statistics every tail lane needs, from the folded
+// histogram: exact integer joint mass, valid count (fixed marginal) and
+// moving-bin column sums, then the marginal pdfs and the fixed marginal's log.
+struct TailStats {
+    unsigned long long valid;
+    double joint_sum, inv_sum, n_factor;
+    bool degenerate;
+};
+struct TailShared {
+    unsigned long long wu[kTail / 64][2];
+    unsigned long long colpart[kTail];  // column partial sums
+    double mm[256], fm[256], logfm[256];
+    long long part[kTail / 64][2];
+};
+
+__device__ __forceinline__ TailStats tail_stats(const unsigned long long *hj,
+                                                const unsigned long long *hm, int B, double mbs,
+                                                TailShared &sh) {
+    constexpr int kWaves = kTail / 64;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int nbin = B * B;
     unsigned long long mass = 0;
     for (int i = t; i < nbin; i += kTail) mass += hj[i];
     unsigned long long cnt = t < B ? hm[t] : 0ull;
+    // per moving bin m: R lanes of column m, each summing rows grp, grp + R, ...
     const int R = kTail / B;  // B <= 255 -> R >= 4
     if (t < R * B) {
         const int col = t % B, grp = t / B;
         unsigned long long c = 0;
         for (int f = grp; f < B; f += R) c += hj[f * B + col];
-        s_colpart[t] = c;
+        sh.colpart[t] = c;
     }
     mass = wave_sum_u64(mass);
     cnt = wave_sum_u64(cnt);
     if (lane == 0) {
-        s_wu[wave][0] = mass;
-        s_wu[wave][1] = cnt;
+        sh.wu[wave][0] = mass;
+        sh.wu[wave][1] = cnt;
     }
     __syncthreads();
     unsigned long long tot = 0, valid = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {
-        tot += s_wu[w][0];
-        valid += s_wu[w][1];
+        tot += sh.wu[w][0];
+        valid += sh.wu[w][1];
     }
-    const double joint_sum = static_cast<double>(tot) * kInvFix;
-    const bool degenerate = valid == 0 || joint_sum < DBL_EPSILON;
-    const double inv_sum = degenerate ? 0.0 : 1.0 / joint_sum;
-    const double n_factor = degenerate ? 0.0 : 1.0 / (mbs * static_cast<double>(valid));
+    TailStats st;
+    st.valid = valid;
+    st.joint_sum = static_cast<double>(tot) * kInvFix;
+    st.degenerate = valid == 0 || st.joint_sum < DBL_EPSILON;
+    st.inv_sum = st.degenerate ? 0.0 : 1.0 / st.joint_sum;
+    st.n_factor = st.degenerate ? 0.0 : 1.0 / (mbs * static_cast<double>(valid));
     if (t < B) {
-        const double fm = static_cast<double>(hm[t]) * inv_sum;
-        s_fm[t] = fm;
-        s_logfm[t] = fm > DBL_EPSILON ? log(fm) : 0.0;
+        const double fm = static_cast<double>(hm[t]) * st.inv_sum;
+        sh.fm[t] = fm;
+        sh.logfm[t] = fm > DBL_EPSILON ? log(fm) : 0.0;
         unsigned long long col = 0;
-        for (int g = 0; g < R; ++g) col += s_colpart[g * B + t];
-        s_mm[t] = static_cast<double>(col) * kInvFix * inv_sum;
+        for (int g = 0; g < R; ++g) col += sh.colpart[g * B + t];
+        sh.mm[t] = static_cast<double>(col) * kInvFix * st.inv_sum;
     }
     __syncthreads();
-    double part = 0.0;
-    for (int i = t; i < nbin; i += kTail) {
+    return st;
+}
+
+// pRatio of bins [i0, i1) (lane t: i0 + t, i0 + t + kTail, ...) and the
+// workgroup's exact value sum over them (valid in lane 0 of wave 0 after
+// the barrier inside).
+__device__ __forceinline__ __int128 tail_bins(const unsigned long long *hj, int B, int i0, int i1,
+                                              const TailStats &st, double *pratio, float *pratio_f,
+                                              TailShared &sh) {
+    constexpr int kWaves = kTail / 64;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    __int128 part = 0;
+    for (int i = i0 + t; i < i1; i += kTail) {
         double pr_out = 0.0;
         const unsigned long long q = hj[i];
-        if (q && !degenerate) {
+        if (q && !st.degenerate) {
             const int f = i / B, m = i - f * B;
-            const double jp = static_cast<double>(q) * kInvFix * inv_sum;
-            if (s_fm[f] > DBL_EPSILON && s_mm[m] > DBL_EPSILON && jp > DBL_EPSILON) {
-                const double pr = log(jp / s_mm[m]);
-                part += jp * (pr - s_logfm[f]);
-                pr_out = pr * n_factor;
+            const double jp = static_cast<double>(q) * kInvFix * st.inv_sum;
+            if (sh.fm[f] > DBL_EPSILON && sh.mm[m] > DBL_EPSILON && jp > DBL_EPSILON) {
+                const double pr = log(jp / sh.mm[m]);
+                part += value_to_fixed(jp * (pr - sh.logfm[f]));
+                pr_out = pr * st.n_factor;
             }
         }
         pratio[i] = pr_out;
         pratio_f[i] = static_cast<float>(pr_out);
     }
-    part = wave_sum_f64(part);
-    if (lane == 0) s_wd[wave] = part;
-    // every read of the histogram is done once all lanes pass this barrier:
-    // clear it for the next evaluation
+    part = wave_sum_i128(part);
+    if (lane == 0) {
+        sh.part[wave][0] = static_cast<long long>(static_cast<unsigned long long>(part));
+        sh.part[wave][1] = static_cast<long long>(part >> 64);
+    }
     __syncthreads();
+    __int128 sum = 0;
+    if (t == 0)
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w)
+            sum += static_cast<__int128>((static_cast<unsigned __int128>(static_cast<unsigned long long>(sh.part[w][1])) << 64) |
+                                         static_cast<unsigned long long>(sh.part[w][0]));
+    return sum;
+}
+
+__device__ __forceinline__ void tail_finish(__int128 sum, const TailStats &st, TailOut *out) {
+    out->value = st.degenerate ? DBL_MAX : -ldexp(static_cast<double>(sum), -kValShift);
+    out->valid = static_cast<double>(st.valid);
+    out->joint_sum = st.joint_sum;
+}
+
+// The whole tail in one workgroup of kTail lanes; clears the histogram copies
+// for the next evaluation when asked.
+__device__ __forceinline__ void mattes_tail_block(unsigned long long *g_joint, unsigned long long *g_marg, int B,
+                                  double mbs, double *pratio, float *pratio_f, TailOut *out,
+                                  unsigned long long *s_hist, bool clear = true) {
+    __shared__ TailShared sh;
+    const int t = threadIdx.x, nbin = B * B;
+    tail_fold(g_joint, g_marg, B, s_hist);
+    __syncthreads();
+    const unsigned long long *hj = s_hist ? s_hist : g_joint;
+    const unsigned long long *hm = s_hist ? s_hist + nbin : g_marg;
+    const TailStats st = tail_stats(hj, hm, B, mbs, sh);
+    const __int128 sum = tail_bins(hj, B, 0, nbin, st, pratio, pratio_f, sh);
+    // every read of the histogram is done once all lanes passed the barrier
+    // inside tail_bins: clear it for the next evaluation
     if (clear) {
         for (int i = t; i < kHistCopies * nbin; i += kTail) g_joint[i] = 0ull;
         for (int i = t; i < kHistCopies * B; i += kTail) g_marg[i] = 0ull;
     }
-    if (t == 0) {
-        double sum = 0.0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) sum += s_wd[w];
-        out->value = degenerate ? DBL_MAX : -sum;
-        out->valid = static_cast<double>(valid);
-        out->joint_sum = joint_sum;
-    }
+    if (t == 0) tail_finish(sum, st, out);
 }
 // ------------------------------------------------------------------ host side
 struct HostResult {
@@ -675,6 +914,9 @@ struct Metric {
     DeviceBuffer<unsigned long long> joint, marg;
     DeviceBuffer<double> pratio;
     DeviceBuffer<float> pratio_f;
+    DeviceBuffer<unsigned long long> tail_part;  // the split tail's partial value sums
+    DeviceBuffer<unsigned> tail_ticket;          // its completion count (zero between tails)
+    DeviceBuffer<unsigned char> recs;            // per-sample records of the value pass
     ~Metric() { pinned_slot_free(res); }
     double fixed_range[2] = {0, 0};
     double moving_range[2] = {0, 0};
