@@ -203,10 +203,8 @@ __global__ __launch_bounds__(kBlock) void batch_fold(const PairDesc *__restrict_
     }
 #pragma unroll
     for (int k = 0; k < 14; ++k) {
-        double x = t[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-        if (lane == 0) fold[k][wave] = x;
+        const double x = wave_total_f64(t[k]);
+        if (lane == 63) fold[k][wave] = x;
     }
     __syncthreads();
     if (threadIdx.x < 14) {

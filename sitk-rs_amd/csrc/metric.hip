@@ -50,6 +50,24 @@
 
 namespace sitkhip {
 
+// Occupancy requests (waves per SIMD) for the two sample passes; 0: none.
+#ifndef SITKHIP_HIST_WAVES
+#define SITKHIP_HIST_WAVES 0
+#endif
+#ifndef SITKHIP_DERIV_WAVES
+#define SITKHIP_DERIV_WAVES 0
+#endif
+#if SITKHIP_HIST_WAVES > 0
+#define SITKHIP_HIST_OCC __attribute__((amdgpu_waves_per_eu(SITKHIP_HIST_WAVES)))
+#else
+#define SITKHIP_HIST_OCC
+#endif
+#if SITKHIP_DERIV_WAVES > 0
+#define SITKHIP_DERIV_OCC __attribute__((amdgpu_waves_per_eu(SITKHIP_DERIV_WAVES)))
+#else
+#define SITKHIP_DERIV_OCC
+#endif
+
 // -------------------------------------------------------------- mean squares
 template <typename R, bool MOMENTS, typename Map>
 __global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Map S, IndexMap X,
@@ -72,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Map 
 // Pass 1: joint histogram (fixed point) + fixed-marginal counts; with REC
 // also the per-sample records for the derivative pass that follows.
 template <typename R, bool LDS, bool REC, typename Map>
-__global__ __launch_bounds__(kBlock) void mattes_hist(FixedGrid F, MovingVol M, Map S,
+__global__ __launch_bounds__(kBlock) SITKHIP_HIST_OCC void mattes_hist(FixedGrid F, MovingVol M, Map S,
                                                       IndexMap X, const ChainConst *__restrict__ C,
                                                       SampleSet SS, const uint8_t *__restrict__ fbin,
                                                       MattesParams P,
@@ -201,41 +219,36 @@ __global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M,
 }
 
 // Pass 2 from the value pass's records (same transform): the same moments
-// as mattes_deriv, in the same order.
-template <typename R, bool LDS>
-__global__ __launch_bounds__(kBlock) void mattes_deriv_rec(FixedGrid F, MovingVol M,
+// as mattes_deriv, in the same order. The pRatio table is read from global
+// memory: a few KB read four times per sample stays in the vector cache, and
+// not staging it spares the workgroup a load round trip and a barrier.
+template <typename R>
+__global__ __launch_bounds__(kBlock) SITKHIP_DERIV_OCC void mattes_deriv_rec(FixedGrid F, MovingVol M,
                                                            const ChainConst *__restrict__ C,
                                                            SampleSet SS, const uint8_t *__restrict__ fbin,
                                                            MattesParams P,
-                                                           const R *__restrict__ g_pratio,
+                                                           const R *__restrict__ pratio,
                                                            const SampleRec<R> *__restrict__ rec,
                                                            double *__restrict__ partials,
                                                            unsigned long long *__restrict__ clr,
                                                            unsigned clr_n,
                                                            unsigned long long *__restrict__ clr2,
                                                            unsigned clr2_n) {
-    extern __shared__ unsigned char s_raw[];
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    // the first round's loads go out before the pRatio staging waits
     RecRound<R> first;
     load_rec_round<R>(SS, fbin, rec, begin + threadIdx.x, begin, end, first);
+    // the value pass's histogram copies, read by the tail, are cleared here
+    // for the next evaluation (every workgroup a slice)
     for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < clr_n; i += gridDim.x * kBlock)
         clr[i] = 0ull;
     for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < clr2_n; i += gridDim.x * kBlock)
         clr2[i] = 0ull;
-    R *s_pr = reinterpret_cast<R *>(s_raw);
-    const int B = P.bins;
-    const R *pr = LDS ? s_pr : g_pratio;
-    if (LDS) {
-        for (int i = threadIdx.x; i < B * B; i += kBlock) s_pr[i] = g_pratio[i];
-        __syncthreads();
-    }
     double acc[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = 0.0;
-    deriv_records<R>(F, M, C, SS, fbin, P, pr, rec, begin, end, first, acc);
+    deriv_records<R>(F, M, C, SS, fbin, P, pratio, rec, begin, end, first, acc);
     block_reduce_store<12>(acc, partials, logical);
 }
 
@@ -784,19 +797,11 @@ static void launch_deriv(Metric &m, const Stages &S, const R *pratio, bool clear
 // clears the histogram copies the tail left.
 template <typename R>
 static void launch_deriv_rec(Metric &m, const R *pratio, const SampleRec<R> *rec) {
-    const int B = m.P.bins;
-    const size_t shm = static_cast<size_t>(B) * B * sizeof(R);
-    const bool lds = shm + 2048 <= 64 * 1024;  // plus the reduction's static LDS
     {
         LaunchScope scope("mattes_deriv");
-        auto go = [&](auto kernel, size_t bytes) {
-            hipLaunchKernelGGL(kernel, dim3(kGrid), dim3(kBlock), bytes, backend().stream, m.F, m.Mv,
-                               m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio, rec, m.partials.ptr,
-                               m.joint.ptr, static_cast<unsigned>(m.joint.len), m.marg.ptr,
-                               static_cast<unsigned>(m.marg.len));
-        };
-        if (lds) go(mattes_deriv_rec<R, true>, shm);
-        else go(mattes_deriv_rec<R, false>, 0);
+        hipLaunchKernelGGL(mattes_deriv_rec<R>, dim3(kGrid), dim3(kBlock), 0, backend().stream, m.F, m.Mv,
+                           m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio, rec, m.partials.ptr, m.joint.ptr,
+                           static_cast<unsigned>(m.joint.len), m.marg.ptr, static_cast<unsigned>(m.marg.len));
         check_launch("mattes_deriv");
     }
     launch_fold<12>(m);

@@ -201,7 +201,32 @@ __device__ __forceinline__ unsigned long long to_fixed<float>(float v) {
     return v > 0.0f ? static_cast<unsigned long long>(__float2uint_rn(v * 4294967296.0f)) : 0ull;
 }
 // ------------------------------------------------------------ FP64 reduction
-// Per-workgroup reduction of NSLOT per-lane FP64 sums (fixed shuffle pattern +
+// A double moved across lanes by DPP (two 32-bit halves); lanes with no
+// source lane, or in rows the row mask leaves out, read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<unsigned>(b)), CTRL,
+                                               ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<unsigned>(b >> 32)),
+                                               CTRL, ROW_MASK, 0xf, true);
+    return __longlong_as_double(static_cast<long long>(
+        (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) | static_cast<unsigned>(lo)));
+}
+// Wave64 sum in a fixed order, all in VALU: an inclusive scan inside each row
+// of 16 lanes (row_shr 1, 2, 4, 8), then across rows (row_bcast 15 into rows
+// 1 and 3, row_bcast 31 into rows 2 and 3). The total is in lane 63.
+__device__ __forceinline__ double wave_total_f64(double v) {
+    v += dpp_f64<0x111, 0xf>(v);
+    v += dpp_f64<0x112, 0xf>(v);
+    v += dpp_f64<0x114, 0xf>(v);
+    v += dpp_f64<0x118, 0xf>(v);
+    v += dpp_f64<0x142, 0xa>(v);
+    v += dpp_f64<0x143, 0xc>(v);
+    return v;
+}
+
+// Per-workgroup reduction of NSLOT per-lane FP64 sums (fixed DPP pattern +
 // LDS) into partials[logical * NSLOT + k]. A separate one-workgroup launch
 // folds the partials in logical-workgroup (= sample) order. Both trees are
 // fixed, so the result is identical on every run. (No cross-workgroup
@@ -216,10 +241,8 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[NSLOT], double 
     const int wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) {
-        double v = acc[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (lane == 0) red[wave][k] = v;
+        const double v = wave_total_f64(acc[k]);
+        if (lane == 63) red[wave][k] = v;
     }
     __syncthreads();
     if (threadIdx.x < NSLOT) {
@@ -262,10 +285,8 @@ __global__ __launch_bounds__(kBlock) void fold_partials(const double *__restrict
     }
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) {
-        double x = t[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-        if (lane == 0) fold[k][wave] = x;
+        const double x = wave_total_f64(t[k]);
+        if (lane == 63) fold[k][wave] = x;
     }
     __syncthreads();
     if (threadIdx.x < NSLOT) {
@@ -546,10 +567,7 @@ __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &
             parzen_values<R>(arg, mid, wv);
             unsigned long long *row = joint + b.fb[u] * B + m0;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const unsigned long long q = to_fixed(wv[k]);
-                if (q) atomicAdd(row + k, q);
-            }
+            for (int k = 0; k < 4; ++k) atomicAdd(row + k, to_fixed(wv[k]));
             atomicAdd(marg + b.fb[u], 1ull);
         }
     }
@@ -613,7 +631,10 @@ __device__ __forceinline__ void deriv_range(const FixedGrid &F, const MovingVol 
 // begin + lane, + kBlock, + 2 kBlock, ... whatever the unroll, so kRecU
 // samples' loads are issued at a time without changing the sums. `first`
 // holds the first round's loads, issued by the caller.
-constexpr int kRecU = 8;
+#ifndef SITKHIP_RECU
+#define SITKHIP_RECU 4
+#endif
+constexpr int kRecU = SITKHIP_RECU;
 
 template <typename R>
 struct RecRound {

@@ -315,12 +315,15 @@ __device__ __forceinline__ bool locate_composed(const MovingVol &M, const IndexM
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // The 8 corner values, in corner order (bit d of the corner selects base+1 on
-// axis d), with neighbour indices clamped into the volume.
+// axis d), with neighbour indices clamped into the volume. A located cell has
+// base in [-1, size - 1] on every axis (the inside test is -0.5 <= c <
+// size - 0.5), so the clamp of base is max(base, 0) and of base + 1 is
+// min(base + 1, size - 1).
 // Offsets are 32-bit: a moving volume holds fewer than 2^31 voxels.
 __device__ __forceinline__ void corners(const MovingVol &M, const Cell &cell, float v[8]) {
-    const int x0 = clampi(cell.base[0], 0, M.size[0] - 1), x1 = clampi(cell.base[0] + 1, 0, M.size[0] - 1);
-    const int y0 = clampi(cell.base[1], 0, M.size[1] - 1), y1 = clampi(cell.base[1] + 1, 0, M.size[1] - 1);
-    const int z0 = clampi(cell.base[2], 0, M.size[2] - 1), z1 = clampi(cell.base[2] + 1, 0, M.size[2] - 1);
+    const int x0 = max(cell.base[0], 0), x1 = min(cell.base[0] + 1, M.size[0] - 1);
+    const int y0 = max(cell.base[1], 0), y1 = min(cell.base[1] + 1, M.size[1] - 1);
+    const int z0 = max(cell.base[2], 0), z1 = min(cell.base[2] + 1, M.size[2] - 1);
     const int s1 = static_cast<int>(M.stride1), s2 = static_cast<int>(M.stride2);
     const int r00 = y0 * s1 + z0 * s2;
     const int r10 = y1 * s1 + z0 * s2;
