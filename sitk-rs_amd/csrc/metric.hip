@@ -124,59 +124,50 @@ __global__ __launch_bounds__(kBlock) SITKHIP_HIST_OCC void mattes_hist(FixedGrid
     }
 }
 
-// The tail split over T = gridDim.x workgroups (T <= kMaxTailSplit): every
-// workgroup folds the whole histogram into its LDS and takes the statistics,
-// then computes the pRatio of its share of the bins and its exact partial
-// value sum. The workgroup that finishes last (a device-scope ticket, which it
-// resets) adds the T partials, writes the result and clears the copies. Without
-// staging (bin counts whose histogram does not fit in LDS) there is one
-// workgroup, reading global memory, and the fold goes into copy 0.
+// The tail in one workgroup (value-only evaluations, and bin counts whose
+// histogram does not fit in LDS: then it reads global memory and folds the
+// copies into copy 0).
 template <bool STAGE>
 __global__ __launch_bounds__(kTail) void mattes_tail_kernel(unsigned long long *g_joint,
                                                              unsigned long long *g_marg, int B,
                                                              double mbs, double *pratio,
                                                              float *pratio_f, TailOut *out,
-                                                             unsigned long long *g_part,
-                                                             unsigned *ticket, int clear) {
+                                                             int clear) {
     extern __shared__ unsigned long long s_stage[];
+    mattes_tail_block(g_joint, g_marg, B, mbs, pratio, pratio_f, out, STAGE ? s_stage : nullptr,
+                      clear != 0);
+}
+
+// The tail of a value+derivative evaluation split over T = gridDim.x
+// workgroups: each folds the whole histogram into its LDS and takes the
+// statistics, then computes the pRatio of its share of the bins and its exact
+// partial value sum (part[2b], part[2b + 1]); workgroup 0 writes the valid
+// count and joint mass, and part[2T] = 1 for a degenerate histogram. The
+// derivative pass that follows adds the partials (finish_split_tail) and
+// clears the histogram copies.
+__global__ __launch_bounds__(kTail) void mattes_tail_split(unsigned long long *g_joint,
+                                                            unsigned long long *g_marg, int B,
+                                                            double mbs, double *pratio,
+                                                            float *pratio_f, TailOut *out,
+                                                            unsigned long long *part) {
+    extern __shared__ unsigned long long s_hist[];
     __shared__ TailShared sh;
-    __shared__ int s_last;
     const int t = threadIdx.x, nbin = B * B, T = static_cast<int>(gridDim.x);
-    unsigned long long *s_hist = STAGE ? s_stage : nullptr;
     tail_fold(g_joint, g_marg, B, s_hist);
     __syncthreads();
-    const unsigned long long *hj = STAGE ? s_hist : g_joint;
-    const unsigned long long *hm = STAGE ? s_hist + nbin : g_marg;
-    const TailStats st = tail_stats(hj, hm, B, mbs, sh);
+    const TailStats st = tail_stats(s_hist, s_hist + nbin, B, mbs, sh);
     const int per = (nbin + T - 1) / T;
     const int i0 = static_cast<int>(blockIdx.x) * per;
     const int i1 = i0 + per < nbin ? i0 + per : nbin;
-    const __int128 sum = tail_bins(hj, B, i0, i1, st, pratio, pratio_f, sh);
+    const __int128 sum = tail_bins(s_hist, B, i0, i1, st, pratio, pratio_f, sh);
     if (t == 0) {
-        g_part[2 * blockIdx.x] = static_cast<unsigned long long>(sum);
-        g_part[2 * blockIdx.x + 1] = static_cast<unsigned long long>(sum >> 64);
-        __threadfence();
-        s_last = atomicAdd(ticket, 1u) == static_cast<unsigned>(T - 1);
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    // every workgroup's reads of the copies ended before its ticket
-    if (clear) {
-        for (int i = t; i < kHistCopies * nbin; i += kTail) g_joint[i] = 0ull;
-        for (int i = t; i < kHistCopies * B; i += kTail) g_marg[i] = 0ull;
-    }
-    if (t == 0) {
-        __int128 total = 0;
-        for (int b = 0; b < T; ++b) {
-            const unsigned long long lo = __hip_atomic_load(g_part + 2 * b, __ATOMIC_RELAXED,
-                                                            __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned long long hi = __hip_atomic_load(g_part + 2 * b + 1, __ATOMIC_RELAXED,
-                                                            __HIP_MEMORY_SCOPE_AGENT);
-            total += static_cast<__int128>((static_cast<unsigned __int128>(hi) << 64) | lo);
+        part[2 * blockIdx.x] = static_cast<unsigned long long>(sum);
+        part[2 * blockIdx.x + 1] = static_cast<unsigned long long>(sum >> 64);
+        if (blockIdx.x == 0) {
+            part[2 * T] = st.degenerate ? 1ull : 0ull;
+            out->valid = static_cast<double>(st.valid);
+            out->joint_sum = st.joint_sum;
         }
-        tail_finish(total, st, out);
-        *ticket = 0u;
     }
 }
 
@@ -233,12 +224,15 @@ __global__ __launch_bounds__(kBlock) SITKHIP_DERIV_OCC void mattes_deriv_rec(Fix
                                                            unsigned long long *__restrict__ clr,
                                                            unsigned clr_n,
                                                            unsigned long long *__restrict__ clr2,
-                                                           unsigned clr2_n) {
+                                                           unsigned clr2_n,
+                                                           const unsigned long long *__restrict__ tail_part,
+                                                           int tail_T, TailOut *tail_out) {
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
     RecRound<R> first;
     load_rec_round<R>(SS, fbin, rec, begin + threadIdx.x, begin, end, first);
+    if (tail_T > 0 && blockIdx.x == 0 && threadIdx.x == 0) finish_split_tail(tail_part, tail_T, tail_out);
     // the value pass's histogram copies, read by the tail, are cleared here
     // for the next evaluation (every workgroup a slice)
     for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < clr_n; i += gridDim.x * kBlock)
@@ -676,9 +670,7 @@ void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b) {
     m.marg.alloc(static_cast<size_t>(kHistCopies) * B);
     m.pratio.alloc(static_cast<size_t>(B) * B);
     m.pratio_f.alloc(static_cast<size_t>(B) * B);
-    m.tail_part.alloc(2 * kMaxTailSplit);
-    m.tail_ticket.alloc(1);
-    HIP_OK(hipMemsetAsync(m.tail_ticket.ptr, 0, m.tail_ticket.bytes(), backend().stream));
+    m.tail_part.alloc(2 * kMaxTailSplit + 1);
     HIP_OK(hipMemsetAsync(m.joint.ptr, 0, m.joint.bytes(), backend().stream));
     HIP_OK(hipMemsetAsync(m.marg.ptr, 0, m.marg.bytes(), backend().stream));
     m.bins_set = true;
@@ -708,19 +700,35 @@ void launch_tail(Metric &m, TailOut *out, bool clear) {
     // staged histogram plus the tail's own ~15 KB of LDS within 64 KB
     const bool lds = B <= 76;
     const size_t shm = lds ? static_cast<size_t>(B * B + B) * sizeof(unsigned long long) : 0;
-    // about one bin per lane; one workgroup without staging
-    int T = lds ? (B * B + kTail - 1) / kTail : 1;
-    if (T > kMaxTailSplit) T = kMaxTailSplit;
     LaunchScope scope("mattes_tail");
     if (lds)
-        hipLaunchKernelGGL((mattes_tail_kernel<true>), dim3(T), dim3(kTail), shm, backend().stream,
+        hipLaunchKernelGGL((mattes_tail_kernel<true>), dim3(1), dim3(kTail), shm, backend().stream,
                            m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out,
-                           m.tail_part.ptr, m.tail_ticket.ptr, clear ? 1 : 0);
+                           clear ? 1 : 0);
     else
         hipLaunchKernelGGL((mattes_tail_kernel<false>), dim3(1), dim3(kTail), 0, backend().stream,
                            m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out,
-                           m.tail_part.ptr, m.tail_ticket.ptr, clear ? 1 : 0);
+                           clear ? 1 : 0);
     check_launch("mattes_tail");
+}
+
+// The split tail when the histogram fits in LDS (returns its workgroup count
+// for the derivative pass to finish), else the one-workgroup tail leaving the
+// histogram for the derivative pass to clear (returns 0).
+static int launch_tail_for_derivative(Metric &m, TailOut *out) {
+    const int B = m.P.bins;
+    if (B > 76) {
+        launch_tail(m, out, false);
+        return 0;
+    }
+    const size_t shm = static_cast<size_t>(B * B + B) * sizeof(unsigned long long);
+    int T = (B * B + kTail - 1) / kTail;  // about one bin per lane
+    if (T > kMaxTailSplit) T = kMaxTailSplit;
+    LaunchScope scope("mattes_tail");
+    hipLaunchKernelGGL(mattes_tail_split, dim3(T), dim3(kTail), shm, backend().stream, m.joint.ptr,
+                       m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out, m.tail_part.ptr);
+    check_launch("mattes_tail");
+    return T;
 }
 
 // tail: run the tail after the value pass; clear: the tail clears the
@@ -796,12 +804,13 @@ static void launch_deriv(Metric &m, const Stages &S, const R *pratio, bool clear
 // The derivative pass from the records of the value pass just run; it also
 // clears the histogram copies the tail left.
 template <typename R>
-static void launch_deriv_rec(Metric &m, const R *pratio, const SampleRec<R> *rec) {
+static void launch_deriv_rec(Metric &m, const R *pratio, const SampleRec<R> *rec, int tail_T) {
     {
         LaunchScope scope("mattes_deriv");
         hipLaunchKernelGGL(mattes_deriv_rec<R>, dim3(kGrid), dim3(kBlock), 0, backend().stream, m.F, m.Mv,
                            m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio, rec, m.partials.ptr, m.joint.ptr,
-                           static_cast<unsigned>(m.joint.len), m.marg.ptr, static_cast<unsigned>(m.marg.len));
+                           static_cast<unsigned>(m.joint.len), m.marg.ptr, static_cast<unsigned>(m.marg.len),
+                           m.tail_part.ptr, tail_T, &m.res_dev->tail);
         check_launch("mattes_deriv");
     }
     launch_fold<12>(m);
@@ -884,12 +893,14 @@ void mattes_value_and_moments(Metric &m, const double *stages, int nstage, doubl
     Stages S = make_stages(stages, nstage);
     if (m.precision == SITKHIP_PRECISION_F64) {
         SampleRec<double> *rec = sample_records<double>(m);
-        launch_hist<double>(m, S, true, false, rec);
-        launch_deriv_rec<double>(m, m.pratio.ptr, rec);
+        launch_hist<double>(m, S, false, false, rec);
+        const int T = launch_tail_for_derivative(m, &m.res_dev->tail);
+        launch_deriv_rec<double>(m, m.pratio.ptr, rec, T);
     } else {
         SampleRec<float> *rec = sample_records<float>(m);
-        launch_hist<float>(m, S, true, false, rec);
-        launch_deriv_rec<float>(m, m.pratio_f.ptr, rec);
+        launch_hist<float>(m, S, false, false, rec);
+        const int T = launch_tail_for_derivative(m, &m.res_dev->tail);
+        launch_deriv_rec<float>(m, m.pratio_f.ptr, rec, T);
     }
     sync();
     const TailOut &t = m.res->tail;

@@ -715,11 +715,6 @@ constexpr int kTail = 1024;
 constexpr int kValShift = 100;
 constexpr int kMaxTailSplit = 16;
 
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 __device__ __forceinline__ __int128 wave_sum_i128(__int128 v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -787,41 +782,62 @@ struct TailStats {
 };
 struct TailShared {
     unsigned long long wu[kTail / 64][2];
-    unsigned long long colpart[kTail];  // column partial sums
+    unsigned long long colsum[256];
     double mm[256], fm[256], logfm[256];
     long long part[kTail / 64][2];
 };
 
+// A 64-bit integer moved across lanes by DPP, as dpp_f64.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<unsigned>(v)), CTRL,
+                                               ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<unsigned>(v >> 32)),
+                                               CTRL, ROW_MASK, 0xf, true);
+    return (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) | static_cast<unsigned>(lo);
+}
+// Wave64 integer sum (exact); the total is in lane 63.
+__device__ __forceinline__ unsigned long long wave_total_u64(unsigned long long v) {
+    v += dpp_u64<0x111, 0xf>(v);
+    v += dpp_u64<0x112, 0xf>(v);
+    v += dpp_u64<0x114, 0xf>(v);
+    v += dpp_u64<0x118, 0xf>(v);
+    v += dpp_u64<0x142, 0xa>(v);
+    v += dpp_u64<0x143, 0xc>(v);
+    return v;
+}
+
+// Column sums: wave w takes moving bins w, w + 16, ..., its lanes the fixed
+// bins of a column, summed across the wave; the joint mass is the sum of the
+// column sums and the valid count that of the fixed marginal (all exact
+// integers, so the order does not matter).
 __device__ __forceinline__ TailStats tail_stats(const unsigned long long *hj,
                                                 const unsigned long long *hm, int B, double mbs,
                                                 TailShared &sh) {
     constexpr int kWaves = kTail / 64;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int nbin = B * B;
     unsigned long long mass = 0;
-    for (int i = t; i < nbin; i += kTail) mass += hj[i];
-    unsigned long long cnt = t < B ? hm[t] : 0ull;
-    // per moving bin m: R lanes of column m, each summing rows grp, grp + R, ...
-    const int R = kTail / B;  // B <= 255 -> R >= 4
-    if (t < R * B) {
-        const int col = t % B, grp = t / B;
+    for (int m = wave; m < B; m += kWaves) {
         unsigned long long c = 0;
-        for (int f = grp; f < B; f += R) c += hj[f * B + col];
-        sh.colpart[t] = c;
+        for (int f = lane; f < B; f += 64) c += hj[f * B + m];
+        c = wave_total_u64(c);
+        if (lane == 63) sh.colsum[m] = c;
+        mass += c;
     }
-    mass = wave_sum_u64(mass);
-    cnt = wave_sum_u64(cnt);
-    if (lane == 0) {
+    unsigned long long cnt = 0;
+    if (wave == kWaves - 1) {
+        for (int f = lane; f < B; f += 64) cnt += hm[f];
+        cnt = wave_total_u64(cnt);
+    }
+    if (lane == 63) {
         sh.wu[wave][0] = mass;
         sh.wu[wave][1] = cnt;
     }
     __syncthreads();
-    unsigned long long tot = 0, valid = 0;
+    unsigned long long tot = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-        tot += sh.wu[w][0];
-        valid += sh.wu[w][1];
-    }
+    for (int w = 0; w < kWaves; ++w) tot += sh.wu[w][0];
+    const unsigned long long valid = sh.wu[kWaves - 1][1];
     TailStats st;
     st.valid = valid;
     st.joint_sum = static_cast<double>(tot) * kInvFix;
@@ -832,9 +848,7 @@ __device__ __forceinline__ TailStats tail_stats(const unsigned long long *hj,
         const double fm = static_cast<double>(hm[t]) * st.inv_sum;
         sh.fm[t] = fm;
         sh.logfm[t] = fm > DBL_EPSILON ? log(fm) : 0.0;
-        unsigned long long col = 0;
-        for (int g = 0; g < R; ++g) col += sh.colpart[g * B + t];
-        sh.mm[t] = static_cast<double>(col) * kInvFix * st.inv_sum;
+        sh.mm[t] = static_cast<double>(sh.colsum[t]) * kInvFix * st.inv_sum;
     }
     __syncthreads();
     return st;
@@ -879,10 +893,23 @@ __device__ __forceinline__ __int128 tail_bins(const unsigned long long *hj, int 
     return sum;
 }
 
+__device__ __forceinline__ double tail_value(__int128 sum, bool degenerate) {
+    return degenerate ? DBL_MAX : -ldexp(static_cast<double>(sum), -kValShift);
+}
 __device__ __forceinline__ void tail_finish(__int128 sum, const TailStats &st, TailOut *out) {
-    out->value = st.degenerate ? DBL_MAX : -ldexp(static_cast<double>(sum), -kValShift);
+    out->value = tail_value(sum, st.degenerate);
     out->valid = static_cast<double>(st.valid);
     out->joint_sum = st.joint_sum;
+}
+// The value of a tail split over T workgroups, from their exact partial sums
+// (part[2b], part[2b + 1]: low and high words) and the degenerate flag
+// (part[2T]); one lane, in a later kernel.
+__device__ __forceinline__ void finish_split_tail(const unsigned long long *part, int T,
+                                                  TailOut *out) {
+    __int128 total = 0;
+    for (int b = 0; b < T; ++b)
+        total += static_cast<__int128>((static_cast<unsigned __int128>(part[2 * b + 1]) << 64) | part[2 * b]);
+    out->value = tail_value(total, part[2 * T] != 0);
 }
 
 // The whole tail in one workgroup of kTail lanes; clears the histogram copies
@@ -935,8 +962,7 @@ struct Metric {
     DeviceBuffer<unsigned long long> joint, marg;
     DeviceBuffer<double> pratio;
     DeviceBuffer<float> pratio_f;
-    DeviceBuffer<unsigned long long> tail_part;  // the split tail's partial value sums
-    DeviceBuffer<unsigned> tail_ticket;          // its completion count (zero between tails)
+    DeviceBuffer<unsigned long long> tail_part;  // the split tail's partial value sums + flag
     DeviceBuffer<unsigned char> recs;            // per-sample records of the value pass
     ~Metric() { pinned_slot_free(res); }
     double fixed_range[2] = {0, 0};
