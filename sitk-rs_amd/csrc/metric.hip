@@ -230,8 +230,10 @@ __global__ __launch_bounds__(kBlock) SITKHIP_DERIV_OCC void mattes_deriv_rec(Fix
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
+    // (a workgroup whose slice is empty -- begin >= end when the slices
+    // overrun n -- loads nothing: begin is not a sample then)
     RecRound<R> first;
-    load_rec_round<R>(SS, fbin, rec, begin + threadIdx.x, begin, end, first);
+    if (begin < end) load_rec_round<R>(SS, fbin, rec, begin + threadIdx.x, begin, end, first);
     if (tail_T > 0 && blockIdx.x == 0 && threadIdx.x == 0) finish_split_tail(tail_part, tail_T, tail_out);
     // the value pass's histogram copies, read by the tail, are cleared here
     // for the next evaluation (every workgroup a slice)

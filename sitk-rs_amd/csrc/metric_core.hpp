@@ -298,9 +298,13 @@ __global__ __launch_bounds__(kBlock) void fold_partials(const double *__restrict
 }
 
 // Samples per lane per iteration: all U samples' loads are issued before any of
-// their values is used, so each lane keeps U independent gathers in flight.
+// their values is used, so each lane keeps U independent gathers in flight. A
+// lane visits samples begin + lane, + kBlock, + 2 kBlock, ... for any U, so U
+// changes no sum. One is fastest on gfx950 with the 8 corner gathers of a
+// sample already in flight together (measured: U = 2 and 4 are 3 us slower on
+// a 2M-sample value pass).
 #ifndef SITKHIP_BATCH
-#define SITKHIP_BATCH 2
+#define SITKHIP_BATCH 1
 #endif
 constexpr int kU = SITKHIP_BATCH;
 
@@ -643,6 +647,8 @@ struct RecRound {
     int fb[kRecU];
 };
 
+// Samples s0, s0 + kBlock, ...; those at or past `end` read sample `begin`
+// instead (so begin < end is required) and are marked absent.
 template <typename R>
 __device__ __forceinline__ void load_rec_round(const SampleSet &SS, const uint8_t *__restrict__ fbin,
                                                const SampleRec<R> *__restrict__ rec, long long s0,

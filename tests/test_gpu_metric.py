@@ -346,3 +346,28 @@ def test_bin_counts_across_the_lds_limits(hip, precision, bins):
     tv, td = (1e-9, 1e-7) if precision == 1 else (1e-6, 1e-4)
     assert abs(v - want_v) <= tv * abs(want_v), (v, want_v)
     assert rel_vec(contract(mom, t), want_d) < td
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+@pytest.mark.parametrize("n", [7, 63, 1030, 2049, 3075])
+def test_sample_counts_that_leave_workgroups_empty(hip, precision, n):
+    """Sample sets whose per-workgroup slices overrun n (ceil(n / G) * G well
+    past n, G the fixed workgroup count), so the last workgroups have no
+    samples: the record-driven derivative pass must load nothing for them.
+    Value, derivative and valid count against the oracle."""
+    f = blob_volume((16, 18, 20), 5, spacing=SP)
+    m = blob_volume((16, 18, 20), 6, spacing=SP) * 0.5 + f * 0.5
+    rng = np.random.default_rng(n)
+    idx = np.sort(rng.choice(f.size, size=n, replace=False)).astype(np.uint32)
+    d, pb = mattes_pair(hip, f, m, precision, bins=24, indices=idx)
+    g = pb.mattes_geometry(24)
+    t = O.make_transform(O.EULER3D, [0.04, -0.02, 0.03, 0.4, -0.3, 0.2], center=[5.0, 7.0, 9.0])
+    want_v, want_d, want_n = pb.mattes_evaluate(g, t)
+    v, mom, got_n = d.value_and_moments(stages_of(t))
+    assert got_n == want_n
+    if want_n == 0:
+        return
+    tv, td = (1e-9, 1e-7) if precision == 1 else (1e-6, 1e-4)
+    assert abs(v - want_v) <= tv * abs(want_v) + 1e-15, (v, want_v)
+    if np.linalg.norm(want_d) > 0:
+        assert rel_vec(contract(mom, t), want_d) < td
