@@ -96,7 +96,9 @@ __global__ __launch_bounds__(kBlock) SITKHIP_HIST_OCC void mattes_hist(FixedGrid
                                                       MattesParams P,
                                                       unsigned long long *__restrict__ g_joint,
                                                       unsigned long long *__restrict__ g_marg,
-                                                      SampleRec<R> *__restrict__ rec) {
+                                                      SampleRec<R> *__restrict__ rec,
+                                                      uint32_t *__restrict__ defer,
+                                                      unsigned defer_cap) {
     extern __shared__ unsigned long long s_hist[];  // bins^2 joint, then bins marginal
     const int B = P.bins;
     const int nb = B * B + B;
@@ -109,12 +111,25 @@ __global__ __launch_bounds__(kBlock) SITKHIP_HIST_OCC void mattes_hist(FixedGrid
         for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
         __syncthreads();
     }
+    // FP32 with an affine map: samples the composed index map cannot decide
+    // are listed (this workgroup's slice of `defer`) and run through the FP64
+    // chain after the loop
+    constexpr bool DEFER = sizeof(R) == 4 && MapTraits<Map>::affine;
+    __shared__ unsigned s_defer_n;
+    if (threadIdx.x == 0) s_defer_n = 0u;
     __shared__ double s_stage[kMaxStages * 12];
-    const StageView V = stage_to_lds(S, s_stage);
-    long long begin, end;
-    unsigned logical;
-    block_slice(SS.n, begin, end, logical);
-    hist_range<R, REC>(F, M, V, X, C, SS, fbin, P, joint, marg, rec, begin, end);
+    const StageView V = stage_to_lds(S, s_stage);  // ends with a barrier
+    long long first, step, end;
+    xcd_walk(SS.n, first, step, end);
+    uint32_t *dq = DEFER ? defer + static_cast<size_t>(blockIdx.x) * defer_cap : nullptr;
+    hist_range<R, REC, StageView, DEFER>(F, M, V, X, C, SS, fbin, P, joint, marg, rec, first, end, step, dq,
+                                         &s_defer_n);
+    if (DEFER) {
+        __syncthreads();
+        const unsigned nq = s_defer_n;
+        for (unsigned i = threadIdx.x; i < nq; i += kBlock)
+            deferred_sample<R, REC>(F, M, V, C, SS, fbin, P, joint, marg, rec, dq[i]);
+    }
     if (LDS) {
         __syncthreads();
         for (int i = threadIdx.x; i < nb; i += kBlock) {
@@ -741,15 +756,27 @@ static void launch_hist(Metric &m, const Stages &S, bool tail, bool clear = true
     const int B = m.P.bins;
     const bool lds = B <= kLdsMaxBins;
     const size_t shm = lds ? static_cast<size_t>(B * B + B) * sizeof(unsigned long long) : 0;
+    // the deferred-sample lists (FP32): one slice per workgroup, as long as
+    // the most samples xcd_walk gives one workgroup
+    uint32_t *dq = nullptr;
+    unsigned cap = 0;
+    if (sizeof(R) == 4) {
+        const long long chunk = (m.SS.n + 7) / 8;
+        const long long stride = static_cast<long long>(kHistGrid / 8) * kBlock;
+        cap = static_cast<unsigned>((chunk + stride - 1) / stride * kBlock);
+        const size_t need = static_cast<size_t>(kHistGrid) * cap;
+        if (m.defer.len < need) m.defer.alloc(need);
+        dq = m.defer.ptr;
+    }
     {
         LaunchScope scope("mattes_hist");
         const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
         with_map(S, [&](const auto &map) {
             using Map = std::decay_t<decltype(map)>;
             auto go = [&](auto kernel, size_t bytes) {
-                hipLaunchKernelGGL(kernel, dim3(kGrid), dim3(kBlock), bytes, backend().stream, m.F, m.Mv,
-                                   map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, m.joint.ptr, m.marg.ptr,
-                                   rec);
+                hipLaunchKernelGGL(kernel, dim3(kHistGrid), dim3(kBlock), bytes, backend().stream, m.F,
+                                   m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, m.joint.ptr,
+                                   m.marg.ptr, rec, dq, cap);
             };
             if (lds) {
                 if (rec) go(mattes_hist<R, true, true, Map>, shm);

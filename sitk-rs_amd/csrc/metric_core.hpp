@@ -28,6 +28,12 @@ constexpr int kBlock = SITKHIP_BLOCK;  // lanes per workgroup of the sample pass
 #define SITKHIP_GRID 1024
 #endif
 constexpr unsigned kGrid = SITKHIP_GRID;  // fixed: the reduction tree must not depend on n
+// The value pass's grid (its integer histogram does not depend on it): eight
+// XCDs times the workgroups that keep each XCD's CUs full.
+#ifndef SITKHIP_HIST_GRID
+#define SITKHIP_HIST_GRID 2048
+#endif
+constexpr unsigned kHistGrid = SITKHIP_HIST_GRID;
 constexpr int kLdsMaxBins = 90;       // bins^2 * 8 B of LDS per workgroup (<= 64 KB)
 constexpr double kFix = 4294967296.0; // 2^32 fixed-point units per histogram unit
 constexpr double kInvFix = 1.0 / 4294967296.0;
@@ -140,6 +146,22 @@ __device__ __forceinline__ void block_slice(long long n, long long &begin, long 
     begin = static_cast<long long>(logical) * per;
     end = begin + per;
     if (end > n) end = n;
+}
+
+// The value pass's walk of the samples: XCD x (workgroup mod 8) takes the
+// x-th eighth of the samples and its G/8 workgroups sweep that range
+// together, kBlock consecutive samples each per step; a lane's samples are
+// first + lane + k * step. The moving voxels one XCD reads at a time then lie
+// in a thin band of slices that stays in its L2 (an integer histogram does
+// not depend on which workgroup adds a sample).
+__device__ __forceinline__ void xcd_walk(long long n, long long &first, long long &step,
+                                         long long &end) {
+    const unsigned G = gridDim.x, b = blockIdx.x;
+    const long long chunk = (n + 7) / 8;
+    const long long xb = static_cast<long long>(b % 8u) * chunk;
+    end = xb + chunk < n ? xb + chunk : n;
+    first = xb + static_cast<long long>(b / 8u) * kBlock;
+    step = static_cast<long long>(G / 8u) * kBlock;
 }
 
 __device__ __forceinline__ uint32_t sample_flat(const SampleSet &S, long long s) {
@@ -325,6 +347,7 @@ struct Located {
     R fr[kU][3];
     float v[kU][8];
     bool ok[kU];
+    bool unsure[kU];  // DEFER: the composed map could not decide; not located
 };
 
 // c = A idx + b for the fixed grid F, the point-map stages (nstage x 12) and
@@ -413,7 +436,10 @@ struct MapTraits {
 // R = float: the composed index map decides, and the reference's FP64 chain
 // runs only for samples within kGuard of a decision boundary. R = double: the
 // chain always (the F64 path follows the reference term for term).
-template <typename R, typename Map>
+// DEFER (FP32, affine maps): a sample the composed map cannot decide is not
+// located here but flagged `unsure`, for the caller to run the chain later
+// (deferred_sample), which keeps the chain's registers out of the loop.
+template <typename R, typename Map, bool DEFER = false>
 __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol &M,
                                              const Map &S, const IndexMap &X,
                                              const ChainConst *__restrict__ C, const Batch &b,
@@ -422,12 +448,17 @@ __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
         bool done = false;
+        L.unsure[u] = false;
         if constexpr (sizeof(R) == 4 && MapTraits<Map>::affine) {
             uint32_t i0, i1, i2;
             split_index(F, b.flat[u], i0, i1, i2);
             bool ok = false;
             if (locate_composed(M, X, i0, i1, i2, ok, cell[u])) {
                 L.ok[u] = b.live[u] && ok;
+                done = true;
+            } else if constexpr (DEFER) {
+                L.unsure[u] = b.live[u];
+                L.ok[u] = false;
                 done = true;
             }
         }
@@ -519,24 +550,59 @@ struct alignas(4 * sizeof(R)) SampleRec {
     R gi[3];
 };
 
-// Mattes value pass (mattes.rs:442-486): per valid sample, the four cubic
-// B-spline taps of the moving Parzen window into row `fixed bin` of `joint`
-// (64-bit fixed point) and one count into marg[fixed bin]. With REC, also the
-// per-sample records (the value is the same: the interpolation computes it
+// One located sample of the Mattes value pass (mattes.rs:442-486): the four
+// cubic B-spline taps of the moving Parzen window into row `fixed bin` of
+// `joint` (64-bit fixed point), one count into marg[fixed bin], and with REC
+// the sample's record (the value is the same: the interpolation computes it
 // the same way with or without the gradient).
-template <typename R, bool REC, typename Map>
+template <typename R, bool REC>
+__device__ __forceinline__ void hist_sample(const MattesParams &P, unsigned long long *joint,
+                                            unsigned long long *marg, SampleRec<R> *__restrict__ rec,
+                                            long long s, bool ok, int fb, const R fr[3],
+                                            const float v[8]) {
+    R mv = R(0), gi[3] = {R(0), R(0), R(0)};
+    bool use = ok;
+    if (use) {
+        Trilinear<R>::template eval<REC>(fr, v, mv, gi);
+        use = Parzen<R>::in_range(P, mv);
+    }
+    if (REC) {
+        SampleRec<R> r;
+        r.mv = use ? mv : static_cast<R>(NAN);
+        r.gi[0] = gi[0];
+        r.gi[1] = gi[1];
+        r.gi[2] = gi[2];
+        rec[s] = r;
+    }
+    if (!use) return;
+    R arg[4], wv[4];
+    bool mid;
+    const int m0 = Parzen<R>::taps(P, mv, arg, mid);
+    parzen_values<R>(arg, mid, wv);
+    unsigned long long *row = joint + fb * P.bins + m0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) atomicAdd(row + k, to_fixed(wv[k]));
+    atomicAdd(marg + fb, 1ull);
+}
+
+// The value pass over samples begin + lane + k * step (< end). DEFER (FP32,
+// affine maps): a sample the composed index map cannot decide is appended to
+// the workgroup's list `defer` (count in *defer_n, LDS) instead, for
+// deferred_sample after the loop.
+template <typename R, bool REC, typename Map, bool DEFER = false>
 __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &M, const Map &S,
                                            const IndexMap &X, const ChainConst *__restrict__ C,
                                            const SampleSet &SS, const uint8_t *__restrict__ fbin,
                                            const MattesParams &P, unsigned long long *joint,
                                            unsigned long long *marg, SampleRec<R> *__restrict__ rec,
-                                           long long begin, long long end) {
-    const int B = P.bins;
-    for (long long s0 = begin + threadIdx.x; s0 < end; s0 += kBlock * kU) {
+                                           long long begin, long long end, long long step = kBlock,
+                                           uint32_t *__restrict__ defer = nullptr,
+                                           unsigned *defer_n = nullptr) {
+    for (long long s0 = begin + threadIdx.x; s0 < end; s0 += step * kU) {
         Batch b;
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            const long long s = s0 + static_cast<long long>(u) * kBlock;
+            const long long s = s0 + static_cast<long long>(u) * step;
             const bool in = s < end;
             const long long sc = in ? s : begin;
             b.fb[u] = fbin[sc];
@@ -544,37 +610,46 @@ __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &
             b.live[u] = in && b.fb[u] != kNoBin;
         }
         Located<R> L;
-        locate_batch<R>(F, M, S, X, C, b, L);
+        locate_batch<R, Map, DEFER>(F, M, S, X, C, b, L);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            R mv = R(0), gi[3] = {R(0), R(0), R(0)};
-            bool use = L.ok[u];
-            if (use) {
-                Trilinear<R>::template eval<REC>(L.fr[u], L.v[u], mv, gi);
-                use = Parzen<R>::in_range(P, mv);
+            const long long s = s0 + static_cast<long long>(u) * step;
+            if (s >= end) continue;
+            if (DEFER && L.unsure[u]) {
+                defer[atomicAdd(defer_n, 1u)] = static_cast<uint32_t>(s);
+                continue;
             }
-            if (REC) {
-                const long long s = s0 + static_cast<long long>(u) * kBlock;
-                if (s < end) {
-                    SampleRec<R> r;
-                    r.mv = use ? mv : static_cast<R>(NAN);
-                    r.gi[0] = gi[0];
-                    r.gi[1] = gi[1];
-                    r.gi[2] = gi[2];
-                    rec[s] = r;
-                }
-            }
-            if (!use) continue;
-            R arg[4], wv[4];
-            bool mid;
-            const int m0 = Parzen<R>::taps(P, mv, arg, mid);
-            parzen_values<R>(arg, mid, wv);
-            unsigned long long *row = joint + b.fb[u] * B + m0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) atomicAdd(row + k, to_fixed(wv[k]));
-            atomicAdd(marg + b.fb[u], 1ull);
+            hist_sample<R, REC>(P, joint, marg, rec, s, L.ok[u], b.fb[u], L.fr[u], L.v[u]);
         }
     }
+}
+
+// A deferred sample of the value pass: the reference's FP64 chain (fixed
+// point, point map, continuous index), then the same accumulation. Runs after
+// the loop, where the loop's registers are free.
+template <typename R, bool REC, typename Map>
+__device__ __forceinline__ void deferred_sample(const FixedGrid &F, const MovingVol &M, const Map &S,
+                                                const ChainConst *__restrict__ C,
+                                                const SampleSet &SS, const uint8_t *__restrict__ fbin,
+                                                const MattesParams &P, unsigned long long *joint,
+                                                unsigned long long *marg,
+                                                SampleRec<R> *__restrict__ rec, long long s) {
+    const int fb = fbin[s];
+    double x[3], p[3];
+    fixed_point(F, *C, sample_flat(SS, s), x);
+    map_point(S, x, p);
+    Cell cell;
+    const bool ok = locate(M, *C, p, cell);
+    if (!ok) {
+        cell.base[0] = cell.base[1] = cell.base[2] = 0;
+        cell.frac[0] = cell.frac[1] = cell.frac[2] = 0.0;
+    }
+    R fr[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) fr[d] = static_cast<R>(cell.frac[d]);
+    float v[8];
+    corners(M, cell, v);
+    hist_sample<R, REC>(P, joint, marg, rec, s, ok, fb, fr, v);
 }
 
 // Mattes derivative pass: per valid sample w = sum_taps pRatio * B3'(arg)
@@ -970,6 +1045,7 @@ struct Metric {
     DeviceBuffer<float> pratio_f;
     DeviceBuffer<unsigned long long> tail_part;  // the split tail's partial value sums + flag
     DeviceBuffer<unsigned char> recs;            // per-sample records of the value pass
+    DeviceBuffer<uint32_t> defer;                // the value pass's deferred-sample lists
     ~Metric() { pinned_slot_free(res); }
     double fixed_range[2] = {0, 0};
     double moving_range[2] = {0, 0};

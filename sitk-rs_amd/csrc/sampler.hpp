@@ -292,7 +292,10 @@ __device__ __forceinline__ bool locate_composed(const MovingVol &M, const IndexM
         c[r] = fma(X.A[r * 3 + 2], d2, fma(X.A[r * 3 + 1], d1, fma(X.A[r * 3 + 0], d0, X.b[r])));
         const double fl = floor(c[r]);
         const double fr = c[r] - fl;
-        sure = sure && fr >= kGuard && fr <= 1.0 - kGuard && fabs(fr - 0.5) >= kGuard;
+        // kGuard from every integer and half-integer: 2c at least 2 kGuard
+        // from every integer (2c, its rounding and their difference are exact)
+        const double t = 2.0 * c[r];
+        sure = sure && fabs(t - rint(t)) >= 2.0 * kGuard;
         inside = inside && c[r] >= -0.5 && c[r] < static_cast<double>(M.size[r]) - 0.5;
         cell.base[r] = static_cast<int>(fl);
         cell.frac[r] = fr;
@@ -319,24 +322,39 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 // base in [-1, size - 1] on every axis (the inside test is -0.5 <= c <
 // size - 0.5), so the clamp of base is max(base, 0) and of base + 1 is
 // min(base + 1, size - 1).
+// Along x the two corners are read as one 8-byte pair (p, p + 1) with
+// p = clamp(base, 0, size - 2): at base = -1 both corners are voxel 0 (the
+// pair's first), at base = size - 1 both are voxel size - 1 (its second).
 // Offsets are 32-bit: a moving volume holds fewer than 2^31 voxels.
+struct alignas(4) FloatPair {
+    float a, b;
+};
+
 __device__ __forceinline__ void corners(const MovingVol &M, const Cell &cell, float v[8]) {
-    const int x0 = max(cell.base[0], 0), x1 = min(cell.base[0] + 1, M.size[0] - 1);
     const int y0 = max(cell.base[1], 0), y1 = min(cell.base[1] + 1, M.size[1] - 1);
     const int z0 = max(cell.base[2], 0), z1 = min(cell.base[2] + 1, M.size[2] - 1);
     const int s1 = static_cast<int>(M.stride1), s2 = static_cast<int>(M.stride2);
-    const int r00 = y0 * s1 + z0 * s2;
-    const int r10 = y1 * s1 + z0 * s2;
-    const int r01 = y0 * s1 + z1 * s2;
-    const int r11 = y1 * s1 + z1 * s2;
-    v[0] = M.vol[r00 + x0];
-    v[1] = M.vol[r00 + x1];
-    v[2] = M.vol[r10 + x0];
-    v[3] = M.vol[r10 + x1];
-    v[4] = M.vol[r01 + x0];
-    v[5] = M.vol[r01 + x1];
-    v[6] = M.vol[r11 + x0];
-    v[7] = M.vol[r11 + x1];
+    const int b0 = cell.base[0], n0 = M.size[0];
+    if (n0 >= 2) {
+        const int p = min(max(b0, 0), n0 - 2);
+        const bool both_hi = b0 == n0 - 1;
+        const bool both_lo = b0 < 0;
+        const int rows[4] = {y0 * s1 + z0 * s2 + p, y1 * s1 + z0 * s2 + p, y0 * s1 + z1 * s2 + p,
+                             y1 * s1 + z1 * s2 + p};
+        FloatPair q[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q[k] = *reinterpret_cast<const FloatPair *>(M.vol + rows[k]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[2 * k] = both_hi ? q[k].b : q[k].a;
+            v[2 * k + 1] = both_lo ? q[k].a : q[k].b;
+        }
+    } else {
+        // one voxel along x: both x corners are voxel 0
+        const int rows[4] = {y0 * s1 + z0 * s2, y1 * s1 + z0 * s2, y0 * s1 + z1 * s2, y1 * s1 + z1 * s2};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[2 * k] = v[2 * k + 1] = M.vol[rows[k]];
+    }
 }
 
 // Trilinear value and index-space gradient from the in-cell fraction `fr` and
