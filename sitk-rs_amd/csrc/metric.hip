@@ -132,7 +132,11 @@ __global__ __launch_bounds__(kBlock) SITKHIP_HIST_OCC void mattes_hist(FixedGrid
     }
     if (LDS) {
         __syncthreads();
-        for (int i = threadIdx.x; i < nb; i += kBlock) {
+        // The workgroups of one XCD finish together; each starts its flush at
+        // its own bin so that they do not queue on the same addresses.
+        const int rot = static_cast<int>((blockIdx.x / kHistCopies) * 193u % static_cast<unsigned>(nb));
+        for (int k = threadIdx.x; k < nb; k += kBlock) {
+            const int i = k + rot < nb ? k + rot : k + rot - nb;
             const unsigned long long v = s_hist[i];
             if (v) atomicAdd(i < B * B ? g_joint + i : g_marg + (i - B * B), v);
         }
