@@ -259,7 +259,8 @@ __device__ __forceinline__ double wave_total_f64(double v) {
 // fixed, so the result is identical on every run. (No cross-workgroup
 // handshake inside a pass: on a multi-XCD device an agent-scope fence in every
 // workgroup writes back and invalidates its XCD's L2 and costs more than the
-// extra launch.)
+// extra launch -- measured on the record-driven derivative pass: a last-
+// workgroup fold took it from 13 to 38 us.)
 template <int NSLOT>
 __device__ __forceinline__ void block_reduce_store(double (&acc)[NSLOT], double *partials,
                                                    unsigned logical) {
@@ -281,10 +282,10 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[NSLOT], double 
 }
 
 // out[k] = sum over G partials of slot k: lane t sums rows t, t+256, ... (all
-// loads issued first), then a fixed shuffle + LDS tree. One workgroup.
+// loads issued first), then a fixed DPP + LDS tree. One workgroup.
 template <int NSLOT>
-__global__ __launch_bounds__(kBlock) void fold_partials(const double *__restrict__ partials,
-                                                        unsigned G, double *__restrict__ out) {
+__device__ __forceinline__ void fold_rows(const double *__restrict__ partials, unsigned G,
+                                          double *__restrict__ out) {
     __shared__ double fold[NSLOT][kBlock / 64];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -323,6 +324,13 @@ __global__ __launch_bounds__(kBlock) void fold_partials(const double *__restrict
         out[threadIdx.x] = x;
     }
 }
+
+template <int NSLOT>
+__global__ __launch_bounds__(kBlock) void fold_partials(const double *__restrict__ partials,
+                                                        unsigned G, double *__restrict__ out) {
+    fold_rows<NSLOT>(partials, G, out);
+}
+
 
 // Samples per lane per iteration: all U samples' loads are issued before any of
 // their values is used, so each lane keeps U independent gathers in flight. A
