@@ -37,6 +37,8 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -586,6 +588,7 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
         void *dev = nullptr;
         m->res = static_cast<HostResult *>(pinned_slot_alloc(sizeof(HostResult), &dev));
         m->res_dev = static_cast<HostResult *>(dev);
+        m->res->done = 0;
 
         // Value ranges (ops/mattes.rs:587-589): fixed over the sample set, moving
         // over its mask.
@@ -796,12 +799,35 @@ static void launch_hist(Metric &m, const Stages &S, bool tail, bool clear = true
     launch_tail(m, &m.res_dev->tail, clear);
 }
 
+// Completion numbers are unique across metrics: a result slot reused by a
+// new metric can never show a number the new metric waits for.
+static std::atomic<unsigned long long> g_eval_seq{0};
+
 template <int NS>
 static void launch_fold(Metric &m) {
     LaunchScope scope("fold_partials");
+    m.seq = g_eval_seq.fetch_add(1) + 1;
     hipLaunchKernelGGL((fold_partials<NS>), dim3(1), dim3(kBlock), 0, backend().stream,
-                       m.partials.ptr, kGrid, m.res_dev->moments);
+                       m.partials.ptr, kGrid, m.res_dev->moments, &m.res_dev->done, m.seq);
     check_launch("fold_partials");
+}
+
+// Waits for the evaluation that ended with launch_fold: spins on the result
+// block's sequence word (a few microseconds sooner than a stream wait), and
+// after 2 ms falls back to the stream's own wait, which also reports any
+// error of the launches.
+static void wait_done(Metric &m) {
+    const volatile unsigned long long *done = &m.res->done;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned n = 1;; ++n) {
+        if (*done == m.seq) {
+            std::atomic_thread_fence(std::memory_order_acquire);
+            return;
+        }
+        if ((n & 255u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
+    }
+    sync();
+    if (*done != m.seq) fail(SITKHIP_ERR_DRIVER, "evaluation finished without its completion word");
 }
 
 // clear_hist: the derivative pass also clears the histogram copies (after a
@@ -935,7 +961,7 @@ void mattes_value_and_moments(Metric &m, const double *stages, int nstage, doubl
         const int T = launch_tail_for_derivative(m, &m.res_dev->tail);
         launch_deriv_rec<float>(m, m.pratio_f.ptr, rec, T);
     }
-    sync();
+    wait_done(m);
     const TailOut &t = m.res->tail;
     if (moments) {
         std::memcpy(moments, m.res->moments, 12 * sizeof(double));
@@ -966,7 +992,7 @@ void ms_moments(Metric &m, const double *stages, int nstage, double out[14]) {
     Stages S = make_stages(stages, nstage);
     if (m.precision == SITKHIP_PRECISION_F64) launch_ms<double, true>(m, S);
     else launch_ms<float, true>(m, S);
-    sync();
+    wait_done(m);
     std::memcpy(out, m.res->moments, 14 * sizeof(double));
     if (m.precision == SITKHIP_PRECISION_F32) moments_to_physical(m.F, out + 1, out + 4);
 }
@@ -977,7 +1003,7 @@ void ms_value(Metric &m, const double *stages, int nstage, double *sq, size_t *v
     Stages S = make_stages(stages, nstage);
     if (m.precision == SITKHIP_PRECISION_F64) launch_ms<double, false>(m, S);
     else launch_ms<float, false>(m, S);
-    sync();
+    wait_done(m);
     if (sq) *sq = m.res->moments[0];
     if (valid) *valid = static_cast<size_t>(m.res->moments[1]);
 }

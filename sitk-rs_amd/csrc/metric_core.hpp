@@ -325,10 +325,20 @@ __device__ __forceinline__ void fold_rows(const double *__restrict__ partials, u
     }
 }
 
+// The fold launch, which ends an evaluation: once the sums are out, lane 0
+// (whose wave wrote them) publishes `seq` to the host-visible `done` after a
+// system-scope fence, so the host can wait on that word (wait_done) instead of
+// on the stream.
 template <int NSLOT>
 __global__ __launch_bounds__(kBlock) void fold_partials(const double *__restrict__ partials,
-                                                        unsigned G, double *__restrict__ out) {
+                                                        unsigned G, double *__restrict__ out,
+                                                        unsigned long long *done,
+                                                        unsigned long long seq) {
     fold_rows<NSLOT>(partials, G, out);
+    if (done && threadIdx.x == 0) {
+        __threadfence_system();
+        *reinterpret_cast<volatile unsigned long long *>(done) = seq;
+    }
 }
 
 
@@ -1031,6 +1041,7 @@ __device__ __forceinline__ void mattes_tail_block(unsigned long long *g_joint, u
 struct HostResult {
     TailOut tail;
     double moments[14];
+    unsigned long long done;  // the sequence number of the last finished evaluation
 };
 
 struct Metric {
@@ -1059,6 +1070,7 @@ struct Metric {
     DeviceBuffer<unsigned long long> tail_part;  // the split tail's partial value sums + flag
     DeviceBuffer<unsigned char> recs;            // per-sample records of the value pass
     DeviceBuffer<uint32_t> defer;                // the value pass's deferred-sample lists
+    unsigned long long seq = 0;                  // evaluations issued (see HostResult::done)
     ~Metric() { pinned_slot_free(res); }
     double fixed_range[2] = {0, 0};
     double moving_range[2] = {0, 0};
