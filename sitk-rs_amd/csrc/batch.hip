@@ -52,6 +52,7 @@ struct PairDesc {
     double *pratio_w;
     float *pratio_fw;
     double *partials;  // ntiles rows of 12 (Mattes) or 14 (mean squares)
+    void *rec;         // Mattes: per-sample records of the value pass (SampleRec<R>)
     long long tile;
     int kind;
     int ntiles;
@@ -112,8 +113,8 @@ __global__ __launch_bounds__(kBlock) void batch_pass1(const PairDesc *__restrict
             const int nb = B * B + B;
             for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
             __syncthreads();
-            hist_range<R, false>(d.F, d.M, S, c.X, d.C, d.SS, d.fbin, d.P, s_hist, s_hist + B * B,
-                                 static_cast<SampleRec<R> *>(nullptr), begin, end);
+            hist_range<R, true>(d.F, d.M, S, c.X, d.C, d.SS, d.fbin, d.P, s_hist, s_hist + B * B,
+                                static_cast<SampleRec<R> *>(d.rec), begin, end);
             __syncthreads();
             const unsigned copy = blockIdx.x % kHistCopies;
             unsigned long long *gj = d.joint + static_cast<size_t>(copy) * B * B;
@@ -146,8 +147,9 @@ __global__ __launch_bounds__(kTail) void batch_tail(const PairDesc *__restrict__
                       s_stage);
 }
 
-// Pass 2: the Mattes derivative moments per tile, pRatio staged in LDS (kept
-// across consecutive tiles of the same pair).
+// Pass 2: the Mattes derivative moments per tile from the value pass's
+// per-sample records, pRatio staged in LDS (kept across consecutive tiles of
+// the same pair).
 template <typename R>
 __global__ __launch_bounds__(kBlock) void batch_pass2(const PairDesc *__restrict__ D,
                                                       const Tile *__restrict__ T, Segments seg,
@@ -168,13 +170,15 @@ __global__ __launch_bounds__(kBlock) void batch_pass2(const PairDesc *__restrict
             staged = tl.pair;
             __syncthreads();
         }
-        const OneStage S = load_stage(c);
         long long begin, end;
         tile_range(d, tl, begin, end);
+        const SampleRec<R> *rec = static_cast<const SampleRec<R> *>(d.rec);
+        RecRound<R> first;
+        load_rec_round<R>(d.SS, d.fbin, rec, begin + threadIdx.x, begin, end, first);
         double acc[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) acc[k] = 0.0;
-        deriv_range<R>(d.F, d.M, S, c.X, d.C, d.SS, d.fbin, d.P, s_pr, begin, end, acc);
+        deriv_records<R>(d.F, d.M, d.C, d.SS, d.fbin, d.P, s_pr, rec, begin, end, first, acc);
         block_reduce_store<12>(acc, d.partials, static_cast<unsigned>(tl.index));
         __syncthreads();
     }
@@ -342,6 +346,7 @@ BatchEngine *batch_create(Metric *const *metrics, size_t n) {
                 d.pratio_f = m->pratio_f.ptr;
                 d.pratio_w = m->pratio.ptr;
                 d.pratio_fw = m->pratio_f.ptr;
+                d.rec = mattes_sample_records(*m);
                 mp.push_back(static_cast<int>(i));
                 maxB = std::max(maxB, m->P.bins);
             }

@@ -882,6 +882,11 @@ static SampleRec<R> *sample_records(Metric &m) {
     return reinterpret_cast<SampleRec<R> *>(m.recs.ptr);
 }
 
+void *mattes_sample_records(Metric &m) {
+    if (m.precision == SITKHIP_PRECISION_F64) return sample_records<double>(m);
+    return sample_records<float>(m);
+}
+
 void mattes_joint_histogram(Metric &m, const double *stages, int nstage, double *joint,
                             double *marginal, size_t *valid) {
     require_bins(m);

@@ -24,6 +24,9 @@ void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b);
 void mattes_value(Metric &m, const double *stages, int nstage, double *value, size_t *valid);
 void mattes_value_and_moments(Metric &m, const double *stages, int nstage, double *value,
                               double moments[12], size_t *valid);
+// The per-sample records a Mattes value pass leaves for the derivative pass
+// (SampleRec<float> or <double> by the metric's precision), allocated on first use.
+void *mattes_sample_records(Metric &m);
 void ms_moments(Metric &m, const double *stages, int nstage, double out[14]);
 void ms_value(Metric &m, const double *stages, int nstage, double *sq, size_t *valid);
 void invert3_host(const double m[9], double out[9]);
