@@ -329,35 +329,50 @@ __global__ __launch_bounds__(kBlock) void range_partials(const float *__restrict
 }
 
 
-// (min, max) of two value sets (either masked) with one synchronisation:
-// both partial reductions are enqueued, then read back together.
-static void ranges_of(const float *v1, const uint8_t *mask1, long long n1, double out1[2],
-                      const float *v2, const uint8_t *mask2, long long n2, double out2[2]) {
+// The two sets' partials folded on the device, in block order (min and max
+// are selections and the counts exact integers, so any order gives the same
+// result): out = fixed (min, max, count), moving (min, max, count).
+__global__ __launch_bounds__(kBlock) void range_fold(const double *__restrict__ part, unsigned G,
+                                                     double *__restrict__ out) {
+    if (threadIdx.x >= 2) return;
+    const double *p = part + static_cast<size_t>(threadIdx.x) * G * 3;
+    double lo = INFINITY, hi = -INFINITY, c = 0.0;
+    for (unsigned b = 0; b < G; ++b) {
+        lo = fmin(lo, p[b * 3 + 0]);
+        hi = fmax(hi, p[b * 3 + 1]);
+        c += p[b * 3 + 2];
+    }
+    out[threadIdx.x * 3 + 0] = lo;
+    out[threadIdx.x * 3 + 1] = hi;
+    out[threadIdx.x * 3 + 2] = c;
+}
+
+// The value ranges of the metric's two sets (ops/mattes.rs:587-589: fixed over
+// the sample set, moving over its mask), enqueued into the pinned result
+// block; read with finish_ranges once the stream has drained.
+static void enqueue_ranges(Metric &m, long long nm) {
     const unsigned G = 512;
-    DeviceBuffer<double> part(2 * G * 3);
-    {
-        LaunchScope scope("range_partials");
-        hipLaunchKernelGGL(range_partials, dim3(G), dim3(kBlock), 0, backend().stream, v1, mask1, n1,
-                           part.ptr);
-        hipLaunchKernelGGL(range_partials, dim3(G), dim3(kBlock), 0, backend().stream, v2, mask2, n2,
-                           part.ptr + G * 3);
-        check_launch("range_partials");
-    }
-    std::vector<double> h(2 * G * 3);
-    part.download(h.data(), h.size());
-    sync();
+    m.range_part.alloc(2 * G * 3);
+    LaunchScope scope("range_partials");
+    hipLaunchKernelGGL(range_partials, dim3(G), dim3(kBlock), 0, backend().stream, m.fval.ptr,
+                       static_cast<const uint8_t *>(nullptr), m.SS.n, m.range_part.ptr);
+    hipLaunchKernelGGL(range_partials, dim3(G), dim3(kBlock), 0, backend().stream, m.moving.ptr,
+                       m.Mv.mask, nm, m.range_part.ptr + G * 3);
+    hipLaunchKernelGGL(range_fold, dim3(1), dim3(64), 0, backend().stream, m.range_part.ptr, G,
+                       m.res_dev->ranges);
+    check_launch("range_partials");
+    m.ranges_pending = true;
+}
+
+static void finish_ranges(Metric &m) {
     for (int k = 0; k < 2; ++k) {
-        double lo = INFINITY, hi = -INFINITY, c = 0.0;
-        for (unsigned b = 0; b < G; ++b) {
-            const double *r = h.data() + (k * G + b) * 3;
-            lo = std::fmin(lo, r[0]);
-            hi = std::fmax(hi, r[1]);
-            c += r[2];
-        }
-        double *out = k == 0 ? out1 : out2;
-        if (c == 0.0) { out[0] = 0.0; out[1] = 0.0; }
-        else { out[0] = lo; out[1] = hi; }
+        const double *r = m.res->ranges + 3 * k;
+        double *out = k == 0 ? m.fixed_range : m.moving_range;
+        if (r[2] == 0.0) { out[0] = 0.0; out[1] = 0.0; }
+        else { out[0] = r[0]; out[1] = r[1]; }
     }
+    m.ranges_pending = false;
+    m.range_part.release();
 }
 
 void invert3_host(const double m[9], double out[9]) {
@@ -446,7 +461,7 @@ static void device_draws(Metric &m, size_t voxels, size_t n, uint64_t seed) {
 static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *indices,
                                   size_t n_indices, const Mask *fmask, const Image &moving,
                                   const Mask *mmask, int precision, bool draw_on_device,
-                                  size_t n_draws, uint64_t draw_seed) {
+                                  size_t n_draws, uint64_t draw_seed, bool defer_ranges = false) {
     if (kind != SITKHIP_METRIC_MEAN_SQUARES && kind != SITKHIP_METRIC_MATTES)
         fail(SITKHIP_ERR_INVALID_ARGUMENT, "unknown metric kind");
     if (precision != SITKHIP_PRECISION_F32 && precision != SITKHIP_PRECISION_F64)
@@ -590,10 +605,11 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
         m->res_dev = static_cast<HostResult *>(dev);
         m->res->done = 0;
 
-        // Value ranges (ops/mattes.rs:587-589): fixed over the sample set, moving
-        // over its mask.
-        ranges_of(m->fval.ptr, nullptr, m->SS.n, m->fixed_range, m->moving.ptr, m->Mv.mask,
-                  static_cast<long long>(nm), m->moving_range);
+        enqueue_ranges(*m, static_cast<long long>(nm));
+        if (!defer_ranges) {
+            sync();
+            finish_ranges(*m);
+        }
     } catch (...) {
         delete m;
         throw;
@@ -608,14 +624,28 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
 }
 
 Metric *metric_create_drawn(int kind, const Image &fixed, size_t n_draws, uint64_t seed,
-                            const Image &moving, int precision) {
+                            const Image &moving, int precision, bool defer_ranges) {
     return metric_create_impl(kind, fixed, nullptr, 0, nullptr, moving, nullptr, precision, true,
-                              n_draws, seed);
+                              n_draws, seed, defer_ranges);
+}
+
+Metric *metric_create_deferred(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
+                               const Image &moving, int precision) {
+    return metric_create_impl(kind, fixed, indices, n_indices, nullptr, moving, nullptr, precision,
+                              false, 0, 0, true);
+}
+
+void metric_finish_ranges(Metric &m) {
+    if (m.ranges_pending) finish_ranges(m);
 }
 
 void metric_destroy(Metric *m) { delete m; }
 size_t metric_sample_count(const Metric &m) { return m.nsamples; }
-void metric_value_ranges(const Metric &m, double fr[2], double mr[2]) {
+void metric_value_ranges(Metric &m, double fr[2], double mr[2]) {
+    if (m.ranges_pending) {
+        sync();
+        finish_ranges(m);
+    }
     fr[0] = m.fixed_range[0]; fr[1] = m.fixed_range[1];
     mr[0] = m.moving_range[0]; mr[1] = m.moving_range[1];
 }

@@ -13,11 +13,16 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
                       const Mask *fmask, const Image &moving, const Mask *mmask, int precision);
 // A metric whose samples are `n_draws` random draws (SplitMix64 from `seed`,
 // with replacement) over the fixed grid, drawn and sorted on the device.
+// defer_ranges: the value ranges are only enqueued (no stream wait); call
+// metric_finish_ranges after the stream has drained, before the bins are set.
 Metric *metric_create_drawn(int kind, const Image &fixed, size_t n_draws, uint64_t seed,
-                            const Image &moving, int precision);
+                            const Image &moving, int precision, bool defer_ranges = false);
+Metric *metric_create_deferred(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
+                               const Image &moving, int precision);
+void metric_finish_ranges(Metric &m);
 void metric_destroy(Metric *m);
 size_t metric_sample_count(const Metric &m);
-void metric_value_ranges(const Metric &m, double fixed_range[2], double moving_range[2]);
+void metric_value_ranges(Metric &m, double fixed_range[2], double moving_range[2]);
 void mattes_bins_from_ranges(const double fr[2], const double mr[2], size_t bins,
                              sitkhip_mattes_bins *out);
 void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b);

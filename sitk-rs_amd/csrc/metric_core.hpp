@@ -1042,6 +1042,7 @@ struct HostResult {
     TailOut tail;
     double moments[14];
     unsigned long long done;  // the sequence number of the last finished evaluation
+    double ranges[6];         // at creation: fixed (min, max, count), moving (min, max, count)
 };
 
 struct Metric {
@@ -1071,6 +1072,8 @@ struct Metric {
     DeviceBuffer<unsigned char> recs;            // per-sample records of the value pass
     DeviceBuffer<uint32_t> defer;                // the value pass's deferred-sample lists
     unsigned long long seq = 0;                  // evaluations issued (see HostResult::done)
+    DeviceBuffer<double> range_part;             // value-range partials (creation only)
+    bool ranges_pending = false;                 // ranges enqueued, not yet read on the host
     ~Metric() { pinned_slot_free(res); }
     double fixed_range[2] = {0, 0};
     double moving_range[2] = {0, 0};

@@ -376,8 +376,10 @@ struct ImageHolder {
     ~ImageHolder() { if (owned) delete img; }
 };
 
-static LevelInputs prepare_level(const Image &fixed, const Image &moving,
-                                 const sitkhip_registration_config &cfg, int level) {
+// Everything a level needs, enqueued without waiting on the stream: the
+// metric's value ranges are left pending (prepare_level_end reads them).
+static LevelInputs prepare_level_begin(const Image &fixed, const Image &moving,
+                                       const sitkhip_registration_config &cfg, int level) {
     const double s = cfg.sigma[level];
     const double sig[3] = {s, s, s};
     const size_t *f = cfg.shrink[level];
@@ -405,7 +407,7 @@ static LevelInputs prepare_level(const Image &fixed, const Image &moving,
         // random draws (metric.rs:419-437), made and sorted on the device
         const size_t cnt = static_cast<size_t>(static_cast<double>(nvox) * cfg.sampling_percentage);
         out.metric.reset(metric_create_drawn(cfg.metric, fixed_level, cnt, cfg.seed, *sm.img,
-                                             cfg.precision));
+                                             cfg.precision, true));
     } else {
         std::vector<int64_t> draws;
         const int64_t *idx = nullptr;
@@ -418,16 +420,30 @@ static LevelInputs prepare_level(const Image &fixed, const Image &moving,
             nidx = draws.size();
         }
         out.metric.reset(
-            metric_create(cfg.metric, fixed_level, idx, nidx, nullptr, *sm.img, nullptr, cfg.precision));
-    }
-    if (cfg.metric == SITKHIP_METRIC_MATTES) {
-        double fr[2], mr[2];
-        metric_value_ranges(*out.metric, fr, mr);
-        sitkhip_mattes_bins b;
-        mattes_bins_from_ranges(fr, mr, static_cast<size_t>(cfg.bins), &b);
-        mattes_set_bins(*out.metric, b);
+            metric_create_deferred(cfg.metric, fixed_level, idx, nidx, *sm.img, cfg.precision));
     }
     return out;
+}
+
+// After the stream has drained: the value ranges, and for Mattes the
+// histogram geometry from them.
+static void prepare_level_end(LevelInputs &in, const sitkhip_registration_config &cfg) {
+    metric_finish_ranges(*in.metric);
+    if (cfg.metric == SITKHIP_METRIC_MATTES) {
+        double fr[2], mr[2];
+        metric_value_ranges(*in.metric, fr, mr);
+        sitkhip_mattes_bins b;
+        mattes_bins_from_ranges(fr, mr, static_cast<size_t>(cfg.bins), &b);
+        mattes_set_bins(*in.metric, b);
+    }
+}
+
+static LevelInputs prepare_level(const Image &fixed, const Image &moving,
+                                 const sitkhip_registration_config &cfg, int level) {
+    LevelInputs in = prepare_level_begin(fixed, moving, cfg, level);
+    sync();
+    prepare_level_end(in, cfg);
+    return in;
 }
 
 void register_images(const Image &fixed, const Image &moving, const sitkhip_registration_config &cfg,
@@ -597,11 +613,15 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
         std::vector<LevelInputs> inputs;
         inputs.reserve(m);
         std::vector<Metric *> metrics(m);
+        // every pair's level enqueued, one stream wait, then the histogram
+        // geometries from the ranges
         for (size_t k = 0; k < m; ++k) {
             const size_t i = who[k];
-            inputs.push_back(prepare_level(*fixed[i], *moving[i], cfg[i], level));
+            inputs.push_back(prepare_level_begin(*fixed[i], *moving[i], cfg[i], level));
             metrics[k] = inputs.back().metric.get();
         }
+        sync();
+        for (size_t k = 0; k < m; ++k) prepare_level_end(inputs[k], cfg[who[k]]);
         const auto tp1 = now();
         std::unique_ptr<BatchEngine, void (*)(BatchEngine *)> eng(batch_create(metrics.data(), m),
                                                                    batch_destroy);
