@@ -125,6 +125,14 @@ void sitkhip_mask_free(sitkhip_mask m);
  * FP64 recursion, one narrowing to float at the end. Output geometry = input. */
 int sitkhip_recursive_gaussian(sitkhip_image in, const double sigma[3], sitkhip_image *out);
 
+/* No counterpart in the reference (which smooths one image per call): the
+ * recursive Gaussian of n images, sigma[3 i .. 3 i + 2] for image i, in a few
+ * multi-image launches (one per axis per group of images). out[i] is the same
+ * image, bit for bit, as sitkhip_recursive_gaussian(in[i], sigma + 3 i) gives.
+ * For the batched registration's level preparation. */
+int sitkhip_recursive_gaussian_batch(size_t n, const sitkhip_image *in, const double *sigma,
+                                     sitkhip_image *out);
+
 /* ------------------------------------------------------ pyramid placement */
 /* shrunk_geometry (src/cuda/ops/pyramid.rs:355-420, ITK ShrinkImageFilter): size
  * max(1, size / f), spacing * f, origin moved by Direction * diag(spacing) *

@@ -21,6 +21,10 @@
 // following steps hit it in cache.
 #include <hip/hip_runtime.h>
 
+#include <array>
+#include <algorithm>
+#include <vector>
+
 #include <cmath>
 #include <mutex>
 
@@ -209,14 +213,13 @@ constexpr int kPairLines = 32;
 constexpr int kPD = 16;
 
 template <typename SrcT, typename DstT>
-__global__ __launch_bounds__(64) void iir_pair_lines(const SrcT *__restrict__ src,
-                                                     DstT *__restrict__ dst, LineSet L, IirCoef k,
-                                                     int H) {
-    extern __shared__ double lds[];  // [2][H][32]: role, step, line
+__device__ __forceinline__ void iir_pair_body(const SrcT *__restrict__ src, DstT *__restrict__ dst,
+                                              const LineSet &L, const IirCoef &k, int H,
+                                              unsigned block, double *lds) {
     const int lane = threadIdx.x;
     const int role = lane >> 5;  // 0 causal, 1 anti-causal
     const int li = lane & 31;
-    long long line = static_cast<long long>(blockIdx.x) * kPairLines + li;
+    long long line = static_cast<long long>(block) * kPairLines + li;
     const bool act = line < L.nlines;
     if (!act) line = L.nlines - 1;  // mirror a valid line; stores are suppressed
     const long long s0 = line_start(L, line);
@@ -317,6 +320,36 @@ __global__ __launch_bounds__(64) void iir_pair_lines(const SrcT *__restrict__ sr
         dst[s0 + static_cast<long long>(mid) * st] =
             narrow<DstT>(mine[mid * kPairLines + li] + partner[mid * kPairLines + li]);
     }
+}
+
+template <typename SrcT, typename DstT>
+__global__ __launch_bounds__(64) void iir_pair_lines(const SrcT *__restrict__ src,
+                                                     DstT *__restrict__ dst, LineSet L, IirCoef k,
+                                                     int H) {
+    extern __shared__ double lds[];  // [2][H][32]: role, step, line
+    iir_pair_body<SrcT, DstT>(src, dst, L, k, H, blockIdx.x, lds);
+}
+
+// Many volumes' lines along one axis in one launch: workgroup b takes
+// volume vol_of[b], lines (b - first_block) * 32 ... of it, with that
+// volume's line geometry and coefficients; the per-line arithmetic is
+// iir_pair_lines'.
+struct BatchLines {
+    const void *src;
+    void *dst;
+    LineSet L;
+    IirCoef k;
+    int H;
+    unsigned first_block;
+};
+
+template <typename SrcT, typename DstT>
+__global__ __launch_bounds__(64) void iir_pair_lines_batch(const BatchLines *__restrict__ vols,
+                                                           const int *__restrict__ vol_of) {
+    extern __shared__ double lds[];
+    const BatchLines &v = vols[vol_of[blockIdx.x]];
+    iir_pair_body<SrcT, DstT>(static_cast<const SrcT *>(v.src), static_cast<DstT *>(v.dst), v.L, v.k,
+                              v.H, blockIdx.x - v.first_block, lds);
 }
 
 static LineSet lines_for(const sitkhip_geometry &g, int axis) {
@@ -438,7 +471,137 @@ Image *recursive_gaussian(const Image &in, const double sigma[3], bool synchrono
     return out;
 }
 
+
+// ----------------------------------------------------------- batched volumes
+template <typename SrcT, typename DstT>
+static void launch_axis_batch(const std::vector<const SrcT *> &src, const std::vector<DstT *> &dst,
+                              const std::vector<const sitkhip_geometry *> &g,
+                              const std::vector<IirCoef> &k, int axis) {
+    const size_t n = src.size();
+    std::vector<BatchLines> hv(n);
+    std::vector<int> map;
+    int Hmax = 1;
+    unsigned blocks = 0;
+    for (size_t i = 0; i < n; ++i) {
+        BatchLines &v = hv[i];
+        v.src = src[i];
+        v.dst = dst[i];
+        v.L = lines_for(*g[i], axis);
+        v.k = k[i];
+        v.H = (v.L.len + 1) / 2;
+        v.first_block = blocks;
+        const unsigned nb = static_cast<unsigned>((v.L.nlines + kPairLines - 1) / kPairLines);
+        map.insert(map.end(), nb, static_cast<int>(i));
+        blocks += nb;
+        Hmax = std::max(Hmax, v.H);
+    }
+    const size_t lds = static_cast<size_t>(2) * Hmax * kPairLines * sizeof(double);
+    DeviceBuffer<BatchLines> dv(n);
+    DeviceBuffer<int> dmap(map.size());
+    dv.upload(hv.data(), n);
+    dmap.upload(map.data(), map.size());
+    static const char *names[3] = {"gauss_iir_axis0", "gauss_iir_axis1", "gauss_iir_axis2"};
+    LaunchScope scope(names[axis]);
+    if (lds > 64 * 1024)
+        HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void *>(&iir_pair_lines_batch<SrcT, DstT>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    hipLaunchKernelGGL((iir_pair_lines_batch<SrcT, DstT>), dim3(blocks), dim3(64), lds, backend().stream,
+                       dv.ptr, dmap.ptr);
+    check_launch(names[axis]);
+}
+
+// recursive_gaussian of many volumes, smoothing all three axes of each
+// (sigma[i] > 0 on every axis): three launches per group of volumes whose
+// FP64 intermediates fit a 1 GiB budget, the per-line arithmetic that of the
+// one-volume path (so the results are the same bits). Volumes the paired-lane
+// kernel cannot take (an axis shorter than 8 voxels, or lines whose LDS does
+// not fit) go through recursive_gaussian one by one. Asynchronous.
+std::vector<Image *> recursive_gaussian_batch(const std::vector<const Image *> &in,
+                                              const std::vector<std::array<double, 3>> &sigma) {
+    const size_t n = in.size();
+    std::vector<Image *> out(n, nullptr);
+    try {
+        std::vector<size_t> group;
+        size_t group_bytes = 0;
+        auto flush = [&]() {
+            if (group.empty()) return;
+            std::vector<DeviceBuffer<double>> A(group.size()), B(group.size());
+            std::vector<const sitkhip_geometry *> g;
+            std::vector<IirCoef> k0, k1, k2;
+            std::vector<const float *> s0;
+            std::vector<double *> d0, d1;
+            std::vector<const double *> s1, s2;
+            std::vector<float *> d2;
+            for (size_t j = 0; j < group.size(); ++j) {
+                const size_t i = group[j];
+                const sitkhip_geometry &gi = in[i]->geom;
+                A[j].alloc(in[i]->len());
+                B[j].alloc(in[i]->len());
+                out[i] = image_like(gi);
+                g.push_back(&gi);
+                k0.push_back(zero_order_coefficients(sigma[i][0] / gi.spacing[0]));
+                k1.push_back(zero_order_coefficients(sigma[i][1] / gi.spacing[1]));
+                k2.push_back(zero_order_coefficients(sigma[i][2] / gi.spacing[2]));
+                s0.push_back(in[i]->buf.ptr);
+                d0.push_back(A[j].ptr);
+                s1.push_back(A[j].ptr);
+                d1.push_back(B[j].ptr);
+                s2.push_back(B[j].ptr);
+                d2.push_back(out[i]->buf.ptr);
+            }
+            launch_axis_batch<float, double>(s0, d0, g, k0, 0);
+            launch_axis_batch<double, double>(s1, d1, g, k1, 1);
+            launch_axis_batch<double, float>(s2, d2, g, k2, 2);
+            group.clear();
+            group_bytes = 0;
+        };
+        for (size_t i = 0; i < n; ++i) {
+            const sitkhip_geometry &gi = in[i]->geom;
+            bool ok = true;
+            for (int d = 0; d < 3; ++d) {
+                if (!(sigma[i][d] > 0.0) || gi.size[d] < 8) ok = false;
+                const size_t lds = static_cast<size_t>(2) * ((gi.size[d] + 1) / 2) * kPairLines * sizeof(double);
+                if (lds > max_lds_per_block()) ok = false;
+            }
+            if (!ok) {
+                out[i] = recursive_gaussian(*in[i], sigma[i].data(), false);
+                continue;
+            }
+            const size_t bytes = in[i]->len() * 2 * sizeof(double);
+            if (!group.empty() && group_bytes + bytes > (static_cast<size_t>(1) << 30)) flush();
+            group.push_back(i);
+            group_bytes += bytes;
+        }
+        flush();
+    } catch (...) {
+        for (Image *p : out) delete p;
+        throw;
+    }
+    return out;
+}
 }  // namespace sitkhip
+
+extern "C" int sitkhip_recursive_gaussian_batch(size_t n, const sitkhip_image *in,
+                                                const double *sigma, sitkhip_image *out) {
+    return sitkhip::run_guarded([&] {
+        if (n == 0) return;
+        if (!in || !sigma || !out) sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "null array");
+        std::vector<const sitkhip::Image *> imgs(n);
+        std::vector<std::array<double, 3>> sig(n);
+        for (size_t i = 0; i < n; ++i) {
+            imgs[i] = reinterpret_cast<const sitkhip::Image *>(in[i]);
+            if (!imgs[i]) sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "null image");
+            for (int d = 0; d < 3; ++d) {
+                sig[i][d] = sigma[3 * i + d];
+                if (!(sig[i][d] >= 0.0))
+                    sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "recursive_gaussian: sigma must be >= 0");
+            }
+        }
+        std::vector<sitkhip::Image *> res = sitkhip::recursive_gaussian_batch(imgs, sig);
+        sitkhip::sync();
+        for (size_t i = 0; i < n; ++i) out[i] = reinterpret_cast<sitkhip_image>(res[i]);
+    });
+}
 
 extern "C" int sitkhip_recursive_gaussian(sitkhip_image in, const double sigma[3],
                                           sitkhip_image *out) {

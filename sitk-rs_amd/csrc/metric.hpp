@@ -2,6 +2,9 @@
 // registration driver in the same library.
 #pragma once
 
+#include <array>
+#include <vector>
+
 #include "common.hpp"
 #include "image.hpp"
 
@@ -44,5 +47,7 @@ void shrunk_geometry(const sitkhip_geometry &g, const size_t f[3], sitkhip_geome
 Image *resample_linear(const Image &in, const sitkhip_geometry &og, const double *stages,
                        int nstage, double dflt, bool synchronous = true);
 Image *recursive_gaussian(const Image &in, const double sigma[3], bool synchronous = true);
+std::vector<Image *> recursive_gaussian_batch(const std::vector<const Image *> &in,
+                                              const std::vector<std::array<double, 3>> &sigma);
 
 }  // namespace sitkhip

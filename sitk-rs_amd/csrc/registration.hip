@@ -14,10 +14,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -378,18 +380,24 @@ struct ImageHolder {
 
 // Everything a level needs, enqueued without waiting on the stream: the
 // metric's value ranges are left pending (prepare_level_end reads them).
+// smoothed_fixed / smoothed_moving: the level's smoothed images when the
+// caller made them (taken over and freed here), else this smooths.
 static LevelInputs prepare_level_begin(const Image &fixed, const Image &moving,
-                                       const sitkhip_registration_config &cfg, int level) {
+                                       const sitkhip_registration_config &cfg, int level,
+                                       Image *smoothed_fixed = nullptr,
+                                       Image *smoothed_moving = nullptr) {
     const double s = cfg.sigma[level];
     const double sig[3] = {s, s, s};
     const size_t *f = cfg.shrink[level];
     const bool smooth = s > 0.0;
     const bool native = f[0] == 1 && f[1] == 1 && f[2] == 1;
     ImageHolder sf, sm, fl;
-    sf.img = smooth ? recursive_gaussian(fixed, sig, false) : const_cast<Image *>(&fixed);
-    sf.owned = smooth;
-    sm.img = smooth ? recursive_gaussian(moving, sig, false) : const_cast<Image *>(&moving);
-    sm.owned = smooth;
+    if (smoothed_fixed) sf.img = smoothed_fixed;
+    else sf.img = smooth ? recursive_gaussian(fixed, sig, false) : const_cast<Image *>(&fixed);
+    sf.owned = smooth || smoothed_fixed;
+    if (smoothed_moving) sm.img = smoothed_moving;
+    else sm.img = smooth ? recursive_gaussian(moving, sig, false) : const_cast<Image *>(&moving);
+    sm.owned = smooth || smoothed_moving;
     if (native) {
         fl.img = sf.img;
     } else {
@@ -613,11 +621,32 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
         std::vector<LevelInputs> inputs;
         inputs.reserve(m);
         std::vector<Metric *> metrics(m);
-        // every pair's level enqueued, one stream wait, then the histogram
-        // geometries from the ranges
+        // every pair's level enqueued (the smoothing of all fixed and moving
+        // images in a few multi-volume launches), one stream wait, then the
+        // histogram geometries from the ranges
+        std::vector<const Image *> vols;
+        std::vector<std::array<double, 3>> sig;
+        std::vector<size_t> vol_of(m, SIZE_MAX);
         for (size_t k = 0; k < m; ++k) {
             const size_t i = who[k];
-            inputs.push_back(prepare_level_begin(*fixed[i], *moving[i], cfg[i], level));
+            const double s = cfg[i].sigma[level];
+            if (!(s > 0.0)) continue;
+            vol_of[k] = vols.size();
+            vols.push_back(fixed[i]);
+            vols.push_back(moving[i]);
+            sig.push_back({s, s, s});
+            sig.push_back({s, s, s});
+        }
+        std::vector<Image *> smoothed = recursive_gaussian_batch(vols, sig);
+        for (size_t k = 0; k < m; ++k) {
+            const size_t i = who[k];
+            Image *sf = nullptr, *sm = nullptr;
+            if (vol_of[k] != SIZE_MAX) {
+                sf = smoothed[vol_of[k]];
+                sm = smoothed[vol_of[k] + 1];
+                smoothed[vol_of[k]] = smoothed[vol_of[k] + 1] = nullptr;
+            }
+            inputs.push_back(prepare_level_begin(*fixed[i], *moving[i], cfg[i], level, sf, sm));
             metrics[k] = inputs.back().metric.get();
         }
         sync();

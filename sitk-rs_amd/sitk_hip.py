@@ -175,6 +175,7 @@ def lib():
         L.sitkhip_mask_free.argtypes = [vp]
         L.sitkhip_mask_free.restype = None
         L.sitkhip_recursive_gaussian.argtypes = [vp, dp, C.POINTER(vp)]
+        L.sitkhip_recursive_gaussian_batch.argtypes = [C.c_size_t, C.POINTER(vp), dp, C.POINTER(vp)]
         L.sitkhip_metric_create.argtypes = [C.c_int, vp, C.POINTER(C.c_int64), C.c_size_t, vp, vp,
                                             vp, C.c_int, C.POINTER(vp)]
         L.sitkhip_metric_create_drawn.argtypes = [C.c_int, vp, C.c_size_t, C.c_uint64, vp, C.c_int,
@@ -395,6 +396,17 @@ def recursive_gaussian(src: DeviceImage, sigma) -> DeviceImage:
     h = C.c_void_p()
     _check(lib().sitkhip_recursive_gaussian(src._h, _dp(s), C.byref(h)))
     return DeviceImage(h)
+
+
+def recursive_gaussian_batch(srcs, sigmas) -> list:
+    """recursive_gaussian of several images in a few multi-image launches
+    (no counterpart in the reference); the same bits as one call per image."""
+    n = len(srcs)
+    s = np.ascontiguousarray(sigmas, dtype=np.float64).reshape(n, 3)
+    ins = (C.c_void_p * n)(*[x._h for x in srcs])
+    outs = (C.c_void_p * n)()
+    _check(lib().sitkhip_recursive_gaussian_batch(C.c_size_t(n), ins, _dp(s), outs))
+    return [DeviceImage(outs[i]) for i in range(n)]
 
 
 class _Resident:

@@ -113,3 +113,21 @@ def test_filled_and_masks(hip):
     mc = hip.DeviceMask.upload(np.ones((6, 5, 5), np.uint8))
     with pytest.raises(hip.CudaError):
         ma.intersect(mc)
+
+
+def test_batch_equals_one_image_at_a_time(hip):
+    """The multi-image launches give every image the bits of its own call,
+    across uneven shapes and spacings, per-image sigmas, and images the batch
+    hands to the one-image path (an axis shorter than 8 voxels)."""
+    rng = np.random.default_rng(17)
+    shapes = [(9, 12, 40), (33, 17, 8), (64, 20, 12), (7, 30, 30), (16, 16, 16), (128, 9, 11)]
+    imgs, sig = [], []
+    for i, shp in enumerate(shapes):
+        v = rng.standard_normal(shp).astype(np.float32)
+        sp = tuple(float(x) for x in rng.uniform(0.6, 1.8, 3))
+        imgs.append(hip.DeviceImage.upload(v, spacing=sp))
+        sig.append([1.0 + 0.5 * i, 2.0, 0.8 + 0.1 * i])
+    got = hip.recursive_gaussian_batch(imgs, sig)
+    for img, s, g in zip(imgs, sig, got):
+        want = hip.recursive_gaussian(img, s).to_host()
+        assert np.array_equal(g.to_host(), want)
