@@ -372,6 +372,7 @@ static void finish_ranges(Metric &m) {
         else { out[0] = r[0]; out[1] = r[1]; }
     }
     m.ranges_pending = false;
+    m.ranges_known = true;
     m.range_part.release();
 }
 
@@ -461,7 +462,8 @@ static void device_draws(Metric &m, size_t voxels, size_t n, uint64_t seed) {
 static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *indices,
                                   size_t n_indices, const Mask *fmask, const Image &moving,
                                   const Mask *mmask, int precision, bool draw_on_device,
-                                  size_t n_draws, uint64_t draw_seed, bool defer_ranges = false) {
+                                  size_t n_draws, uint64_t draw_seed,
+                                  const CreateOptions &opt = CreateOptions{}) {
     if (kind != SITKHIP_METRIC_MEAN_SQUARES && kind != SITKHIP_METRIC_MATTES)
         fail(SITKHIP_ERR_INVALID_ARGUMENT, "unknown metric kind");
     if (precision != SITKHIP_PRECISION_F32 && precision != SITKHIP_PRECISION_F64)
@@ -551,23 +553,32 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
         }
         m->nsamples = static_cast<size_t>(m->SS.n);
 
-        // Per-sample fixed values (grid: a private copy of the fixed volume).
-        m->fval.alloc(m->nsamples);
-        if (grid) {
+        // Per-sample fixed values (grid: a private copy of the fixed volume,
+        // or the volume itself when the caller hands it over).
+        if (grid && opt.adopt_fixed == &fixed) {
+            m->fval = std::move(opt.adopt_fixed->buf);
+        } else if (grid) {
+            m->fval.alloc(m->nsamples);
             HIP_OK(hipMemcpyAsync(m->fval.ptr, fixed.buf.ptr, nf * sizeof(float),
                                   hipMemcpyDeviceToDevice, backend().stream));
         } else if (m->nsamples > 0) {
+            m->fval.alloc(m->nsamples);
             LaunchScope scope("gather_values");
             hipLaunchKernelGGL(gather_values, dim3(grid_for(m->nsamples, 256, 4096)), dim3(256), 0,
                                backend().stream, m->idx.ptr, m->SS.n, fixed.buf.ptr, m->fval.ptr);
             check_launch("gather_values");
         }
 
-        // Moving volume and mask: private device copies.
+        // Moving volume and mask: private device copies (the volume itself when
+        // the caller hands it over).
         const size_t nm = moving.len();
-        m->moving.alloc(nm);
-        HIP_OK(hipMemcpyAsync(m->moving.ptr, moving.buf.ptr, nm * sizeof(float),
-                              hipMemcpyDeviceToDevice, backend().stream));
+        if (opt.adopt_moving == &moving) {
+            m->moving = std::move(opt.adopt_moving->buf);
+        } else {
+            m->moving.alloc(nm);
+            HIP_OK(hipMemcpyAsync(m->moving.ptr, moving.buf.ptr, nm * sizeof(float),
+                                  hipMemcpyDeviceToDevice, backend().stream));
+        }
         if (mmask) {
             m->mmask.alloc(nm);
             HIP_OK(hipMemcpyAsync(m->mmask.ptr, mmask->buf.ptr, nm, hipMemcpyDeviceToDevice,
@@ -605,10 +616,13 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
         m->res_dev = static_cast<HostResult *>(dev);
         m->res->done = 0;
 
-        enqueue_ranges(*m, static_cast<long long>(nm));
-        if (!defer_ranges) {
-            sync();
-            finish_ranges(*m);
+        m->moving_voxels = static_cast<long long>(nm);
+        if (opt.ranges) {
+            enqueue_ranges(*m, static_cast<long long>(nm));
+            if (!opt.defer_ranges) {
+                sync();
+                finish_ranges(*m);
+            }
         }
     } catch (...) {
         delete m;
@@ -624,15 +638,15 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
 }
 
 Metric *metric_create_drawn(int kind, const Image &fixed, size_t n_draws, uint64_t seed,
-                            const Image &moving, int precision, bool defer_ranges) {
+                            const Image &moving, int precision, const CreateOptions &opt) {
     return metric_create_impl(kind, fixed, nullptr, 0, nullptr, moving, nullptr, precision, true,
-                              n_draws, seed, defer_ranges);
+                              n_draws, seed, opt);
 }
 
-Metric *metric_create_deferred(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
-                               const Image &moving, int precision) {
+Metric *metric_create_with(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
+                           const Image &moving, int precision, const CreateOptions &opt) {
     return metric_create_impl(kind, fixed, indices, n_indices, nullptr, moving, nullptr, precision,
-                              false, 0, 0, true);
+                              false, 0, 0, opt);
 }
 
 void metric_finish_ranges(Metric &m) {
@@ -642,6 +656,7 @@ void metric_finish_ranges(Metric &m) {
 void metric_destroy(Metric *m) { delete m; }
 size_t metric_sample_count(const Metric &m) { return m.nsamples; }
 void metric_value_ranges(Metric &m, double fr[2], double mr[2]) {
+    if (!m.ranges_pending && !m.ranges_known) enqueue_ranges(m, m.moving_voxels);
     if (m.ranges_pending) {
         sync();
         finish_ranges(m);
@@ -1099,9 +1114,10 @@ int sitkhip_metric_volume_bytes(sitkhip_metric h, size_t *out) {
 
 int sitkhip_metric_value_ranges(sitkhip_metric h, double fixed_range[2], double moving_range[2]) {
     return run_guarded([&] {
-        Metric &m = metric_ref(h);
-        if (fixed_range) { fixed_range[0] = m.fixed_range[0]; fixed_range[1] = m.fixed_range[1]; }
-        if (moving_range) { moving_range[0] = m.moving_range[0]; moving_range[1] = m.moving_range[1]; }
+        double fr[2], mr[2];
+        metric_value_ranges(metric_ref(h), fr, mr);
+        if (fixed_range) { fixed_range[0] = fr[0]; fixed_range[1] = fr[1]; }
+        if (moving_range) { moving_range[0] = mr[0]; moving_range[1] = mr[1]; }
     });
 }
 

@@ -16,12 +16,23 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
                       const Mask *fmask, const Image &moving, const Mask *mmask, int precision);
 // A metric whose samples are `n_draws` random draws (SplitMix64 from `seed`,
 // with replacement) over the fixed grid, drawn and sorted on the device.
-// defer_ranges: the value ranges are only enqueued (no stream wait); call
-// metric_finish_ranges after the stream has drained, before the bins are set.
+// Creation options for the registration driver:
+//   defer_ranges : the value ranges are only enqueued (no stream wait); call
+//                  metric_finish_ranges after the stream has drained;
+//   ranges       : compute them at all (else on the first metric_value_ranges);
+//   adopt_fixed / adopt_moving : the caller hands over that image's buffer (it
+//                  must be the `fixed` / `moving` argument) instead of a copy.
+struct CreateOptions {
+    bool defer_ranges = false;
+    bool ranges = true;
+    Image *adopt_fixed = nullptr;
+    Image *adopt_moving = nullptr;
+};
 Metric *metric_create_drawn(int kind, const Image &fixed, size_t n_draws, uint64_t seed,
-                            const Image &moving, int precision, bool defer_ranges = false);
-Metric *metric_create_deferred(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
-                               const Image &moving, int precision);
+                            const Image &moving, int precision,
+                            const CreateOptions &opt = CreateOptions{});
+Metric *metric_create_with(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
+                           const Image &moving, int precision, const CreateOptions &opt);
 void metric_finish_ranges(Metric &m);
 void metric_destroy(Metric *m);
 size_t metric_sample_count(const Metric &m);

@@ -1074,6 +1074,8 @@ struct Metric {
     unsigned long long seq = 0;                  // evaluations issued (see HostResult::done)
     DeviceBuffer<double> range_part;             // value-range partials (creation only)
     bool ranges_pending = false;                 // ranges enqueued, not yet read on the host
+    bool ranges_known = false;                   // fixed_range / moving_range hold them
+    long long moving_voxels = 0;
     ~Metric() { pinned_slot_free(res); }
     double fixed_range[2] = {0, 0};
     double moving_range[2] = {0, 0};

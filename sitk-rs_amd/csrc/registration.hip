@@ -411,11 +411,18 @@ static LevelInputs prepare_level_begin(const Image &fixed, const Image &moving,
     const size_t nvox = fixed_level.len();
     LevelInputs out;
     out.grid = fixed_level.geom;
+    // the level's own images go to the metric instead of being copied; the
+    // value ranges only feed the Mattes histogram
+    CreateOptions opt;
+    opt.defer_ranges = true;
+    opt.ranges = cfg.metric == SITKHIP_METRIC_MATTES;
+    opt.adopt_moving = sm.owned ? sm.img : nullptr;
+    opt.adopt_fixed = fl.owned || (fl.img == sf.img && sf.owned) ? fl.img : nullptr;
     if (cfg.sampling == 2) {
         // random draws (metric.rs:419-437), made and sorted on the device
         const size_t cnt = static_cast<size_t>(static_cast<double>(nvox) * cfg.sampling_percentage);
         out.metric.reset(metric_create_drawn(cfg.metric, fixed_level, cnt, cfg.seed, *sm.img,
-                                             cfg.precision, true));
+                                             cfg.precision, opt));
     } else {
         std::vector<int64_t> draws;
         const int64_t *idx = nullptr;
@@ -427,8 +434,7 @@ static LevelInputs prepare_level_begin(const Image &fixed, const Image &moving,
             idx = draws.data();
             nidx = draws.size();
         }
-        out.metric.reset(
-            metric_create_deferred(cfg.metric, fixed_level, idx, nidx, *sm.img, cfg.precision));
+        out.metric.reset(metric_create_with(cfg.metric, fixed_level, idx, nidx, *sm.img, cfg.precision, opt));
     }
     return out;
 }
