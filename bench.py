@@ -83,7 +83,9 @@ def batched_pair_configs(S, pairs):
 
 def registrations_record(S, W, n_pairs, n_plain):
     """Registrations/s of the batched rigid workload: every pair registered in
-    one batched run (one batched evaluation per iteration, autotuned tiling)
+    one batched run (one batched evaluation per iteration, default tiling: the
+    autotuner's 6-18 ms per level does not pay back over these levels' 30-101
+    evaluations of 0.27-0.66 ms)
     against the same registrations run one at a time; both from volumes
     resident on the device, success = TRE <= 0.5 mm against the inverse of the
     generating warp."""
@@ -107,7 +109,7 @@ def registrations_record(S, W, n_pairs, n_plain):
         return W.tre(est, p["T_gt"].inverse(), v)
 
     t0 = time.perf_counter()
-    got, res = S.register_batch(fixed, moving, cfgs, kinds, p0, centres, autotune=True)
+    got, res = S.register_batch(fixed, moving, cfgs, kinds, p0, centres, autotune=False)
     t_batch = time.perf_counter() - t0
     tre_b = np.array([tre_of(g, p) for g, p in zip(got, pairs)])
     log(f"batched: {n_pairs} registrations in {t_batch:.2f}s, "

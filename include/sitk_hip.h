@@ -379,6 +379,18 @@ int sitkhip_register_batch(size_t n, const sitkhip_image *fixed, const sitkhip_i
                            sitkhip_transform *transforms, sitkhip_registration_result *results,
                            int autotune);
 
+/* sitkhip_register_batch, also reporting each pair's metric value (-MI or mean
+ * squares, as the optimizer sees it before its own evaluation) at a second
+ * parameter set probe_params[12 i ..] (the transform kind's parameter count
+ * used), evaluated on the first level's prepared images before optimizing:
+ * probe_values[i]. For an initializer's fallback rule (keep the optimized
+ * result only if its value beats the probe's) without preparing the level
+ * twice. No counterpart in the reference. */
+int sitkhip_register_batch_probe(size_t n, const sitkhip_image *fixed, const sitkhip_image *moving,
+                                 const sitkhip_registration_config *configs,
+                                 sitkhip_transform *transforms, sitkhip_registration_result *results,
+                                 int autotune, const double *probe_params, double *probe_values);
+
 /* ---------------------------------------------------------------- B-spline */
 /* Mattes MI through a cubic B-spline transform: the value and the
  * sparse-support derivative (src/registration/mattes.rs:722-796, transform

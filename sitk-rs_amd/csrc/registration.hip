@@ -591,7 +591,8 @@ struct RsgdRun {
 
 void register_batch(size_t n, const Image *const *fixed, const Image *const *moving,
                     const sitkhip_registration_config *cfg, sitkhip_transform *tr,
-                    sitkhip_registration_result *res, bool autotune) {
+                    sitkhip_registration_result *res, bool autotune, const double *probe,
+                    double *probe_values) {
     const auto t0 = std::chrono::steady_clock::now();
     // SITKHIP_TRACE=1: per-level phase times on stderr
     const bool trace = std::getenv("SITKHIP_TRACE") != nullptr;
@@ -678,6 +679,19 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
             r.T.set_parameters(r.p.data());
             r.T.stage(&stages[12 * k]);
         }
+        // the metric value of each pair at its probe parameters (first level,
+        // before optimizing), from the same prepared level
+        if (probe && level == 0) {
+            std::vector<double> pst(12 * m);
+            for (size_t k = 0; k < m; ++k) {
+                Transform T = Ts[who[k]];
+                T.set_parameters(probe + 12 * who[k]);
+                T.stage(&pst[12 * k]);
+            }
+            std::vector<BatchOutput> po(m);
+            batch_evaluate(*eng, pst.data(), nullptr, po);
+            for (size_t k = 0; k < m; ++k) probe_values[who[k]] = po[k].value;
+        }
         // tuning pays only when one evaluation is long enough to time
         size_t total_samples = 0;
         for (size_t k = 0; k < m; ++k) total_samples += metric_sample_count(*metrics[k]);
@@ -759,11 +773,23 @@ extern "C" int sitkhip_register_batch(size_t n, const sitkhip_image *fixed,
                                       const sitkhip_registration_config *configs,
                                       sitkhip_transform *transforms,
                                       sitkhip_registration_result *results, int autotune) {
+    return sitkhip_register_batch_probe(n, fixed, moving, configs, transforms, results, autotune,
+                                        nullptr, nullptr);
+}
+
+extern "C" int sitkhip_register_batch_probe(size_t n, const sitkhip_image *fixed,
+                                            const sitkhip_image *moving,
+                                            const sitkhip_registration_config *configs,
+                                            sitkhip_transform *transforms,
+                                            sitkhip_registration_result *results, int autotune,
+                                            const double *probe_params, double *probe_values) {
     return sitkhip::run_guarded([&] {
         if (!fixed || !moving || !configs || !transforms || !results)
             sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "null argument");
+        if ((probe_params == nullptr) != (probe_values == nullptr))
+            sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "probe parameters and values go together");
         sitkhip::register_batch(n, reinterpret_cast<sitkhip::Image *const *>(fixed),
                                 reinterpret_cast<sitkhip::Image *const *>(moving), configs,
-                                transforms, results, autotune != 0);
+                                transforms, results, autotune != 0, probe_params, probe_values);
     });
 }

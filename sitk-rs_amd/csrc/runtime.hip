@@ -46,9 +46,35 @@ void check_launch(const char *name) {
 }
 
 // ------------------------------------------------------ caching allocator
+// Freed blocks go to a free list per size class and serve the next request of
+// that class. Blocks below kSlabMax are carved from 256 MiB slabs (one
+// hipMalloc per slab instead of one per block: a batch of registrations asks
+// for thousands of small volumes); larger ones are allocated one by one.
+// empty_cache returns the free large blocks and every slab none of whose
+// blocks is in use.
 namespace {
 std::mutex g_alloc_mu;
 std::unordered_map<size_t, std::vector<void *>> g_free_blocks;
+
+constexpr size_t kSlabBytes = static_cast<size_t>(256) << 20;
+constexpr size_t kSlabMax = static_cast<size_t>(16) << 20;
+
+struct Slab {
+    char *base;
+    size_t used;
+    size_t live;  // carved blocks not in a free list
+};
+std::map<char *, Slab> g_slabs;  // by base address
+Slab *g_open_slab = nullptr;     // the slab new blocks are carved from
+
+Slab *slab_of(void *ptr) {
+    if (g_slabs.empty()) return nullptr;
+    auto it = g_slabs.upper_bound(static_cast<char *>(ptr));
+    if (it == g_slabs.begin()) return nullptr;
+    --it;
+    Slab &sl = it->second;
+    return static_cast<char *>(ptr) < sl.base + kSlabBytes ? &sl : nullptr;
+}
 
 // Size classes: 8 steps per power of two (at most 12.5% rounding), >= 256 B.
 size_t size_class(size_t bytes) {
@@ -56,6 +82,25 @@ size_t size_class(size_t bytes) {
     int lg = 63 - __builtin_clzll(static_cast<unsigned long long>(bytes));
     const size_t step = static_cast<size_t>(1) << (lg >= 3 ? lg - 3 : 0);
     return (bytes + step - 1) / step * step;
+}
+
+// A block of class c from the open slab (a new slab when it is full); nullptr
+// when the device has no room for a slab. Called with g_alloc_mu held.
+void *carve(size_t c) {
+    c = (c + 255) & ~static_cast<size_t>(255);  // blocks start 256-byte aligned, as hipMalloc's do
+    if (!g_open_slab || g_open_slab->used + c > kSlabBytes) {
+        void *base = nullptr;
+        if (hipMalloc(&base, kSlabBytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        char *b = static_cast<char *>(base);
+        g_open_slab = &(g_slabs[b] = Slab{b, 0, 0});
+    }
+    void *p = g_open_slab->base + g_open_slab->used;
+    g_open_slab->used += c;
+    ++g_open_slab->live;
+    return p;
 }
 }  // namespace
 
@@ -67,7 +112,11 @@ void *device_alloc(size_t bytes) {
         if (it != g_free_blocks.end() && !it->second.empty()) {
             void *p = it->second.back();
             it->second.pop_back();
+            if (Slab *sl = slab_of(p)) ++sl->live;
             return p;
+        }
+        if (c < kSlabMax) {
+            if (void *p = carve(c)) return p;
         }
     }
     void *p = nullptr;
@@ -85,18 +134,37 @@ void device_free(void *ptr, size_t bytes) {
     if (!ptr) return;
     std::lock_guard<std::mutex> lk(g_alloc_mu);
     g_free_blocks[size_class(bytes)].push_back(ptr);
+    if (Slab *sl = slab_of(ptr)) --sl->live;
 }
 
 void device_empty_cache() {
-    std::unordered_map<size_t, std::vector<void *>> blocks;
+    std::vector<void *> release;
     {
         std::lock_guard<std::mutex> lk(g_alloc_mu);
-        blocks.swap(g_free_blocks);
+        // free blocks of idle slabs are dropped with their slab; free large
+        // blocks are released one by one; free blocks of busy slabs stay
+        for (auto &kv : g_free_blocks) {
+            std::vector<void *> keep;
+            for (void *p : kv.second) {
+                Slab *sl = slab_of(p);
+                if (!sl) release.push_back(p);
+                else if (sl->live != 0) keep.push_back(p);
+            }
+            kv.second.swap(keep);
+        }
+        for (auto it = g_slabs.begin(); it != g_slabs.end();) {
+            if (it->second.live == 0) {
+                if (g_open_slab == &it->second) g_open_slab = nullptr;
+                release.push_back(it->second.base);
+                it = g_slabs.erase(it);
+            } else {
+                ++it;
+            }
+        }
     }
-    if (blocks.empty()) return;
+    if (release.empty()) return;
     (void)hipStreamSynchronize(backend().stream);
-    for (auto &kv : blocks)
-        for (void *p : kv.second) (void)hipFree(p);
+    for (void *p : release) (void)hipFree(p);
 }
 
 namespace {

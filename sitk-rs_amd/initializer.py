@@ -179,14 +179,13 @@ def evaluate(net: S.CNN, n_pairs: int, seed: int = 12345, refine_iterations: int
     pred = predict(net, [(f, m, c) for (f, m), c in zip(pairs, centres)])
     t_pred = time.perf_counter() - t0
     p0 = [affine_params(T) for T in pred]
-    got, res = S.register_batch([f for f, _ in pairs], [m for _, m in pairs], [cfg] * n_pairs,
-                                [S.AFFINE] * n_pairs, p0, centres)
-    # the centered initializer's value (identity about the centre, same geometry)
+    # the refinement, also probing the centered initializer's value (identity
+    # about the centre) on the same prepared level
     ident = [list(np.eye(3).reshape(9)) + [0.0, 0.0, 0.0]] * n_pairs
-    zero_cfg = refine_config_fast(0)
-    _, res0 = S.register_batch([f for f, _ in pairs], [m for _, m in pairs], [zero_cfg] * n_pairs,
-                               [S.AFFINE] * n_pairs, ident, centres)
-    worse = [i for i in range(n_pairs) if res[i].value > res0[i].value]
+    got, res, v_ident = S.register_batch([f for f, _ in pairs], [m for _, m in pairs],
+                                         [cfg] * n_pairs, [S.AFFINE] * n_pairs, p0, centres,
+                                         probe=ident)
+    worse = [i for i in range(n_pairs) if res[i].value > v_ident[i]]
     if worse:
         got_f, res_f = S.register_batch([pairs[i][0] for i in worse], [pairs[i][1] for i in worse],
                                         [cfg] * len(worse), [S.AFFINE] * len(worse),

@@ -215,6 +215,10 @@ def lib():
                                              C.POINTER(RegistrationConfig),
                                              C.POINTER(TransformSpec),
                                              C.POINTER(RegistrationResult), C.c_int]
+        L.sitkhip_register_batch_probe.argtypes = [C.c_size_t, C.POINTER(vp), C.POINTER(vp),
+                                                   C.POINTER(RegistrationConfig),
+                                                   C.POINTER(TransformSpec),
+                                                   C.POINTER(RegistrationResult), C.c_int, dp, dp]
         fp = C.POINTER(C.c_float)
         L.sitkhip_bspline_create.argtypes = [vp, szp, dp, dp, dp, C.POINTER(vp)]
         L.sitkhip_bspline_free.argtypes = [vp]
@@ -745,9 +749,11 @@ class Batch:
 
 
 def register_batch(fixed, moving, configs, kinds, params, centers, autotune=False,
-                   compute_zyx=False):
+                   compute_zyx=False, probe=None):
     """n registrations run together, one batched evaluation per iteration.
-    Returns (list of parameter arrays, list of RegistrationResult)."""
+    Returns (list of parameter arrays, list of RegistrationResult); with
+    `probe` (n parameter lists), also each pair's metric value at its probe
+    parameters on the first level (sitkhip_register_batch_probe)."""
     n = len(fixed)
     fa = (C.c_void_p * n)(*[f._h for f in fixed])
     ma = (C.c_void_p * n)(*[m._h for m in moving])
@@ -755,9 +761,18 @@ def register_batch(fixed, moving, configs, kinds, params, centers, autotune=Fals
     ta = (TransformSpec * n)(*[_transform_spec(k, p, c, compute_zyx)
                                for k, p, c in zip(kinds, params, centers)])
     ra = (RegistrationResult * n)()
-    _check(lib().sitkhip_register_batch(n, fa, ma, ca, ta, ra, int(autotune)))
     npar = {TRANSLATION: 3, EULER3D: 6, AFFINE: 12}
-    return [np.array(ta[i].params[:npar[kinds[i]]]) for i in range(n)], [ra[i] for i in range(n)]
+    if probe is None:
+        _check(lib().sitkhip_register_batch(n, fa, ma, ca, ta, ra, int(autotune)))
+    else:
+        pp = np.zeros((n, 12))
+        for i, p in enumerate(probe):
+            pp[i, :len(p)] = p
+        pv = np.zeros(n)
+        _check(lib().sitkhip_register_batch_probe(n, fa, ma, ca, ta, ra, int(autotune), _dp(pp),
+                                                  _dp(pv)))
+    out = ([np.array(ta[i].params[:npar[kinds[i]]]) for i in range(n)], [ra[i] for i in range(n)])
+    return out if probe is None else out + (pv,)
 
 
 # ------------------------------------------------------------- learned initializer

@@ -199,3 +199,40 @@ def test_batched_registration_matches_one_at_a_time(hip):
         assert res[i].iterations == r1.iterations, (i, res[i].iterations, r1.iterations)
         assert res[i].level_stop[0] == r1.level_stop[0]
         assert res[i].valid_points == r1.valid_points
+
+
+def test_probe_values_are_the_zero_iteration_values(hip):
+    """register_batch's probe: each pair's metric value at the probe
+    parameters on the first prepared level equals the value a zero-iteration
+    batched registration started there reports (same level, same samples;
+    Mattes bit for bit, mean squares to 1e-12), and the probe does not change
+    the registration itself."""
+    import workload as W
+
+    rng = np.random.default_rng(21)
+    fixed, moving, configs, zero, kinds, p0, probe, centres = [], [], [], [], [], [], [], []
+    for i, shp in enumerate([(28, 30, 24), (26, 24, 30), (30, 28, 26)]):
+        vol = W.blob_volume(shp, (1.0, 1.0, 1.0), seed=70 + i, nblobs=20)
+        (ang, t), T_gt = W.random_euler(rng, 5.0, 3.0, vol.center())
+        mov = W.resample_linear(vol, T_gt)
+        fixed.append(hip.DeviceImage.upload(vol.data, spacing=vol.spacing, origin=vol.origin))
+        moving.append(hip.DeviceImage.upload(mov.data, spacing=mov.spacing, origin=mov.origin))
+        metric = hip.MATTES if i != 1 else hip.MEAN_SQUARES
+        kw = dict(metric=metric, bins=32, sampling=2 if metric == hip.MATTES else 0,
+                  sampling_percentage=0.3, seed=3 + i, minimum_step=1e-3,
+                  learning_rate_mode=hip.LR_ONCE)
+        configs.append(hip.registration_config(iterations=20, **kw))
+        zero.append(hip.registration_config(iterations=0, **kw))
+        kinds.append(hip.EULER3D)
+        p0.append([0.02, -0.01, 0.03, 0.5, -0.4, 0.2])
+        probe.append([0.0] * 6)
+        centres.append(vol.center())
+    got, res, pv = hip.register_batch(fixed, moving, configs, kinds, p0, centres, probe=probe)
+    got2, res2 = hip.register_batch(fixed, moving, configs, kinds, p0, centres)
+    _, res0 = hip.register_batch(fixed, moving, zero, kinds, probe, centres)
+    for i in range(3):
+        assert np.array_equal(got[i], got2[i]) and res[i].value == res2[i].value
+        if configs[i].metric == hip.MATTES:
+            assert pv[i] == res0[i].value, (i, pv[i], res0[i].value)
+        else:
+            assert abs(pv[i] - res0[i].value) <= 1e-12 * abs(res0[i].value)
