@@ -329,22 +329,39 @@ __global__ __launch_bounds__(kBlock) void range_partials(const float *__restrict
 }
 
 
-// The two sets' partials folded on the device, in block order (min and max
+// The two sets' partials folded on the device by a tree (min and max
 // are selections and the counts exact integers, so any order gives the same
 // result): out = fixed (min, max, count), moving (min, max, count).
 __global__ __launch_bounds__(kBlock) void range_fold(const double *__restrict__ part, unsigned G,
                                                      double *__restrict__ out) {
-    if (threadIdx.x >= 2) return;
-    const double *p = part + static_cast<size_t>(threadIdx.x) * G * 3;
-    double lo = INFINITY, hi = -INFINITY, c = 0.0;
-    for (unsigned b = 0; b < G; ++b) {
-        lo = fmin(lo, p[b * 3 + 0]);
-        hi = fmax(hi, p[b * 3 + 1]);
-        c += p[b * 3 + 2];
+    __shared__ double smin[kBlock], smax[kBlock], scnt[kBlock];
+    for (int k = 0; k < 2; ++k) {
+        const double *p = part + static_cast<size_t>(k) * G * 3;
+        double lo = INFINITY, hi = -INFINITY, c = 0.0;
+        for (unsigned b = threadIdx.x; b < G; b += kBlock) {
+            lo = fmin(lo, p[b * 3 + 0]);
+            hi = fmax(hi, p[b * 3 + 1]);
+            c += p[b * 3 + 2];
+        }
+        smin[threadIdx.x] = lo;
+        smax[threadIdx.x] = hi;
+        scnt[threadIdx.x] = c;
+        __syncthreads();
+        for (int w = kBlock / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) {
+                smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+                smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+                scnt[threadIdx.x] += scnt[threadIdx.x + w];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            out[k * 3 + 0] = smin[0];
+            out[k * 3 + 1] = smax[0];
+            out[k * 3 + 2] = scnt[0];
+        }
+        __syncthreads();
     }
-    out[threadIdx.x * 3 + 0] = lo;
-    out[threadIdx.x * 3 + 1] = hi;
-    out[threadIdx.x * 3 + 2] = c;
 }
 
 // The value ranges of the metric's two sets (ops/mattes.rs:587-589: fixed over
@@ -358,7 +375,7 @@ static void enqueue_ranges(Metric &m, long long nm) {
                        static_cast<const uint8_t *>(nullptr), m.SS.n, m.range_part.ptr);
     hipLaunchKernelGGL(range_partials, dim3(G), dim3(kBlock), 0, backend().stream, m.moving.ptr,
                        m.Mv.mask, nm, m.range_part.ptr + G * 3);
-    hipLaunchKernelGGL(range_fold, dim3(1), dim3(64), 0, backend().stream, m.range_part.ptr, G,
+    hipLaunchKernelGGL(range_fold, dim3(1), dim3(kBlock), 0, backend().stream, m.range_part.ptr, G,
                        m.res_dev->ranges);
     check_launch("range_partials");
     m.ranges_pending = true;
