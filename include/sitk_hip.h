@@ -391,6 +391,41 @@ int sitkhip_register_batch_probe(size_t n, const sitkhip_image *fixed, const sit
                                  sitkhip_transform *transforms, sitkhip_registration_result *results,
                                  int autotune, const double *probe_params, double *probe_values);
 
+/* ---------------------------------------------------------------- L-BFGS-B */
+/* The bound-constrained limited-memory BFGS optimizer
+ * (src/registration/lbfgsb.rs:1590-1746, LBFGSBOptimizer: Byrd, Lu, Nocedal
+ * & Zhu with the Moré-Thuente line search and ITK's stopping rules), on the
+ * host, driven by reverse communication instead of an `eval` closure: start
+ * and tell write the point where they want the value and gradient to x (n
+ * doubles) and set *want = 1, or *want = 0 once the optimizer has stopped;
+ * result then gives the best point evaluated (the reference returns that
+ * point, not the last iterate), its value, the iteration and evaluation counts
+ * and the stop reason (sitkhip_bspline_result.stop numbering: 0 iterations,
+ * 1 projected gradient, 2 relative reduction, 3 evaluations, 4 line search).
+ * bound_selection (ITK BoundSelection, lbfgsb.rs:1632-1640): 0 unbounded,
+ * 1 lower only, 2 both, 3 upper only, null = every variable unbounded;
+ * lower / upper are read where a selection uses them. An infeasible pair
+ * (lower > upper under selection 2) is SITKHIP_ERR_INVALID_ARGUMENT, as the
+ * reference panics (lbfgsb.rs:1711-1720). Host code: no device is touched. */
+typedef struct sitkhip_lbfgsb_s *sitkhip_lbfgsb;
+
+typedef struct {
+    int corrections;            /* m, stored corrections (5) */
+    double factr;               /* relative-reduction factor (1e7) */
+    double pgtol;               /* projected-gradient tolerance (1e-5) */
+    int max_iterations;
+    int max_evaluations;        /* (2000) */
+} sitkhip_lbfgsb_options;
+
+int sitkhip_lbfgsb_create(size_t n, const sitkhip_lbfgsb_options *options, const double *lower,
+                          const double *upper, const int *bound_selection, sitkhip_lbfgsb *out);
+void sitkhip_lbfgsb_free(sitkhip_lbfgsb optimizer);
+int sitkhip_lbfgsb_start(sitkhip_lbfgsb optimizer, const double *initial, double *x, int *want);
+int sitkhip_lbfgsb_tell(sitkhip_lbfgsb optimizer, double value, const double *gradient, double *x,
+                        int *want);
+int sitkhip_lbfgsb_result(sitkhip_lbfgsb optimizer, double *best, double *value, int *iterations,
+                          int *evaluations, int *stop);
+
 /* ---------------------------------------------------------------- B-spline */
 /* Mattes MI through a cubic B-spline transform: the value and the
  * sparse-support derivative (src/registration/mattes.rs:722-796, transform
