@@ -448,11 +448,34 @@ int sitkhip_bspline_mattes_value(sitkhip_bspline b, const double *coeffs, double
 int sitkhip_bspline_mattes_value_and_derivative(sitkhip_bspline b, const double *coeffs,
                                                 double *value, double *deriv, size_t *valid);
 
+/* Sets (stage non-null) or clears the moving-initial transform applied
+ * after the B-spline: moving point = stage(B(x)), stage = matrix row-major
+ * [9], offset [3] (ImageRegistrationMethod::set_moving_initial_transform and
+ * the composition of registration/method.rs:198-340, whose sparse Jacobian
+ * is left-multiplied by the stage's matrix). */
+int sitkhip_bspline_set_moving_initial(sitkhip_bspline b, const double stage[12]);
+
+/* Many starts evaluated together: one B-spline metric (its images, sample
+ * set and grid) with nstarts coefficient vectors and moving-initial stages
+ * (stages: nstarts x 12, null = none). value_and_derivative evaluates the
+ * starts listed in active[0..nactive) at coeffs[s * 3 ncp ..] in one set of
+ * launches and writes values[s], derivs[s * 3 ncp ..], valid[s] of those
+ * starts; each start's results are bit-equal to a single-start evaluation
+ * with the same stage. No counterpart in the reference (its device path
+ * refuses B-spline, registration/device.rs:79-100). */
+typedef struct sitkhip_bspline_multi_s *sitkhip_bspline_multi;
+
+int sitkhip_bspline_multi_create(sitkhip_bspline b, size_t nstarts, const double *stages,
+                                 sitkhip_bspline_multi *out);
+void sitkhip_bspline_multi_free(sitkhip_bspline_multi m);
+int sitkhip_bspline_multi_value_and_derivative(sitkhip_bspline_multi m, size_t nactive,
+                                               const int *active, const double *coeffs,
+                                               double *values, double *derivs, size_t *valid);
+
 /* A deformable registration: Mattes MI through a cubic B-spline transform
  * whose domain is the fixed image's extent (bspline.rs:324-339) with `mesh`
- * cells per axis, optimised by limited-memory BFGS on the host
- * (LBFGSBOptimizer with every variable unbounded, lbfgsb.rs:1590-1674;
- * ITK's stopping rules; backtracking Armijo steps). coeffs: the initial
+ * cells per axis, optimised by L-BFGS-B on the host (LBFGSBOptimizer,
+ * lbfgsb.rs:1590-1746, every variable unbounded). coeffs: the initial
  * coefficients in (3 x (mesh+3)^3, [d][cp]), the best ones out. */
 typedef struct {
     int bins;
@@ -480,6 +503,21 @@ typedef struct {
 int sitkhip_register_bspline(sitkhip_image fixed, sitkhip_image moving,
                              const sitkhip_bspline_config *config, double *coeffs, size_t ncoeffs,
                              sitkhip_bspline_result *result);
+
+/* nstarts registrations of one pair at once, start s from its own
+ * moving-initial stage stages[12 s ..] (null: none) and coefficients
+ * coeffs[s * ncoeffs ..] (the best ones out), each with its own L-BFGS-B;
+ * every round, all optimizers that want an evaluation get it from one
+ * batched set of launches. bound_selection / lower / upper: per coefficient,
+ * shared by every start (null selection: unbounded). results[s] per start;
+ * *best (may be null) = the start with the lowest final value. Each start's
+ * result equals a one-start call's. No counterpart in the reference. */
+int sitkhip_register_bspline_multistart(sitkhip_image fixed, sitkhip_image moving,
+                                        const sitkhip_bspline_config *config, size_t nstarts,
+                                        const double *stages, double *coeffs, size_t ncoeffs,
+                                        const int *bound_selection, const double *lower,
+                                        const double *upper, sitkhip_bspline_result *results,
+                                        size_t *best);
 
 /* ------------------------------------------------------------- generators */
 /* Seeded synthetic volume (no counterpart in the reference; the benchmark

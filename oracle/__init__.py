@@ -124,6 +124,8 @@ def lib():
         L.orc_mattes_value.restype = C.c_double
         L.orc_mattes_evaluate.argtypes = [pp, gp, tp, dp, szp]
         L.orc_mattes_evaluate.restype = C.c_double
+        L.orc_mattes_evaluate_bspline_composed.argtypes = [pp, gp, tp, tp, dp, szp]
+        L.orc_mattes_evaluate_bspline_composed.restype = C.c_double
         L.orc_mattes_moments.argtypes = [pp, gp, tp, dp, dp]
         L.orc_ms_evaluate.argtypes = [pp, tp, dp, szp]
         L.orc_ms_evaluate.restype = C.c_double
@@ -355,6 +357,16 @@ class MetricProblem:
         valid = C.c_size_t(0)
         v = lib().orc_mattes_evaluate(C.byref(self.pb), C.byref(g), C.byref(t), _dp(deriv),
                                       C.byref(valid))
+        return v, deriv, valid.value
+
+    def mattes_evaluate_composed(self, g: MattesGeom, t: Transform, init):
+        """B-spline t followed by the linear moving-initial transform init
+        (None = no composition)."""
+        deriv = np.empty(t.nparams, dtype=np.float64)
+        valid = C.c_size_t(0)
+        v = lib().orc_mattes_evaluate_bspline_composed(
+            C.byref(self.pb), C.byref(g), C.byref(t), C.byref(init) if init is not None else None,
+            _dp(deriv), C.byref(valid))
         return v, deriv, valid.value
 
     def mattes_tail(self, joint, marg, valid, g: MattesGeom):

@@ -768,10 +768,34 @@ double orc_compensated_sum(const double *v, size_t n) {
     return sum;
 }
 
+/* moving_point = moving_initial(optimized(x)) (registration/method.rs:198-240):
+ * the optimized transform, then the moving-initial one, each rounding as its
+ * own transform_point. */
+static void composed_point(const orc_transform *t, const orc_transform *init, const double fp[3],
+                           double mp[3]) {
+    if (!init) {
+        orc_transform_point(t, fp, mp);
+        return;
+    }
+    double y[3];
+    orc_transform_point(t, fp, y);
+    orc_transform_point(init, y, mp);
+}
+
+static void mattes_histogram_composed(const orc_problem *pb, const orc_mattes_geom *g,
+                                      const orc_transform *t, const orc_transform *init,
+                                      double *joint, double *marginal, size_t *valid_out);
+
 /* mattes.rs:442-486 */
 void orc_mattes_histogram(const orc_problem *pb, const orc_mattes_geom *g,
                           const orc_transform *t, double *joint, double *marginal,
                           size_t *valid_out) {
+    mattes_histogram_composed(pb, g, t, NULL, joint, marginal, valid_out);
+}
+
+static void mattes_histogram_composed(const orc_problem *pb, const orc_mattes_geom *g,
+                                      const orc_transform *t, const orc_transform *init,
+                                      double *joint, double *marginal, size_t *valid_out) {
     size_t bins = g->num_bins;
     memset(joint, 0, bins * bins * sizeof(double));
     memset(marginal, 0, bins * sizeof(double));
@@ -783,7 +807,7 @@ void orc_mattes_histogram(const orc_problem *pb, const orc_mattes_geom *g,
         double fp[3], mp[3], mv;
         write_point(pb, flat, fp);
         double fv = (double)pb->fvals[flat];
-        orc_transform_point(t, fp, mp);
+        composed_point(t, init, fp, mp);
         if (!orc_moving_value_at(pb, mp, &mv)) continue;
         if (mv < g->moving_true_min || mv > g->moving_true_max) continue;
         double mterm = mv / g->moving_bin_size - g->moving_normalized_min;
@@ -848,7 +872,8 @@ double orc_mattes_value(const orc_problem *pb, const orc_mattes_geom *g, const o
 }
 
 static double mattes_evaluate_sparse(const orc_problem *pb, const orc_mattes_geom *g,
-                                     const orc_transform *t, double *deriv, size_t *valid_out);
+                                     const orc_transform *t, const orc_transform *init,
+                                     double *deriv, size_t *valid_out);
 
 /* mattes.rs:522-676 */
 static double mattes_evaluate_global(const orc_problem *pb, const orc_mattes_geom *g,
@@ -931,8 +956,16 @@ done:
 
 double orc_mattes_evaluate(const orc_problem *pb, const orc_mattes_geom *g,
                            const orc_transform *t, double *deriv, size_t *valid) {
-    if (t->kind == ORC_BSPLINE) return mattes_evaluate_sparse(pb, g, t, deriv, valid);
+    if (t->kind == ORC_BSPLINE) return mattes_evaluate_sparse(pb, g, t, NULL, deriv, valid);
     return mattes_evaluate_global(pb, g, t, deriv, valid);
+}
+
+double orc_mattes_evaluate_bspline_composed(const orc_problem *pb, const orc_mattes_geom *g,
+                                            const orc_transform *t, const orc_transform *init,
+                                            double *deriv, size_t *valid) {
+    if (t->kind != ORC_BSPLINE) return -1.0;
+    if (init && init->kind == ORC_BSPLINE) return -1.0;
+    return mattes_evaluate_sparse(pb, g, t, init, deriv, valid);
 }
 
 /* The device derivative's moment form (src/cuda/ops/mattes.rs:75-91):
@@ -1119,15 +1152,19 @@ int orc_bspline_from_image_domain(orc_transform *t, const size_t img_size[3],
     return 0;
 }
 
+/* mattes.rs:722-796. With a moving-initial transform g (linear), the sparse
+ * Jacobian's columns are left-multiplied by g's matrix (method.rs:313-340), so
+ * the moving gradient enters as J_g' grad. */
 static double mattes_evaluate_sparse(const orc_problem *pb, const orc_mattes_geom *g,
-                                     const orc_transform *t, double *deriv, size_t *valid_out) {
+                                     const orc_transform *t, const orc_transform *init,
+                                     double *deriv, size_t *valid_out) {
     size_t bins = g->num_bins;
     int np = t->nparams;
     double *joint = (double *)malloc(bins * bins * sizeof(double));
     double *marg = (double *)malloc(bins * sizeof(double));
     double *pratio = (double *)malloc(bins * bins * sizeof(double));
     size_t valid;
-    orc_mattes_histogram(pb, g, t, joint, marg, &valid);
+    mattes_histogram_composed(pb, g, t, init, joint, marg, &valid);
     *valid_out = valid;
     double value;
     for (int k = 0; k < np; ++k) deriv[k] = 0.0;
@@ -1141,12 +1178,20 @@ static double mattes_evaluate_sparse(const orc_problem *pb, const orc_mattes_geo
             double fp[3], mp[3], mv, grad[3];
             write_point(pb, flat, fp);
             double fv = (double)pb->fvals[flat];
-            orc_transform_point(t, fp, mp);
+            composed_point(t, init, fp, mp);
             if (!orc_moving_value_and_physical_gradient(pb, mp, &mv, grad)) continue;
             if (mv < g->moving_true_min || mv > g->moving_true_max) continue;
             long start[3];
             double w[3][4];
             if (!bspline_support(t, fp, start, w)) continue;
+            if (init) {
+                double gq[3];
+                for (int d = 0; d < 3; ++d) {
+                    gq[d] = 0.0;
+                    for (int r = 0; r < 3; ++r) gq[d] += init->matrix[r * 3 + d] * grad[r];
+                }
+                grad[0] = gq[0]; grad[1] = gq[1]; grad[2] = gq[2];
+            }
             double mterm = mv / g->moving_bin_size - g->moving_normalized_min;
             size_t mi = orc_parzen_window_index(g, mv, g->moving_bin_size, g->moving_normalized_min);
             size_t fi = orc_parzen_window_index(g, fv, g->fixed_bin_size, g->fixed_normalized_min);

@@ -177,6 +177,13 @@ double orc_mattes_value(const orc_problem *pb, const orc_mattes_geom *g,
 /* evaluate(): global support for dense transforms, sparse for B-spline. */
 double orc_mattes_evaluate(const orc_problem *pb, const orc_mattes_geom *g,
                            const orc_transform *t, double *deriv, size_t *valid);
+/* The sparse (B-spline) evaluation with a moving-initial transform applied
+ * after it: moving_point = init(t(x)) (registration/method.rs:198-340); init
+ * must be linear (translation, Euler3D, affine) or NULL. Returns -1 for a
+ * non-B-spline t. */
+double orc_mattes_evaluate_bspline_composed(const orc_problem *pb, const orc_mattes_geom *g,
+                                            const orc_transform *t, const orc_transform *init,
+                                            double *deriv, size_t *valid);
 /* The 12 pRatio-weighted moments A[3], B[3][3] (src/cuda/ops/mattes.rs:75-91)
  * in FP64, for pinning the moment form against the dense form. */
 void orc_mattes_moments(const orc_problem *pb, const orc_mattes_geom *g,

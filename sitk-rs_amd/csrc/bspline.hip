@@ -27,6 +27,8 @@
 #include <string>
 #include <vector>
 
+#include "lbfgsb.hpp"
+
 #include "common.hpp"
 #include "image.hpp"
 #include "metric.hpp"
@@ -40,6 +42,11 @@ struct BSplineMap {
     double origin[3];   // grid origin
     double p2i[9];      // physical -> grid continuous index
     const double *coef; // [3][ncp]
+    // A moving-initial transform applied after the B-spline, as one
+    // matrix-offset stage (registration/method.rs:198-240: moving point =
+    // moving_initial(optimized(x))); post == 0: none.
+    int post;
+    double stage[12];   // matrix row-major [9], offset [3]
 };
 
 template <>
@@ -94,11 +101,26 @@ __host__ __device__ inline bool bspline_support(const BSplineMap &B, const doubl
     return true;
 }
 
+// p = stage(q), one matrix-offset stage as the stage replay rounds it.
+__device__ __forceinline__ void apply_post_stage(const BSplineMap &B, double p[3]) {
+    if (!B.post) return;
+    const double *m = B.stage;
+    const double q0 = p[0], q1 = p[1], q2 = p[2];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    a0 += m[0] * q0; a0 += m[1] * q1; a0 += m[2] * q2;
+    a1 += m[3] * q0; a1 += m[4] * q1; a1 += m[5] * q2;
+    a2 += m[6] * q0; a2 += m[7] * q1; a2 += m[8] * q2;
+    p[0] = a0 + m[9];
+    p[1] = a1 + m[10];
+    p[2] = a2 + m[11];
+}
+
 __device__ __forceinline__ void map_point(const BSplineMap &B, const double x[3], double p[3]) {
     int start[3];
     double w[3][4];
     if (!bspline_support(B, x, start, w)) {
         p[0] = x[0]; p[1] = x[1]; p[2] = x[2];
+        apply_post_stage(B, p);
         return;
     }
     double d0 = 0.0, d1 = 0.0, d2 = 0.0;
@@ -115,6 +137,7 @@ __device__ __forceinline__ void map_point(const BSplineMap &B, const double x[3]
     p[0] = x[0] + d0;
     p[1] = x[1] + d1;
     p[2] = x[2] + d2;
+    apply_post_stage(B, p);
 }
 
 // Value pass over the cell-grouped sample list (any order gives the same
@@ -149,12 +172,12 @@ constexpr int kCellChunk = 256;
 // One workgroup per cell: out[cell][d * 64 + (k * 16 + j * 4 + i)] =
 // sum over the cell's samples of g_d * w0[i] w1[j] w2[k].
 template <typename R>
-__global__ __launch_bounds__(kCellChunk) void bs_cells(FixedGrid F, MovingVol M, BSplineMap B,
-                                                       const uint32_t *__restrict__ idx,
-                                                       const uint8_t *__restrict__ fbin,
-                                                       const int *__restrict__ cell_begin,
-                                                       MattesParams P, const R *__restrict__ pratio,
-                                                       double *__restrict__ out) {
+__device__ __forceinline__ void bs_cells_body(const FixedGrid &F, const MovingVol &M, const BSplineMap &B,
+                                              const uint32_t *__restrict__ idx,
+                                              const uint8_t *__restrict__ fbin,
+                                              const int *__restrict__ cell_begin,
+                                              const MattesParams &P, const R *__restrict__ pratio,
+                                              double *__restrict__ out) {
     __shared__ double s_g[kCellChunk][3];
     __shared__ double s_w[kCellChunk][12];
     const int cell = blockIdx.x;
@@ -190,9 +213,22 @@ __global__ __launch_bounds__(kCellChunk) void bs_cells(FixedGrid F, MovingVol M,
                         R a = R(0);
 #pragma unroll
                         for (int k = 0; k < 4; ++k) a += row[k] * dv[k];
+                        double gq[3] = {static_cast<double>(gp[0]), static_cast<double>(gp[1]),
+                                        static_cast<double>(gp[2])};
+                        if (B.post) {
+                            // the sparse Jacobian's columns left-multiplied by the
+                            // stage's matrix: the gradient enters as M' grad
 #pragma unroll
-                        for (int d = 0; d < 3; ++d)
-                            g[d] = static_cast<double>(a) * static_cast<double>(gp[d]);
+                            for (int d = 0; d < 3; ++d) {
+                                double acc = 0.0;
+#pragma unroll
+                                for (int r = 0; r < 3; ++r)
+                                    acc += B.stage[r * 3 + d] * static_cast<double>(gp[r]);
+                                gq[d] = acc;
+                            }
+                        }
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) g[d] = static_cast<double>(a) * gq[d];
                     }
                 }
             }
@@ -214,8 +250,19 @@ __global__ __launch_bounds__(kCellChunk) void bs_cells(FixedGrid F, MovingVol M,
     if (t < 192) out[static_cast<size_t>(cell) * 192 + t] = acc;
 }
 
+template <typename R>
+__global__ __launch_bounds__(kCellChunk) void bs_cells(FixedGrid F, MovingVol M, BSplineMap B,
+                                                       const uint32_t *__restrict__ idx,
+                                                       const uint8_t *__restrict__ fbin,
+                                                       const int *__restrict__ cell_begin,
+                                                       MattesParams P, const R *__restrict__ pratio,
+                                                       double *__restrict__ out) {
+    bs_cells_body<R>(F, M, B, idx, fbin, cell_begin, P, pratio, out);
+}
+
 // deriv[d][cp] = sum over the cells whose support holds cp (cell order).
-__global__ void bs_gather(BSplineMap B, const double *__restrict__ cellT, double *__restrict__ deriv) {
+__device__ __forceinline__ void bs_gather_body(const BSplineMap &B, const double *__restrict__ cellT,
+                                               double *__restrict__ deriv) {
     const int cp = blockIdx.x * blockDim.x + threadIdx.x;
     if (cp >= B.ncp) return;
     const int m0 = B.gs[0] - 3, m1 = B.gs[1] - 3, m2 = B.gs[2] - 3;
@@ -231,6 +278,97 @@ __global__ void bs_gather(BSplineMap B, const double *__restrict__ cellT, double
             }
 #pragma unroll
     for (int d = 0; d < 3; ++d) deriv[static_cast<size_t>(d) * B.ncp + cp] = s[d];
+}
+
+__global__ void bs_gather(BSplineMap B, const double *__restrict__ cellT, double *__restrict__ deriv) {
+    bs_gather_body(B, cellT, deriv);
+}
+
+// ------------------------------------------------------- many starts at once
+// The same passes for S independent starts that share the sample set and the
+// images: blockIdx.y picks a start from the active list; each start has its
+// own coefficients, moving-initial stage, histogram copies, pRatio, tail,
+// cell partials and derivative. Per start the arithmetic is the single-start
+// passes' (bit-equal results).
+__device__ __forceinline__ BSplineMap start_map(const BSplineMap &B0, const double *__restrict__ coef,
+                                                const double *__restrict__ stage, int s) {
+    BSplineMap B = B0;
+    B.coef = coef + static_cast<size_t>(s) * 3 * B0.ncp;
+    B.post = 1;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) B.stage[i] = stage[s * 12 + i];
+    return B;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void bs_hist_multi(FixedGrid F, MovingVol M, BSplineMap B0,
+                                                        const double *__restrict__ coef,
+                                                        const double *__restrict__ stage,
+                                                        const int *__restrict__ active,
+                                                        const ChainConst *__restrict__ C, SampleSet SS,
+                                                        const uint8_t *__restrict__ fbin, MattesParams P,
+                                                        unsigned long long *__restrict__ g_joint,
+                                                        unsigned long long *__restrict__ g_marg) {
+    extern __shared__ unsigned long long s_hist[];
+    const int st = active[blockIdx.y];
+    const BSplineMap B = start_map(B0, coef, stage, st);
+    const int nb = P.bins * P.bins + P.bins;
+    const unsigned copy = blockIdx.x % kHistCopies;
+    g_joint += (static_cast<size_t>(st) * kHistCopies + copy) * P.bins * P.bins;
+    g_marg += (static_cast<size_t>(st) * kHistCopies + copy) * P.bins;
+    for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
+    __syncthreads();
+    long long begin, end;
+    unsigned logical;
+    block_slice(SS.n, begin, end, logical);
+    const IndexMap none{};
+    hist_range<R, false>(F, M, B, none, C, SS, fbin, P, s_hist, s_hist + P.bins * P.bins,
+                         static_cast<SampleRec<R> *>(nullptr), begin, end);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += kBlock) {
+        const unsigned long long v = s_hist[i];
+        if (v) atomicAdd(i < P.bins * P.bins ? g_joint + i : g_marg + (i - P.bins * P.bins), v);
+    }
+}
+
+// One workgroup per active start: the Mattes tail on its histogram copies
+// (cleared for the next evaluation), pRatio in double and float.
+__global__ __launch_bounds__(kTail) void bs_tail_multi(const int *__restrict__ active, int B, double mbs,
+                                                       unsigned long long *__restrict__ g_joint,
+                                                       unsigned long long *__restrict__ g_marg,
+                                                       double *__restrict__ pratio,
+                                                       float *__restrict__ pratio_f,
+                                                       TailOut *__restrict__ tail) {
+    extern __shared__ unsigned long long s_fold[];
+    const int st = active[blockIdx.x];
+    const size_t nb = static_cast<size_t>(B) * B;
+    mattes_tail_block(g_joint + static_cast<size_t>(st) * kHistCopies * nb,
+                      g_marg + static_cast<size_t>(st) * kHistCopies * B, B, mbs,
+                      pratio + st * nb, pratio_f + st * nb, tail + st, s_fold);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kCellChunk) void bs_cells_multi(FixedGrid F, MovingVol M, BSplineMap B0,
+                                                             const double *__restrict__ coef,
+                                                             const double *__restrict__ stage,
+                                                             const int *__restrict__ active,
+                                                             const uint32_t *__restrict__ idx,
+                                                             const uint8_t *__restrict__ fbin,
+                                                             const int *__restrict__ cell_begin,
+                                                             MattesParams P, const R *__restrict__ pratio,
+                                                             int ncells, double *__restrict__ out) {
+    const int st = active[blockIdx.y];
+    const BSplineMap B = start_map(B0, coef, stage, st);
+    bs_cells_body<R>(F, M, B, idx, fbin, cell_begin, P,
+                     pratio + static_cast<size_t>(st) * P.bins * P.bins,
+                     out + static_cast<size_t>(st) * ncells * 192);
+}
+
+__global__ void bs_gather_multi(BSplineMap B, const int *__restrict__ active, int ncells,
+                                const double *__restrict__ cellT, double *__restrict__ deriv) {
+    const int st = active[blockIdx.y];
+    bs_gather_body(B, cellT + static_cast<size_t>(st) * ncells * 192,
+                   deriv + static_cast<size_t>(st) * 3 * B.ncp);
 }
 
 // ------------------------------------------------------------------ host side
@@ -436,161 +574,309 @@ int sitkhip_bspline_mattes_value_and_derivative(sitkhip_bspline h, const double 
     });
 }
 
-}  // extern "C"
-
-// ============================================================ registration
-// A B-spline registration on the device metric, driven by a limited-memory
-// BFGS optimizer on the host (LBFGSBOptimizer with every variable unbounded,
-// src/registration/lbfgsb.rs:1590-1674: the case the reference documents as
-// reducing to unconstrained L-BFGS). Its stopping rules are ITK's: relative
-// reduction (f_k - f_{k+1}) / max(|f_k|, |f_{k+1}|, 1) <= factr * eps, the
-// gradient's infinity norm <= pgtol, the iteration and evaluation caps; the
-// result is the best point evaluated. The step search is a backtracking
-// Armijo search from the unit step (the reference uses the netlib Moré-Thuente
-// search; recorded as a deviation).
-namespace sitkhip {
-
-struct LbfgsResult {
-    double value;
-    size_t valid;
-    int iterations;
-    int evaluations;
-    int stop;  // 0 max iterations, 1 gradient, 2 relative reduction, 3 evaluations, 4 no descent
-};
-
-static double dot(const std::vector<double> &a, const std::vector<double> &b) {
-    double s = 0.0;
-    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
-    return s;
+int sitkhip_bspline_set_moving_initial(sitkhip_bspline h, const double stage[12]) {
+    return run_guarded([&] {
+        auto b = reinterpret_cast<BSplineMetric *>(h);
+        if (!b) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null handle");
+        b->B.post = stage ? 1 : 0;
+        for (int i = 0; i < 12; ++i) b->B.stage[i] = stage ? stage[i] : 0.0;
+    });
 }
 
-static LbfgsResult lbfgs(BSplineMetric &b, std::vector<double> &x, int max_it, int max_eval, int m,
-                         double factr, double pgtol) {
-    const size_t n = x.size();
-    std::vector<double> g(n), gn(n), xn(n), d(n), q(n);
-    std::vector<std::vector<double>> S, Y;
-    std::vector<double> rho;
-    LbfgsResult r{DBL_MAX, 0, 0, 0, 0};
-    size_t valid = 0;
-    double f = 0.0;
-    bspline_evaluate(b, x.data(), true, &f, g.data(), &valid);
-    r.evaluations = 1;
-    std::vector<double> best = x;
-    double fbest = f;
-    size_t vbest = valid;
-    for (;;) {
-        double ginf = 0.0;
-        for (double v : g) ginf = std::max(ginf, std::fabs(v));
-        if (ginf <= pgtol) { r.stop = 1; break; }
-        if (r.iterations >= max_it) { r.stop = 0; break; }
-        // two-loop recursion (Nocedal 1980) for d = -H g
-        q = g;
-        const int k = static_cast<int>(S.size());
-        std::vector<double> alpha(k);
-        for (int i = k - 1; i >= 0; --i) {
-            alpha[i] = rho[i] * dot(S[i], q);
-            for (size_t j = 0; j < n; ++j) q[j] -= alpha[i] * Y[i][j];
-        }
-        double gamma = 1.0 / std::max(ginf, 1e-300);  // first step: unit infinity-norm move
-        if (k > 0) gamma = dot(S[k - 1], Y[k - 1]) / dot(Y[k - 1], Y[k - 1]);
-        for (size_t j = 0; j < n; ++j) q[j] *= gamma;
-        for (int i = 0; i < k; ++i) {
-            const double beta = rho[i] * dot(Y[i], q);
-            for (size_t j = 0; j < n; ++j) q[j] += S[i][j] * (alpha[i] - beta);
-        }
-        for (size_t j = 0; j < n; ++j) d[j] = -q[j];
-        double gd = dot(g, d);
-        if (!(gd < 0.0)) {  // not a descent direction: restart from steepest descent
-            S.clear(); Y.clear(); rho.clear();
-            for (size_t j = 0; j < n; ++j) d[j] = -g[j] / std::max(ginf, 1e-300);
-            gd = dot(g, d);
-        }
-        double t = 1.0, fn = 0.0;
-        size_t vn = 0;
-        bool accepted = false;
-        for (int tries = 0; tries < 30 && r.evaluations < max_eval; ++tries) {
-            for (size_t j = 0; j < n; ++j) xn[j] = x[j] + t * d[j];
-            bspline_evaluate(b, xn.data(), true, &fn, gn.data(), &vn);
-            ++r.evaluations;
-            if (fn <= f + 1e-4 * t * gd) { accepted = true; break; }
-            t *= 0.5;
-        }
-        if (!accepted) { r.stop = r.evaluations >= max_eval ? 3 : 4; break; }
-        ++r.iterations;
-        std::vector<double> s(n), y(n);
-        for (size_t j = 0; j < n; ++j) { s[j] = xn[j] - x[j]; y[j] = gn[j] - g[j]; }
-        const double sy = dot(s, y);
-        if (sy > DBL_EPSILON * dot(y, y)) {
-            if (static_cast<int>(S.size()) == m) {
-                S.erase(S.begin()); Y.erase(Y.begin()); rho.erase(rho.begin());
-            }
-            S.push_back(s); Y.push_back(y); rho.push_back(1.0 / sy);
-        }
-        const double fprev = f;
-        x = xn; g = gn; f = fn; valid = vn;
-        if (f < fbest) { fbest = f; best = x; vbest = valid; }
-        if ((fprev - f) / std::max({std::fabs(fprev), std::fabs(f), 1.0}) <= factr * DBL_EPSILON) {
-            r.stop = 2;
-            break;
-        }
-        if (r.evaluations >= max_eval) { r.stop = 3; break; }
+}  // extern "C"
+
+// ============================================================ many starts
+namespace sitkhip {
+
+struct BSplineMulti {
+    BSplineMetric *b = nullptr;  // sample grouping and images; not owned
+    int S = 0;
+    size_t np = 0;               // parameters per start (3 ncp)
+    unsigned G = 0;              // value-pass workgroups per start
+    DeviceBuffer<double> coef, stage, pratio, cellT, deriv;
+    DeviceBuffer<float> pratio_f;
+    DeviceBuffer<unsigned long long> joint, marg;
+    DeviceBuffer<TailOut> tail;
+    DeviceBuffer<int> active;
+    double *h_coef = nullptr, *h_deriv = nullptr;  // pinned, S x np
+    TailOut *h_tail = nullptr;
+    int *h_active = nullptr;
+    ~BSplineMulti() {
+        if (h_coef) (void)hipHostFree(h_coef);
+        if (h_deriv) (void)hipHostFree(h_deriv);
+        if (h_tail) (void)hipHostFree(h_tail);
+        if (h_active) (void)hipHostFree(h_active);
     }
-    x = best;
-    r.value = fbest;
-    r.valid = vbest;
-    return r;
+};
+
+static const double kIdentityStage[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+
+// stages: S x 12 (null: every start without a moving-initial transform).
+BSplineMulti *bspline_multi_create(BSplineMetric &b, int S, const double *stages) {
+    if (S < 1 || S > 4096) fail(SITKHIP_ERR_INVALID_ARGUMENT, "start count must be 1..4096");
+    const int B = b.m->P.bins;
+    if (B > kLdsMaxBins) fail(SITKHIP_ERR_TOO_MANY_BINS, "the B-spline value pass holds at most 90 bins in LDS");
+    auto *u = new BSplineMulti();
+    try {
+        u->b = &b;
+        u->S = S;
+        u->np = 3 * static_cast<size_t>(b.B.ncp);
+        // about 4096 samples per workgroup, a multiple of the 8 histogram copies
+        size_t g = (static_cast<size_t>(b.SS.n) + 4095) / 4096;
+        g = std::max<size_t>(8, std::min<size_t>(1024, (g + 7) / 8 * 8));
+        u->G = static_cast<unsigned>(g);
+        const size_t nb = static_cast<size_t>(B) * B;
+        u->coef.alloc(S * u->np);
+        u->stage.alloc(static_cast<size_t>(S) * 12);
+        u->pratio.alloc(S * nb);
+        u->pratio_f.alloc(S * nb);
+        u->cellT.alloc(static_cast<size_t>(S) * std::max(1, b.ncells) * 192);
+        u->deriv.alloc(S * u->np);
+        u->joint.alloc(static_cast<size_t>(S) * kHistCopies * nb);
+        u->marg.alloc(static_cast<size_t>(S) * kHistCopies * B);
+        u->tail.alloc(S);
+        u->active.alloc(S);
+        HIP_OK(hipMemsetAsync(u->joint.ptr, 0, u->joint.bytes(), backend().stream));
+        HIP_OK(hipMemsetAsync(u->marg.ptr, 0, u->marg.bytes(), backend().stream));
+        HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&u->h_coef), S * u->np * sizeof(double), hipHostMallocDefault));
+        HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&u->h_deriv), S * u->np * sizeof(double), hipHostMallocDefault));
+        HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&u->h_tail), S * sizeof(TailOut), hipHostMallocDefault));
+        HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&u->h_active), S * sizeof(int), hipHostMallocDefault));
+        std::vector<double> st(static_cast<size_t>(S) * 12);
+        for (int i = 0; i < S; ++i)
+            std::memcpy(st.data() + 12 * i, stages ? stages + 12 * i : kIdentityStage, 12 * sizeof(double));
+        u->stage.upload(st.data(), st.size());
+        std::memset(u->h_coef, 0, S * u->np * sizeof(double));
+        sync();
+    } catch (...) {
+        delete u;
+        throw;
+    }
+    return u;
+}
+
+template <typename R>
+static void multi_passes(BSplineMulti &u, int nact) {
+    BSplineMetric &b = *u.b;
+    Metric &m = *b.m;
+    const int B = m.P.bins;
+    const hipStream_t st = backend().stream;
+    const size_t shm = static_cast<size_t>(B * B + B) * sizeof(unsigned long long);
+    {
+        LaunchScope scope("bs_hist_multi");
+        hipLaunchKernelGGL(bs_hist_multi<R>, dim3(u.G, nact), dim3(kBlock), shm, st, m.F, m.Mv, b.B,
+                           u.coef.ptr, u.stage.ptr, u.active.ptr, m.chain.ptr, b.SS, b.fbin.ptr, m.P,
+                           u.joint.ptr, u.marg.ptr);
+        check_launch("bs_hist_multi");
+    }
+    {
+        LaunchScope scope("bs_tail_multi");
+        hipLaunchKernelGGL(bs_tail_multi, dim3(nact), dim3(kTail), shm, st, u.active.ptr, B, m.P.mbs,
+                           u.joint.ptr, u.marg.ptr, u.pratio.ptr, u.pratio_f.ptr, u.tail.ptr);
+        check_launch("bs_tail_multi");
+    }
+    const R *pr = sizeof(R) == 4 ? reinterpret_cast<const R *>(u.pratio_f.ptr)
+                                 : reinterpret_cast<const R *>(u.pratio.ptr);
+    if (b.ncells > 0) {
+        LaunchScope scope("bs_cells_multi");
+        hipLaunchKernelGGL(bs_cells_multi<R>, dim3(b.ncells, nact), dim3(kCellChunk), 0, st, m.F, m.Mv,
+                           b.B, u.coef.ptr, u.stage.ptr, u.active.ptr, b.idx.ptr, b.fbin.ptr,
+                           b.cell_begin.ptr, m.P, pr, b.ncells, u.cellT.ptr);
+        check_launch("bs_cells_multi");
+    }
+    LaunchScope scope("bs_gather_multi");
+    hipLaunchKernelGGL(bs_gather_multi, dim3((b.B.ncp + 255) / 256, nact), dim3(256), 0, st, b.B,
+                       u.active.ptr, b.ncells, u.cellT.ptr, u.deriv.ptr);
+    check_launch("bs_gather_multi");
+}
+
+// Evaluates value and derivative of the listed starts at the coefficients
+// already in u.h_coef (start s at h_coef + s * np). Results land in u.h_tail
+// and u.h_deriv (zero derivative for a start without valid samples).
+void bspline_multi_evaluate(BSplineMulti &u, const int *act, int nact) {
+    if (nact <= 0) return;
+    if (nact > u.S) fail(SITKHIP_ERR_INVALID_ARGUMENT, "more active starts than starts");
+    for (int i = 0; i < nact; ++i) {
+        if (act[i] < 0 || act[i] >= u.S) fail(SITKHIP_ERR_INVALID_ARGUMENT, "active start out of range");
+        u.h_active[i] = act[i];
+    }
+    const hipStream_t st = backend().stream;
+    const size_t bytes = u.np * sizeof(double);
+    for (int i = 0; i < nact; ++i)
+        HIP_OK(hipMemcpyAsync(u.coef.ptr + act[i] * u.np, u.h_coef + act[i] * u.np, bytes,
+                              hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(u.active.ptr, u.h_active, nact * sizeof(int), hipMemcpyHostToDevice, st));
+    if (u.b->m->precision == SITKHIP_PRECISION_F64) multi_passes<double>(u, nact);
+    else multi_passes<float>(u, nact);
+    for (int i = 0; i < nact; ++i) {
+        HIP_OK(hipMemcpyAsync(u.h_deriv + act[i] * u.np, u.deriv.ptr + act[i] * u.np, bytes,
+                              hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(u.h_tail + act[i], u.tail.ptr + act[i], sizeof(TailOut),
+                              hipMemcpyDeviceToHost, st));
+    }
+    sync();
+    for (int i = 0; i < nact; ++i)
+        if (u.h_tail[act[i]].valid == 0.0) std::memset(u.h_deriv + act[i] * u.np, 0, bytes);
+}
+
+// Every start optimised by its own L-BFGS-B (lbfgsb.rs:1590-1746); each round
+// evaluates all starts whose optimizer asks for a point in one batched set of
+// launches. x: S x np, in (starts) and out (best points).
+static void register_starts(BSplineMulti &u, const sitkhip_bspline_config &cfg, double *x,
+                            const int *nbd, const double *lo, const double *hi,
+                            sitkhip_bspline_result *res) {
+    LbfgsbOptions o;
+    o.corrections = cfg.corrections;
+    o.factr = cfg.factr;
+    o.pgtol = cfg.pgtol;
+    o.max_iterations = cfg.iterations;
+    o.max_evaluations = cfg.max_evaluations;
+    const int S = u.S;
+    const size_t np = u.np;
+    std::vector<std::unique_ptr<Lbfgsb>> opt(S);
+    std::vector<int> act, next;
+    std::vector<double> fbest(S, 0.0);
+    std::vector<size_t> vbest(S, 0);
+    std::vector<int> nev(S, 0);
+    for (int s = 0; s < S; ++s) {
+        opt[s].reset(new Lbfgsb(np, o, lo, hi, nbd));
+        if (opt[s]->start(x + s * np)) act.push_back(s);
+    }
+    while (!act.empty()) {
+        for (int s : act) std::memcpy(u.h_coef + s * np, opt[s]->point(), np * sizeof(double));
+        bspline_multi_evaluate(u, act.data(), static_cast<int>(act.size()));
+        next.clear();
+        for (int s : act) {
+            const double f = u.h_tail[s].value;
+            if (nev[s] == 0 || f < fbest[s]) {
+                fbest[s] = f;
+                vbest[s] = static_cast<size_t>(u.h_tail[s].valid);
+            }
+            ++nev[s];
+            if (opt[s]->tell(f, u.h_deriv + s * np)) next.push_back(s);
+        }
+        act.swap(next);
+    }
+    for (int s = 0; s < S; ++s) {
+        std::memcpy(x + s * np, opt[s]->best(), np * sizeof(double));
+        res[s].value = opt[s]->best_value();
+        res[s].valid_points = vbest[s];
+        res[s].iterations = opt[s]->iterations();
+        res[s].evaluations = opt[s]->evaluations();
+        res[s].stop = opt[s]->stop();
+    }
+}
+
+// The Mattes metric and B-spline grid of a deformable registration: the
+// transform's domain is the fixed image's extent (bspline.rs:324-339) with
+// `mesh` cells per axis.
+struct BSplineSetup {
+    std::unique_ptr<Metric, void (*)(Metric *)> metric{nullptr, metric_destroy};
+    std::unique_ptr<BSplineMetric> bs;
+};
+
+static void bspline_setup(Image &f, Image &mv, const sitkhip_bspline_config &cfg, size_t ncoeffs,
+                          BSplineSetup &out) {
+    const sitkhip_geometry &g = f.geom;
+    size_t gs[3];
+    double origin[3], spacing[3], shift[3];
+    for (int d = 0; d < 3; ++d) {
+        if (cfg.mesh[d] < 1) fail(SITKHIP_ERR_INVALID_ARGUMENT, "mesh must be >= 1 per axis");
+        gs[d] = cfg.mesh[d] + 3;
+        spacing[d] = static_cast<double>(g.size[d]) * g.spacing[d] / static_cast<double>(cfg.mesh[d]);
+        shift[d] = -0.5 * spacing[d] * (3.0 - 1.0);
+    }
+    for (int r = 0; r < 3; ++r) {
+        double acc = 0.0;
+        for (int c = 0; c < 3; ++c) acc += g.direction[r * 3 + c] * shift[c];
+        origin[r] = g.origin[r] + acc;
+    }
+    const size_t ncp = gs[0] * gs[1] * gs[2];
+    if (ncoeffs != 3 * ncp) fail(SITKHIP_ERR_INVALID_ARGUMENT, "coefficient count must be 3 x (mesh+3)^3");
+    Metric *m = nullptr;
+    if (cfg.sampling == 2) {
+        const size_t cnt = static_cast<size_t>(static_cast<double>(f.len()) * cfg.sampling_percentage);
+        m = metric_create_drawn(SITKHIP_METRIC_MATTES, f, cnt, cfg.seed, mv, cfg.precision);
+    } else {
+        m = metric_create(SITKHIP_METRIC_MATTES, f, nullptr, 0, nullptr, mv, nullptr, cfg.precision);
+    }
+    out.metric.reset(m);
+    double fr[2], mr[2];
+    metric_value_ranges(*m, fr, mr);
+    sitkhip_mattes_bins bins;
+    mattes_bins_from_ranges(fr, mr, static_cast<size_t>(cfg.bins), &bins);
+    mattes_set_bins(*m, bins);
+    out.bs.reset(bspline_create(*m, gs, origin, spacing, g.direction));
 }
 
 }  // namespace sitkhip
 
-extern "C" int sitkhip_register_bspline(sitkhip_image fixed, sitkhip_image moving,
-                                        const sitkhip_bspline_config *cfg, double *coeffs,
-                                        size_t ncoeffs, sitkhip_bspline_result *result) {
+extern "C" {
+
+int sitkhip_bspline_multi_create(sitkhip_bspline h, size_t nstarts, const double *stages,
+                                 sitkhip_bspline_multi *out) {
+    return run_guarded([&] {
+        auto b = reinterpret_cast<BSplineMetric *>(h);
+        if (!b || !out) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null argument");
+        if (nstarts < 1 || nstarts > 4096) fail(SITKHIP_ERR_INVALID_ARGUMENT, "start count must be 1..4096");
+        *out = reinterpret_cast<sitkhip_bspline_multi>(
+            bspline_multi_create(*b, static_cast<int>(nstarts), stages));
+    });
+}
+
+void sitkhip_bspline_multi_free(sitkhip_bspline_multi h) { delete reinterpret_cast<BSplineMulti *>(h); }
+
+int sitkhip_bspline_multi_value_and_derivative(sitkhip_bspline_multi h, size_t nactive,
+                                               const int *active, const double *coeffs,
+                                               double *values, double *derivs, size_t *valid) {
+    return run_guarded([&] {
+        auto u = reinterpret_cast<BSplineMulti *>(h);
+        if (!u || !active || !coeffs) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null argument");
+        if (nactive > static_cast<size_t>(u->S)) fail(SITKHIP_ERR_INVALID_ARGUMENT, "more active starts than starts");
+        std::vector<int> act(active, active + nactive);
+        for (int s : act) {
+            if (s < 0 || s >= u->S) fail(SITKHIP_ERR_INVALID_ARGUMENT, "active start out of range");
+            std::memcpy(u->h_coef + s * u->np, coeffs + s * u->np, u->np * sizeof(double));
+        }
+        bspline_multi_evaluate(*u, act.data(), static_cast<int>(act.size()));
+        for (int s : act) {
+            if (values) values[s] = u->h_tail[s].value;
+            if (valid) valid[s] = static_cast<size_t>(u->h_tail[s].valid);
+            if (derivs) std::memcpy(derivs + s * u->np, u->h_deriv + s * u->np, u->np * sizeof(double));
+        }
+    });
+}
+
+int sitkhip_register_bspline_multistart(sitkhip_image fixed, sitkhip_image moving,
+                                        const sitkhip_bspline_config *cfg, size_t nstarts,
+                                        const double *stages, double *coeffs, size_t ncoeffs,
+                                        const int *bound_selection, const double *lower,
+                                        const double *upper, sitkhip_bspline_result *results,
+                                        size_t *best) {
     return run_guarded([&] {
         auto f = reinterpret_cast<Image *>(fixed);
         auto mv = reinterpret_cast<Image *>(moving);
-        if (!f || !mv || !cfg || !coeffs || !result) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null argument");
+        if (!f || !mv || !cfg || !coeffs || !results) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null argument");
+        if (nstarts < 1 || nstarts > 4096) fail(SITKHIP_ERR_INVALID_ARGUMENT, "start count must be 1..4096");
         const auto t0 = std::chrono::steady_clock::now();
-        const sitkhip_geometry &g = f->geom;
-        size_t gs[3];
-        double origin[3], spacing[3], shift[3];
-        for (int d = 0; d < 3; ++d) {
-            if (cfg->mesh[d] < 1) fail(SITKHIP_ERR_INVALID_ARGUMENT, "mesh must be >= 1 per axis");
-            gs[d] = cfg->mesh[d] + 3;
-            // the transform domain is the fixed image's extent (bspline.rs:324-339)
-            spacing[d] = static_cast<double>(g.size[d]) * g.spacing[d] / static_cast<double>(cfg->mesh[d]);
-            shift[d] = -0.5 * spacing[d] * (3.0 - 1.0);
+        BSplineSetup su;
+        bspline_setup(*f, *mv, *cfg, ncoeffs, su);
+        std::unique_ptr<BSplineMulti> u(bspline_multi_create(*su.bs, static_cast<int>(nstarts), stages));
+        register_starts(*u, *cfg, coeffs, bound_selection, lower, upper, results);
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        size_t bi = 0;
+        for (size_t s = 0; s < nstarts; ++s) {
+            results[s].seconds = secs;
+            if (results[s].value < results[bi].value) bi = s;
         }
-        for (int r = 0; r < 3; ++r) {
-            double acc = 0.0;
-            for (int c = 0; c < 3; ++c) acc += g.direction[r * 3 + c] * shift[c];
-            origin[r] = g.origin[r] + acc;
-        }
-        const size_t ncp = gs[0] * gs[1] * gs[2];
-        if (ncoeffs != 3 * ncp) fail(SITKHIP_ERR_INVALID_ARGUMENT, "coefficient count must be 3 x (mesh+3)^3");
-        Metric *m = nullptr;
-        if (cfg->sampling == 2) {
-            const size_t cnt = static_cast<size_t>(static_cast<double>(f->len()) * cfg->sampling_percentage);
-            m = metric_create_drawn(SITKHIP_METRIC_MATTES, *f, cnt, cfg->seed, *mv, cfg->precision);
-        } else {
-            m = metric_create(SITKHIP_METRIC_MATTES, *f, nullptr, 0, nullptr, *mv, nullptr, cfg->precision);
-        }
-        std::unique_ptr<Metric, void (*)(Metric *)> mh(m, metric_destroy);
-        double fr[2], mr[2];
-        metric_value_ranges(*m, fr, mr);
-        sitkhip_mattes_bins bins;
-        mattes_bins_from_ranges(fr, mr, static_cast<size_t>(cfg->bins), &bins);
-        mattes_set_bins(*m, bins);
-        std::unique_ptr<BSplineMetric> b(bspline_create(*m, gs, origin, spacing, g.direction));
-        std::vector<double> x(coeffs, coeffs + ncoeffs);
-        LbfgsResult r = lbfgs(*b, x, cfg->iterations, cfg->max_evaluations, cfg->corrections,
-                              cfg->factr, cfg->pgtol);
-        std::memcpy(coeffs, x.data(), ncoeffs * sizeof(double));
-        result->value = r.value;
-        result->valid_points = r.valid;
-        result->iterations = r.iterations;
-        result->evaluations = r.evaluations;
-        result->stop = r.stop;
-        result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (best) *best = bi;
     });
 }
+
+int sitkhip_register_bspline(sitkhip_image fixed, sitkhip_image moving, const sitkhip_bspline_config *cfg,
+                             double *coeffs, size_t ncoeffs, sitkhip_bspline_result *result) {
+    return sitkhip_register_bspline_multistart(fixed, moving, cfg, 1, nullptr, coeffs, ncoeffs, nullptr,
+                                               nullptr, nullptr, result, nullptr);
+}
+
+}  // extern "C"

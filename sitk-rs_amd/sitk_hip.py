@@ -233,6 +233,15 @@ def lib():
         L.sitkhip_bspline_mattes_value_and_derivative.argtypes = [vp, dp, dp, dp, szp]
         L.sitkhip_register_bspline.argtypes = [vp, vp, C.POINTER(BSplineConfig), dp, C.c_size_t,
                                                C.POINTER(BSplineResult)]
+        L.sitkhip_bspline_set_moving_initial.argtypes = [vp, dp]
+        L.sitkhip_bspline_multi_create.argtypes = [vp, C.c_size_t, dp, C.POINTER(vp)]
+        L.sitkhip_bspline_multi_free.argtypes = [vp]
+        L.sitkhip_bspline_multi_free.restype = None
+        L.sitkhip_bspline_multi_value_and_derivative.argtypes = [vp, C.c_size_t, C.POINTER(C.c_int),
+                                                                 dp, dp, dp, szp]
+        L.sitkhip_register_bspline_multistart.argtypes = [
+            vp, vp, C.POINTER(BSplineConfig), C.c_size_t, dp, dp, C.c_size_t, C.POINTER(C.c_int),
+            dp, dp, C.POINTER(BSplineResult), szp]
         ip = C.POINTER(C.c_int)
         L.sitkhip_lbfgsb_create.argtypes = [C.c_size_t, C.POINTER(LbfgsbOptions), dp, dp, ip,
                                             C.POINTER(vp)]
@@ -924,9 +933,62 @@ class BSplineMattes:
                                                                  C.byref(n)))
         return v.value, d, n.value
 
+    def set_moving_initial(self, stage=None):
+        """The moving-initial transform applied after the B-spline, as one
+        matrix-offset stage (12 doubles), or None for none."""
+        if stage is None:
+            _check(lib().sitkhip_bspline_set_moving_initial(self._h, None))
+        else:
+            st = np.ascontiguousarray(stage, dtype=np.float64).reshape(12)
+            _check(lib().sitkhip_bspline_set_moving_initial(self._h, _dp(st)))
+
     def free(self) -> None:
         if self._h:
             lib().sitkhip_bspline_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class BSplineMulti:
+    """Several starts of one BSplineMattes evaluated in one set of launches:
+    start s has its own coefficients and moving-initial stage (stages:
+    nstarts x 12 or None). Uses `bspline`, which must outlive it."""
+
+    def __init__(self, bspline: BSplineMattes, nstarts: int, stages=None):
+        self._bs = bspline
+        self.nstarts = int(nstarts)
+        self.nparams = bspline.nparams
+        h = C.c_void_p()
+        st = None if stages is None else np.ascontiguousarray(stages, dtype=np.float64).reshape(-1)
+        if st is not None and st.size != 12 * self.nstarts:
+            raise ValueError("12 doubles per start expected")
+        _check(lib().sitkhip_bspline_multi_create(bspline._h, self.nstarts,
+                                                  _dp(st) if st is not None else None, C.byref(h)))
+        self._h = h
+
+    def value_and_derivative(self, coeffs, active=None):
+        """coeffs: nstarts x nparams; evaluates the starts in `active` (all by
+        default); returns values, derivatives and valid counts per start (only
+        the active rows are written)."""
+        c = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(self.nstarts, self.nparams)
+        act = np.arange(self.nstarts, dtype=np.int32) if active is None else \
+            np.ascontiguousarray(active, dtype=np.int32)
+        vals = np.zeros(self.nstarts)
+        ders = np.zeros((self.nstarts, self.nparams))
+        valid = np.zeros(self.nstarts, dtype=np.uint64)
+        _check(lib().sitkhip_bspline_multi_value_and_derivative(
+            self._h, act.size, act.ctypes.data_as(C.POINTER(C.c_int)), _dp(c), _dp(vals), _dp(ders),
+            valid.ctypes.data_as(C.POINTER(C.c_size_t))))
+        return vals, ders, valid
+
+    def free(self) -> None:
+        if self._h:
+            lib().sitkhip_bspline_multi_free(self._h)
             self._h = None
 
     def __del__(self):
@@ -958,6 +1020,41 @@ def register_bspline(fixed: DeviceImage, moving: DeviceImage, mesh, coeffs=None,
     _check(lib().sitkhip_register_bspline(fixed._h, moving._h, C.byref(c), _dp(x), x.size,
                                           C.byref(r)))
     return x, r
+
+
+def register_bspline_multistart(fixed: DeviceImage, moving: DeviceImage, mesh, nstarts, stages=None,
+                                coeffs=None, bins=32, sampling=SAMPLING_NONE,
+                                sampling_percentage=1.0, seed=0, precision=PRECISION_F32,
+                                iterations=100, max_evaluations=2000, corrections=5, factr=1e7,
+                                pgtol=1e-5, bound_selection=None, lower=None, upper=None):
+    """nstarts deformable registrations of one pair, evaluated together each
+    round; start s from moving-initial stage stages[s] (12 doubles) and
+    coefficients coeffs[s]. Returns (best coefficients per start [nstarts x
+    3 ncp], [BSplineResult per start], index of the best start)."""
+    c = BSplineConfig()
+    c.bins, c.sampling, c.sampling_percentage, c.seed = bins, sampling, sampling_percentage, seed
+    c.precision, c.iterations, c.max_evaluations = precision, iterations, max_evaluations
+    c.corrections, c.factr, c.pgtol = corrections, factr, pgtol
+    c.mesh[:] = [int(v) for v in mesh]
+    ncp = int(np.prod([m + 3 for m in mesh]))
+    npar = 3 * ncp
+    x = np.zeros((nstarts, npar)) if coeffs is None else \
+        np.ascontiguousarray(coeffs, dtype=np.float64).reshape(nstarts, npar).copy()
+    st = None if stages is None else np.ascontiguousarray(stages, dtype=np.float64).reshape(-1)
+    sel = None if bound_selection is None else np.ascontiguousarray(bound_selection, dtype=np.int32)
+    lo = None if lower is None else np.ascontiguousarray(lower, dtype=np.float64)
+    hi = None if upper is None else np.ascontiguousarray(upper, dtype=np.float64)
+    for a in (sel, lo, hi):
+        if a is not None and a.size != npar:
+            raise ValueError("bounds are per coefficient")
+    res = (BSplineResult * nstarts)()
+    best = C.c_size_t()
+    _check(lib().sitkhip_register_bspline_multistart(
+        fixed._h, moving._h, C.byref(c), nstarts, _dp(st) if st is not None else None, _dp(x), npar,
+        sel.ctypes.data_as(C.POINTER(C.c_int)) if sel is not None else None,
+        _dp(lo) if lo is not None else None, _dp(hi) if hi is not None else None, res,
+        C.byref(best)))
+    return x, list(res), best.value
 
 
 # ------------------------------------------------------------------ L-BFGS-B

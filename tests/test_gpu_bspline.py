@@ -102,3 +102,101 @@ def test_bspline_registration_descends_and_reports_the_oracles_value(hip):
     assert res.iterations > 0 and res.evaluations >= res.iterations
     assert abs(res.value - got) <= 1e-9 * abs(got), (res.value, got)
     assert got < start - 1e-3, (start, got)
+
+
+# ------------------------------------- moving-initial transform and many starts
+def _stage(params, center):
+    return O.transform_stage(O.make_transform(O.EULER3D, params, center=center))
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_moving_initial_stage_against_the_oracle(hip, precision):
+    """moving point = moving_initial(B(x)) (registration/method.rs:198-340):
+    value, derivative (the chain rule through the stage's matrix) and valid
+    count against the oracle's composed sparse evaluation."""
+    bs, pb, g, t, coeffs, met = _case(hip, precision, (22, 20, 18), (3, 3, 3), True, seed=11)
+    params, center = [0.12, -0.08, 0.15, 0.7, -0.4, 0.3], [9.0, 11.0, 10.0]
+    rig = O.make_transform(O.EULER3D, params, center=center)
+    bs.set_moving_initial(O.transform_stage(rig))
+    want_v, want_d, want_n = pb.mattes_evaluate_composed(g, t, rig)
+    v, d, n = bs.value_and_derivative(coeffs)
+    assert n == want_n
+    tv, td = (1e-9, 1e-7) if precision == 1 else (1e-6, 1e-4)
+    assert abs(v - want_v) <= tv * abs(want_v), (v, want_v)
+    assert rel_vec(d, want_d) < td, rel_vec(d, want_d)
+    bs.set_moving_initial(None)
+    v0, d0, _ = bs.value_and_derivative(coeffs)
+    want0, _, _ = pb.mattes_evaluate(g, t)
+    assert abs(v0 - want0) <= tv * abs(want0)
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_many_starts_equal_one_start_at_a_time(hip, precision):
+    """Each start of the batched evaluation is bit-equal to the single-start
+    evaluation with the same stage, whichever starts are active."""
+    bs, pb, g, t, coeffs, met = _case(hip, precision, (22, 20, 18), (3, 3, 3), True, seed=12)
+    rng = np.random.default_rng(7)
+    S = 5
+    stages = [None if s == 0 else _stage(rng.uniform(-0.15, 0.15, 3).tolist()
+                                         + rng.uniform(-1, 1, 3).tolist(), [9.0, 11.0, 10.0])
+              for s in range(S)]
+    st = np.stack([np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0.0]) if x is None else x for x in stages])
+    cs = np.stack([rng.normal(0.0, 0.5, bs.nparams) for _ in range(S)])
+    multi = hip.BSplineMulti(bs, S, st)
+    for active in (None, [3, 1], [4]):
+        vals, ders, valid = multi.value_and_derivative(cs, active)
+        for s in (range(S) if active is None else active):
+            bs.set_moving_initial(stages[s])
+            v, d, n = bs.value_and_derivative(cs[s])
+            assert vals[s] == v and valid[s] == n and np.array_equal(ders[s], d), s
+    bs.set_moving_initial(None)
+
+
+def _warped_pair(hip, n=32, seed=4):
+    sp, org = (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)
+    f = blob_volume((n, n, n), 123, nblobs=16)
+    rng = np.random.default_rng(seed)
+    mesh = (2, 2, 2)
+    true = rng.normal(0.0, 1.2, 3 * 5 ** 3)
+    t_true = O.make_bspline((n, n, n), sp, org, np.eye(3), mesh, true)
+    m = O.resample_linear(f, sp, org, (n, n, n), sp, org, t_true)
+    return (hip.DeviceImage.upload(f, spacing=sp, origin=org),
+            hip.DeviceImage.upload(m, spacing=sp, origin=org), mesh)
+
+
+def test_multistart_registration_equals_one_start_at_a_time(hip):
+    """Each start of a batched multi-start registration ends exactly where the
+    same start registered alone ends (same evaluations, so the same L-BFGS-B
+    iterates), and the best start is the lowest value."""
+    df, dm, mesh = _warped_pair(hip)
+    S = 4
+    rng = np.random.default_rng(9)
+    stages = np.stack([_stage(rng.uniform(-0.05, 0.05, 3).tolist() + rng.uniform(-0.8, 0.8, 3).tolist(),
+                              [16.0, 16.0, 16.0]) for _ in range(S)])
+    kw = dict(bins=24, iterations=15, sampling=hip.SAMPLING_RANDOM, sampling_percentage=0.5, seed=5)
+    xs, rs, best = hip.register_bspline_multistart(df, dm, mesh, S, stages=stages, **kw)
+    for s in range(S):
+        x1, r1, _ = hip.register_bspline_multistart(df, dm, mesh, 1, stages=stages[s:s + 1], **kw)
+        assert np.array_equal(x1[0], xs[s]), s
+        assert (r1[0].value, r1[0].iterations, r1[0].evaluations, r1[0].stop) == \
+            (rs[s].value, rs[s].iterations, rs[s].evaluations, rs[s].stop), s
+    assert rs[best].value == min(r.value for r in rs)
+    assert all(r.iterations > 0 for r in rs)
+
+
+def test_bounded_multistart_stays_in_the_box(hip):
+    df, dm, mesh = _warped_pair(hip, seed=6)
+    npar = 3 * 5 ** 3
+    lo, hi = -0.4 * np.ones(npar), 0.3 * np.ones(npar)
+    sel = np.full(npar, 2, dtype=np.int32)
+    sel[::7] = 0  # some unbounded
+    xs, rs, best = hip.register_bspline_multistart(
+        df, dm, mesh, 2, bins=24, iterations=20, precision=hip.PRECISION_F64,
+        coeffs=np.stack([np.zeros(npar), np.full(npar, 0.1)]),
+        bound_selection=sel, lower=lo, upper=hi)
+    for s in range(2):
+        b = xs[s][sel == 2]
+        assert np.all((b >= -0.4) & (b <= 0.3)), s
+        assert rs[s].iterations > 0
+    # some coefficients end on a bound: the box is active
+    assert np.any(np.isclose(xs[0][sel == 2], -0.4) | np.isclose(xs[0][sel == 2], 0.3))

@@ -494,3 +494,60 @@ def test_bspline_grid_geometry_is_the_references():
     assert list(t.grid_spacing) == [2.0, 2.0, 2.0]
     assert list(t.grid_origin) == [-1.0, 0.0, 1.0]
     assert t.nparams == 3 * 5 * 7 * 6
+
+
+# ------------------------- B-spline with a moving-initial transform (oracle)
+def _composed_problem():
+    n = 16
+    f = O.fixture_volume(n, [0, 0, 0])
+    m = O.fixture_volume(n, [0.6, -0.4, 0.3])
+    sp, org = [1.0, 1.0, 1.0], [0.0, 0.0, 0.0]
+    pb = O.MetricProblem(f, sp, org, m, sp, org)
+    return pb, pb.mattes_geometry(20)
+
+
+def test_composition_without_initial_is_the_plain_sparse_evaluation():
+    pb, g = _composed_problem()
+    rng = np.random.default_rng(4)
+    t, c, ncp = _bspline(lambda c, n_: c.__setitem__(slice(None), rng.normal(0, 0.2, c.size)),
+                         mesh=(2, 2, 2))
+    v0, d0, n0 = pb.mattes_evaluate(g, t)
+    v1, d1, n1 = pb.mattes_evaluate_composed(g, t, None)
+    assert v1 == v0 and n1 == n0 and np.array_equal(d1, d0)
+
+
+def test_zero_bspline_then_rigid_is_the_rigid_transform():
+    # moving_initial(optimized(x)) (registration/method.rs:198-240): with every
+    # coefficient zero the B-spline maps x to x exactly, so the composition is
+    # the moving-initial transform alone, bit for bit.
+    pb, g = _composed_problem()
+    t, c, ncp = _bspline(lambda c, n_: None, mesh=(2, 2, 2))
+    rig = O.make_transform(O.EULER3D, [0.04, -0.03, 0.05, 0.4, -0.3, 0.2], center=[7.5, 7.5, 7.5])
+    v0, _, n0 = pb.mattes_evaluate(g, rig)
+    v1, d1, n1 = pb.mattes_evaluate_composed(g, t, rig)
+    assert v1 == v0 and n1 == n0
+    assert np.any(d1 != 0.0)
+
+
+def test_composed_sparse_derivative_matches_finite_difference():
+    # The sparse Jacobian's columns are left-multiplied by the moving-initial
+    # transform's matrix (method.rs:313-340); a rotation is not symmetric, so
+    # applying its matrix untransposed, or not at all, fails this check.
+    pb, g = _composed_problem()
+    rng = np.random.default_rng(5)
+    t, c, ncp = _bspline(lambda c, n_: c.__setitem__(slice(None), rng.normal(0, 0.2, c.size)),
+                         mesh=(2, 2, 2))
+    rig = O.make_transform(O.EULER3D, [0.25, -0.2, 0.3, 0.4, -0.3, 0.2], center=[7.5, 7.5, 7.5])
+    v0, d0, _ = pb.mattes_evaluate_composed(g, t, rig)
+    _, d_plain, _ = pb.mattes_evaluate(g, t)
+    assert not np.allclose(d0, d_plain, rtol=1e-3)
+    big = np.argsort(-np.abs(d0))[:8]
+    for k in big:
+        h = 1e-5
+        c[k] += h
+        vp, _, _ = pb.mattes_evaluate_composed(g, t, rig)
+        c[k] -= 2 * h
+        vm, _, _ = pb.mattes_evaluate_composed(g, t, rig)
+        c[k] += h
+        fd = (vp - vm) / (2 * h)
+        assert abs(fd - d0[k]) < 5e-3 * max(1.0, abs(d0[k])), (k, fd, d0[k])
