@@ -140,191 +140,132 @@ __device__ __forceinline__ void map_point(const BSplineMap &B, const double x[3]
     apply_post_stage(B, p);
 }
 
-// Value pass over the cell-grouped sample list (any order gives the same
-// integer histogram).
-template <typename R>
-__global__ __launch_bounds__(kBlock) void bs_hist(FixedGrid F, MovingVol M, BSplineMap B,
-                                                  const ChainConst *__restrict__ C, SampleSet SS, const uint8_t *__restrict__ fbin,
-                                                  MattesParams P, unsigned long long *__restrict__ g_joint,
-                                                  unsigned long long *__restrict__ g_marg) {
-    extern __shared__ unsigned long long s_hist[];
-    const int nb = P.bins * P.bins + P.bins;
-    const unsigned copy = blockIdx.x % kHistCopies;
-    g_joint += static_cast<size_t>(copy) * P.bins * P.bins;
-    g_marg += copy * P.bins;
-    for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
-    __syncthreads();
-    long long begin, end;
-    unsigned logical;
-    block_slice(SS.n, begin, end, logical);
-    const IndexMap none{};
-    hist_range<R, false>(F, M, B, none, C, SS, fbin, P, s_hist, s_hist + P.bins * P.bins,
-                         static_cast<SampleRec<R> *>(nullptr), begin, end);
-    __syncthreads();
-    for (int i = threadIdx.x; i < nb; i += kBlock) {
-        const unsigned long long v = s_hist[i];
-        if (v) atomicAdd(i < P.bins * P.bins ? g_joint + i : g_marg + (i - P.bins * P.bins), v);
-    }
-}
-
-constexpr int kCellChunk = 256;
-
-// One workgroup per cell: out[cell][d * 64 + (k * 16 + j * 4 + i)] =
-// sum over the cell's samples of g_d * w0[i] w1[j] w2[k].
-template <typename R>
-__device__ __forceinline__ void bs_cells_body(const FixedGrid &F, const MovingVol &M, const BSplineMap &B,
-                                              const uint32_t *__restrict__ idx,
-                                              const uint8_t *__restrict__ fbin,
-                                              const int *__restrict__ cell_begin,
-                                              const MattesParams &P, const R *__restrict__ pratio,
-                                              double *__restrict__ out) {
-    __shared__ double s_g[kCellChunk][3];
-    __shared__ double s_w[kCellChunk][12];
-    const int cell = blockIdx.x;
-    const int b = cell_begin[cell], e = cell_begin[cell + 1];
-    const int t = threadIdx.x;
-    double acc = 0.0;
-    const int od = t / 64, oi = t & 3, oj = (t >> 2) & 3, ok = (t >> 4) & 3;
-    for (int s0 = b; s0 < e; s0 += kCellChunk) {
-        const int s = s0 + t;
-        double g[3] = {0.0, 0.0, 0.0};
-        double w[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
-        if (s < e && fbin[s] != kNoBin) {
-            double x[3], p[3];
-            fixed_point(F, idx[s], x);
-            int start[3];
-            if (bspline_support(B, x, start, w)) {
-                map_point(B, x, p);
-                Cell cl;
-                if (locate(M, p, cl)) {
-                    float v[8];
-                    corners(M, cl, v);
-                    R fr[3] = {static_cast<R>(cl.frac[0]), static_cast<R>(cl.frac[1]),
-                               static_cast<R>(cl.frac[2])};
-                    R mv, gi[3], gp[3];
-                    Trilinear<R>::template eval<true>(fr, v, mv, gi);
-                    if (Parzen<R>::in_range(P, mv)) {
-                        physical_gradient<R>(M, gi, gp);
-                        R arg[4], dv[4];
-                        bool mid;
-                        const int m0 = Parzen<R>::taps(P, mv, arg, mid);
-                        parzen_derivs<R>(arg, mid, dv);
-                        const R *row = pratio + fbin[s] * P.bins + m0;
-                        R a = R(0);
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) a += row[k] * dv[k];
-                        double gq[3] = {static_cast<double>(gp[0]), static_cast<double>(gp[1]),
-                                        static_cast<double>(gp[2])};
-                        if (B.post) {
-                            // the sparse Jacobian's columns left-multiplied by the
-                            // stage's matrix: the gradient enters as M' grad
-#pragma unroll
-                            for (int d = 0; d < 3; ++d) {
-                                double acc = 0.0;
-#pragma unroll
-                                for (int r = 0; r < 3; ++r)
-                                    acc += B.stage[r * 3 + d] * static_cast<double>(gp[r]);
-                                gq[d] = acc;
-                            }
-                        }
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) g[d] = static_cast<double>(a) * gq[d];
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int d = 0; d < 3; ++d) s_g[t][d] = g[d];
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) s_w[t][d * 4 + k] = w[d][k];
-        __syncthreads();
-        if (t < 192) {
-            const int n = min(kCellChunk, e - s0);
-            for (int q = 0; q < n; ++q)
-                acc += s_g[q][od] * ((s_w[q][oi] * s_w[q][4 + oj]) * s_w[q][8 + ok]);
-        }
-        __syncthreads();
-    }
-    if (t < 192) out[static_cast<size_t>(cell) * 192 + t] = acc;
-}
-
-template <typename R>
-__global__ __launch_bounds__(kCellChunk) void bs_cells(FixedGrid F, MovingVol M, BSplineMap B,
-                                                       const uint32_t *__restrict__ idx,
-                                                       const uint8_t *__restrict__ fbin,
-                                                       const int *__restrict__ cell_begin,
-                                                       MattesParams P, const R *__restrict__ pratio,
-                                                       double *__restrict__ out) {
-    bs_cells_body<R>(F, M, B, idx, fbin, cell_begin, P, pratio, out);
-}
-
-// deriv[d][cp] = sum over the cells whose support holds cp (cell order).
-__device__ __forceinline__ void bs_gather_body(const BSplineMap &B, const double *__restrict__ cellT,
-                                               double *__restrict__ deriv) {
-    const int cp = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cp >= B.ncp) return;
-    const int m0 = B.gs[0] - 3, m1 = B.gs[1] - 3, m2 = B.gs[2] - 3;
-    const int px = cp % B.gs[0], py = (cp / B.gs[0]) % B.gs[1], pz = cp / (B.gs[0] * B.gs[1]);
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int sz = max(0, pz - 3); sz <= min(m2 - 1, pz); ++sz)
-        for (int sy = max(0, py - 3); sy <= min(m1 - 1, py); ++sy)
-            for (int sx = max(0, px - 3); sx <= min(m0 - 1, px); ++sx) {
-                const int cell = sx + m0 * (sy + m1 * sz);
-                const int o = (px - sx) + 4 * (py - sy) + 16 * (pz - sz);
-#pragma unroll
-                for (int d = 0; d < 3; ++d) s[d] += cellT[static_cast<size_t>(cell) * 192 + d * 64 + o];
-            }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) deriv[static_cast<size_t>(d) * B.ncp + cp] = s[d];
-}
-
-__global__ void bs_gather(BSplineMap B, const double *__restrict__ cellT, double *__restrict__ deriv) {
-    bs_gather_body(B, cellT, deriv);
-}
-
-// ------------------------------------------------------- many starts at once
-// The same passes for S independent starts that share the sample set and the
-// images: blockIdx.y picks a start from the active list; each start has its
-// own coefficients, moving-initial stage, histogram copies, pRatio, tail,
-// cell partials and derivative. Per start the arithmetic is the single-start
-// passes' (bit-equal results).
+// ----------------------------------------------------------------- passes
+// One launch serves any number of starts (independent optimisations of one
+// pair that share the sample set and the images): blockIdx.y picks a start
+// from the active list; each start has its own coefficients, moving-initial
+// stage, histogram copies, pRatio, tail, records, cell partials and
+// derivative. A single evaluation is the one-start case.
+//
+// The start's map: coefficients coef + s * 3 ncp and stage s (when given; else
+// the base map's own).
 __device__ __forceinline__ BSplineMap start_map(const BSplineMap &B0, const double *__restrict__ coef,
                                                 const double *__restrict__ stage, int s) {
     BSplineMap B = B0;
-    B.coef = coef + static_cast<size_t>(s) * 3 * B0.ncp;
-    B.post = 1;
+    if (coef) B.coef = coef + static_cast<size_t>(s) * 3 * B0.ncp;
+    if (stage) {
+        B.post = 1;
 #pragma unroll
-    for (int i = 0; i < 12; ++i) B.stage[i] = stage[s * 12 + i];
+        for (int i = 0; i < 12; ++i) B.stage[i] = stage[s * 12 + i];
+    }
     return B;
 }
 
+constexpr int kCellsPerBlock = 4;  // cells per value-pass workgroup
+
+// The 3 x 64 coefficients of a cell's support, s[d * 64 + k * 16 + j * 4 + i].
+__device__ __forceinline__ void stage_cell_coefs(const BSplineMap &B, int cell, double *s) {
+    const int m0 = B.gs[0] - 3, m1 = B.gs[1] - 3;
+    const int sx = cell % m0, sy = (cell / m0) % m1, sz = cell / (m0 * m1);
+    for (int t = threadIdx.x; t < 192; t += blockDim.x) {
+        const int d = t >> 6, o = t & 63;
+        const int i = o & 3, j = (o >> 2) & 3, k = o >> 4;
+        const int cp = (sx + i) + B.gs[0] * ((sy + j) + B.gs[1] * (sz + k));
+        s[t] = B.coef[static_cast<size_t>(d) * B.ncp + cp];
+    }
+}
+
+// map_point with the support's coefficients read from LDS: the same sums in
+// the same order. A point of the cell always has the cell's support (the
+// samples were grouped by this very function).
+__device__ __forceinline__ void map_point_cell(const BSplineMap &B, const double x[3],
+                                               const double *sc, double p[3]) {
+    int start[3];
+    double w[3][4];
+    if (!bspline_support(B, x, start, w)) {
+        p[0] = x[0]; p[1] = x[1]; p[2] = x[2];
+        apply_post_stage(B, p);
+        return;
+    }
+    // one k-slab (48 LDS reads) unrolled so the reads are issued ahead of the
+    // dependent sums
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double wt = (w[0][i] * w[1][j]) * w[2][k];
+                const int o = k * 16 + j * 4 + i;
+                d0 += wt * sc[o];
+                d1 += wt * sc[64 + o];
+                d2 += wt * sc[128 + o];
+            }
+    p[0] = x[0] + d0;
+    p[1] = x[1] + d1;
+    p[2] = x[2] + d2;
+    apply_post_stage(B, p);
+}
+
+// Value pass, kCellsPerBlock cells per workgroup (the last bucket, cell ==
+// ncells, holds the samples outside the valid region): the cell's support
+// coefficients in LDS, the joint histogram in LDS, one record per sample (the
+// moving value and its index gradient, NaN value = dropped) for the
+// derivative pass. Any grouping gives the same integer histogram.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void bs_hist_multi(FixedGrid F, MovingVol M, BSplineMap B0,
-                                                        const double *__restrict__ coef,
-                                                        const double *__restrict__ stage,
-                                                        const int *__restrict__ active,
-                                                        const ChainConst *__restrict__ C, SampleSet SS,
-                                                        const uint8_t *__restrict__ fbin, MattesParams P,
-                                                        unsigned long long *__restrict__ g_joint,
-                                                        unsigned long long *__restrict__ g_marg) {
+__global__ __launch_bounds__(kBlock) void bs_value(FixedGrid F, MovingVol M, BSplineMap B0,
+                                                   const double *__restrict__ coef,
+                                                   const double *__restrict__ stage,
+                                                   const int *__restrict__ active,
+                                                   const ChainConst *__restrict__ C,
+                                                   const uint32_t *__restrict__ idx,
+                                                   const uint8_t *__restrict__ fbin,
+                                                   const int *__restrict__ cell_begin, int ncells,
+                                                   MattesParams P, unsigned long long *__restrict__ g_joint,
+                                                   unsigned long long *__restrict__ g_marg,
+                                                   SampleRec<R> *__restrict__ rec, long long n) {
     extern __shared__ unsigned long long s_hist[];
+    __shared__ double s_coef[192];
     const int st = active[blockIdx.y];
     const BSplineMap B = start_map(B0, coef, stage, st);
     const int nb = P.bins * P.bins + P.bins;
+    unsigned long long *h_joint = s_hist, *h_marg = s_hist + P.bins * P.bins;
+    SampleRec<R> *r = rec + static_cast<size_t>(st) * n;
+    for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
+    for (int q = 0; q < kCellsPerBlock; ++q) {
+        const int cell = blockIdx.x * kCellsPerBlock + q;
+        if (cell > ncells) break;  // the same for every lane
+        __syncthreads();           // the previous cell's coefficients are no longer read
+        if (cell < ncells) stage_cell_coefs(B, cell, s_coef);
+        __syncthreads();
+        const int e = cell_begin[cell + 1];
+        for (int s = cell_begin[cell] + threadIdx.x; s < e; s += kBlock) {
+            const int fb = fbin[s];
+            Cell cl;
+            bool ok = false;
+            if (fb != kNoBin) {
+                double x[3], p[3];
+                fixed_point(F, *C, idx[s], x);
+                map_point_cell(B, x, s_coef, p);
+                ok = locate(M, *C, p, cl);
+            }
+            if (!ok) {
+                cl.base[0] = cl.base[1] = cl.base[2] = 0;
+                cl.frac[0] = cl.frac[1] = cl.frac[2] = 0.0;
+            }
+            R fr[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) fr[d] = static_cast<R>(cl.frac[d]);
+            float v[8];
+            corners(M, cl, v);
+            hist_sample<R, true>(P, h_joint, h_marg, r, s, ok, fb, fr, v);
+        }
+    }
+    __syncthreads();
     const unsigned copy = blockIdx.x % kHistCopies;
     g_joint += (static_cast<size_t>(st) * kHistCopies + copy) * P.bins * P.bins;
     g_marg += (static_cast<size_t>(st) * kHistCopies + copy) * P.bins;
-    for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
-    __syncthreads();
-    long long begin, end;
-    unsigned logical;
-    block_slice(SS.n, begin, end, logical);
-    const IndexMap none{};
-    hist_range<R, false>(F, M, B, none, C, SS, fbin, P, s_hist, s_hist + P.bins * P.bins,
-                         static_cast<SampleRec<R> *>(nullptr), begin, end);
-    __syncthreads();
     for (int i = threadIdx.x; i < nb; i += kBlock) {
         const unsigned long long v = s_hist[i];
         if (v) atomicAdd(i < P.bins * P.bins ? g_joint + i : g_marg + (i - P.bins * P.bins), v);
@@ -333,12 +274,12 @@ __global__ __launch_bounds__(kBlock) void bs_hist_multi(FixedGrid F, MovingVol M
 
 // One workgroup per active start: the Mattes tail on its histogram copies
 // (cleared for the next evaluation), pRatio in double and float.
-__global__ __launch_bounds__(kTail) void bs_tail_multi(const int *__restrict__ active, int B, double mbs,
-                                                       unsigned long long *__restrict__ g_joint,
-                                                       unsigned long long *__restrict__ g_marg,
-                                                       double *__restrict__ pratio,
-                                                       float *__restrict__ pratio_f,
-                                                       TailOut *__restrict__ tail) {
+__global__ __launch_bounds__(kTail) void bs_tail(const int *__restrict__ active, int B, double mbs,
+                                                 unsigned long long *__restrict__ g_joint,
+                                                 unsigned long long *__restrict__ g_marg,
+                                                 double *__restrict__ pratio,
+                                                 float *__restrict__ pratio_f,
+                                                 TailOut *__restrict__ tail) {
     extern __shared__ unsigned long long s_fold[];
     const int st = active[blockIdx.x];
     const size_t nb = static_cast<size_t>(B) * B;
@@ -347,42 +288,149 @@ __global__ __launch_bounds__(kTail) void bs_tail_multi(const int *__restrict__ a
                       pratio + st * nb, pratio_f + st * nb, tail + st, s_fold);
 }
 
+// Derivative pass, one workgroup per (cell, start), from the value pass's
+// records: per valid sample the pRatio-weighted B3' sum a times the moving
+// gradient (through the stage's transposed matrix when there is one), g_d;
+// then out[cell][16 n + m] = sum over the cell's samples of u_m t_n with
+// u_m = w0[m % 4] w1[m / 4] and t_n = g_{n / 4} w2[n % 4] (n < 12), i.e.
+// out[d * 64 + k * 16 + j * 4 + i] = sum of g_d w0[i] w1[j] w2[k]. The sum
+// is a 16 x 4 by 4 x 16 FP64 matrix product per four samples
+// (v_mfma_f64_16x16x4f64): each lane stages its sample's weights and g in
+// LDS, then lane l feeds A[l % 16][l / 16] = u and B[l / 16][l % 16] = t of
+// sample 4 r + l / 16. The four waves' tiles are added in a fixed order.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+static_assert(kBlock == 256, "the derivative pass adds four waves' tiles");
+
 template <typename R>
-__global__ __launch_bounds__(kCellChunk) void bs_cells_multi(FixedGrid F, MovingVol M, BSplineMap B0,
-                                                             const double *__restrict__ coef,
-                                                             const double *__restrict__ stage,
-                                                             const int *__restrict__ active,
-                                                             const uint32_t *__restrict__ idx,
-                                                             const uint8_t *__restrict__ fbin,
-                                                             const int *__restrict__ cell_begin,
-                                                             MattesParams P, const R *__restrict__ pratio,
-                                                             int ncells, double *__restrict__ out) {
+__global__ __launch_bounds__(kBlock) void bs_deriv(FixedGrid F, MovingVol M, BSplineMap B0,
+                                                   const double *__restrict__ stage,
+                                                   const int *__restrict__ active,
+                                                   const uint32_t *__restrict__ idx,
+                                                   const int *__restrict__ cell_begin,
+                                                   const uint8_t *__restrict__ fbin,
+                                                   MattesParams P, const R *__restrict__ pratio,
+                                                   const SampleRec<R> *__restrict__ rec, long long n,
+                                                   int ncells, double *__restrict__ out) {
+    __shared__ double s_x[kBlock / 64][64][16];  // per lane: w[3][4], g[3], pad
     const int st = active[blockIdx.y];
-    const BSplineMap B = start_map(B0, coef, stage, st);
-    bs_cells_body<R>(F, M, B, idx, fbin, cell_begin, P,
-                     pratio + static_cast<size_t>(st) * P.bins * P.bins,
-                     out + static_cast<size_t>(st) * ncells * 192);
+    const BSplineMap B = start_map(B0, nullptr, stage, st);
+    const SampleRec<R> *r = rec + static_cast<size_t>(st) * n;
+    pratio += static_cast<size_t>(st) * P.bins * P.bins;
+    const int cell = blockIdx.x;
+    const int b = cell_begin[cell], e = cell_begin[cell + 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c = lane & 15, g4 = lane >> 4;
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int s0 = b; s0 < e; s0 += kBlock) {
+        const int s = s0 + threadIdx.x;
+        double g[3] = {0.0, 0.0, 0.0};
+        double w[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+        if (s < e) {
+            const SampleRec<R> rr = r[s];
+            if (rr.mv == rr.mv) {  // a sample the value pass kept
+                double x[3];
+                fixed_point(F, idx[s], x);
+                int start[3];
+                bspline_support(B, x, start, w);
+                R gp[3];
+                physical_gradient<R>(M, rr.gi, gp);
+                R arg[4], dv[4];
+                bool mid;
+                const int m0 = Parzen<R>::taps(P, rr.mv, arg, mid);
+                parzen_derivs<R>(arg, mid, dv);
+                const R *row = pratio + fbin[s] * P.bins + m0;
+                R a = R(0);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) a += row[k] * dv[k];
+                double gq[3] = {static_cast<double>(gp[0]), static_cast<double>(gp[1]),
+                                static_cast<double>(gp[2])};
+                if (B.post) {
+                    // the sparse Jacobian's columns left-multiplied by the
+                    // stage's matrix: the gradient enters as M' grad
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        double acc2 = 0.0;
+#pragma unroll
+                        for (int q = 0; q < 3; ++q)
+                            acc2 += B.stage[q * 3 + d] * static_cast<double>(gp[q]);
+                        gq[d] = acc2;
+                    }
+                }
+#pragma unroll
+                for (int d = 0; d < 3; ++d) g[d] = static_cast<double>(a) * gq[d];
+            }
+        }
+        double *x = s_x[wv][lane];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            x[k] = w[0][k];
+            x[4 + k] = w[1][k];
+            x[8 + k] = w[2][k];
+        }
+        x[12] = g[0];
+        x[13] = g[1];
+        x[14] = g[2];
+        __syncthreads();
+#pragma unroll 4
+        for (int q = 0; q < 16; ++q) {
+            const double *y = s_x[wv][4 * q + g4];
+            const double ua = y[c & 3] * y[4 + (c >> 2)];
+            const double tb = c < 12 ? y[12 + (c >> 2)] * y[8 + (c & 3)] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ua, tb, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // accumulator i of lane l is tile element (row l / 16 + 4 i, column l % 16);
+    // wave w's tile goes to red[w][row][column]
+    double *red = &s_x[0][0][0];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[wv * 256 + (g4 + 4 * i) * 16 + c] = acc[i];
+    __syncthreads();
+    if (threadIdx.x < 192) {
+        const int t = threadIdx.x;
+        const int o = (t & 15) * 16 + (t >> 4);
+        out[(static_cast<size_t>(st) * ncells + cell) * 192 + t] =
+            (red[o] + red[256 + o]) + (red[512 + o] + red[768 + o]);
+    }
 }
 
-__global__ void bs_gather_multi(BSplineMap B, const int *__restrict__ active, int ncells,
-                                const double *__restrict__ cellT, double *__restrict__ deriv) {
+// deriv[d][cp] = sum over the cells whose support holds cp (cell order).
+__global__ void bs_gather(BSplineMap B, const int *__restrict__ active, int ncells,
+                          const double *__restrict__ cellT, double *__restrict__ deriv) {
     const int st = active[blockIdx.y];
-    bs_gather_body(B, cellT + static_cast<size_t>(st) * ncells * 192,
-                   deriv + static_cast<size_t>(st) * 3 * B.ncp);
+    cellT += static_cast<size_t>(st) * ncells * 192;
+    deriv += static_cast<size_t>(st) * 3 * B.ncp;
+    const int cp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cp >= B.ncp) return;
+    const int m0 = B.gs[0] - 3, m1 = B.gs[1] - 3, m2 = B.gs[2] - 3;
+    const int px = cp % B.gs[0], py = (cp / B.gs[0]) % B.gs[1], pz = cp / (B.gs[0] * B.gs[1]);
+    double sum[3] = {0.0, 0.0, 0.0};
+    for (int sz = max(0, pz - 3); sz <= min(m2 - 1, pz); ++sz)
+        for (int sy = max(0, py - 3); sy <= min(m1 - 1, py); ++sy)
+            for (int sx = max(0, px - 3); sx <= min(m0 - 1, px); ++sx) {
+                const int cell = sx + m0 * (sy + m1 * sz);
+                const int o = (px - sx) + 4 * (py - sy) + 16 * (pz - sz);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) sum[d] += cellT[static_cast<size_t>(cell) * 192 + d * 64 + o];
+            }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) deriv[static_cast<size_t>(d) * B.ncp + cp] = sum[d];
 }
 
 // ------------------------------------------------------------------ host side
+struct BSplineMulti;
+void bspline_multi_free(BSplineMulti *u);
+
 struct BSplineMetric {
     Metric *m = nullptr;        // the Mattes metric (bins set); not owned
-    BSplineMap B{};
+    BSplineMap B{};             // coef unused: every evaluation brings its coefficients
     int ncells = 0;
     SampleSet SS{};             // cell-grouped samples
     DeviceBuffer<uint32_t> idx;
     DeviceBuffer<uint8_t> fbin;
-    DeviceBuffer<int> cell_begin;
-    DeviceBuffer<double> coef, cellT, deriv;
-    HostResult *res = nullptr, *res_dev = nullptr;
-    ~BSplineMetric() { pinned_slot_free(res); }
+    DeviceBuffer<int> cell_begin;  // ncells + 2 bounds: the cells, then the outside bucket
+    BSplineMulti *one = nullptr;   // the single-start evaluation (its stage: B's own)
+    ~BSplineMetric() { bspline_multi_free(one); }
 };
 
 BSplineMetric *bspline_create(Metric &m, const size_t gs[3], const double origin[3],
@@ -457,17 +505,11 @@ BSplineMetric *bspline_create(Metric &m, const size_t gs[3], const double origin
             b->idx.upload(sflat.data(), n);
             b->fbin.upload(sfb.data(), n);
         }
-        b->cell_begin.alloc(b->ncells + 1);
-        b->cell_begin.upload(begin.data(), b->ncells + 1);
+        b->cell_begin.alloc(b->ncells + 2);
+        b->cell_begin.upload(begin.data(), b->ncells + 2);
         b->SS.idx = b->idx.ptr;
         b->SS.n = static_cast<long long>(n);
-        b->coef.alloc(3 * ncp);
-        b->B.coef = b->coef.ptr;
-        b->cellT.alloc(static_cast<size_t>(b->ncells) * 192);
-        b->deriv.alloc(3 * ncp);
-        void *dev = nullptr;
-        b->res = static_cast<HostResult *>(pinned_slot_alloc(sizeof(HostResult), &dev));
-        b->res_dev = static_cast<HostResult *>(dev);
+        b->B.coef = nullptr;
         sync();
     } catch (...) {
         delete b;
@@ -476,57 +518,8 @@ BSplineMetric *bspline_create(Metric &m, const size_t gs[3], const double origin
     return b;
 }
 
-template <typename R>
-static void bs_value_pass(BSplineMetric &b) {
-    Metric &m = *b.m;
-    const int B = m.P.bins;
-    if (B > kLdsMaxBins) fail(SITKHIP_ERR_TOO_MANY_BINS, "the B-spline value pass holds at most 90 bins in LDS");
-    const size_t shm = static_cast<size_t>(B * B + B) * sizeof(unsigned long long);
-    {
-        LaunchScope scope("bs_hist");
-        hipLaunchKernelGGL(bs_hist<R>, dim3(kGrid), dim3(kBlock), shm, backend().stream, m.F, m.Mv,
-                           b.B, m.chain.ptr, b.SS, b.fbin.ptr, m.P, m.joint.ptr, m.marg.ptr);
-        check_launch("bs_hist");
-    }
-    launch_tail(m, &b.res_dev->tail);
-}
-
-template <typename R>
-static void bs_deriv_pass(BSplineMetric &b) {
-    Metric &m = *b.m;
-    const R *pr = sizeof(R) == 4 ? reinterpret_cast<const R *>(m.pratio_f.ptr)
-                                 : reinterpret_cast<const R *>(m.pratio.ptr);
-    if (b.ncells > 0) {
-        LaunchScope scope("bs_cells");
-        hipLaunchKernelGGL(bs_cells<R>, dim3(b.ncells), dim3(kCellChunk), 0, backend().stream, m.F,
-                           m.Mv, b.B, b.idx.ptr, b.fbin.ptr, b.cell_begin.ptr, m.P, pr, b.cellT.ptr);
-        check_launch("bs_cells");
-    }
-    LaunchScope scope("bs_gather");
-    hipLaunchKernelGGL(bs_gather, dim3((b.B.ncp + 255) / 256), dim3(256), 0, backend().stream, b.B,
-                       b.cellT.ptr, b.deriv.ptr);
-    check_launch("bs_gather");
-}
-
 void bspline_evaluate(BSplineMetric &b, const double *coeffs, bool derivative, double *value,
-                      double *deriv, size_t *valid) {
-    if (!coeffs) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null coefficients");
-    const size_t n = 3 * static_cast<size_t>(b.B.ncp);
-    b.coef.upload(coeffs, n);
-    const bool f64 = b.m->precision == SITKHIP_PRECISION_F64;
-    if (f64) bs_value_pass<double>(b);
-    else bs_value_pass<float>(b);
-    if (derivative) {
-        if (f64) bs_deriv_pass<double>(b);
-        else bs_deriv_pass<float>(b);
-        if (deriv) b.deriv.download(deriv, n);
-    }
-    sync();
-    if (value) *value = b.res->tail.value;
-    if (valid) *valid = static_cast<size_t>(b.res->tail.valid);
-    if (derivative && deriv && b.res->tail.valid == 0.0)
-        for (size_t i = 0; i < n; ++i) deriv[i] = 0.0;
-}
+                      double *deriv, size_t *valid);
 
 }  // namespace sitkhip
 
@@ -592,12 +585,13 @@ struct BSplineMulti {
     BSplineMetric *b = nullptr;  // sample grouping and images; not owned
     int S = 0;
     size_t np = 0;               // parameters per start (3 ncp)
-    unsigned G = 0;              // value-pass workgroups per start
+    bool staged = false;         // per-start stages (else the metric's own)
     DeviceBuffer<double> coef, stage, pratio, cellT, deriv;
     DeviceBuffer<float> pratio_f;
     DeviceBuffer<unsigned long long> joint, marg;
     DeviceBuffer<TailOut> tail;
     DeviceBuffer<int> active;
+    DeviceBuffer<unsigned char> rec;  // S x n sample records
     double *h_coef = nullptr, *h_deriv = nullptr;  // pinned, S x np
     TailOut *h_tail = nullptr;
     int *h_active = nullptr;
@@ -609,9 +603,9 @@ struct BSplineMulti {
     }
 };
 
-static const double kIdentityStage[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+void bspline_multi_free(BSplineMulti *u) { delete u; }
 
-// stages: S x 12 (null: every start without a moving-initial transform).
+// stages: S x 12, or null for the metric's own moving-initial stage.
 BSplineMulti *bspline_multi_create(BSplineMetric &b, int S, const double *stages) {
     if (S < 1 || S > 4096) fail(SITKHIP_ERR_INVALID_ARGUMENT, "start count must be 1..4096");
     const int B = b.m->P.bins;
@@ -621,11 +615,10 @@ BSplineMulti *bspline_multi_create(BSplineMetric &b, int S, const double *stages
         u->b = &b;
         u->S = S;
         u->np = 3 * static_cast<size_t>(b.B.ncp);
-        // about 4096 samples per workgroup, a multiple of the 8 histogram copies
-        size_t g = (static_cast<size_t>(b.SS.n) + 4095) / 4096;
-        g = std::max<size_t>(8, std::min<size_t>(1024, (g + 7) / 8 * 8));
-        u->G = static_cast<unsigned>(g);
+        u->staged = stages != nullptr;
         const size_t nb = static_cast<size_t>(B) * B;
+        const size_t recb = b.m->precision == SITKHIP_PRECISION_F64 ? sizeof(SampleRec<double>)
+                                                                    : sizeof(SampleRec<float>);
         u->coef.alloc(S * u->np);
         u->stage.alloc(static_cast<size_t>(S) * 12);
         u->pratio.alloc(S * nb);
@@ -636,16 +629,14 @@ BSplineMulti *bspline_multi_create(BSplineMetric &b, int S, const double *stages
         u->marg.alloc(static_cast<size_t>(S) * kHistCopies * B);
         u->tail.alloc(S);
         u->active.alloc(S);
+        u->rec.alloc(static_cast<size_t>(S) * static_cast<size_t>(b.SS.n) * recb);
         HIP_OK(hipMemsetAsync(u->joint.ptr, 0, u->joint.bytes(), backend().stream));
         HIP_OK(hipMemsetAsync(u->marg.ptr, 0, u->marg.bytes(), backend().stream));
         HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&u->h_coef), S * u->np * sizeof(double), hipHostMallocDefault));
         HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&u->h_deriv), S * u->np * sizeof(double), hipHostMallocDefault));
         HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&u->h_tail), S * sizeof(TailOut), hipHostMallocDefault));
         HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&u->h_active), S * sizeof(int), hipHostMallocDefault));
-        std::vector<double> st(static_cast<size_t>(S) * 12);
-        for (int i = 0; i < S; ++i)
-            std::memcpy(st.data() + 12 * i, stages ? stages + 12 * i : kIdentityStage, 12 * sizeof(double));
-        u->stage.upload(st.data(), st.size());
+        if (stages) u->stage.upload(stages, static_cast<size_t>(S) * 12);
         std::memset(u->h_coef, 0, S * u->np * sizeof(double));
         sync();
     } catch (...) {
@@ -656,44 +647,48 @@ BSplineMulti *bspline_multi_create(BSplineMetric &b, int S, const double *stages
 }
 
 template <typename R>
-static void multi_passes(BSplineMulti &u, int nact) {
+static void multi_passes(BSplineMulti &u, int nact, bool derivative) {
     BSplineMetric &b = *u.b;
     Metric &m = *b.m;
     const int B = m.P.bins;
     const hipStream_t st = backend().stream;
     const size_t shm = static_cast<size_t>(B * B + B) * sizeof(unsigned long long);
+    const double *stage = u.staged ? u.stage.ptr : nullptr;
+    auto *rec = reinterpret_cast<SampleRec<R> *>(u.rec.ptr);
+    const unsigned groups = static_cast<unsigned>((b.ncells + 1 + kCellsPerBlock - 1) / kCellsPerBlock);
     {
-        LaunchScope scope("bs_hist_multi");
-        hipLaunchKernelGGL(bs_hist_multi<R>, dim3(u.G, nact), dim3(kBlock), shm, st, m.F, m.Mv, b.B,
-                           u.coef.ptr, u.stage.ptr, u.active.ptr, m.chain.ptr, b.SS, b.fbin.ptr, m.P,
-                           u.joint.ptr, u.marg.ptr);
-        check_launch("bs_hist_multi");
+        LaunchScope scope("bs_value");
+        hipLaunchKernelGGL(bs_value<R>, dim3(groups, nact), dim3(kBlock), shm, st, m.F, m.Mv, b.B,
+                           u.coef.ptr, stage, u.active.ptr, m.chain.ptr, b.idx.ptr, b.fbin.ptr,
+                           b.cell_begin.ptr, b.ncells, m.P, u.joint.ptr, u.marg.ptr, rec, b.SS.n);
+        check_launch("bs_value");
     }
     {
-        LaunchScope scope("bs_tail_multi");
-        hipLaunchKernelGGL(bs_tail_multi, dim3(nact), dim3(kTail), shm, st, u.active.ptr, B, m.P.mbs,
+        LaunchScope scope("bs_tail");
+        hipLaunchKernelGGL(bs_tail, dim3(nact), dim3(kTail), shm, st, u.active.ptr, B, m.P.mbs,
                            u.joint.ptr, u.marg.ptr, u.pratio.ptr, u.pratio_f.ptr, u.tail.ptr);
-        check_launch("bs_tail_multi");
+        check_launch("bs_tail");
     }
+    if (!derivative) return;
     const R *pr = sizeof(R) == 4 ? reinterpret_cast<const R *>(u.pratio_f.ptr)
                                  : reinterpret_cast<const R *>(u.pratio.ptr);
     if (b.ncells > 0) {
-        LaunchScope scope("bs_cells_multi");
-        hipLaunchKernelGGL(bs_cells_multi<R>, dim3(b.ncells, nact), dim3(kCellChunk), 0, st, m.F, m.Mv,
-                           b.B, u.coef.ptr, u.stage.ptr, u.active.ptr, b.idx.ptr, b.fbin.ptr,
-                           b.cell_begin.ptr, m.P, pr, b.ncells, u.cellT.ptr);
-        check_launch("bs_cells_multi");
+        LaunchScope scope("bs_deriv");
+        hipLaunchKernelGGL(bs_deriv<R>, dim3(b.ncells, nact), dim3(kBlock), 0, st, m.F, m.Mv, b.B,
+                           stage, u.active.ptr, b.idx.ptr, b.cell_begin.ptr, b.fbin.ptr, m.P, pr, rec,
+                           b.SS.n, b.ncells, u.cellT.ptr);
+        check_launch("bs_deriv");
     }
-    LaunchScope scope("bs_gather_multi");
-    hipLaunchKernelGGL(bs_gather_multi, dim3((b.B.ncp + 255) / 256, nact), dim3(256), 0, st, b.B,
+    LaunchScope scope("bs_gather");
+    hipLaunchKernelGGL(bs_gather, dim3((b.B.ncp + 255) / 256, nact), dim3(256), 0, st, b.B,
                        u.active.ptr, b.ncells, u.cellT.ptr, u.deriv.ptr);
-    check_launch("bs_gather_multi");
+    check_launch("bs_gather");
 }
 
-// Evaluates value and derivative of the listed starts at the coefficients
-// already in u.h_coef (start s at h_coef + s * np). Results land in u.h_tail
-// and u.h_deriv (zero derivative for a start without valid samples).
-void bspline_multi_evaluate(BSplineMulti &u, const int *act, int nact) {
+// Evaluates the listed starts at the coefficients already in u.h_coef (start
+// s at h_coef + s * np). Results land in u.h_tail and, with the derivative,
+// u.h_deriv (zero for a start without valid samples).
+void bspline_multi_evaluate(BSplineMulti &u, const int *act, int nact, bool derivative = true) {
     if (nact <= 0) return;
     if (nact > u.S) fail(SITKHIP_ERR_INVALID_ARGUMENT, "more active starts than starts");
     for (int i = 0; i < nact; ++i) {
@@ -706,17 +701,32 @@ void bspline_multi_evaluate(BSplineMulti &u, const int *act, int nact) {
         HIP_OK(hipMemcpyAsync(u.coef.ptr + act[i] * u.np, u.h_coef + act[i] * u.np, bytes,
                               hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(u.active.ptr, u.h_active, nact * sizeof(int), hipMemcpyHostToDevice, st));
-    if (u.b->m->precision == SITKHIP_PRECISION_F64) multi_passes<double>(u, nact);
-    else multi_passes<float>(u, nact);
+    if (u.b->m->precision == SITKHIP_PRECISION_F64) multi_passes<double>(u, nact, derivative);
+    else multi_passes<float>(u, nact, derivative);
     for (int i = 0; i < nact; ++i) {
-        HIP_OK(hipMemcpyAsync(u.h_deriv + act[i] * u.np, u.deriv.ptr + act[i] * u.np, bytes,
-                              hipMemcpyDeviceToHost, st));
+        if (derivative)
+            HIP_OK(hipMemcpyAsync(u.h_deriv + act[i] * u.np, u.deriv.ptr + act[i] * u.np, bytes,
+                                  hipMemcpyDeviceToHost, st));
         HIP_OK(hipMemcpyAsync(u.h_tail + act[i], u.tail.ptr + act[i], sizeof(TailOut),
                               hipMemcpyDeviceToHost, st));
     }
     sync();
-    for (int i = 0; i < nact; ++i)
-        if (u.h_tail[act[i]].valid == 0.0) std::memset(u.h_deriv + act[i] * u.np, 0, bytes);
+    if (derivative)
+        for (int i = 0; i < nact; ++i)
+            if (u.h_tail[act[i]].valid == 0.0) std::memset(u.h_deriv + act[i] * u.np, 0, bytes);
+}
+
+void bspline_evaluate(BSplineMetric &b, const double *coeffs, bool derivative, double *value,
+                      double *deriv, size_t *valid) {
+    if (!coeffs) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null coefficients");
+    if (!b.one) b.one = bspline_multi_create(b, 1, nullptr);
+    BSplineMulti &u = *b.one;
+    std::memcpy(u.h_coef, coeffs, u.np * sizeof(double));
+    const int act = 0;
+    bspline_multi_evaluate(u, &act, 1, derivative);
+    if (value) *value = u.h_tail[0].value;
+    if (valid) *valid = static_cast<size_t>(u.h_tail[0].valid);
+    if (derivative && deriv) std::memcpy(deriv, u.h_deriv, u.np * sizeof(double));
 }
 
 // Every start optimised by its own L-BFGS-B (lbfgsb.rs:1590-1746); each round

@@ -189,3 +189,143 @@ def config_batched_pairs(n: int = 512, seed: int = 3):
         _, T_gt = random_euler(rng, 15.0, 15.0, vol.center())
         pairs.append({"fixed": vol, "T_gt": T_gt, "metric": "ms" if i % 2 == 0 else "mattes"})
     return pairs
+
+
+# ------------------------------------------------------- deformable workload
+@dataclass
+class RbfField:
+    """A smooth displacement u(x) = sum_k a_k exp(-|x - c_k|^2 / (2 s_k^2)) (mm),
+    a_k a 3-vector per bump."""
+    centres: np.ndarray   # (K, 3)
+    vectors: np.ndarray   # (K, 3)
+    sigmas: np.ndarray    # (K,)
+
+    def __call__(self, pts) -> np.ndarray:
+        pts = np.asarray(pts, dtype=np.float64)
+        out = np.zeros_like(pts)
+        for c, a, s in zip(self.centres, self.vectors, self.sigmas):
+            d2 = np.sum((pts - c) ** 2, axis=-1)
+            out += np.exp(-d2 / (2.0 * s * s))[..., None] * a
+        return out
+
+
+def rbf_field(rng, vol: Volume, nbumps=20, peak=4.0, sigma=(15.0, 30.0)) -> RbfField:
+    """`nbumps` Gaussian bumps with centres uniform over the middle 80% of the
+    volume, sigma U[sigma] mm and random directions, scaled so the largest
+    displacement over a 4-voxel grid of the volume is `peak` mm."""
+    size = np.asarray(vol.size, dtype=np.float64)
+    sp = np.asarray(vol.spacing)
+    org = np.asarray(vol.origin)
+    ext = size * sp
+    centres = org + rng.uniform(0.1, 0.9, (nbumps, 3)) * ext
+    dirs = rng.normal(0.0, 1.0, (nbumps, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    vectors = dirs * rng.uniform(0.5, 1.0, (nbumps, 1))
+    sigmas = rng.uniform(sigma[0], sigma[1], nbumps)
+    f = RbfField(centres, vectors, sigmas)
+    g = np.stack(np.meshgrid(*[np.arange(0, n, 4) for n in vol.size], indexing="ij"), -1).reshape(-1, 3)
+    m = np.linalg.norm(f(org + g * sp), axis=1).max()
+    return RbfField(centres, vectors * (peak / m), sigmas)
+
+
+@dataclass
+class DeformableTruth:
+    """The registration map that aligns a deformable pair, T(x) = R(z) with
+    z = x + u(z): the moving image was made as moving(y) = fixed(V(y)),
+    V(y) = z - u(z) with z = R^-1(y), so V(T(x)) = x."""
+    R: AffineMap
+    u: RbfField
+
+    def __call__(self, pts, iterations=30) -> np.ndarray:
+        x = np.asarray(pts, dtype=np.float64)
+        z = x.copy()
+        for _ in range(iterations):  # |grad u| < 1: a contraction
+            z = x + self.u(z)
+        return self.R.apply(z)
+
+
+def config_deformable(seed: int = 5, edge: int = 192, nblobs: int = 135, peak: float = 4.0,
+                      max_deg: float = 5.0, max_mm: float = 5.0):
+    """192^3 float32, 1 mm, the blob generator at the 128^3 pairs' density
+    (135 blobs); moving(y) = fixed(V(y)) with V(y) = z - u(z), z = R^-1(y), R a
+    seeded Euler3D offset (+-5 deg, +-5 mm about the centre) and u 20 Gaussian
+    RBF bumps of peak 4 mm. Returns (fixed, moving, DeformableTruth)."""
+    from scipy import ndimage
+
+    fixed = blob_volume((edge,) * 3, (1.0, 1.0, 1.0), seed, nblobs=nblobs)
+    rng = np.random.default_rng(seed + 2000)
+    _, R = random_euler(rng, max_deg, max_mm, fixed.center())
+    u = rbf_field(rng, fixed, peak=peak)
+    Ri = R.inverse()
+    out = np.empty_like(fixed.data)
+    n = edge
+    xs = np.arange(n, dtype=np.float64)
+    for k in range(n):  # one z slice at a time keeps the temporaries small
+        X, Y = np.meshgrid(xs, xs, indexing="xy")
+        y = np.stack([X.ravel(), Y.ravel(), np.full(X.size, float(k))], -1)
+        z = Ri.apply(y)
+        v = z - u(z)
+        coords = np.stack([v[:, 2], v[:, 1], v[:, 0]])  # (z, y, x) array order, spacing 1, origin 0
+        out[k] = ndimage.map_coordinates(fixed.data, coords, order=1, mode="constant",
+                                         cval=0.0, prefilter=False).reshape(n, n)
+    moving = Volume(out, fixed.spacing, fixed.origin)
+    return fixed, moving, DeformableTruth(R, u)
+
+
+def bspline_grid(vol: Volume, mesh):
+    """The control grid of a B-spline transform over the volume's extent
+    (crates/sitk/src/transform/bspline.rs:324-339; identity direction):
+    (grid size, grid origin, grid spacing)."""
+    size = np.asarray(vol.size, dtype=np.float64)
+    sp = np.asarray(vol.spacing)
+    gsp = size * sp / np.asarray(mesh, dtype=np.float64)
+    return np.asarray(mesh) + 3, np.asarray(vol.origin) - gsp, gsp
+
+
+def _b3(u):
+    a = np.abs(u)
+    return np.where(a < 1.0, (4.0 - 6.0 * a * a + 3.0 * a ** 3) / 6.0,
+                    np.where(a < 2.0, (2.0 - a) ** 3 / 6.0, 0.0))
+
+
+def bspline_displacement(pts, grid_size, grid_origin, grid_spacing, coeffs) -> np.ndarray:
+    """The cubic B-spline displacement at physical points (identity grid
+    direction; points outside the valid region get zero), coefficients
+    [3][ncp], cp = x + gs0 (y + gs1 z) (bspline.rs:485-583)."""
+    pts = np.asarray(pts, dtype=np.float64)
+    gs = np.asarray(grid_size, dtype=np.int64)
+    c = (pts - np.asarray(grid_origin)) / np.asarray(grid_spacing)
+    ok = np.all((c >= 1.0) & (c < gs - 2.0), axis=1)
+    start = np.floor(c - 1.0).astype(np.int64)
+    start = np.clip(start, 0, gs - 4)
+    u = c - start
+    ncp = int(np.prod(gs))
+    coef = np.asarray(coeffs, dtype=np.float64).reshape(3, ncp)
+    out = np.zeros_like(pts)
+    for k in range(4):
+        wk = _b3(u[:, 2] - k)
+        for j in range(4):
+            wj = _b3(u[:, 1] - j)
+            for i in range(4):
+                w = _b3(u[:, 0] - i) * wj * wk
+                cp = (start[:, 0] + i) + gs[0] * ((start[:, 1] + j) + gs[1] * (start[:, 2] + k))
+                out += w[:, None] * coef[:, cp].T
+    out[~ok] = 0.0
+    return out
+
+
+def deformable_tre(truth: DeformableTruth, vol: Volume, mesh, coeffs, stage, stride=8,
+                   margin=16.0) -> float:
+    """Mean |T_est(x) - T(x)| (mm) over every `stride`-th voxel at least `margin`
+    mm inside the volume, T_est(x) = stage(x + B(x))."""
+    g = np.stack(np.meshgrid(*[np.arange(0, n, stride) for n in vol.size], indexing="ij"),
+                 -1).reshape(-1, 3)
+    pts = np.asarray(vol.origin) + g * np.asarray(vol.spacing)
+    lo = np.asarray(vol.origin) + margin
+    hi = np.asarray(vol.origin) + (np.asarray(vol.size) - 1) * np.asarray(vol.spacing) - margin
+    pts = pts[np.all((pts >= lo) & (pts <= hi), axis=1)]
+    gs, go, gsp = bspline_grid(vol, mesh)
+    y = pts + bspline_displacement(pts, gs, go, gsp, coeffs)
+    st = np.asarray(stage, dtype=np.float64).reshape(12)
+    est = y @ st[:9].reshape(3, 3).T + st[9:]
+    return float(np.linalg.norm(est - truth(pts), axis=1).mean())
