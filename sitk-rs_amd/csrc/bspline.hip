@@ -161,7 +161,8 @@ __device__ __forceinline__ BSplineMap start_map(const BSplineMap &B0, const doub
     return B;
 }
 
-constexpr int kCellsPerBlock = 4;  // cells per value-pass workgroup
+// Cells per value-pass workgroup: 4 when many starts fill the chip, else 1.
+constexpr int kCellsPerBlockMany = 4;
 
 // The 3 x 64 coefficients of a cell's support, s[d * 64 + k * 16 + j * 4 + i].
 __device__ __forceinline__ void stage_cell_coefs(const BSplineMap &B, int cell, double *s) {
@@ -208,8 +209,8 @@ __device__ __forceinline__ void map_point_cell(const BSplineMap &B, const double
     apply_post_stage(B, p);
 }
 
-// Value pass, kCellsPerBlock cells per workgroup (the last bucket, cell ==
-// ncells, holds the samples outside the valid region): the cell's support
+// Value pass, `cpb` cells per workgroup (the last bucket, cell == ncells,
+// holds the samples outside the valid region): the cell's support
 // coefficients in LDS, the joint histogram in LDS, one record per sample (the
 // moving value and its index gradient, NaN value = dropped) for the
 // derivative pass. Any grouping gives the same integer histogram.
@@ -222,7 +223,8 @@ __global__ __launch_bounds__(kBlock) void bs_value(FixedGrid F, MovingVol M, BSp
                                                    const uint32_t *__restrict__ idx,
                                                    const uint8_t *__restrict__ fbin,
                                                    const int *__restrict__ cell_begin, int ncells,
-                                                   MattesParams P, unsigned long long *__restrict__ g_joint,
+                                                   int cpb, MattesParams P,
+                                                   unsigned long long *__restrict__ g_joint,
                                                    unsigned long long *__restrict__ g_marg,
                                                    SampleRec<R> *__restrict__ rec, long long n) {
     extern __shared__ unsigned long long s_hist[];
@@ -233,8 +235,8 @@ __global__ __launch_bounds__(kBlock) void bs_value(FixedGrid F, MovingVol M, BSp
     unsigned long long *h_joint = s_hist, *h_marg = s_hist + P.bins * P.bins;
     SampleRec<R> *r = rec + static_cast<size_t>(st) * n;
     for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
-    for (int q = 0; q < kCellsPerBlock; ++q) {
-        const int cell = blockIdx.x * kCellsPerBlock + q;
+    for (int q = 0; q < cpb; ++q) {
+        const int cell = blockIdx.x * cpb + q;
         if (cell > ncells) break;  // the same for every lane
         __syncthreads();           // the previous cell's coefficients are no longer read
         if (cell < ncells) stage_cell_coefs(B, cell, s_coef);
@@ -655,12 +657,14 @@ static void multi_passes(BSplineMulti &u, int nact, bool derivative) {
     const size_t shm = static_cast<size_t>(B * B + B) * sizeof(unsigned long long);
     const double *stage = u.staged ? u.stage.ptr : nullptr;
     auto *rec = reinterpret_cast<SampleRec<R> *>(u.rec.ptr);
-    const unsigned groups = static_cast<unsigned>((b.ncells + 1 + kCellsPerBlock - 1) / kCellsPerBlock);
+    const int cpb = nact >= 16 ? kCellsPerBlockMany : 1;
+    const unsigned groups = static_cast<unsigned>((b.ncells + 1 + cpb - 1) / cpb);
     {
         LaunchScope scope("bs_value");
         hipLaunchKernelGGL(bs_value<R>, dim3(groups, nact), dim3(kBlock), shm, st, m.F, m.Mv, b.B,
                            u.coef.ptr, stage, u.active.ptr, m.chain.ptr, b.idx.ptr, b.fbin.ptr,
-                           b.cell_begin.ptr, b.ncells, m.P, u.joint.ptr, u.marg.ptr, rec, b.SS.n);
+                           b.cell_begin.ptr, b.ncells, cpb, m.P, u.joint.ptr, u.marg.ptr, rec,
+                           b.SS.n);
         check_launch("bs_value");
     }
     {
