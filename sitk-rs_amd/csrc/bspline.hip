@@ -19,12 +19,19 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <cstdlib>
+#include <exception>
+#include <functional>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lbfgsb.hpp"
@@ -701,19 +708,20 @@ void bspline_multi_evaluate(BSplineMulti &u, const int *act, int nact, bool deri
     }
     const hipStream_t st = backend().stream;
     const size_t bytes = u.np * sizeof(double);
-    for (int i = 0; i < nact; ++i)
-        HIP_OK(hipMemcpyAsync(u.coef.ptr + act[i] * u.np, u.h_coef + act[i] * u.np, bytes,
-                              hipMemcpyHostToDevice, st));
+    // one copy each way over the span of listed starts (the rows of starts in
+    // between that are not listed travel too and are not used)
+    const int lo = *std::min_element(act, act + nact), hi = *std::max_element(act, act + nact);
+    const size_t span = static_cast<size_t>(hi - lo + 1);
+    HIP_OK(hipMemcpyAsync(u.coef.ptr + lo * u.np, u.h_coef + lo * u.np, span * bytes,
+                          hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(u.active.ptr, u.h_active, nact * sizeof(int), hipMemcpyHostToDevice, st));
     if (u.b->m->precision == SITKHIP_PRECISION_F64) multi_passes<double>(u, nact, derivative);
     else multi_passes<float>(u, nact, derivative);
-    for (int i = 0; i < nact; ++i) {
-        if (derivative)
-            HIP_OK(hipMemcpyAsync(u.h_deriv + act[i] * u.np, u.deriv.ptr + act[i] * u.np, bytes,
-                                  hipMemcpyDeviceToHost, st));
-        HIP_OK(hipMemcpyAsync(u.h_tail + act[i], u.tail.ptr + act[i], sizeof(TailOut),
+    if (derivative)
+        HIP_OK(hipMemcpyAsync(u.h_deriv + lo * u.np, u.deriv.ptr + lo * u.np, span * bytes,
                               hipMemcpyDeviceToHost, st));
-    }
+    HIP_OK(hipMemcpyAsync(u.h_tail + lo, u.tail.ptr + lo, span * sizeof(TailOut),
+                          hipMemcpyDeviceToHost, st));
     sync();
     if (derivative)
         for (int i = 0; i < nact; ++i)
@@ -732,6 +740,84 @@ void bspline_evaluate(BSplineMetric &b, const double *coeffs, bool derivative, d
     if (valid) *valid = static_cast<size_t>(u.h_tail[0].valid);
     if (derivative && deriv) std::memcpy(deriv, u.h_deriv, u.np * sizeof(double));
 }
+
+// A fixed set of host threads for the optimizers' updates between batched
+// evaluations: run(n, f) calls f(i) for every i < n across the threads (the
+// calling one included) and returns when all are done, rethrowing the first
+// exception. SITKHIP_HOST_THREADS caps the count (default: up to 16).
+class HostPool {
+public:
+    explicit HostPool(int want) {
+        int t = static_cast<int>(std::thread::hardware_concurrency());
+        if (const char *e = std::getenv("SITKHIP_HOST_THREADS")) t = std::atoi(e);
+        t = std::max(1, std::min({t, 16, want}));
+        for (int i = 1; i < t; ++i) th_.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            quit_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    template <typename F>
+    void run(int n, F f) {
+        std::function<void(int)> job = f;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            job_ = &job;
+            n_ = n;
+            next_ = 0;
+            busy_ = static_cast<int>(th_.size());
+            err_ = nullptr;
+            ++gen_;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> g(mu_);
+        done_.wait(g, [this] { return busy_ == 0; });
+        job_ = nullptr;
+        if (err_) std::rethrow_exception(err_);
+    }
+
+private:
+    void work() {
+        for (;;) {
+            const int i = next_.fetch_add(1);
+            if (i >= n_) break;
+            try {
+                (*job_)(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(mu_);
+                if (!err_) err_ = std::current_exception();
+            }
+        }
+    }
+    void loop() {
+        unsigned long long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [&] { return quit_ || gen_ != seen; });
+                if (quit_) return;
+                seen = gen_;
+            }
+            work();
+            std::lock_guard<std::mutex> g(mu_);
+            if (--busy_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    std::function<void(int)> *job_ = nullptr;
+    std::atomic<int> next_{0};
+    int n_ = 0, busy_ = 0;
+    bool quit_ = false;
+    unsigned long long gen_ = 0;
+    std::exception_ptr err_;
+};
 
 // Every start optimised by its own L-BFGS-B (lbfgsb.rs:1590-1746); each round
 // evaluates all starts whose optimizer asks for a point in one batched set of
@@ -752,23 +838,32 @@ static void register_starts(BSplineMulti &u, const sitkhip_bspline_config &cfg, 
     std::vector<double> fbest(S, 0.0);
     std::vector<size_t> vbest(S, 0);
     std::vector<int> nev(S, 0);
+    std::vector<char> want(S, 0);
     for (int s = 0; s < S; ++s) {
         opt[s].reset(new Lbfgsb(np, o, lo, hi, nbd));
-        if (opt[s]->start(x + s * np)) act.push_back(s);
+        if (opt[s]->start(x + s * np)) {
+            act.push_back(s);
+            std::memcpy(u.h_coef + s * np, opt[s]->point(), np * sizeof(double));
+        }
     }
+    HostPool pool(S);
     while (!act.empty()) {
-        for (int s : act) std::memcpy(u.h_coef + s * np, opt[s]->point(), np * sizeof(double));
         bspline_multi_evaluate(u, act.data(), static_cast<int>(act.size()));
-        next.clear();
-        for (int s : act) {
+        // the optimizers' updates are independent: one start per task
+        pool.run(static_cast<int>(act.size()), [&](int i) {
+            const int s = act[i];
             const double f = u.h_tail[s].value;
             if (nev[s] == 0 || f < fbest[s]) {
                 fbest[s] = f;
                 vbest[s] = static_cast<size_t>(u.h_tail[s].valid);
             }
             ++nev[s];
-            if (opt[s]->tell(f, u.h_deriv + s * np)) next.push_back(s);
-        }
+            want[s] = opt[s]->tell(f, u.h_deriv + s * np) ? 1 : 0;
+            if (want[s]) std::memcpy(u.h_coef + s * np, opt[s]->point(), np * sizeof(double));
+        });
+        next.clear();
+        for (int s : act)
+            if (want[s]) next.push_back(s);
         act.swap(next);
     }
     for (int s = 0; s < S; ++s) {

@@ -277,6 +277,8 @@ Lbfgsb::Lbfgsb(size_t n, const LbfgsbOptions &opt, const double *lower, const do
     Y_.assign(n * static_cast<size_t>(m_), 0.0);
     sy_.assign(static_cast<size_t>(m_) * m_, 0.0);
     ss_.assign(static_cast<size_t>(m_) * m_, 0.0);
+    syf_.assign(static_cast<size_t>(m_) * m_, 0.0);
+    yy_.assign(static_cast<size_t>(m_) * m_, 0.0);
     tchol_.assign(static_cast<size_t>(m_) * m_, 0.0);
     c_.assign(2 * static_cast<size_t>(m_), 0.0);
     xbest_.assign(n, 0.0);
@@ -679,32 +681,43 @@ bool Lbfgsb::subspace_minimization() {
 
     // K's blocks: P = D + Y'ZZ'Y / theta, K12 = L_a' - R_z', K22 = theta S'AA'S.
     std::vector<double> P(c * c), Q(c * c), S2(c * c);
-    for (int i = 0; i < c; ++i)
-        for (int j = i; j < c; ++j) {
-            const double *yi = y(i), *yj = y(j);
-            double acc = 0.0;
-            for (int k : free_) acc += yi[k] * yj[k];
-            P[i * c + j] = acc / theta_ + (i == j ? sy_[i * m_ + i] : 0.0);
-        }
-    for (int i = 0; i < c; ++i)        // row: y index
-        for (int j = 0; j < c; ++j) {  // column: s index
-            const double *yi = y(i), *sj = s(j);
-            double acc = 0.0;
-            if (j > i) {
-                for (int k : active_) acc += sj[k] * yi[k];
-                Q[i * c + j] = acc;
-            } else {
-                for (int k : free_) acc += sj[k] * yi[k];
-                Q[i * c + j] = -acc;
+    if (active_.empty()) {
+        // every variable free: the sums over Z are the stored full products
+        for (int i = 0; i < c; ++i)
+            for (int j = i; j < c; ++j) {
+                P[i * c + j] = yy_[i * m_ + j] / theta_ + (i == j ? sy_[i * m_ + i] : 0.0);
+                S2[i * c + j] = theta_ * 0.0;
             }
-        }
-    for (int i = 0; i < c; ++i)
-        for (int j = i; j < c; ++j) {
-            const double *si = s(i), *sj = s(j);
-            double acc = 0.0;
-            for (int k : active_) acc += si[k] * sj[k];
-            S2[i * c + j] = theta_ * acc;
-        }
+        for (int i = 0; i < c; ++i)
+            for (int j = 0; j < c; ++j) Q[i * c + j] = j > i ? 0.0 : -syf_[j * m_ + i];
+    } else {
+        for (int i = 0; i < c; ++i)
+            for (int j = i; j < c; ++j) {
+                const double *yi = y(i), *yj = y(j);
+                double acc = 0.0;
+                for (int k : free_) acc += yi[k] * yj[k];
+                P[i * c + j] = acc / theta_ + (i == j ? sy_[i * m_ + i] : 0.0);
+            }
+        for (int i = 0; i < c; ++i)        // row: y index
+            for (int j = 0; j < c; ++j) {  // column: s index
+                const double *yi = y(i), *sj = s(j);
+                double acc = 0.0;
+                if (j > i) {
+                    for (int k : active_) acc += sj[k] * yi[k];
+                    Q[i * c + j] = acc;
+                } else {
+                    for (int k : free_) acc += sj[k] * yi[k];
+                    Q[i * c + j] = -acc;
+                }
+            }
+        for (int i = 0; i < c; ++i)
+            for (int j = i; j < c; ++j) {
+                const double *si = s(i), *sj = s(j);
+                double acc = 0.0;
+                for (int k : active_) acc += si[k] * sj[k];
+                S2[i * c + j] = theta_ * acc;
+            }
+    }
     if (!cholesky_upper(P.data(), c, c)) return false;
     // Q := U1'^-1 K12, column by column; S2 := K22 + Q'Q
     for (int j = 0; j < c; ++j) {
@@ -857,6 +870,8 @@ void Lbfgsb::update_memory(double gd) {
             for (int j = 0; j + 1 < col_; ++j) {
                 sy_[i * m_ + j] = sy_[(i + 1) * m_ + j + 1];
                 ss_[i * m_ + j] = ss_[(i + 1) * m_ + j + 1];
+                syf_[i * m_ + j] = syf_[(i + 1) * m_ + j + 1];
+                yy_[i * m_ + j] = yy_[(i + 1) * m_ + j + 1];
             }
     }
     std::memcpy(S_.data() + static_cast<size_t>(slot_new) * n_, d_.data(), n_ * sizeof(double));
@@ -871,6 +886,28 @@ void Lbfgsb::update_memory(double gd) {
     }
     ss_[cn * m_ + cn] = stp_ == 1.0 ? dtd_ : stp_ * stp_ * dtd_;
     sy_[cn * m_ + cn] = dr;
+    // the products the all-free subspace step reads, each summed over the
+    // variables in index order with the general branch's operand order
+    for (int i = 0; i < cn; ++i) {
+        const double *si = s(i), *yi = y(i);
+        double a = 0.0, b = 0.0;
+        for (size_t k = 0; k < n_; ++k) {
+            a += si[k] * r_[k];
+            b += yi[k] * r_[k];
+        }
+        syf_[i * m_ + cn] = a;
+        yy_[i * m_ + cn] = b;
+        yy_[cn * m_ + i] = b;
+    }
+    {
+        double a = 0.0, b = 0.0;
+        for (size_t k = 0; k < n_; ++k) {
+            a += d_[k] * r_[k];
+            b += r_[k] * r_[k];
+        }
+        syf_[cn * m_ + cn] = a;
+        yy_[cn * m_ + cn] = b;
+    }
     if (!form_t()) reset_memory();
 }
 

@@ -104,6 +104,11 @@ private:
     // corrections: column j (0 oldest) of S and Y lives in slot (head_ + j) % m
     std::vector<double> S_, Y_;
     std::vector<double> sy_, ss_;   // m x m, logical order, row-major: sy_[i*m+j] = s_i'y_j (i >= j), ss_ symmetric
+    // For steps with every variable free: syf_[i*m+j] = s_i'y_j for i <= j
+    // (the diagonal as the dot product, not the step's dr) and yy_ = Y'Y,
+    // kept as the corrections arrive so the subspace step needs no O(m^2 n)
+    // sums (the same sums, in the same order, as the general branch).
+    std::vector<double> syf_, yy_;
     std::vector<double> tchol_;     // Cholesky factor (upper, row-major) of T = theta SS + L D^-1 L'
     int col_ = 0, head_ = 0;
     double theta_ = 1.0;
