@@ -149,8 +149,8 @@ __device__ __forceinline__ void map_point(const BSplineMap &B, const double x[3]
 
 // ----------------------------------------------------------------- passes
 // One launch serves any number of starts (independent optimisations of one
-// pair that share the sample set and the images): blockIdx.y picks a start
-// from the active list; each start has its own coefficients, moving-initial
+// pair that share the sample set and the images): blockIdx.x picks a start
+// from the active list (so consecutive workgroups share a cell's samples); each start has its own coefficients, moving-initial
 // stage, histogram copies, pRatio, tail, records, cell partials and
 // derivative. A single evaluation is the one-start case.
 //
@@ -183,20 +183,28 @@ __device__ __forceinline__ void stage_cell_coefs(const BSplineMap &B, int cell, 
     }
 }
 
-// map_point with the support's coefficients read from LDS: the same sums in
-// the same order. A point of the cell always has the cell's support (the
-// samples were grouped by this very function).
-__device__ __forceinline__ void map_point_cell(const BSplineMap &B, const double x[3],
-                                               const double *sc, double p[3]) {
-    int start[3];
-    double w[3][4];
-    if (!bspline_support(B, x, start, w)) {
+// map_point for a sample of a cell: the stored support weights ws (w0, w1,
+// w2; 12 doubles) and the cell's coefficients from LDS, the same sums in the
+// same order as the general map; a sample of the outside bucket maps to
+// itself. Then the stage, if any.
+__device__ __forceinline__ void map_point_cell(const BSplineMap &B, const double x[3], bool in_cell,
+                                               const double *__restrict__ ws, const double *sc,
+                                               double p[3]) {
+    if (!in_cell) {
         p[0] = x[0]; p[1] = x[1]; p[2] = x[2];
         apply_post_stage(B, p);
         return;
     }
-    // one k-slab (48 LDS reads) unrolled so the reads are issued ahead of the
-    // dependent sums
+    double w2[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w2[k] = ws[8 + k];
+    // (w0[i] w1[j]) once per sample, then one k-slab (48 LDS reads) unrolled
+    // so the reads are issued ahead of the dependent sums
+    double w01[16];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w01[j * 4 + i] = ws[i] * ws[4 + j];
     double d0 = 0.0, d1 = 0.0, d2 = 0.0;
 #pragma unroll 1
     for (int k = 0; k < 4; ++k)
@@ -204,7 +212,7 @@ __device__ __forceinline__ void map_point_cell(const BSplineMap &B, const double
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const double wt = (w[0][i] * w[1][j]) * w[2][k];
+                const double wt = w01[j * 4 + i] * w2[k];
                 const int o = k * 16 + j * 4 + i;
                 d0 += wt * sc[o];
                 d1 += wt * sc[64 + o];
@@ -214,6 +222,31 @@ __device__ __forceinline__ void map_point_cell(const BSplineMap &B, const double
     p[1] = x[1] + d1;
     p[2] = x[2] + d2;
     apply_post_stage(B, p);
+}
+
+// The support weights of every sample of a cell (they depend only on the
+// fixed point): w[12] per sample in the cell-grouped order, computed once.
+// A sample whose support on the device is not its cell's (the grouping was
+// made on the host) raises *mismatch.
+__global__ void bs_weights(FixedGrid F, BSplineMap B, const ChainConst *__restrict__ C,
+                           const uint32_t *__restrict__ idx, const int *__restrict__ cell_begin,
+                           int ncells, double *__restrict__ wts, int *__restrict__ mismatch) {
+    const int cell = blockIdx.y;
+    const int m0 = B.gs[0] - 3, m1 = B.gs[1] - 3;
+    const int sx = cell % m0, sy = (cell / m0) % m1, sz = cell / (m0 * m1);
+    const int e = cell_begin[cell + 1];
+    for (int s = cell_begin[cell] + blockIdx.x * blockDim.x + threadIdx.x; s < e;
+         s += gridDim.x * blockDim.x) {
+        double x[3], w[3][4];
+        int start[3];
+        fixed_point(F, *C, idx[s], x);
+        const bool in = bspline_support(B, x, start, w);
+        if (!in || start[0] != sx || start[1] != sy || start[2] != sz) *mismatch = 1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) wts[static_cast<size_t>(s) * 12 + d * 4 + k] = w[d][k];
+    }
 }
 
 // Value pass, `cpb` cells per workgroup (the last bucket, cell == ncells,
@@ -229,21 +262,22 @@ __global__ __launch_bounds__(kBlock) void bs_value(FixedGrid F, MovingVol M, BSp
                                                    const ChainConst *__restrict__ C,
                                                    const uint32_t *__restrict__ idx,
                                                    const uint8_t *__restrict__ fbin,
-                                                   const int *__restrict__ cell_begin, int ncells,
+                                                   const int *__restrict__ cell_begin,
+                                                   const double *__restrict__ wts, int ncells,
                                                    int cpb, MattesParams P,
                                                    unsigned long long *__restrict__ g_joint,
                                                    unsigned long long *__restrict__ g_marg,
                                                    SampleRec<R> *__restrict__ rec, long long n) {
     extern __shared__ unsigned long long s_hist[];
     __shared__ double s_coef[192];
-    const int st = active[blockIdx.y];
+    const int st = active[blockIdx.x];
     const BSplineMap B = start_map(B0, coef, stage, st);
     const int nb = P.bins * P.bins + P.bins;
     unsigned long long *h_joint = s_hist, *h_marg = s_hist + P.bins * P.bins;
     SampleRec<R> *r = rec + static_cast<size_t>(st) * n;
     for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
     for (int q = 0; q < cpb; ++q) {
-        const int cell = blockIdx.x * cpb + q;
+        const int cell = blockIdx.y * cpb + q;
         if (cell > ncells) break;  // the same for every lane
         __syncthreads();           // the previous cell's coefficients are no longer read
         if (cell < ncells) stage_cell_coefs(B, cell, s_coef);
@@ -256,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void bs_value(FixedGrid F, MovingVol M, BSp
             if (fb != kNoBin) {
                 double x[3], p[3];
                 fixed_point(F, *C, idx[s], x);
-                map_point_cell(B, x, s_coef, p);
+                map_point_cell(B, x, cell < ncells, wts + static_cast<size_t>(s) * 12, s_coef, p);
                 ok = locate(M, *C, p, cl);
             }
             if (!ok) {
@@ -272,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void bs_value(FixedGrid F, MovingVol M, BSp
         }
     }
     __syncthreads();
-    const unsigned copy = blockIdx.x % kHistCopies;
+    const unsigned copy = (blockIdx.y * gridDim.x + blockIdx.x) % kHistCopies;
     g_joint += (static_cast<size_t>(st) * kHistCopies + copy) * P.bins * P.bins;
     g_marg += (static_cast<size_t>(st) * kHistCopies + copy) * P.bins;
     for (int i = threadIdx.x; i < nb; i += kBlock) {
@@ -316,16 +350,17 @@ __global__ __launch_bounds__(kBlock) void bs_deriv(FixedGrid F, MovingVol M, BSp
                                                    const int *__restrict__ active,
                                                    const uint32_t *__restrict__ idx,
                                                    const int *__restrict__ cell_begin,
+                                                   const double *__restrict__ wts,
                                                    const uint8_t *__restrict__ fbin,
                                                    MattesParams P, const R *__restrict__ pratio,
                                                    const SampleRec<R> *__restrict__ rec, long long n,
                                                    int ncells, double *__restrict__ out) {
     __shared__ double s_x[kBlock / 64][64][16];  // per lane: w[3][4], g[3], pad
-    const int st = active[blockIdx.y];
+    const int st = active[blockIdx.x];
     const BSplineMap B = start_map(B0, nullptr, stage, st);
     const SampleRec<R> *r = rec + static_cast<size_t>(st) * n;
     pratio += static_cast<size_t>(st) * P.bins * P.bins;
-    const int cell = blockIdx.x;
+    const int cell = blockIdx.y;
     const int b = cell_begin[cell], e = cell_begin[cell + 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int c = lane & 15, g4 = lane >> 4;
@@ -337,10 +372,10 @@ __global__ __launch_bounds__(kBlock) void bs_deriv(FixedGrid F, MovingVol M, BSp
         if (s < e) {
             const SampleRec<R> rr = r[s];
             if (rr.mv == rr.mv) {  // a sample the value pass kept
-                double x[3];
-                fixed_point(F, idx[s], x);
-                int start[3];
-                bspline_support(B, x, start, w);
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) w[d][k] = wts[static_cast<size_t>(s) * 12 + d * 4 + k];
                 R gp[3];
                 physical_gradient<R>(M, rr.gi, gp);
                 R arg[4], dv[4];
@@ -438,6 +473,7 @@ struct BSplineMetric {
     DeviceBuffer<uint32_t> idx;
     DeviceBuffer<uint8_t> fbin;
     DeviceBuffer<int> cell_begin;  // ncells + 2 bounds: the cells, then the outside bucket
+    DeviceBuffer<double> wts;      // support weights w[3][4] per sample of a cell
     BSplineMulti *one = nullptr;   // the single-start evaluation (its stage: B's own)
     ~BSplineMetric() { bspline_multi_free(one); }
 };
@@ -519,6 +555,20 @@ BSplineMetric *bspline_create(Metric &m, const size_t gs[3], const double origin
         b->SS.idx = b->idx.ptr;
         b->SS.n = static_cast<long long>(n);
         b->B.coef = nullptr;
+        b->wts.alloc(n * 12);
+        if (b->ncells > 0 && n > 0) {
+            DeviceBuffer<int> bad(1);
+            HIP_OK(hipMemsetAsync(bad.ptr, 0, sizeof(int), backend().stream));
+            hipLaunchKernelGGL(bs_weights, dim3(8, b->ncells), dim3(256), 0, backend().stream, m.F,
+                               b->B, m.chain.ptr, b->idx.ptr, b->cell_begin.ptr, b->ncells,
+                               b->wts.ptr, bad.ptr);
+            check_launch("bs_weights");
+            int flag = 0;
+            HIP_OK(hipMemcpyAsync(&flag, bad.ptr, sizeof(int), hipMemcpyDeviceToHost, backend().stream));
+            sync();
+            if (flag)
+                fail(SITKHIP_ERR_DRIVER, "a sample's B-spline support on the device differs from its cell");
+        }
         sync();
     } catch (...) {
         delete b;
@@ -668,10 +718,10 @@ static void multi_passes(BSplineMulti &u, int nact, bool derivative) {
     const unsigned groups = static_cast<unsigned>((b.ncells + 1 + cpb - 1) / cpb);
     {
         LaunchScope scope("bs_value");
-        hipLaunchKernelGGL(bs_value<R>, dim3(groups, nact), dim3(kBlock), shm, st, m.F, m.Mv, b.B,
+        hipLaunchKernelGGL(bs_value<R>, dim3(nact, groups), dim3(kBlock), shm, st, m.F, m.Mv, b.B,
                            u.coef.ptr, stage, u.active.ptr, m.chain.ptr, b.idx.ptr, b.fbin.ptr,
-                           b.cell_begin.ptr, b.ncells, cpb, m.P, u.joint.ptr, u.marg.ptr, rec,
-                           b.SS.n);
+                           b.cell_begin.ptr, b.wts.ptr, b.ncells, cpb, m.P, u.joint.ptr, u.marg.ptr,
+                           rec, b.SS.n);
         check_launch("bs_value");
     }
     {
@@ -685,8 +735,9 @@ static void multi_passes(BSplineMulti &u, int nact, bool derivative) {
                                  : reinterpret_cast<const R *>(u.pratio.ptr);
     if (b.ncells > 0) {
         LaunchScope scope("bs_deriv");
-        hipLaunchKernelGGL(bs_deriv<R>, dim3(b.ncells, nact), dim3(kBlock), 0, st, m.F, m.Mv, b.B,
-                           stage, u.active.ptr, b.idx.ptr, b.cell_begin.ptr, b.fbin.ptr, m.P, pr, rec,
+        hipLaunchKernelGGL(bs_deriv<R>, dim3(nact, b.ncells), dim3(kBlock), 0, st, m.F, m.Mv, b.B,
+                           stage, u.active.ptr, b.idx.ptr, b.cell_begin.ptr, b.wts.ptr, b.fbin.ptr, m.P,
+                           pr, rec,
                            b.SS.n, b.ncells, u.cellT.ptr);
         check_launch("bs_deriv");
     }
