@@ -14,8 +14,9 @@
 // support, which depends only on the fixed point. Per evaluation one workgroup
 // per cell contracts its samples into the 3 x 64 partial derivatives of the
 // cell's support (sum over samples of (pRatio-weighted B3' sum) * dM/dx_d *
-// weight), in sample order; then every control point gathers the partials of
-// the (up to 64) cells whose support holds it, in cell order.
+// weight, as FP64 matrix products in a fixed order); then every control point
+// gathers the partials of the (up to 64) cells whose support holds it, in
+// cell order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

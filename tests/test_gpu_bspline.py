@@ -152,6 +152,25 @@ def test_many_starts_equal_one_start_at_a_time(hip, precision):
     bs.set_moving_initial(None)
 
 
+def test_many_starts_grouped_four_cells_per_workgroup(hip):
+    """With 16 or more active starts the value pass groups four cells per
+    workgroup: still bit-equal per start, the outside-the-grid bucket
+    included (the grid covers only part of the image here)."""
+    bs, pb, g, t, coeffs, met = _case(hip, 0, (12, 10, 14), (2, 2, 2), True, seed=13)
+    rng = np.random.default_rng(8)
+    S = 20
+    st = np.stack([_stage(rng.uniform(-0.1, 0.1, 3).tolist() + rng.uniform(-1, 1, 3).tolist(),
+                          [9.0, 11.0, 10.0]) for _ in range(S)])
+    cs = np.stack([rng.normal(0.0, 0.5, bs.nparams) for _ in range(S)])
+    multi = hip.BSplineMulti(bs, S, st)
+    vals, ders, valid = multi.value_and_derivative(cs)
+    for s in (0, 7, 19):
+        bs.set_moving_initial(st[s])
+        v, d, n = bs.value_and_derivative(cs[s])
+        assert vals[s] == v and valid[s] == n and np.array_equal(ders[s], d), s
+    bs.set_moving_initial(None)
+
+
 def _warped_pair(hip, n=32, seed=4):
     sp, org = (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)
     f = blob_volume((n, n, n), 123, nblobs=16)
