@@ -755,8 +755,12 @@ static void multi_passes(BSplineMulti &u, int nact, bool derivative) {
 void bspline_multi_evaluate(BSplineMulti &u, const int *act, int nact, bool derivative = true) {
     if (nact <= 0) return;
     if (nact > u.S) fail(SITKHIP_ERR_INVALID_ARGUMENT, "more active starts than starts");
+    std::vector<char> seen(u.S, 0);
     for (int i = 0; i < nact; ++i) {
         if (act[i] < 0 || act[i] >= u.S) fail(SITKHIP_ERR_INVALID_ARGUMENT, "active start out of range");
+        // a start listed twice would add its samples to its histogram twice
+        if (seen[act[i]]) fail(SITKHIP_ERR_INVALID_ARGUMENT, "a start is listed twice");
+        seen[act[i]] = 1;
         u.h_active[i] = act[i];
     }
     const hipStream_t st = backend().stream;

@@ -150,6 +150,9 @@ def test_many_starts_equal_one_start_at_a_time(hip, precision):
             v, d, n = bs.value_and_derivative(cs[s])
             assert vals[s] == v and valid[s] == n and np.array_equal(ders[s], d), s
     bs.set_moving_initial(None)
+    for bad in ([1, 1], [5], [-1]):
+        with pytest.raises(hip.CudaError, match="InvalidArgument"):
+            multi.value_and_derivative(cs, bad)
 
 
 def test_many_starts_grouped_four_cells_per_workgroup(hip):
