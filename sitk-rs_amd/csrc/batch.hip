@@ -244,9 +244,11 @@ struct BatchEngine {
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;  // recorded after each launched evaluation
 
     ~BatchEngine() {
         drop_graph();
+        if (done) (void)hipEventDestroy(done);
         if (ctl_host) (void)hipHostFree(ctl_host);
         if (res) (void)hipHostFree(res);
     }
@@ -467,8 +469,9 @@ static void ensure_graph(BatchEngine &b) {
     HIP_OK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
 }
 
-// One evaluation of the active pairs, results left in b.res.
-static void run_once(BatchEngine &b) {
+// Enqueues one evaluation of the active pairs (results land in b.res) and
+// records b.done after it.
+static void enqueue_once(BatchEngine &b) {
     if (b.use_graph && !profiling_enabled()) {
         ensure_graph(b);
         HIP_OK(hipGraphLaunch(b.exec, backend().stream));
@@ -479,6 +482,13 @@ static void run_once(BatchEngine &b) {
     } else {
         enqueue_any(b, profiling_enabled());
     }
+    if (!b.done) HIP_OK(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(b.done, backend().stream));
+}
+
+// One evaluation of the active pairs, results left in b.res.
+static void run_once(BatchEngine &b) {
+    enqueue_once(b);
     sync();
 }
 
@@ -491,10 +501,31 @@ static void set_control(BatchEngine &b, const double *stages, const uint8_t *act
     }
 }
 
+void batch_prepare(BatchEngine &b) {
+    if (b.use_graph) ensure_graph(b);
+}
+
+void batch_launch(BatchEngine &b, const double *stages, const uint8_t *active) {
+    set_control(b, stages, active);
+    enqueue_once(b);
+}
+
+static void read_outputs(BatchEngine &b, std::vector<BatchOutput> &out);
+
+void batch_collect(BatchEngine &b, std::vector<BatchOutput> &out) {
+    if (!b.done) fail(SITKHIP_ERR_INVALID_ARGUMENT, "no evaluation was launched");
+    HIP_OK(hipEventSynchronize(b.done));
+    read_outputs(b, out);
+}
+
 void batch_evaluate(BatchEngine &b, const double *stages, const uint8_t *active,
                     std::vector<BatchOutput> &out) {
     set_control(b, stages, active);
     run_once(b);
+    read_outputs(b, out);
+}
+
+static void read_outputs(BatchEngine &b, std::vector<BatchOutput> &out) {
     out.resize(b.n);
     for (size_t i = 0; i < b.n; ++i) {
         if (!b.ctl_host[i].active) continue;

@@ -35,6 +35,14 @@ BatchTiling batch_tiling(const BatchEngine &b);
 // nullptr (all). Inactive pairs cost nothing and keep their previous output.
 void batch_evaluate(BatchEngine &b, const double *stages, const uint8_t *active,
                     std::vector<BatchOutput> &out);
+// The same evaluation in two halves, so the host can work while the device
+// evaluates: batch_launch enqueues it and returns (the stages and flags are
+// copied; the engine must not be launched again before batch_collect),
+// batch_collect waits for that evaluation only and reads its outputs.
+// batch_prepare builds the engine's graph ahead of the first launch.
+void batch_prepare(BatchEngine &b);
+void batch_launch(BatchEngine &b, const double *stages, const uint8_t *active);
+void batch_collect(BatchEngine &b, std::vector<BatchOutput> &out);
 // Times every candidate tiling on `stages` and keeps the fastest.
 BatchTiling batch_autotune(BatchEngine &b, const double *stages, double *best_ms);
 

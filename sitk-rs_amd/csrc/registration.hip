@@ -17,6 +17,7 @@
 #include <array>
 #include <cfloat>
 #include <chrono>
+#include <exception>
 #include <cmath>
 #include <cstdio>
 #include <cstdint>
@@ -659,69 +660,112 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
         sync();
         for (size_t k = 0; k < m; ++k) prepare_level_end(inputs[k], cfg[who[k]]);
         const auto tp1 = now();
-        std::unique_ptr<BatchEngine, void (*)(BatchEngine *)> eng(batch_create(metrics.data(), m),
-                                                                   batch_destroy);
+        // Two engines over alternate pairs once the batch is large: the host
+        // updates one half's optimizers while the device evaluates the other
+        // (each pair's evaluation is the same in either engine).
+        const size_t H = m >= 64 ? 2 : 1;
+        struct Half {
+            std::vector<size_t> ks;  // level-local pair indices
+            std::unique_ptr<BatchEngine, void (*)(BatchEngine *)> eng{nullptr, batch_destroy};
+            std::vector<double> stages;
+            std::vector<uint8_t> active;
+            std::vector<BatchOutput> out;
+            bool pending = false;
+            int rounds = 0;
+        };
+        std::vector<Half> half(H);
+        for (size_t k = 0; k < m; ++k) half[k % H].ks.push_back(k);
+        for (Half &h : half) {
+            std::vector<Metric *> mh;
+            for (size_t k : h.ks) mh.push_back(metrics[k]);
+            h.eng.reset(batch_create(mh.data(), mh.size()));
+            h.stages.assign(12 * h.ks.size(), 0.0);
+            h.active.assign(h.ks.size(), 1);
+        }
         const auto tp2 = now();
         std::vector<RsgdRun> runs(m);
-        std::vector<double> stages(12 * m);
-        std::vector<uint8_t> active(m, 1);
-        for (size_t k = 0; k < m; ++k) {
-            const size_t i = who[k];
-            RsgdRun &r = runs[k];
-            r.cfg = &cfg[i];
-            r.T = Ts[i];
-            r.p = P[i];
-            r.prev.assign(r.p.size(), 0.0);
-            r.est.build(inputs[k].grid, r.T);
-            r.scales = cfg[i].scales_mode == SITKHIP_SCALES_PHYSICAL_SHIFT
-                           ? r.est.scales()
-                           : std::vector<double>(r.T.nparams, 1.0);
-            r.T.set_parameters(r.p.data());
-            r.T.stage(&stages[12 * k]);
-        }
+        for (Half &h : half)
+            for (size_t j = 0; j < h.ks.size(); ++j) {
+                const size_t k = h.ks[j];
+                const size_t i = who[k];
+                RsgdRun &r = runs[k];
+                r.cfg = &cfg[i];
+                r.T = Ts[i];
+                r.p = P[i];
+                r.prev.assign(r.p.size(), 0.0);
+                r.est.build(inputs[k].grid, r.T);
+                r.scales = cfg[i].scales_mode == SITKHIP_SCALES_PHYSICAL_SHIFT
+                               ? r.est.scales()
+                               : std::vector<double>(r.T.nparams, 1.0);
+                r.T.set_parameters(r.p.data());
+                r.T.stage(&h.stages[12 * j]);
+            }
         // the metric value of each pair at its probe parameters (first level,
         // before optimizing), from the same prepared level
         if (probe && level == 0) {
-            std::vector<double> pst(12 * m);
-            for (size_t k = 0; k < m; ++k) {
-                Transform T = Ts[who[k]];
-                T.set_parameters(probe + 12 * who[k]);
-                T.stage(&pst[12 * k]);
+            for (Half &h : half) {
+                std::vector<double> pst(12 * h.ks.size());
+                for (size_t j = 0; j < h.ks.size(); ++j) {
+                    Transform T = Ts[who[h.ks[j]]];
+                    T.set_parameters(probe + 12 * who[h.ks[j]]);
+                    T.stage(&pst[12 * j]);
+                }
+                std::vector<BatchOutput> po(h.ks.size());
+                batch_evaluate(*h.eng, pst.data(), nullptr, po);
+                for (size_t j = 0; j < h.ks.size(); ++j) probe_values[who[h.ks[j]]] = po[j].value;
             }
-            std::vector<BatchOutput> po(m);
-            batch_evaluate(*eng, pst.data(), nullptr, po);
-            for (size_t k = 0; k < m; ++k) probe_values[who[k]] = po[k].value;
         }
         // tuning pays only when one evaluation is long enough to time
-        size_t total_samples = 0;
-        for (size_t k = 0; k < m; ++k) total_samples += metric_sample_count(*metrics[k]);
-        if (autotune && total_samples >= (static_cast<size_t>(1) << 22))
-            batch_autotune(*eng, stages.data(), nullptr);
-        const auto tp3 = now();
-        int rounds = 0;
-        std::vector<BatchOutput> out(m);
-        for (;;) {
-            ++rounds;
-            batch_evaluate(*eng, stages.data(), active.data(), out);
-            bool any = false;
-            for (size_t k = 0; k < m; ++k) {
-                if (!active[k]) continue;
-                RsgdRun &r = runs[k];
-                ++r.evaluations;
-                size_t valid = out[k].valid;
-                r.value = contract_objective(cfg[who[k]].metric, out[k].value, out[k].moments, r.T,
-                                             r.grad, valid);
-                r.valid = out[k].valid;
-                if (r.advance()) {
-                    r.T.set_parameters(r.p.data());
-                    r.T.stage(&stages[12 * k]);
-                    any = true;
-                } else {
-                    active[k] = 0;
-                }
-            }
-            if (!any) break;
+        for (Half &h : half) {
+            size_t total_samples = 0;
+            for (size_t k : h.ks) total_samples += metric_sample_count(*metrics[k]);
+            if (autotune && total_samples >= (static_cast<size_t>(1) << 22) / H)
+                batch_autotune(*h.eng, h.stages.data(), nullptr);
+            batch_prepare(*h.eng);
         }
+        const auto tp3 = now();
+        // on an error, let the evaluations in flight finish before the
+        // engines and metrics go
+        struct DrainOnError {
+            ~DrainOnError() {
+                if (std::uncaught_exceptions() > 0) (void)hipStreamSynchronize(backend().stream);
+            }
+        } drain;
+        for (Half &h : half) {
+            batch_launch(*h.eng, h.stages.data(), h.active.data());
+            h.pending = true;
+        }
+        for (bool busy = true; busy;) {
+            busy = false;
+            for (Half &h : half) {
+                if (!h.pending) continue;
+                batch_collect(*h.eng, h.out);
+                ++h.rounds;
+                bool any = false;
+                for (size_t j = 0; j < h.ks.size(); ++j) {
+                    if (!h.active[j]) continue;
+                    const size_t k = h.ks[j];
+                    RsgdRun &r = runs[k];
+                    ++r.evaluations;
+                    size_t valid = h.out[j].valid;
+                    r.value = contract_objective(cfg[who[k]].metric, h.out[j].value, h.out[j].moments,
+                                                 r.T, r.grad, valid);
+                    r.valid = h.out[j].valid;
+                    if (r.advance()) {
+                        r.T.set_parameters(r.p.data());
+                        r.T.stage(&h.stages[12 * j]);
+                        any = true;
+                    } else {
+                        h.active[j] = 0;
+                    }
+                }
+                if (any) batch_launch(*h.eng, h.stages.data(), h.active.data());
+                h.pending = any;
+                busy = busy || any;
+            }
+        }
+        int rounds = 0;
+        for (const Half &h : half) rounds = std::max(rounds, h.rounds);
         if (trace) {
             const auto tp4 = now();
             std::fprintf(stderr,
