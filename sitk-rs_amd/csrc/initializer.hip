@@ -502,10 +502,26 @@ __global__ __launch_bounds__(256) void moments_partials(const float *__restrict_
 
 // One input slot: channel `ch` of [64^3][8] = (v - mean) * inv_std as BF16;
 // channels 2..7 are zero.
-__global__ void pack_channel(const float *__restrict__ v, float mean, float inv_std, int ch,
+// The channel's normalisation from the 256 partial (sum, sum of squares)
+// pairs, added in order by one lane: norm = {mean, 1 / std} as floats.
+__global__ void moments_finish(const double *__restrict__ part, size_t nv, float *__restrict__ norm) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s1 = 0.0, s2 = 0.0;
+    for (int b = 0; b < 256; ++b) {
+        s1 += part[2 * b];
+        s2 += part[2 * b + 1];
+    }
+    const double mean = s1 / static_cast<double>(nv);
+    const double var = fmax(s2 / static_cast<double>(nv) - mean * mean, 1e-12);
+    norm[0] = static_cast<float>(mean);
+    norm[1] = static_cast<float>(1.0 / sqrt(var));
+}
+
+__global__ void pack_channel(const float *__restrict__ v, const float *__restrict__ norm, int ch,
                              uint16_t *__restrict__ slot) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kEdge * kEdge * kEdge) return;
+    const float mean = norm[0], inv_std = norm[1];
     slot[static_cast<size_t>(i) * kCin0 + ch] = f2bf((v[i] - mean) * inv_std);
     if (ch == 0) {
 #pragma unroll
@@ -773,24 +789,20 @@ static void pack(Net &net, int slot, const float *fixed64, const float *moving64
     const size_t nv = static_cast<size_t>(kEdge) * kEdge * kEdge;
     uint16_t *dst = net.act[0].ptr + static_cast<size_t>(slot) * nv * kCin0;
     const float *src[2] = {fixed64, moving64};
+    // the normalisation is computed on the device: no host round trip
     DeviceBuffer<double> part(2 * 256);
+    DeviceBuffer<float> norm(2);
     for (int ch = 0; ch < 2; ++ch) {
         hipLaunchKernelGGL(moments_partials, dim3(256), dim3(256), 0, backend().stream, src[ch], nv,
                            part.ptr);
         check_launch("moments_partials");
-        std::vector<double> h(512);
-        part.download(h.data(), h.size());
-        sync();
-        double s1 = 0.0, s2 = 0.0;
-        for (int b = 0; b < 256; ++b) { s1 += h[2 * b]; s2 += h[2 * b + 1]; }
-        const double mean = s1 / static_cast<double>(nv);
-        const double var = std::max(s2 / static_cast<double>(nv) - mean * mean, 1e-12);
+        hipLaunchKernelGGL(moments_finish, dim3(1), dim3(64), 0, backend().stream, part.ptr, nv,
+                           norm.ptr);
+        check_launch("moments_finish");
         hipLaunchKernelGGL(pack_channel, dim3((nv + 255) / 256), dim3(256), 0, backend().stream,
-                           src[ch], static_cast<float>(mean), static_cast<float>(1.0 / std::sqrt(var)),
-                           ch, dst);
+                           src[ch], norm.ptr, ch, dst);
         check_launch("pack_channel");
     }
-    sync();
 }
 
 // The 64^3 grid spanning an image's physical extent (same origin corner and
@@ -812,8 +824,10 @@ static sitkhip_geometry grid64(const sitkhip_geometry &g) {
 }
 
 void set_input(Net &net, int slot, const Image &fixed, const Image &moving) {
-    std::unique_ptr<Image> f(resample_linear(fixed, grid64(fixed.geom), nullptr, 0, 0.0));
-    std::unique_ptr<Image> m(resample_linear(moving, grid64(moving.geom), nullptr, 0, 0.0));
+    // enqueued only: the buffers are freed after the pack is enqueued, and a
+    // later use of the same blocks is ordered after it on the stream
+    std::unique_ptr<Image> f(resample_linear(fixed, grid64(fixed.geom), nullptr, 0, 0.0, false));
+    std::unique_ptr<Image> m(resample_linear(moving, grid64(moving.geom), nullptr, 0, 0.0, false));
     pack(net, slot, f->buf.ptr, m->buf.ptr);
 }
 
@@ -823,6 +837,7 @@ void set_input_raw(Net &net, int slot, const float *fixed_host, const float *mov
     f.upload(fixed_host, nv);
     m.upload(moving_host, nv);
     pack(net, slot, f.ptr, m.ptr);
+    sync();  // the caller's host arrays may go once this returns
 }
 
 }  // namespace cnn
