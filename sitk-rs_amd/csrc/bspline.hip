@@ -708,7 +708,7 @@ BSplineMulti *bspline_multi_create(BSplineMetric &b, int S, const double *stages
 }
 
 template <typename R>
-static void multi_passes(BSplineMulti &u, int nact, bool derivative) {
+static void multi_passes(BSplineMulti &u, const int *active, int nact, bool derivative) {
     BSplineMetric &b = *u.b;
     Metric &m = *b.m;
     const int B = m.P.bins;
@@ -721,14 +721,14 @@ static void multi_passes(BSplineMulti &u, int nact, bool derivative) {
     {
         LaunchScope scope("bs_value");
         hipLaunchKernelGGL(bs_value<R>, dim3(nact, groups), dim3(kBlock), shm, st, m.F, m.Mv, b.B,
-                           u.coef.ptr, stage, u.active.ptr, m.chain.ptr, b.idx.ptr, b.fbin.ptr,
+                           u.coef.ptr, stage, active, m.chain.ptr, b.idx.ptr, b.fbin.ptr,
                            b.cell_begin.ptr, b.wts.ptr, b.ncells, cpb, m.P, u.joint.ptr, u.marg.ptr,
                            rec, b.SS.n);
         check_launch("bs_value");
     }
     {
         LaunchScope scope("bs_tail");
-        hipLaunchKernelGGL(bs_tail, dim3(nact), dim3(kTail), shm, st, u.active.ptr, B, m.P.mbs,
+        hipLaunchKernelGGL(bs_tail, dim3(nact), dim3(kTail), shm, st, active, B, m.P.mbs,
                            u.joint.ptr, u.marg.ptr, u.pratio.ptr, u.pratio_f.ptr, u.tail.ptr);
         check_launch("bs_tail");
     }
@@ -738,14 +738,14 @@ static void multi_passes(BSplineMulti &u, int nact, bool derivative) {
     if (b.ncells > 0) {
         LaunchScope scope("bs_deriv");
         hipLaunchKernelGGL(bs_deriv<R>, dim3(nact, b.ncells), dim3(kBlock), 0, st, m.F, m.Mv, b.B,
-                           stage, u.active.ptr, b.idx.ptr, b.cell_begin.ptr, b.wts.ptr, b.fbin.ptr, m.P,
+                           stage, active, b.idx.ptr, b.cell_begin.ptr, b.wts.ptr, b.fbin.ptr, m.P,
                            pr, rec,
                            b.SS.n, b.ncells, u.cellT.ptr);
         check_launch("bs_deriv");
     }
     LaunchScope scope("bs_gather");
     hipLaunchKernelGGL(bs_gather, dim3((b.B.ncp + 255) / 256, nact), dim3(256), 0, st, b.B,
-                       u.active.ptr, b.ncells, u.cellT.ptr, u.deriv.ptr);
+                       active, b.ncells, u.cellT.ptr, u.deriv.ptr);
     check_launch("bs_gather");
 }
 
@@ -772,8 +772,8 @@ void bspline_multi_evaluate(BSplineMulti &u, const int *act, int nact, bool deri
     HIP_OK(hipMemcpyAsync(u.coef.ptr + lo * u.np, u.h_coef + lo * u.np, span * bytes,
                           hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(u.active.ptr, u.h_active, nact * sizeof(int), hipMemcpyHostToDevice, st));
-    if (u.b->m->precision == SITKHIP_PRECISION_F64) multi_passes<double>(u, nact, derivative);
-    else multi_passes<float>(u, nact, derivative);
+    if (u.b->m->precision == SITKHIP_PRECISION_F64) multi_passes<double>(u, u.active.ptr, nact, derivative);
+    else multi_passes<float>(u, u.active.ptr, nact, derivative);
     if (derivative)
         HIP_OK(hipMemcpyAsync(u.h_deriv + lo * u.np, u.deriv.ptr + lo * u.np, span * bytes,
                               hipMemcpyDeviceToHost, st));
@@ -796,6 +796,54 @@ void bspline_evaluate(BSplineMetric &b, const double *coeffs, bool derivative, d
     if (value) *value = u.h_tail[0].value;
     if (valid) *valid = static_cast<size_t>(u.h_tail[0].valid);
     if (derivative && deriv) std::memcpy(deriv, u.h_deriv, u.np * sizeof(double));
+}
+
+// A contiguous range of starts evaluated on its own (register_starts keeps
+// two, so the host updates one range's optimizers while the device evaluates
+// the other's). The ranges are disjoint, so each evaluation touches only its
+// own rows of the coefficient, derivative and tail buffers.
+struct StartLane {
+    int lo = 0, hi = 0;        // starts [lo, hi)
+    std::vector<int> act;      // the starts wanting an evaluation
+    DeviceBuffer<int> active;
+    int *h_active = nullptr;   // pinned
+    hipEvent_t done = nullptr;
+    bool pending = false;
+    StartLane() = default;
+    StartLane(const StartLane &) = delete;
+    StartLane &operator=(const StartLane &) = delete;
+    ~StartLane() {
+        if (h_active) (void)hipHostFree(h_active);
+        if (done) (void)hipEventDestroy(done);
+    }
+};
+
+static void lane_launch(BSplineMulti &u, StartLane &L) {
+    const int nact = static_cast<int>(L.act.size());
+    const hipStream_t st = backend().stream;
+    const size_t bytes = u.np * sizeof(double);
+    const int lo = *std::min_element(L.act.begin(), L.act.end());
+    const int hi = *std::max_element(L.act.begin(), L.act.end());
+    const size_t span = static_cast<size_t>(hi - lo + 1);
+    std::memcpy(L.h_active, L.act.data(), nact * sizeof(int));
+    HIP_OK(hipMemcpyAsync(u.coef.ptr + lo * u.np, u.h_coef + lo * u.np, span * bytes,
+                          hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(L.active.ptr, L.h_active, nact * sizeof(int), hipMemcpyHostToDevice, st));
+    if (u.b->m->precision == SITKHIP_PRECISION_F64) multi_passes<double>(u, L.active.ptr, nact, true);
+    else multi_passes<float>(u, L.active.ptr, nact, true);
+    HIP_OK(hipMemcpyAsync(u.h_deriv + lo * u.np, u.deriv.ptr + lo * u.np, span * bytes,
+                          hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(u.h_tail + lo, u.tail.ptr + lo, span * sizeof(TailOut),
+                          hipMemcpyDeviceToHost, st));
+    HIP_OK(hipEventRecord(L.done, st));
+    L.pending = true;
+}
+
+static void lane_wait(BSplineMulti &u, StartLane &L) {
+    HIP_OK(hipEventSynchronize(L.done));
+    L.pending = false;
+    for (int s : L.act)
+        if (u.h_tail[s].valid == 0.0) std::memset(u.h_deriv + s * u.np, 0, u.np * sizeof(double));
 }
 
 // Every start optimised by its own L-BFGS-B (lbfgsb.rs:1590-1746); each round
@@ -826,24 +874,56 @@ static void register_starts(BSplineMulti &u, const sitkhip_bspline_config &cfg, 
         }
     }
     HostPool pool(S);
-    while (!act.empty()) {
-        bspline_multi_evaluate(u, act.data(), static_cast<int>(act.size()));
-        // the optimizers' updates are independent: one start per task
-        pool.run(static_cast<int>(act.size()), [&](int i) {
-            const int s = act[i];
-            const double f = u.h_tail[s].value;
-            if (nev[s] == 0 || f < fbest[s]) {
-                fbest[s] = f;
-                vbest[s] = static_cast<size_t>(u.h_tail[s].valid);
-            }
-            ++nev[s];
-            want[s] = opt[s]->tell(f, u.h_deriv + s * np) ? 1 : 0;
-            if (want[s]) std::memcpy(u.h_coef + s * np, opt[s]->point(), np * sizeof(double));
-        });
-        next.clear();
+    // two lanes of contiguous starts once there are enough to share the
+    // device: while one lane's evaluation runs, the other's optimizers step
+    const int nlanes = S >= 8 ? 2 : 1;
+    StartLane lanes[2];
+    for (int l = 0; l < nlanes; ++l) {
+        StartLane &L = lanes[l];
+        L.lo = S * l / nlanes;
+        L.hi = S * (l + 1) / nlanes;
         for (int s : act)
-            if (want[s]) next.push_back(s);
-        act.swap(next);
+            if (s >= L.lo && s < L.hi) L.act.push_back(s);
+        L.active.alloc(static_cast<size_t>(L.hi - L.lo));
+        HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&L.h_active), (L.hi - L.lo) * sizeof(int),
+                             hipHostMallocDefault));
+        HIP_OK(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+    }
+    // on an error, let the evaluations in flight finish before the buffers go
+    struct DrainOnError {
+        ~DrainOnError() {
+            if (std::uncaught_exceptions() > 0) (void)hipStreamSynchronize(backend().stream);
+        }
+    } drain;
+    for (int l = 0; l < nlanes; ++l)
+        if (!lanes[l].act.empty()) lane_launch(u, lanes[l]);
+    for (bool busy = true; busy;) {
+        busy = false;
+        for (int l = 0; l < nlanes; ++l) {
+            StartLane &L = lanes[l];
+            if (!L.pending) continue;
+            lane_wait(u, L);
+            // the optimizers' updates are independent: one start per task
+            pool.run(static_cast<int>(L.act.size()), [&](int i) {
+                const int s = L.act[i];
+                const double f = u.h_tail[s].value;
+                if (nev[s] == 0 || f < fbest[s]) {
+                    fbest[s] = f;
+                    vbest[s] = static_cast<size_t>(u.h_tail[s].valid);
+                }
+                ++nev[s];
+                want[s] = opt[s]->tell(f, u.h_deriv + s * np) ? 1 : 0;
+                if (want[s]) std::memcpy(u.h_coef + s * np, opt[s]->point(), np * sizeof(double));
+            });
+            next.clear();
+            for (int s : L.act)
+                if (want[s]) next.push_back(s);
+            L.act.swap(next);
+            if (!L.act.empty()) {
+                lane_launch(u, L);
+                busy = true;
+            }
+        }
     }
     for (int s = 0; s < S; ++s) {
         std::memcpy(x + s * np, opt[s]->best(), np * sizeof(double));

@@ -191,13 +191,13 @@ def test_multistart_registration_equals_one_start_at_a_time(hip):
     same start registered alone ends (same evaluations, so the same L-BFGS-B
     iterates), and the best start is the lowest value."""
     df, dm, mesh = _warped_pair(hip)
-    S = 4
+    S = 8  # two lanes of four starts
     rng = np.random.default_rng(9)
     stages = np.stack([_stage(rng.uniform(-0.05, 0.05, 3).tolist() + rng.uniform(-0.8, 0.8, 3).tolist(),
                               [16.0, 16.0, 16.0]) for _ in range(S)])
     kw = dict(bins=24, iterations=15, sampling=hip.SAMPLING_RANDOM, sampling_percentage=0.5, seed=5)
     xs, rs, best = hip.register_bspline_multistart(df, dm, mesh, S, stages=stages, **kw)
-    for s in range(S):
+    for s in (0, 3, 4, 7):
         x1, r1, _ = hip.register_bspline_multistart(df, dm, mesh, 1, stages=stages[s:s + 1], **kw)
         assert np.array_equal(x1[0], xs[s]), s
         assert (r1[0].value, r1[0].iterations, r1[0].evaluations, r1[0].stop) == \
