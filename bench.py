@@ -255,7 +255,7 @@ def main() -> int:
                     help="starts of the deformable multi-start record (0 skips it)")
     ap.add_argument("--plain-starts", type=int, default=8,
                     help="deformable starts registered one at a time for the plain rate")
-    ap.add_argument("--init-pairs", type=int, default=256,
+    ap.add_argument("--init-pairs", type=int, default=1024,
                     help="held-out pairs for the learned-initializer record (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
