@@ -157,6 +157,40 @@ def learned_init_record(S, n_pairs, plain_pairs):
     return rec
 
 
+def large_pair_record(S, W, fixed, moving, T_gt, reps=3):
+    """The single large pair registered end to end by the plain method on the
+    device: affine from the identity about the centre, three-level pyramid
+    (shrink 4 / 2 / 1, sigma 2 / 1 / 0), Mattes 50 bins at 20% sampling,
+    gradient descent with the golden-section line search (100 iterations per
+    level, physical-shift scales, rate estimated once). Median wall clock of
+    `reps` runs from resident volumes; TRE against the inverse generating
+    affine."""
+    df = S.DeviceImage.upload(fixed.data, spacing=fixed.spacing, origin=fixed.origin)
+    dm = S.DeviceImage.upload(moving.data, spacing=moving.spacing, origin=moving.origin)
+    ident = list(np.eye(3).reshape(9)) + [0.0, 0.0, 0.0]
+    cfg = S.registration_config(metric=S.MATTES, bins=50, sampling=S.SAMPLING_RANDOM,
+                                sampling_percentage=0.2, seed=1, optimizer=S.LINE_SEARCH,
+                                iterations=100, learning_rate_mode=S.LR_ONCE,
+                                shrink_factors=((4, 4, 4), (2, 2, 2), (1, 1, 1)),
+                                sigmas=(2.0, 1.0, 0.0))
+    times, res, p = [], None, None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        p, res = S.register(df, dm, cfg, S.AFFINE, ident, center=fixed.center())
+        times.append(time.perf_counter() - t0)
+    est = W.AffineMap(np.asarray(p[:9]).reshape(3, 3), np.asarray(p[9:]), fixed.center())
+    tre = W.tre(est, T_gt.inverse(), fixed)
+    df.free()
+    dm.free()
+    return {"workload": "256x256x160 pair (0.9x0.9x1.5 mm, +-20% bias), affine +-10 deg / +-5% / "
+                        "+-10 mm; 3-level pyramid 4/2/1, sigma 2/1/0; Mattes 50 bins, 20%; "
+                        "gradient descent with line search, affine from the identity",
+            "seconds": float(np.median(times)), "registrations_per_s": 1.0 / float(np.median(times)),
+            "tre_mm": float(tre), "iterations": int(res.iterations),
+            "value_and_gradient_evaluations": int(res.evaluations),
+            "value_evaluations": int(res.value_evaluations)}
+
+
 def deformable_record(S, W, n_starts, n_plain, iterations=100):
     """The deformable multi-start workload: a 192^3 pair related by a seeded
     Euler3D offset and a 20-bump Gaussian-RBF displacement (peak 4 mm); a
@@ -406,6 +440,7 @@ def main() -> int:
         "result": {"value": value, "valid": int(valid)},
         "cpu_baseline": cpu,
     }
+    out["large_pair_registration"] = large_pair_record(S, W, fixed, moving, T_gt)
     if args.pairs > 0:
         out["registrations"] = registrations_record(S, W, args.pairs, args.plain_pairs)
     if args.init_pairs > 0:
