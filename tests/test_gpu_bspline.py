@@ -222,3 +222,36 @@ def test_bounded_multistart_stays_in_the_box(hip):
         assert rs[s].iterations > 0
     # some coefficients end on a bound: the box is active
     assert np.any(np.isclose(xs[0][sel == 2], -0.4) | np.isclose(xs[0][sel == 2], 0.3))
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_deformable_pair_at_scale_against_the_oracle(hip, precision):
+    """The deformable workload's generator at 96^3 (RBF bumps on an Euler3D
+    offset), 10% random draws, a 10^3-point control grid over the image and a
+    rigid moving-initial stage: a batch of two starts against the oracle's
+    composed sparse evaluation, value, derivative and valid count."""
+    import workload as W
+
+    f, m, truth = W.config_deformable(edge=96, nblobs=17, seed=21)
+    idx = hip.draw_samples(f.data.size, hip.SAMPLING_RANDOM, 0.10, 9)
+    df = hip.DeviceImage.upload(f.data, spacing=f.spacing, origin=f.origin)
+    dm = hip.DeviceImage.upload(m.data, spacing=m.spacing, origin=m.origin)
+    met = hip.ResidentMattes(df, dm, bins=32, indices=idx, precision=precision)
+    mesh = (7, 7, 7)
+    gs, go, gsp = W.bspline_grid(f, mesh)
+    bs = hip.BSplineMattes(met, list(gs), list(go), list(gsp))
+    rng = np.random.default_rng(3)
+    coeffs = np.stack([rng.normal(0.0, 0.8, bs.nparams) for _ in range(2)])
+    rig = [O.make_transform(O.EULER3D, list(rng.uniform(-0.05, 0.05, 3)) + list(rng.uniform(-2, 2, 3)),
+                            center=list(f.center())) for _ in range(2)]
+    multi = hip.BSplineMulti(bs, 2, np.stack([O.transform_stage(r) for r in rig]))
+    vals, ders, valid = multi.value_and_derivative(coeffs)
+    pb = O.MetricProblem(f.data, f.spacing, f.origin, m.data, m.spacing, m.origin, indices=idx)
+    g = pb.mattes_geometry(32)
+    tv, td = (1e-9, 1e-7) if precision == 1 else (1e-6, 1e-4)
+    for s in range(2):
+        t = O.make_bspline(f.size, f.spacing, f.origin, np.eye(3), mesh, coeffs[s])
+        want_v, want_d, want_n = pb.mattes_evaluate_composed(g, t, rig[s])
+        assert valid[s] == want_n, s
+        assert abs(vals[s] - want_v) <= tv * abs(want_v), (s, vals[s], want_v)
+        assert rel_vec(ders[s], want_d) < td, (s, rel_vec(ders[s], want_d))
