@@ -815,7 +815,10 @@ struct TailOut {
 // (2^-kValShift units) and rounded once, so the value does not depend on how
 // the bins are split over lanes and workgroups: the one-workgroup tail of the
 // batched engine and the split tail below give the same bits.
-constexpr int kTail = 1024;
+#ifndef SITKHIP_TAIL
+#define SITKHIP_TAIL 512
+#endif
+constexpr int kTail = SITKHIP_TAIL;  // lanes of a tail workgroup
 constexpr int kValShift = 100;
 constexpr int kMaxTailSplit = 16;
 
