@@ -285,6 +285,8 @@ def main() -> int:
                     help="batched rigid pairs for the registrations/s record (0 skips it)")
     ap.add_argument("--plain-pairs", type=int, default=64,
                     help="pairs registered one at a time for the plain rate")
+    ap.add_argument("--lp-reps", type=int, default=3,
+                    help="end-to-end registrations of the large pair (0 skips the record)")
     ap.add_argument("--starts", type=int, default=64,
                     help="starts of the deformable multi-start record (0 skips it)")
     ap.add_argument("--plain-starts", type=int, default=8,
@@ -440,7 +442,8 @@ def main() -> int:
         "result": {"value": value, "valid": int(valid)},
         "cpu_baseline": cpu,
     }
-    out["large_pair_registration"] = large_pair_record(S, W, fixed, moving, T_gt)
+    if args.lp_reps > 0:
+        out["large_pair_registration"] = large_pair_record(S, W, fixed, moving, T_gt, args.lp_reps)
     if args.pairs > 0:
         out["registrations"] = registrations_record(S, W, args.pairs, args.plain_pairs)
     if args.init_pairs > 0:
