@@ -280,7 +280,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-evals", type=int, default=8)
+    ap.add_argument("--cpu-evals", type=int, default=16)
     ap.add_argument("--pairs", type=int, default=512,
                     help="batched rigid pairs for the registrations/s record (0 skips it)")
     ap.add_argument("--plain-pairs", type=int, default=64,
