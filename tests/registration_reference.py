@@ -106,3 +106,106 @@ def register_rsgd(pb, geom, kind, p0, center, size, spacing, origin, minimum_ste
         taken += 1
         v, g = evaluate(p)
     return np.array(p), taken, stop
+
+
+# GradientDescentLineSearchOptimizer (crates/sitk/src/registration/optimizer.rs:
+# 587-785) with the golden-section search (optimizer.rs:458-545) and the window
+# convergence monitor (registration/convergence.rs:35-95), restated on the host.
+GOLDEN_PHI = 1.618034
+GOLDEN_RESPHI = 2.0 - GOLDEN_PHI
+
+
+def golden_section(eps, max_it, a, b, c, fb, counter, f):
+    if counter[0] > max_it:
+        return (c + a) / 2.0
+    counter[0] += 1
+    x = b + GOLDEN_RESPHI * (c - b) if c - b > b - a else b - GOLDEN_RESPHI * (b - a)
+    if abs(c - a) < eps * (abs(b) + abs(x)):
+        return (c + a) / 2.0
+    fx = f(x)
+    if fb is None:
+        fb = f(b)
+    if fx < fb:
+        if c - b > b - a:
+            return golden_section(eps, max_it, b, x, c, fx, counter, f)
+        return golden_section(eps, max_it, a, x, b, fx, counter, f)
+    if c - b > b - a:
+        return golden_section(eps, max_it, a, b, x, fb, counter, f)
+    return golden_section(eps, max_it, x, b, c, fb, counter, f)
+
+
+class WindowConvergence:
+    def __init__(self, window):
+        self.window = window
+        self.values = []
+        self.total = 0.0
+
+    def add(self, v):
+        self.values.append(v)
+        if len(self.values) > self.window:
+            self.values.pop(0)
+        self.total += abs(v)
+
+    def value(self):
+        if len(self.values) < self.window:
+            return None
+        if self.total == 0.0:
+            return 0.0
+        w = float(self.window)
+        o0 = o1 = d0 = d1 = 0.0
+        for n, e in enumerate(self.values):
+            x = n / (w - 1.0)
+            a, b = 1.0 - x, x
+            y = e / self.total
+            d = a * a + b * b
+            o0 += a * a
+            o1 += b * b
+            d0 += a * a * a * y / d
+            d1 += b * b * b * y / d
+        c0 = d0 / o0 if o0 != 0.0 else 0.0
+        c1 = d1 / o1 if o1 != 0.0 else 0.0
+        return c0 - c1
+
+
+def register_gd_line_search(pb, geom, kind, p0, center, size, spacing, origin, iterations,
+                            window=10, min_convergence=1e-6):
+    """The estimated-rate line-search method (method.rs:1047-1057, 3361-3373):
+    rate from the physical-shift estimator at the start, physical-shift
+    scales. Returns (best params, iterations, stop reason name)."""
+    t = O.make_transform(kind, p0, center=center)
+    est = Estimator(t, size, spacing, origin)
+    scales = est.scales()
+
+    def evaluate(p):
+        tt = O.make_transform(kind, p, center=center)
+        v, d, _ = pb.mattes_evaluate(geom, tt)
+        return v, d
+
+    def value_of(p):
+        return pb.mattes_value(geom, O.make_transform(kind, p, center=center))
+
+    p = list(p0)
+    _, g = evaluate(p)
+    lr = est.learning_rate([g[k] / scales[k] for k in range(len(p))])
+    value, grad = evaluate(p)
+    best, best_p = value, list(p)
+    mon = WindowConvergence(window)
+    taken, stop = 0, "MaxIterations"
+    while True:
+        if value < best:
+            best, best_p = value, list(p)
+        mon.add(value)
+        cv = mon.value()
+        if cv is not None and cv <= min_convergence:
+            stop = "Converged"
+            break
+        if taken >= iterations:
+            break
+        d = [grad[k] / scales[k] for k in range(len(p))]
+        base = p
+        lr = golden_section(0.01, 20, lr * 0.0, lr, lr * 5.0, None, [0],
+                            lambda x: value_of([base[k] - x * d[k] for k in range(len(base))]))
+        p = [p[k] - lr * d[k] for k in range(len(p))]
+        taken += 1
+        value, grad = evaluate(p)
+    return np.array(best_p), taken, stop

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from registration_reference import register_rsgd
+from registration_reference import register_gd_line_search, register_rsgd
 
 pytestmark = pytest.mark.gpu
 
@@ -85,6 +85,35 @@ def test_single_level_mattes_registration_matches_the_host_loop(hip, precision):
     if precision == 1:
         assert res.iterations == want_it, (res.iterations, want_it, want_stop)
     assert hip.STOP_NAMES[res.level_stop[0]] == want_stop or precision == 0
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_single_level_line_search_registration_matches_the_host_loop(hip, precision):
+    """Gradient descent with the golden-section line search and the window
+    convergence stop (optimizer.rs:587-785) against its host restatement on
+    the oracle's metric: the same iterates, so the same result, iteration
+    count and stop reason (F64)."""
+    n = 32
+    sp, org = [0.8, 0.9, 1.1], [-3.0, 2.0, 1.0]
+    f = O.fixture_volume(n, [0, 0, 0])
+    m = O.fixture_inverted(n, [-2.0, 1.0, 0.0])
+    pb = O.MetricProblem(f, sp, org, m, sp, org)
+    g = pb.mattes_geometry(50)
+    want, want_it, want_stop = register_gd_line_search(pb, g, O.TRANSLATION, [0.0, 0.0, 0.0],
+                                                       [0, 0, 0], (n, n, n), sp, org,
+                                                       iterations=40)
+    df = hip.DeviceImage.upload(f, spacing=sp, origin=org)
+    dm = hip.DeviceImage.upload(m, spacing=sp, origin=org)
+    cfg = hip.registration_config(metric=hip.MATTES, bins=50, precision=precision,
+                                  optimizer=hip.LINE_SEARCH, iterations=40,
+                                  learning_rate_mode=hip.LR_ONCE)
+    got, res = hip.register(df, dm, cfg, hip.TRANSLATION, [0.0, 0.0, 0.0])
+    assert np.max(np.abs(got)) > 0.5  # it moved
+    tol = 1e-6 if precision == 1 else 1e-2
+    assert np.max(np.abs(got - want)) < tol, (got, want)
+    if precision == 1:
+        assert res.iterations == want_it, (res.iterations, want_it, want_stop)
+        assert hip.STOP_NAMES[res.level_stop[0]] == want_stop
 
 
 def test_three_level_registration_recovers_the_warp(hip):
