@@ -623,8 +623,14 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
                 cc.f_i2p[i] = m->F.i2p[i];
                 cc.m_p2i[i] = m->Mv.p2i[i];
             }
+            // from a pinned slot: a copy from pageable memory would wait for
+            // the stream to reach it
+            void *unused = nullptr;
+            m->chain_host = pinned_slot_alloc(sizeof(ChainConst), &unused);
+            std::memcpy(m->chain_host, &cc, sizeof(ChainConst));
             m->chain.alloc(1);
-            m->chain.upload(&cc, 1);
+            HIP_OK(hipMemcpyAsync(m->chain.ptr, m->chain_host, sizeof(ChainConst),
+                                  hipMemcpyHostToDevice, backend().stream));
         }
 
         m->partials.alloc(kGrid * 14);

@@ -1079,7 +1079,11 @@ struct Metric {
     bool ranges_pending = false;                 // ranges enqueued, not yet read on the host
     bool ranges_known = false;                   // fixed_range / moving_range hold them
     long long moving_voxels = 0;
-    ~Metric() { pinned_slot_free(res); }
+    void *chain_host = nullptr;                  // pinned source of the chain's upload
+    ~Metric() {
+        pinned_slot_free(res);
+        pinned_slot_free(chain_host);
+    }
     double fixed_range[2] = {0, 0};
     double moving_range[2] = {0, 0};
     size_t nsamples = 0;

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "host_pool.hpp"
 #include "common.hpp"
 #include "image.hpp"
 #include "metric.hpp"
@@ -453,6 +454,25 @@ static void prepare_level_end(LevelInputs &in, const sitkhip_registration_config
     }
 }
 
+// Streams for the level set-up of many pairs (the library stream and three
+// more, default priority) and the host threads that feed them, made once per
+// process.
+struct PrepLanes {
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ready = nullptr;
+    HostPool pool{4};
+};
+
+static PrepLanes &prep_lanes() {
+    static PrepLanes *lanes = [] {
+        auto *l = new PrepLanes();
+        for (auto &s : l->side) HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&l->ready, hipEventDisableTiming));
+        return l;
+    }();
+    return *lanes;
+}
+
 static LevelInputs prepare_level(const Image &fixed, const Image &moving,
                                  const sitkhip_registration_config &cfg, int level) {
     LevelInputs in = prepare_level_begin(fixed, moving, cfg, level);
@@ -626,8 +646,7 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
             if (level < cfg[i].levels) who.push_back(i);
         const size_t m = who.size();
         const auto tp0 = now();
-        std::vector<LevelInputs> inputs;
-        inputs.reserve(m);
+        std::vector<LevelInputs> inputs(m);
         std::vector<Metric *> metrics(m);
         // every pair's level enqueued (the smoothing of all fixed and moving
         // images in a few multi-volume launches), one stream wait, then the
@@ -646,18 +665,49 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
             sig.push_back({s, s, s});
         }
         std::vector<Image *> smoothed = recursive_gaussian_batch(vols, sig);
-        for (size_t k = 0; k < m; ++k) {
-            const size_t i = who[k];
-            Image *sf = nullptr, *sm = nullptr;
-            if (vol_of[k] != SIZE_MAX) {
-                sf = smoothed[vol_of[k]];
-                sm = smoothed[vol_of[k] + 1];
-                smoothed[vol_of[k]] = smoothed[vol_of[k] + 1] = nullptr;
-            }
-            inputs.push_back(prepare_level_begin(*fixed[i], *moving[i], cfg[i], level, sf, sm));
-            metrics[k] = inputs.back().metric.get();
+        // Each pair's set-up is some ten small launches (resampling, draws and
+        // their counting sort, fixed values, ranges); spread over four streams,
+        // each fed by its own host thread, their kernels overlap instead of
+        // queueing one after another and the launches are made four at a
+        // time. The side streams start after the smoothing; blocks freed
+        // meanwhile are held back until every stream has drained.
+        const hipStream_t main_stream = backend().stream;
+        PrepLanes &lanes = prep_lanes();
+        const size_t L = m >= 8 ? 4 : 1;
+        if (L > 1) {
+            HIP_OK(hipEventRecord(lanes.ready, main_stream));
+            for (size_t l = 1; l < L; ++l) HIP_OK(hipStreamWaitEvent(lanes.side[l - 1], lanes.ready, 0));
         }
-        sync();
+        auto drain_lanes = [&] {
+            for (size_t l = 1; l < L; ++l) HIP_OK(hipStreamSynchronize(lanes.side[l - 1]));
+            sync();
+        };
+        device_defer_frees(L > 1);
+        auto tpe = now();
+        try {
+            lanes.pool.run(static_cast<int>(L), [&](int l) {
+                StreamScope scope(l == 0 ? main_stream : lanes.side[l - 1]);
+                for (size_t k = static_cast<size_t>(l); k < m; k += L) {
+                    const size_t i = who[k];
+                    Image *sf = nullptr, *sm = nullptr;
+                    if (vol_of[k] != SIZE_MAX) {
+                        sf = smoothed[vol_of[k]];
+                        sm = smoothed[vol_of[k] + 1];
+                        smoothed[vol_of[k]] = smoothed[vol_of[k] + 1] = nullptr;
+                    }
+                    inputs[k] = prepare_level_begin(*fixed[i], *moving[i], cfg[i], level, sf, sm);
+                    metrics[k] = inputs[k].metric.get();
+                }
+            });
+            tpe = now();
+            drain_lanes();
+        } catch (...) {
+            for (size_t l = 1; l < L; ++l) (void)hipStreamSynchronize(lanes.side[l - 1]);
+            (void)hipStreamSynchronize(main_stream);
+            device_defer_frees(false);
+            throw;
+        }
+        device_defer_frees(false);
         for (size_t k = 0; k < m; ++k) prepare_level_end(inputs[k], cfg[who[k]]);
         const auto tp1 = now();
         // Two engines over alternate pairs once the batch is large: the host
@@ -769,9 +819,10 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
         if (trace) {
             const auto tp4 = now();
             std::fprintf(stderr,
-                         "register_batch level %d: %zu pairs, prepare %.1f ms, batch %.1f ms, "
-                         "tune %.1f ms, %d evaluations %.1f ms (%.3f ms each)\n",
-                         level, m, ms(tp0, tp1), ms(tp1, tp2), ms(tp2, tp3), rounds, ms(tp3, tp4),
+                         "register_batch level %d: %zu pairs, prepare %.1f ms (enqueued at %.1f ms), "
+                         "batch %.1f ms, tune %.1f ms, %d evaluations %.1f ms (%.3f ms each)\n",
+                         level, m, ms(tp0, tp1), ms(tp0, tpe), ms(tp1, tp2), ms(tp2, tp3), rounds,
+                         ms(tp3, tp4),
                          ms(tp3, tp4) / rounds);
         }
         for (size_t k = 0; k < m; ++k) {

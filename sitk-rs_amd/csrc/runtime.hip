@@ -12,6 +12,7 @@
 #include <map>
 #include <mutex>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -130,11 +131,29 @@ void *device_alloc(size_t bytes) {
     return p;
 }
 
+static bool g_defer_frees = false;
+static std::vector<std::pair<void *, size_t>> g_deferred;  // (block, size class)
+
 void device_free(void *ptr, size_t bytes) {
     if (!ptr) return;
     std::lock_guard<std::mutex> lk(g_alloc_mu);
+    if (g_defer_frees) {
+        g_deferred.emplace_back(ptr, size_class(bytes));
+        return;
+    }
     g_free_blocks[size_class(bytes)].push_back(ptr);
     if (Slab *sl = slab_of(ptr)) --sl->live;
+}
+
+void device_defer_frees(bool on) {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    g_defer_frees = on;
+    if (on) return;
+    for (auto &d : g_deferred) {
+        g_free_blocks[d.second].push_back(d.first);
+        if (Slab *sl = slab_of(d.first)) --sl->live;
+    }
+    g_deferred.clear();
 }
 
 void device_empty_cache() {
@@ -229,11 +248,27 @@ static void init_backend_locked(int device) {
     g_backend_ready = true;
 }
 
+// the stream a StreamScope set on this thread (nullptr: the library stream)
+static thread_local hipStream_t t_stream = nullptr;
+static thread_local Backend t_view;
+
 Backend &backend() {
     std::lock_guard<std::mutex> lk(g_backend_mu);
     if (!g_backend_ready) init_backend_locked(0);
     HIP_OK(hipSetDevice(g_backend.device));
-    return g_backend;
+    return t_stream ? t_view : g_backend;
+}
+
+StreamScope::StreamScope(hipStream_t s) : prev_(t_stream) {
+    const Backend b = backend();
+    t_view = b;
+    t_view.stream = s;
+    t_stream = s;
+}
+
+StreamScope::~StreamScope() {
+    t_stream = prev_;
+    t_view.stream = prev_;
 }
 
 void sync() { HIP_OK(hipStreamSynchronize(backend().stream)); }

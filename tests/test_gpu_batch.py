@@ -201,6 +201,41 @@ def test_batched_registration_matches_one_at_a_time(hip):
         assert res[i].valid_points == r1.valid_points
 
 
+def test_batched_pyramid_registrations_match_one_at_a_time(hip):
+    """Twelve pairs with two-level pyramids (shrink 2 and sigma 1, then full
+    resolution): from eight pairs on, the levels are set up over several
+    streams, so this checks that set-up against the one-at-a-time path for
+    every sampling mode (device draws, a regular grid, every voxel)."""
+    import workload as W
+
+    rng = np.random.default_rng(17)
+    fixed, moving, configs, kinds, p0, centres = [], [], [], [], [], []
+    for i in range(12):
+        shp = (22 + 2 * (i % 4), 26 - (i % 3), 20 + 3 * (i % 2))
+        vol = W.blob_volume(shp, (1.0, 1.0, 1.0), seed=90 + i, nblobs=16)
+        (ang, t), T_gt = W.random_euler(rng, 4.0, 2.0, vol.center())
+        mov = W.resample_linear(vol, T_gt)
+        fixed.append(hip.DeviceImage.upload(vol.data, spacing=vol.spacing, origin=vol.origin))
+        moving.append(hip.DeviceImage.upload(mov.data, spacing=mov.spacing, origin=mov.origin))
+        metric = hip.MATTES if i % 3 != 1 else hip.MEAN_SQUARES
+        sampling = (2, 0, 1)[i % 3]
+        configs.append(hip.registration_config(metric=metric, bins=24, sampling=sampling,
+                                               sampling_percentage=0.3, seed=5 + i,
+                                               minimum_step=1e-3, iterations=25,
+                                               learning_rate_mode=hip.LR_ONCE,
+                                               shrink_factors=((2, 2, 2), (1, 1, 1)),
+                                               sigmas=(1.0, 0.0)))
+        kinds.append(hip.EULER3D)
+        p0.append([0.0] * 6)
+        centres.append(vol.center())
+    got, res = hip.register_batch(fixed, moving, configs, kinds, p0, centres)
+    for i in range(12):
+        want, r1 = hip.register(fixed[i], moving[i], configs[i], kinds[i], p0[i], center=centres[i])
+        assert np.max(np.abs(got[i] - want)) < 1e-6, (i, got[i], want)
+        assert res[i].iterations == r1.iterations, (i, res[i].iterations, r1.iterations)
+        assert res[i].valid_points == r1.valid_points
+
+
 def test_probe_values_are_the_zero_iteration_values(hip):
     """register_batch's probe: each pair's metric value at the probe
     parameters on the first prepared level equals the value a zero-iteration
