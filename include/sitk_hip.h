@@ -199,6 +199,60 @@ int sitkhip_metric_create(int kind, sitkhip_image fixed, const int64_t *fixed_in
  * ascending voxel order by a device counting sort. No fixed mask. */
 int sitkhip_metric_create_drawn(int kind, sitkhip_image fixed, size_t n_draws, uint64_t seed,
                                 sitkhip_image moving, int precision, sitkhip_metric *out);
+
+/* FixedPoints (src/cuda/ops/resident.rs:464-484): where the fixed samples'
+ * physical points come from.
+ *   GRID     : every voxel of the grid (size, origin, idx_to_phys), dim-0
+ *              fastest; the sample count is the voxel count;
+ *   INDICES  : sample s is grid voxel indices[s] (duplicates count twice);
+ *   EXPLICIT : sample s is the physical point points[3 s .. 3 s + 2] and has
+ *              no grid voxel (the grid fields are unused). */
+typedef enum {
+    SITKHIP_POINTS_GRID = 0,
+    SITKHIP_POINTS_EXPLICIT = 1,
+    SITKHIP_POINTS_INDICES = 2
+} sitkhip_points_mode;
+typedef struct {
+    int mode;                   /* sitkhip_points_mode */
+    const double *points;       /* EXPLICIT: n x 3, row-major */
+    const int64_t *indices;     /* INDICES: n flat grid indices */
+    size_t size[3];             /* GRID / INDICES: the fixed grid */
+    double origin[3];
+    double idx_to_phys[9];      /* row-major, Direction * diag(spacing) */
+} sitkhip_fixed_points;
+
+/* MovingGeometry (src/cuda/ops/resident.rs:424-437). strides must be the
+ * dense layout (1, size[0], size[0] * size[1]); mask may be NULL (len bytes,
+ * non-zero = inside). phys_to_index is the inverse of the composed
+ * Direction * diag(spacing), row-major. */
+typedef struct {
+    size_t len;
+    size_t size[3];
+    size_t strides[3];
+    double origin[3];
+    double phys_to_index[9];
+    const uint8_t *mask;
+} sitkhip_moving_geometry;
+
+/* ResidentMetric::new (src/cuda/ops/mean_squares.rs:201-256), the constructor
+ * CudaMetricBackend calls (src/registration/cuda.rs:195-200): both volumes
+ * from host values instead of device images.
+ *   fixed_values  : n values (EXPLICIT: the host's gathered per-sample values)
+ *                   or one per grid voxel (GRID, INDICES);
+ *   moving_values : moving->len values;
+ *   fixed_mask    : optional, one byte per grid voxel; refused with EXPLICIT
+ *                   points (SITKHIP_ERR_MASKED_EXPLICIT_POINTS, error.rs).
+ * The reference's producers widen each pixel to f64; here the values are
+ * narrowed to float32 on upload, (float)x, as every device volume of this
+ * library is float32 (exact for float32 and integer pixels below 2^24).
+ * kind may be mean squares or Mattes (then set the bins as for any metric).
+ * Refusals in the order of Resident::build (resident.rs:530-660): n == 0 or a
+ * length that does not match its size -> SITKHIP_ERR_DEGENERATE_INPUT, an
+ * index outside the grid -> SITKHIP_ERR_SAMPLE_INDEX_OUT_OF_GRID. */
+int sitkhip_metric_create_host(int kind, size_t n, const double *fixed_values,
+                               const sitkhip_fixed_points *points, const uint8_t *fixed_mask,
+                               const sitkhip_moving_geometry *moving, const double *moving_values,
+                               int precision, sitkhip_metric *out);
 void sitkhip_metric_free(sitkhip_metric m);
 /* sample_count (mean_squares.rs:313-318): samples the metric walks (after the
  * fixed mask). */

@@ -51,6 +51,7 @@ class Problem(C.Structure):
         ("fvals", C.POINTER(C.c_float)),
         ("idx", C.POINTER(C.c_int64)),
         ("nidx", C.c_size_t),
+        ("pts", C.POINTER(C.c_double)),
         ("fmask", C.POINTER(C.c_uint8)),
         ("msize", C.c_size_t * 3),
         ("morigin", C.c_double * 3),
@@ -286,12 +287,21 @@ class MetricProblem:
     """
 
     def __init__(self, fixed, fspacing, forigin, moving, mspacing, morigin,
-                 fdir=None, mdir=None, indices=None, fixed_mask=None, moving_mask=None):
+                 fdir=None, mdir=None, indices=None, fixed_mask=None, moving_mask=None,
+                 points=None):
+        """points: explicit sample points (n x 3, physical), with `fixed` then
+        the n gathered sample values (FixedPoints::Explicit); the fixed
+        geometry is unused."""
         eye = np.eye(3).reshape(9)
         fdir = eye if fdir is None else np.asarray(fdir, dtype=np.float64).reshape(9)
         mdir = eye if mdir is None else np.asarray(mdir, dtype=np.float64).reshape(9)
         self.fixed = np.ascontiguousarray(fixed, dtype=np.float32)
         self.moving = np.ascontiguousarray(moving, dtype=np.float32)
+        self._pts = None
+        if points is not None:
+            self._pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            self.fixed = self.fixed.reshape(1, 1, -1)
+            assert self.fixed.size == self._pts.shape[0]
         fz, fy, fx = self.fixed.shape
         mz, my, mx = self.moving.shape
         pb = Problem()
@@ -305,6 +315,9 @@ class MetricProblem:
             self._idx = np.ascontiguousarray(indices, dtype=np.int64)
             pb.idx = self._idx.ctypes.data_as(C.POINTER(C.c_int64))
             pb.nidx = self._idx.size
+        if self._pts is not None:
+            pb.pts = self._pts.ctypes.data_as(C.POINTER(C.c_double))
+            pb.nidx = self._pts.shape[0]
         self._fmask = None
         if fixed_mask is not None:
             self._fmask = np.ascontiguousarray(fixed_mask, dtype=np.uint8)

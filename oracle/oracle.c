@@ -514,18 +514,26 @@ typedef struct {
     size_t n;          /* raw selection length */
 } sample_iter;
 
+/* Explicit points (FixedPoints::Explicit, cuda/ops/resident.rs:464-484):
+ * sample s is the point pts[s] with the gathered value fvals[s]; there is no
+ * grid index, so no fixed mask applies (error.rs MaskedExplicitPoints). */
 static size_t selection_len(const orc_problem *pb) {
+    if (pb->pts) return pb->nidx;
     return pb->idx ? pb->nidx : pb->fsize[0] * pb->fsize[1] * pb->fsize[2];
 }
 static size_t selection_flat(const orc_problem *pb, size_t s) {
     return pb->idx ? (size_t)pb->idx[s] : s;
 }
 static int selection_allowed(const orc_problem *pb, size_t flat) {
-    return pb->fmask ? pb->fmask[flat] != 0 : 1;
+    return pb->fmask && !pb->pts ? pb->fmask[flat] != 0 : 1;
 }
 
 /* src/registration/scales.rs:259-271 (origin-first fold). */
 static void write_point(const orc_problem *pb, size_t flat, double out[3]) {
+    if (pb->pts) {
+        for (int r = 0; r < 3; ++r) out[r] = pb->pts[3 * flat + r];
+        return;
+    }
     size_t id[3];
     for (int d = 0; d < 3; ++d) { id[d] = flat % pb->fsize[d]; flat /= pb->fsize[d]; }
     for (int r = 0; r < 3; ++r) {

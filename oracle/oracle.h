@@ -116,9 +116,10 @@ typedef struct {
     size_t fsize[3];
     double forigin[3];
     double fi2p[9];          /* index -> physical, Direction * diag(spacing) */
-    const float *fvals;      /* whole fixed grid, dim-0-fastest */
+    const float *fvals;      /* whole fixed grid, dim-0-fastest (explicit points: one per sample) */
     const int64_t *idx;      /* sample list (flat indices, sample order), or NULL = grid */
-    size_t nidx;
+    size_t nidx;             /* length of idx, or the number of explicit points */
+    const double *pts;       /* explicit points (resident.rs:464-484), nidx x 3, or NULL */
     const uint8_t *fmask;    /* optional fixed mask on the fixed grid */
     size_t msize[3];
     double morigin[3];

@@ -548,6 +548,7 @@ static void read_outputs(BatchEngine &b, std::vector<BatchOutput> &out) {
 }
 
 BatchTiling batch_autotune(BatchEngine &b, const double *stages, double *best_ms) {
+    LaunchScope scope("batch_autotune");
     set_control(b, stages, nullptr);
     const int cu = std::max(8, backend().cu_count);
     const size_t tiles[] = {2048, 8192};
@@ -599,6 +600,9 @@ extern "C" {
 int sitkhip_batch_create(const sitkhip_metric *metrics, size_t n, sitkhip_batch *out) {
     return run_guarded([&] {
         if (!metrics || !out) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null argument");
+        for (size_t i = 0; i < n; ++i)
+            if (metrics[i] && metric_has_explicit_points(*reinterpret_cast<Metric *>(metrics[i])))
+                fail(SITKHIP_ERR_INVALID_ARGUMENT, "metrics over explicit points are evaluated one at a time");
         *out = reinterpret_cast<sitkhip_batch>(
             batch_create(reinterpret_cast<Metric *const *>(metrics), n));
     });

@@ -61,6 +61,7 @@ struct BSplineMap {
 template <>
 struct MapTraits<BSplineMap> {
     static constexpr bool affine = false;
+    static constexpr bool points = false;
 };
 
 __host__ __device__ inline double bspline_b3(double u) {
@@ -595,6 +596,9 @@ int sitkhip_bspline_create(sitkhip_metric metric, const size_t grid_size[3],
         auto m = reinterpret_cast<Metric *>(metric);
         if (!m || !grid_size || !grid_origin || !grid_spacing || !grid_direction || !out)
             fail(SITKHIP_ERR_INVALID_ARGUMENT, "null argument");
+        if (metric_has_explicit_points(*m))
+            fail(SITKHIP_ERR_INVALID_ARGUMENT, "the B-spline metric groups samples by grid voxel; "
+                                               "explicit points have none");
         *out = reinterpret_cast<sitkhip_bspline>(
             bspline_create(*m, grid_size, grid_origin, grid_spacing, grid_direction));
     });
@@ -962,6 +966,9 @@ static void bspline_setup(Image &f, Image &mv, const sitkhip_bspline_config &cfg
     const size_t ncp = gs[0] * gs[1] * gs[2];
     if (ncoeffs != 3 * ncp) fail(SITKHIP_ERR_INVALID_ARGUMENT, "coefficient count must be 3 x (mesh+3)^3");
     Metric *m = nullptr;
+    if (cfg.sampling != 0 && cfg.sampling != 2)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "B-spline registration samples every voxel or draws at random");
+    check_sampling_fraction(cfg.sampling, cfg.sampling_percentage, f.len());
     if (cfg.sampling == 2) {
         const size_t cnt = static_cast<size_t>(static_cast<double>(f.len()) * cfg.sampling_percentage);
         m = metric_create_drawn(SITKHIP_METRIC_MATTES, f, cnt, cfg.seed, mv, cfg.precision);

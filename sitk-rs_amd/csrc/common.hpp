@@ -149,4 +149,16 @@ inline unsigned grid_for(size_t n, unsigned block, unsigned cap) {
     return static_cast<unsigned>(g);
 }
 
+// Refuses a sampling fraction that the draw cannot turn into a count: the
+// reference casts ceil(1 / p) (regular) and voxels * p (random) to usize, which
+// saturates in Rust but is undefined in C++ for NaN, infinities and negatives.
+// Only strategies 1 (regular) and 2 (random) read the fraction.
+inline void check_sampling_fraction(int strategy, double p, size_t voxels) {
+    if (strategy != 1 && strategy != 2) return;
+    const double inv = 1.0 / p;
+    if (!(p > 0.0) || !(inv < 9.0e18) || !(static_cast<double>(voxels) * p < 9.0e18))
+        fail(SITKHIP_ERR_INVALID_ARGUMENT,
+             "sampling percentage must be a finite number above 0 (got " + std::to_string(p) + ")");
+}
+
 }  // namespace sitkhip

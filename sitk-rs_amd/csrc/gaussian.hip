@@ -439,13 +439,12 @@ Image *recursive_gaussian(const Image &in, const double sigma[3], bool synchrono
             if (synchronous) sync();
             return out;
         }
-        // FP64 intermediates, kept between calls (every call is synchronous and
-        // the library has one stream, so one pair of buffers serves them all).
-        static std::mutex mu;
-        static DeviceBuffer<double> A, B;
-        std::lock_guard<std::mutex> lk(mu);
-        if (A.len < n) A.alloc(n);
-        if (na >= 2 && B.len < n) B.alloc(n);
+        // FP64 intermediates of this call, from the caching allocator: freed
+        // at return while the launches are still queued, a block is reused
+        // only by later work on the same stream (or held back while work is
+        // spread over several streams), so calls on different streams never
+        // share one.
+        DeviceBuffer<double> A(n), B(na >= 2 ? n : 1);
         const float *src32 = in.buf.ptr;
         if (na == 1) {
             IirCoef k = zero_order_coefficients(sigma[axes[0]] / g.spacing[axes[0]]);

@@ -41,6 +41,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <type_traits>
 #include <vector>
 
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Map 
 #pragma unroll
     for (int k = 0; k < NS; ++k) acc[k] = 0.0;
     __shared__ double s_stage[kMaxStages * 12];
-    const StageView V = stage_to_lds(S, s_stage);
+    const auto V = stage_to_lds(S, s_stage);
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
@@ -120,11 +121,11 @@ __global__ __launch_bounds__(kBlock) SITKHIP_HIST_OCC void mattes_hist(FixedGrid
     __shared__ unsigned s_defer_n;
     if (threadIdx.x == 0) s_defer_n = 0u;
     __shared__ double s_stage[kMaxStages * 12];
-    const StageView V = stage_to_lds(S, s_stage);  // ends with a barrier
+    const auto V = stage_to_lds(S, s_stage);  // ends with a barrier
     long long first, step, end;
     xcd_walk(SS.n, first, step, end);
     uint32_t *dq = DEFER ? defer + static_cast<size_t>(blockIdx.x) * defer_cap : nullptr;
-    hist_range<R, REC, StageView, DEFER>(F, M, V, X, C, SS, fbin, P, joint, marg, rec, first, end, step, dq,
+    hist_range<R, REC, std::decay_t<decltype(V)>, DEFER>(F, M, V, X, C, SS, fbin, P, joint, marg, rec, first, end, step, dq,
                                          &s_defer_n);
     if (DEFER) {
         __syncthreads();
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M,
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = 0.0;
     __shared__ double s_stage[kMaxStages * 12];
-    const StageView V = stage_to_lds(S, s_stage);
+    const auto V = stage_to_lds(S, s_stage);
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
@@ -513,6 +514,14 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
         m->F.div0 = make_fastdiv(m->F.size[0]);
         m->F.div1 = make_fastdiv(m->F.size[1]);
         index_to_physical(fg, m->F.i2p);
+        if (opt.f_i2p) std::memcpy(m->F.i2p, opt.f_i2p, sizeof(m->F.i2p));
+        if (opt.points) {
+            // n x 1 x 1 grid of gathered values, origin 0, identity i2p: the
+            // index-space moments are then the physical ones
+            m->pts.alloc(3 * nf);
+            m->pts.upload(opt.points, 3 * nf);
+            m->F.pts = m->pts.ptr;
+        }
 
         // Sample list (host-side validation and mask filter, as FixedSamples does).
         std::vector<uint8_t> hmask;
@@ -526,6 +535,10 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
         if (draw_on_device) {
             if (fmask) fail(SITKHIP_ERR_INVALID_ARGUMENT, "device draws take no fixed mask");
             if (n_draws >= (1ull << 31)) fail(SITKHIP_ERR_INVALID_ARGUMENT, "too many draws");
+            // the counting sort scans one count per voxel with an int item count
+            if (nf >= (1ull << 31))
+                fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY,
+                     "device draws need a fixed grid below 2^31 voxels");
             device_draws(*m, nf, n_draws, draw_seed);
         } else if (!grid) {
             list.reserve(n_indices);
@@ -612,6 +625,7 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
         double mi2p[9];
         index_to_physical(mg, mi2p);
         invert3_host(mi2p, m->Mv.p2i);
+        if (opt.m_p2i) std::memcpy(m->Mv.p2i, opt.m_p2i, sizeof(m->Mv.p2i));
         for (int i = 0; i < 9; ++i) m->Mv.p2i_f[i] = static_cast<float>(m->Mv.p2i[i]);
         {
             ChainConst cc;
@@ -671,6 +685,75 @@ Metric *metric_create_with(int kind, const Image &fixed, const int64_t *indices,
     return metric_create_impl(kind, fixed, indices, n_indices, nullptr, moving, nullptr, precision,
                               false, 0, 0, opt);
 }
+
+Metric *metric_create_host(int kind, size_t n, const double *fixed_values,
+                           const sitkhip_fixed_points &pts, const uint8_t *fixed_mask,
+                           const sitkhip_moving_geometry &mg, const double *moving_values,
+                           int precision) {
+    // The checks of Resident::build (resident.rs:530-660), in its order.
+    if (n == 0) fail(SITKHIP_ERR_DEGENERATE_INPUT, "no fixed samples");
+    if (!fixed_values || !moving_values) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null value array");
+    const size_t mlen = mg.size[0] * mg.size[1] * mg.size[2];
+    if (mg.len != mlen || mlen == 0)
+        fail(SITKHIP_ERR_DEGENERATE_INPUT, "moving length does not match its size");
+    if (mg.strides[0] != 1 || mg.strides[1] != mg.size[0] || mg.strides[2] != mg.size[0] * mg.size[1])
+        fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "the moving volume must be dense, dimension 0 fastest");
+    sitkhip_geometry fg{}, mgeo{};
+    for (int d = 0; d < 3; ++d) {
+        mgeo.size[d] = mg.size[d];
+        mgeo.spacing[d] = 1.0;
+        mgeo.origin[d] = mg.origin[d];
+        mgeo.direction[d * 4] = 1.0;
+        fg.spacing[d] = 1.0;
+        fg.direction[d * 4] = 1.0;
+    }
+    CreateOptions opt;
+    opt.m_p2i = mg.phys_to_index;
+    const int64_t *indices = nullptr;
+    size_t n_indices = 0, fixed_len = 0;
+    switch (pts.mode) {
+        case SITKHIP_POINTS_EXPLICIT:
+            if (!pts.points) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null explicit points");
+            fg.size[0] = n;
+            fg.size[1] = fg.size[2] = 1;
+            fixed_len = n;
+            opt.points = pts.points;
+            break;
+        case SITKHIP_POINTS_GRID:
+        case SITKHIP_POINTS_INDICES:
+            for (int d = 0; d < 3; ++d) {
+                fg.size[d] = pts.size[d];
+                fg.origin[d] = pts.origin[d];
+            }
+            fixed_len = pts.size[0] * pts.size[1] * pts.size[2];
+            if (fixed_len == 0) fail(SITKHIP_ERR_DEGENERATE_INPUT, "empty fixed grid");
+            if (pts.mode == SITKHIP_POINTS_GRID && fixed_len != n)
+                fail(SITKHIP_ERR_DEGENERATE_INPUT, "a grid sample set has one sample per voxel");
+            if (pts.mode == SITKHIP_POINTS_INDICES) {
+                if (!pts.indices) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null sample indices");
+                indices = pts.indices;
+                n_indices = n;
+            }
+            opt.f_i2p = pts.idx_to_phys;
+            break;
+        default:
+            fail(SITKHIP_ERR_INVALID_ARGUMENT, "unknown fixed point mode");
+    }
+    if (fixed_mask && pts.mode == SITKHIP_POINTS_EXPLICIT)
+        fail(SITKHIP_ERR_MASKED_EXPLICIT_POINTS,
+             "a fixed mask gates samples by grid voxel; explicit points have none");
+    // A grid sample set whose index list would be the grid itself: no list
+    // (the list form is only needed with a subset or repeats).
+    std::unique_ptr<Image> fimg(image_upload(SITKHIP_FLOAT64, fixed_values, fixed_len, fg));
+    std::unique_ptr<Image> mimg(image_upload(SITKHIP_FLOAT64, moving_values, mlen, mgeo));
+    std::unique_ptr<Mask> fmask, mmask;
+    if (fixed_mask) fmask.reset(mask_upload(fixed_mask, fixed_len, fg));
+    if (mg.mask) mmask.reset(mask_upload(mg.mask, mlen, mgeo));
+    return metric_create_impl(kind, *fimg, indices, n_indices, fmask.get(), *mimg, mmask.get(),
+                              precision, false, 0, 0, opt);
+}
+
+bool metric_has_explicit_points(const Metric &m) { return m.F.pts != nullptr; }
 
 void metric_finish_ranges(Metric &m) {
     if (m.ranges_pending) finish_ranges(m);
@@ -776,9 +859,15 @@ static void require_bins(const Metric &m) {
 
 // Calls f with the point map as the kernels take it: one stage in registers
 // (the common case, 12 doubles of kernel argument) or the general replay.
+// Metrics over explicit points take PointStages (every sample on the exact
+// chain).
 template <typename Fn>
-static void with_map(const Stages &S, Fn &&f) {
-    if (S.n == 1) {
+static void with_map(const Metric &m, const Stages &S, Fn &&f) {
+    if (m.F.pts) {
+        PointStages P;
+        static_cast<Stages &>(P) = S;
+        f(P);
+    } else if (S.n == 1) {
         OneStage one;
         std::memcpy(one.m, S.s, sizeof(one.m));
         f(one);
@@ -846,7 +935,7 @@ static void launch_hist(Metric &m, const Stages &S, bool tail, bool clear = true
     {
         LaunchScope scope("mattes_hist");
         const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
-        with_map(S, [&](const auto &map) {
+        with_map(m, S, [&](const auto &map) {
             using Map = std::decay_t<decltype(map)>;
             auto go = [&](auto kernel, size_t bytes) {
                 hipLaunchKernelGGL(kernel, dim3(kHistGrid), dim3(kBlock), bytes, backend().stream, m.F,
@@ -912,7 +1001,7 @@ static void launch_deriv(Metric &m, const Stages &S, const R *pratio, bool clear
     {
         LaunchScope scope("mattes_deriv");
         const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
-        with_map(S, [&](const auto &map) {
+        with_map(m, S, [&](const auto &map) {
             using Map = std::decay_t<decltype(map)>;
             if (lds)
                 hipLaunchKernelGGL((mattes_deriv<R, true, Map>), dim3(kGrid), dim3(kBlock), shm,
@@ -1049,7 +1138,7 @@ static void launch_ms(Metric &m, const Stages &S) {
     {
         LaunchScope scope(MOM ? "ms_moments" : "ms_value");
         const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
-        with_map(S, [&](const auto &map) {
+        with_map(m, S, [&](const auto &map) {
             using Map = std::decay_t<decltype(map)>;
             hipLaunchKernelGGL((ms_pass<R, MOM, Map>), dim3(kGrid), dim3(kBlock), 0, backend().stream,
                                m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fval.ptr, m.partials.ptr);
@@ -1109,6 +1198,17 @@ int sitkhip_metric_create_drawn(int kind, sitkhip_image fixed, size_t n_draws, u
         if (!f || !mv || !out) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null image or out-pointer");
         *out = reinterpret_cast<sitkhip_metric>(
             metric_create_drawn(kind, *f, n_draws, seed, *mv, precision));
+    });
+}
+
+int sitkhip_metric_create_host(int kind, size_t n, const double *fixed_values,
+                               const sitkhip_fixed_points *points, const uint8_t *fixed_mask,
+                               const sitkhip_moving_geometry *moving, const double *moving_values,
+                               int precision, sitkhip_metric *out) {
+    return run_guarded([&] {
+        if (!points || !moving || !out) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null argument");
+        *out = reinterpret_cast<sitkhip_metric>(metric_create_host(
+            kind, n, fixed_values, *points, fixed_mask, *moving, moving_values, precision));
     });
 }
 
@@ -1257,6 +1357,7 @@ extern "C" int sitkhip_draw_samples(size_t voxels, int strategy, double percenta
             return;
         }
         if (voxels == 0) sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "no voxels to draw from");
+        sitkhip::check_sampling_fraction(strategy, percentage, voxels);
         if (strategy == 1) {
             size_t stride = static_cast<size_t>(std::ceil(1.0 / percentage));
             if (stride < 1) stride = 1;

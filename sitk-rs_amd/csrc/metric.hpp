@@ -22,17 +22,33 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
 //   ranges       : compute them at all (else on the first metric_value_ranges);
 //   adopt_fixed / adopt_moving : the caller hands over that image's buffer (it
 //                  must be the `fixed` / `moving` argument) instead of a copy.
+//   f_i2p / m_p2i : the fixed grid's index->physical matrix / the moving
+//                  physical->index matrix as given (row-major), instead of the
+//                  ones derived from the images' spacing and direction;
+//   points       : explicit sample points (host, n x 3): the fixed image is then
+//                  the n gathered sample values as an n x 1 x 1 grid.
 struct CreateOptions {
     bool defer_ranges = false;
     bool ranges = true;
     Image *adopt_fixed = nullptr;
     Image *adopt_moving = nullptr;
+    const double *f_i2p = nullptr;
+    const double *m_p2i = nullptr;
+    const double *points = nullptr;
 };
 Metric *metric_create_drawn(int kind, const Image &fixed, size_t n_draws, uint64_t seed,
                             const Image &moving, int precision,
                             const CreateOptions &opt = CreateOptions{});
 Metric *metric_create_with(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
                            const Image &moving, int precision, const CreateOptions &opt);
+// ResidentMetric::new (mean_squares.rs:201-256): both volumes from host values
+// (narrowed to float32 as an upload is), the sample set as FixedPoints.
+Metric *metric_create_host(int kind, size_t n, const double *fixed_values,
+                           const sitkhip_fixed_points &points, const uint8_t *fixed_mask,
+                           const sitkhip_moving_geometry &moving, const double *moving_values,
+                           int precision);
+// Whether the metric's samples are explicit points (no grid index).
+bool metric_has_explicit_points(const Metric &m);
 void metric_finish_ranges(Metric &m);
 void metric_destroy(Metric *m);
 size_t metric_sample_count(const Metric &m);

@@ -423,10 +423,15 @@ inline IndexMap make_index_map(const FixedGrid &F, const MovingVol &M, const dou
 // no geometry in registers) -- its moments B'[d][c] become the physical ones
 // on the host, B[d][e] = origin_e A_d + sum_c I2P[e][c] B'[d][c]
 // (moments_to_physical).
-template <typename R>
+// PTS: the metric's samples are explicit points (the physical point is
+// itself the moment point: the grid is the identity); a separate
+// instantiation, so the grid kernels carry no such branch.
+template <typename R, bool PTS = false>
 __device__ __forceinline__ void moment_point(const FixedGrid &F, const ChainConst *__restrict__ C,
                                              uint32_t flat, double x[3]) {
-    if constexpr (sizeof(R) == 4) {
+    if constexpr (PTS) {
+        sample_point<true>(F, *C, flat, x);
+    } else if constexpr (sizeof(R) == 4) {
         uint32_t i0, i1, i2;
         split_index(F, flat, i0, i1, i2);
         x[0] = static_cast<double>(i0);
@@ -454,6 +459,21 @@ inline void moments_to_physical(const FixedGrid &F, const double *a, double *b) 
 template <typename Map>
 struct MapTraits {
     static constexpr bool affine = true;
+    static constexpr bool points = false;
+};
+
+// The stages of a metric over explicit points (and their LDS view): the
+// samples have no grid index to compose the map with, so every sample takes
+// the exact chain.
+template <>
+struct MapTraits<PointStages> {
+    static constexpr bool affine = false;
+    static constexpr bool points = true;
+};
+template <>
+struct MapTraits<PointStageView> {
+    static constexpr bool affine = false;
+    static constexpr bool points = true;
 };
 
 // R = float: the composed index map decides, and the reference's FP64 chain
@@ -493,7 +513,7 @@ __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol
             const ChainConst *Cv = C;
             asm volatile("" : "+v"(Cv));
             double x[3], p[3];
-            fixed_point(F, *Cv, b.flat[u], x);
+            sample_point<MapTraits<Map>::points>(F, *Cv, b.flat[u], x);
             map_point(S, x, p);
             L.ok[u] = b.live[u] && locate(M, *Cv, p, cell[u]);
         }
@@ -545,7 +565,7 @@ __device__ __forceinline__ void ms_range(const FixedGrid &F, const MovingVol &M,
             acc[0] = fma(diff, diff, acc[0]);
             if (MOMENTS) {
                 double x[3];
-                moment_point<R>(F, C, b.flat[u], x);
+                moment_point<R, MapTraits<Map>::points>(F, C, b.flat[u], x);
                 physical_gradient<R>(M, gi, g);
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
@@ -715,7 +735,7 @@ __device__ __forceinline__ void deriv_range(const FixedGrid &F, const MovingVol 
             for (int k = 0; k < 4; ++k) w += row[k] * dv[k];
             const double wd = static_cast<double>(w);
             double x[3];
-            moment_point<R>(F, C, b.flat[u], x);
+            moment_point<R, MapTraits<Map>::points>(F, C, b.flat[u], x);
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const double wg = wd * static_cast<double>(g[d]);
@@ -788,7 +808,10 @@ __device__ __forceinline__ void deriv_records(const FixedGrid &F, const MovingVo
             for (int k = 0; k < 4; ++k) w += row[k] * dv[k];
             const double wd = static_cast<double>(w);
             double x[3];
-            moment_point<R>(F, C, q.flat[u], x);
+            if (F.pts)
+                moment_point<R, true>(F, C, q.flat[u], x);
+            else
+                moment_point<R>(F, C, q.flat[u], x);
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const double wg = wd * static_cast<double>(g[d]);
@@ -1058,6 +1081,7 @@ struct Metric {
     DeviceBuffer<float> moving;
     DeviceBuffer<uint8_t> mmask;
     DeviceBuffer<uint32_t> idx;
+    DeviceBuffer<double> pts;     // explicit sample points (n x 3), else empty
     DeviceBuffer<float> fval;     // per-sample fixed values (MS, and the range/bins source)
     DeviceBuffer<uint8_t> fbin;   // per-sample fixed Parzen bins (Mattes)
     DeviceBuffer<double> partials;

@@ -151,7 +151,7 @@ static void index_to_physical(const sitkhip_geometry &g, double out[9]) {
 }
 
 static GridMap grid_map(const sitkhip_geometry &in, const sitkhip_geometry &out) {
-    GridMap G;
+    GridMap G{};
     for (int d = 0; d < 3; ++d) {
         if (out.size[d] >= (1u << 31) || in.size[d] >= (1u << 31))
             fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "axis longer than 2^31 voxels");

@@ -352,6 +352,8 @@ static void check_config(const sitkhip_registration_config &cfg, int kind) {
         fail(SITKHIP_ERR_INVALID_ARGUMENT, "unsupported transform kind");
     if (cfg.optimizer != SITKHIP_OPTIMIZER_REGULAR_STEP && cfg.optimizer != SITKHIP_OPTIMIZER_LINE_SEARCH)
         fail(SITKHIP_ERR_INVALID_ARGUMENT, "unsupported optimizer");
+    if (cfg.sampling < 0 || cfg.sampling > 2) fail(SITKHIP_ERR_INVALID_ARGUMENT, "unknown sampling strategy");
+    check_sampling_fraction(cfg.sampling, cfg.sampling_percentage, 1);
 }
 
 static Transform initial_transform(const sitkhip_transform &tr) {
@@ -420,6 +422,7 @@ static LevelInputs prepare_level_begin(const Image &fixed, const Image &moving,
     opt.ranges = cfg.metric == SITKHIP_METRIC_MATTES;
     opt.adopt_moving = sm.owned ? sm.img : nullptr;
     opt.adopt_fixed = fl.owned || (fl.img == sf.img && sf.owned) ? fl.img : nullptr;
+    check_sampling_fraction(cfg.sampling, cfg.sampling_percentage, nvox);
     if (cfg.sampling == 2) {
         // random draws (metric.rs:419-437), made and sorted on the device
         const size_t cnt = static_cast<size_t>(static_cast<double>(nvox) * cfg.sampling_percentage);

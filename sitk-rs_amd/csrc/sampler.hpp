@@ -58,6 +58,11 @@ struct FixedGrid {
     FastDiv div0, div1;  // flat -> (i0, i1, i2)
     double origin[3];
     double i2p[9];
+    // Explicit points (FixedPoints::Explicit, resident.rs:464-484): sample s
+    // sits at pts[3 s .. 3 s + 2] and has no grid voxel; nullptr for a grid.
+    // The "grid" is then n x 1 x 1 with origin 0 and an identity i2p, so that
+    // index-space moments are the physical ones.
+    const double *pts;
 };
 
 struct MovingVol {
@@ -118,6 +123,20 @@ __device__ __forceinline__ void fixed_point(const FixedGrid &F, const ChainConst
     }
 }
 
+// The physical point of sample `flat`: PTS (explicit points), the stored
+// point; else the grid voxel's point as above.
+template <bool PTS>
+__device__ __forceinline__ void sample_point(const FixedGrid &F, const ChainConst &C, uint32_t flat,
+                                             double x[3]) {
+    if constexpr (PTS) {
+        x[0] = F.pts[3 * static_cast<size_t>(flat) + 0];
+        x[1] = F.pts[3 * static_cast<size_t>(flat) + 1];
+        x[2] = F.pts[3 * static_cast<size_t>(flat) + 2];
+    } else {
+        fixed_point(F, C, flat, x);
+    }
+}
+
 // replay_stages: p <- mat_vec(M, p) + t per stage, mat_vec seeded at 0.0.
 __device__ __forceinline__ void map_point(const Stages &S, const double x[3], double p[3]) {
     double q0 = x[0], q1 = x[1], q2 = x[2];
@@ -174,8 +193,22 @@ __device__ __forceinline__ void map_point(const StageView &S, const double x[3],
     p[0] = q0; p[1] = q1; p[2] = q2;
 }
 
+// The stages of a metric whose samples are explicit points, and their LDS
+// view: the same replay, a distinct type so that kernels are instantiated
+// separately for such metrics (every sample on the exact chain).
+struct PointStages : Stages {};
+struct PointStageView : StageView {};
+
 // Copies the stages into the workgroup's LDS (s_stage: kMaxStages * 12
 // doubles); all lanes must call it, and it ends with a barrier.
+__device__ __forceinline__ PointStageView stage_to_lds(const PointStages &S, double *s_stage) {
+    for (int i = threadIdx.x; i < S.n * 12; i += blockDim.x) s_stage[i] = S.s[i];
+    __syncthreads();
+    PointStageView v;
+    v.m = s_stage;
+    v.n = S.n;
+    return v;
+}
 __device__ __forceinline__ StageView stage_to_lds(const Stages &S, double *s_stage) {
     for (int i = threadIdx.x; i < S.n * 12; i += blockDim.x) s_stage[i] = S.s[i];
     __syncthreads();

@@ -41,6 +41,33 @@ class CudaError(RuntimeError):
         self.message = message
 
 
+POINTS_GRID, POINTS_EXPLICIT, POINTS_INDICES = 0, 1, 2
+
+
+class FixedPointsSpec(C.Structure):
+    """sitkhip_fixed_points (FixedPoints, crates/sitk/src/cuda/ops/resident.rs:464-484)."""
+    _fields_ = [
+        ("mode", C.c_int),
+        ("points", C.POINTER(C.c_double)),
+        ("indices", C.POINTER(C.c_int64)),
+        ("size", C.c_size_t * 3),
+        ("origin", C.c_double * 3),
+        ("idx_to_phys", C.c_double * 9),
+    ]
+
+
+class MovingGeometrySpec(C.Structure):
+    """sitkhip_moving_geometry (MovingGeometry, resident.rs:424-437)."""
+    _fields_ = [
+        ("len", C.c_size_t),
+        ("size", C.c_size_t * 3),
+        ("strides", C.c_size_t * 3),
+        ("origin", C.c_double * 3),
+        ("phys_to_index", C.c_double * 9),
+        ("mask", C.POINTER(C.c_uint8)),
+    ]
+
+
 class Geometry(C.Structure):
     _fields_ = [
         ("size", C.c_size_t * 3),
@@ -155,6 +182,9 @@ def lib():
         L = C.CDLL(LIB_PATH)
         vp, dp, szp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_size_t)
         gp = C.POINTER(Geometry)
+        L.sitkhip_metric_create_host.argtypes = [C.c_int, C.c_size_t, dp, C.POINTER(FixedPointsSpec),
+                                                 C.POINTER(C.c_uint8), C.POINTER(MovingGeometrySpec),
+                                                 dp, C.c_int, C.POINTER(vp)]
         L.sitkhip_status_name.restype = C.c_char_p
         L.sitkhip_status_name.argtypes = [C.c_int]
         L.sitkhip_last_error.restype = C.c_char_p
@@ -456,6 +486,69 @@ class _Resident:
                                            moving_mask._h if moving_mask else None, precision,
                                            C.byref(h)))
         self._h = h
+
+    @classmethod
+    def from_host(cls, fixed_values, moving_values, moving_size, moving_origin, moving_p2i,
+                  points=None, indices=None, grid=None, fixed_mask=None, moving_mask=None,
+                  precision: int = PRECISION_F32, bins=None):
+        """ResidentMetric::new (crates/sitk/src/cuda/ops/mean_squares.rs:201-256):
+        both volumes from host values. The sample set is `points` (explicit,
+        n x 3 physical; fixed_values then holds one value per point), or the
+        fixed grid `grid` = (size, origin, idx_to_phys 3x3) with every voxel
+        or the voxels `indices` (fixed_values then holds the whole grid)."""
+        self = cls.__new__(cls)
+        self._h = None
+        fv = np.ascontiguousarray(fixed_values, dtype=np.float64).reshape(-1)
+        mv = np.ascontiguousarray(moving_values, dtype=np.float64).reshape(-1)
+        spec = FixedPointsSpec()
+        keep = [fv, mv]
+        if points is not None:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            keep.append(pts)
+            spec.mode = POINTS_EXPLICIT
+            spec.points = _dp(pts)
+            n = pts.shape[0]
+        else:
+            size, origin, i2p = grid
+            spec.size[:] = [int(v) for v in size]
+            spec.origin[:] = [float(v) for v in origin]
+            spec.idx_to_phys[:] = [float(v) for v in np.asarray(i2p, dtype=np.float64).reshape(9)]
+            if indices is not None:
+                idx = np.ascontiguousarray(indices, dtype=np.int64)
+                keep.append(idx)
+                spec.mode = POINTS_INDICES
+                spec.indices = idx.ctypes.data_as(C.POINTER(C.c_int64))
+                n = idx.size
+            else:
+                spec.mode = POINTS_GRID
+                n = int(np.prod(size))
+        mg = MovingGeometrySpec()
+        ms = [int(v) for v in moving_size]
+        mg.len = ms[0] * ms[1] * ms[2]
+        mg.size[:] = ms
+        mg.strides[:] = [1, ms[0], ms[0] * ms[1]]
+        mg.origin[:] = [float(v) for v in moving_origin]
+        mg.phys_to_index[:] = [float(v) for v in np.asarray(moving_p2i, dtype=np.float64).reshape(9)]
+        if moving_mask is not None:
+            mm = np.ascontiguousarray(moving_mask, dtype=np.uint8).reshape(-1)
+            keep.append(mm)
+            mg.mask = mm.ctypes.data_as(C.POINTER(C.c_uint8))
+        fmp = None
+        if fixed_mask is not None:
+            fm = np.ascontiguousarray(fixed_mask, dtype=np.uint8).reshape(-1)
+            keep.append(fm)
+            fmp = fm.ctypes.data_as(C.POINTER(C.c_uint8))
+        kind = MATTES if cls is ResidentMattes else MEAN_SQUARES
+        h = C.c_void_p()
+        _check(lib().sitkhip_metric_create_host(kind, n, _dp(fv), C.byref(spec), fmp, C.byref(mg),
+                                                _dp(mv), precision, C.byref(h)))
+        self._h = h
+        if cls is ResidentMattes:
+            self.bins = None
+            if bins is not None:
+                fr, mr = self.value_ranges()
+                self.set_bins(mattes_bins(fr, mr, bins))
+        return self
 
     def sample_count(self) -> int:
         v = C.c_size_t()

@@ -163,6 +163,19 @@ def tre(T_est: AffineMap, T_true_inv: AffineMap, vol: Volume, stride: int = 8) -
 
 
 # --------------------------------------------------------------- configurations
+def config_cpu_pair(seed: int = 1):
+    """The CPU-runnable rigid pair: 64^3 float32, spacing 1 mm, identity
+    direction, 40 blobs (amplitude U[50,200], sigma U[2,8] mm) plus N(0,5)
+    noise; moving = fixed resampled through a seeded Euler3D (rotations
+    +-5 deg, translations +-3 mm about the volume centre), linear
+    interpolation. Returns (fixed, moving, T_gt)."""
+    fixed = blob_volume((64, 64, 64), (1.0, 1.0, 1.0), seed, nblobs=40)
+    rng = np.random.default_rng(seed + 500)
+    _, T_gt = random_euler(rng, 5.0, 3.0, fixed.center())
+    moving = resample_linear(fixed, T_gt)
+    return fixed, moving, T_gt
+
+
 def config_large_pair(seed: int = 2):
     """256x256x160 float32, spacing 0.9x0.9x1.5 mm, 40 blobs, +-20% bias field;
     moving = fixed resampled through a seeded affine (rotations +-10 deg, scale

@@ -53,3 +53,18 @@ def blob_volume(shape, seed, nblobs=12, spacing=(1.0, 1.0, 1.0)):
     v += 0.3 * x + 0.2 * y
     v += rng.normal(0, 2.0, shape)
     return v.astype(np.float32)
+
+
+def record_error(config: str, **fields) -> None:
+    """Appends one JSON line {config, **fields} to the file named by the
+    SITKHIP_ERROR_LOG environment variable (no-op when it is unset): the
+    measured errors against the oracle, per configuration."""
+    import json
+    import os
+
+    path = os.environ.get("SITKHIP_ERROR_LOG")
+    if not path:
+        return
+    with open(path, "a") as fh:
+        fh.write(json.dumps({"config": config, **{k: (float(v) if isinstance(v, (np.floating, float))
+                                                       else v) for k, v in fields.items()}}) + "\n")

@@ -112,3 +112,55 @@ def test_draw_samples_is_the_references():
         got = S.draw_samples(10007, strategy, pct, seed)
         want = O.draw_samples(10007, strategy, pct, seed)
         assert np.array_equal(got, want)
+
+
+def test_draw_samples_refuses_fractions_it_cannot_count():
+    """ceil(1/p) and voxels*p are cast to sizes: NaN, infinities, zero and
+    negative fractions are refused by name instead of being undefined casts.
+    A fraction above 1 stays allowed, as in the reference (stride 1 / more draws
+    than voxels)."""
+    for strategy in (1, 2):
+        for bad in (0.0, -0.2, float("nan"), float("inf"), 1e-320):
+            with pytest.raises(S.CudaError) as e:
+                S.draw_samples(1000, strategy, bad, 1)
+            assert e.value.status == "InvalidArgument"
+    assert S.draw_samples(10, 1, 1.5, 0).tolist() == list(range(10))
+    assert S.draw_samples(10, 2, 1.5, 3).size == 15
+
+
+def test_host_constructor_refusals_before_any_device_work():
+    """sitkhip_metric_create_host validates on the host, in the order of
+    Resident::build (crates/sitk/src/cuda/ops/resident.rs:530-660), before it
+    touches a device: an empty sample set and a moving length that does not
+    match its size are DegenerateInput, a non-dense moving layout is refused,
+    and a fixed mask with explicit points is MaskedExplicitPoints
+    (src/cuda/error.rs)."""
+    m = np.ones((4, 4, 4))
+    p2i = np.eye(3)
+    pts = np.zeros((5, 3))
+    with pytest.raises(S.CudaError) as e:
+        S.ResidentMetric.from_host(np.zeros(0), m, (4, 4, 4), (0, 0, 0), p2i, points=np.zeros((0, 3)))
+    assert e.value.status == "DegenerateInput"
+    with pytest.raises(S.CudaError) as e:
+        S.ResidentMetric.from_host(np.zeros(5), m, (4, 4, 4), (0, 0, 0), p2i, points=pts,
+                                   fixed_mask=np.ones(5, np.uint8))
+    assert e.value.status == "MaskedExplicitPoints"
+    spec = S.FixedPointsSpec()
+    spec.mode = S.POINTS_EXPLICIT
+    p = np.zeros(15)
+    spec.points = p.ctypes.data_as(C.POINTER(C.c_double))
+    mg = S.MovingGeometrySpec()
+    mg.len = 63
+    mg.size[:] = [4, 4, 4]
+    mg.strides[:] = [1, 4, 16]
+    fv, mv = np.zeros(5), np.zeros(64)
+    h = C.c_void_p()
+    dp = C.POINTER(C.c_double)
+    rc = S.lib().sitkhip_metric_create_host(0, 5, fv.ctypes.data_as(dp), C.byref(spec), None,
+                                            C.byref(mg), mv.ctypes.data_as(dp), 0, C.byref(h))
+    assert S.lib().sitkhip_status_name(rc).decode() == "DegenerateInput"
+    mg.len = 64
+    mg.strides[:] = [1, 4, 17]
+    rc = S.lib().sitkhip_metric_create_host(0, 5, fv.ctypes.data_as(dp), C.byref(spec), None,
+                                            C.byref(mg), mv.ctypes.data_as(dp), 0, C.byref(h))
+    assert S.lib().sitkhip_status_name(rc).decode() == "UnsupportedGeometry"

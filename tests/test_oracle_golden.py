@@ -551,3 +551,66 @@ def test_composed_sparse_derivative_matches_finite_difference():
         c[k] += h
         fd = (vp - vm) / (2 * h)
         assert abs(fd - d0[k]) < 5e-3 * max(1.0, abs(d0[k])), (k, fd, d0[k])
+
+
+# ---------------------------------------------------------- explicit points
+def _grid_points(shape, spacing, origin, direction):
+    """Every voxel's physical point in dim-0-fastest order, by the origin-first
+    fold of VirtualGrid::write_point (scales.rs:259-271): ((o + a i) + b j) + c k,
+    one rounding per step, as the oracle evaluates it."""
+    i2p = O.index_to_physical(np.asarray(direction, float).reshape(9), spacing).reshape(3, 3)
+    nz, ny, nx = shape
+    k, j, i = np.meshgrid(np.arange(nz, dtype=float), np.arange(ny, dtype=float),
+                          np.arange(nx, dtype=float), indexing="ij")
+    i, j, k = i.reshape(-1), j.reshape(-1), k.reshape(-1)
+    pts = np.empty((i.size, 3))
+    for r in range(3):
+        pts[:, r] = ((origin[r] + i2p[r, 0] * i) + i2p[r, 1] * j) + i2p[r, 2] * k
+    return pts
+
+
+def test_explicit_points_on_the_grid_are_the_grid():
+    """FixedPoints::Explicit (cuda/ops/resident.rs:464-484) carrying exactly the
+    grid's own points and values evaluates the same samples as the grid: mean
+    squares and Mattes value, derivative and valid count bit for bit, under an
+    oblique direction and a multi-stage-free Euler map."""
+    rng = np.random.default_rng(4)
+    f = (rng.random((9, 10, 11)) * 100).astype(np.float32)
+    m = (f * 0.7 + rng.random(f.shape).astype(np.float32) * 20).astype(np.float32)
+    sp, org = [0.9, 1.1, 1.3], [-2.0, 1.5, 0.5]
+    c, s = math.cos(0.2), math.sin(0.2)
+    fdir = [c, -s, 0.0, s, c, 0.0, 0.0, 0.0, 1.0]
+    grid = O.MetricProblem(f, sp, org, m, sp, org, fdir=fdir, mdir=fdir)
+    pts = _grid_points(f.shape, sp, org, fdir)
+    expl = O.MetricProblem(f.reshape(-1), sp, org, m, sp, org, mdir=fdir, points=pts)
+    t = O.make_transform(O.EULER3D, [0.03, -0.02, 0.05, 0.4, -0.3, 0.2], center=[3.0, 5.0, 5.0])
+    assert grid.ms_evaluate(t)[0] == expl.ms_evaluate(t)[0]
+    assert np.array_equal(grid.ms_evaluate(t)[1], expl.ms_evaluate(t)[1])
+    g1, g2 = grid.mattes_geometry(20), expl.mattes_geometry(20)
+    v1, d1, n1 = grid.mattes_evaluate(g1, t)
+    v2, d2, n2 = expl.mattes_evaluate(g2, t)
+    assert v1 == v2 and n1 == n2 and np.array_equal(d1, d2)
+
+
+def test_explicit_point_derivative_matches_finite_difference():
+    """Off-grid explicit points: the mean-squares derivative is the derivative
+    of its value."""
+    rng = np.random.default_rng(6)
+    m = np.zeros((14, 15, 16), np.float32)
+    z, y, x = np.meshgrid(np.arange(14), np.arange(15), np.arange(16), indexing="ij")
+    m[:] = (50 * np.exp(-((x - 8) ** 2 + (y - 7) ** 2 + (z - 6) ** 2) / 18.0)
+            + 0.5 * x).astype(np.float32)
+    pts = rng.uniform(3.0, 10.0, (400, 3))
+    vals = rng.uniform(0.0, 40.0, 400).astype(np.float32)
+    pb = O.MetricProblem(vals, [1, 1, 1], [0, 0, 0], m, [1, 1, 1], [0, 0, 0], points=pts)
+    p0 = [0.02, -0.01, 0.03, 0.3, -0.2, 0.1]
+    _, an, n = pb.ms_evaluate(O.make_transform(O.EULER3D, p0, center=[7, 7, 7]))
+    assert n == 400
+    for k in range(6):
+        h = 1e-6
+        pp, pm = list(p0), list(p0)
+        pp[k] += h
+        pm[k] -= h
+        fd = (pb.ms_value(O.make_transform(O.EULER3D, pp, center=[7, 7, 7]))[0]
+              - pb.ms_value(O.make_transform(O.EULER3D, pm, center=[7, 7, 7]))[0]) / (2 * h)
+        assert abs(fd - an[k]) < 1e-4 * max(1.0, abs(an[k])), (k, fd, an[k])
