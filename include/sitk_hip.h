@@ -53,7 +53,8 @@ typedef enum {
     SITKHIP_ERR_TOO_FEW_BINS = 12,          /* RegistrationError::TooFewHistogramBins */
     SITKHIP_ERR_CONSTANT_INTENSITY = 13,    /* RegistrationError::ConstantIntensity */
     SITKHIP_ERR_TOO_MANY_BINS = 14,         /* bins above SITKHIP_MAX_BINS */
-    SITKHIP_ERR_NOT_BUILT = 15              /* library has no code object for this device */
+    SITKHIP_ERR_NOT_BUILT = 15,             /* library has no code object for this device */
+    SITKHIP_ERR_MASK_SIZE_MISMATCH = 16     /* RegistrationError::MaskSizeMismatch */
 } sitkhip_status;
 
 const char *sitkhip_status_name(int status);
@@ -372,6 +373,15 @@ typedef struct {
     int levels;
     size_t shrink[SITKHIP_MAX_LEVELS][3];
     double sigma[SITKHIP_MAX_LEVELS]; /* physical smoothing sigma per level, all axes */
+    /* set_metric_fixed_mask / set_metric_moving_mask (method.rs:1555-1590),
+     * NULL for none. The fixed mask lies on the fixed image's grid and is
+     * placed on every level's grid by the nearest-neighbour resample with
+     * default 0 (level_mask_on_device, method.rs:3068-3110); the moving mask
+     * lies on the moving image's grid and is used as is at every level (the
+     * moving volume is never shrunk). A mask whose size differs from its
+     * image's is SITKHIP_ERR_MASK_SIZE_MISMATCH (method.rs:2070-2083). */
+    sitkhip_mask fixed_mask;
+    sitkhip_mask moving_mask;
 } sitkhip_registration_config;
 
 typedef struct {

@@ -133,6 +133,7 @@ def lib():
         L.orc_ms_value.argtypes = [pp, tp, szp]
         L.orc_ms_value.restype = C.c_double
         L.orc_resample_linear.argtypes = [fp, szp, dp, dp, dp, szp, dp, dp, dp, tp, C.c_double, fp]
+        L.orc_resample_nearest.argtypes = L.orc_resample_linear.argtypes
         L.orc_shrunk_geometry.argtypes = [szp, dp, dp, dp, szp, szp, dp, dp]
         L.orc_fixture_volume.argtypes = [C.c_size_t, dp, fp]
         L.orc_fixture_inverted.argtypes = [C.c_size_t, dp, fp]
@@ -452,8 +453,16 @@ def compensated_sum(v) -> float:
     return lib().orc_compensated_sum(_dp(a), a.size)
 
 
+def resample_nearest(vol, in_spacing, in_origin, out_size, out_spacing, out_origin, t,
+                     default=0.0, in_dir=None, out_dir=None):
+    """As resample_linear with the nearest-neighbour interpolator (round half
+    up, transform/interpolator.rs:37-54)."""
+    return resample_linear(vol, in_spacing, in_origin, out_size, out_spacing, out_origin, t,
+                           default, in_dir, out_dir, _fn="orc_resample_nearest")
+
+
 def resample_linear(vol, in_spacing, in_origin, out_size, out_spacing, out_origin, t,
-                    default=0.0, in_dir=None, out_dir=None):
+                    default=0.0, in_dir=None, out_dir=None, _fn="orc_resample_linear"):
     """vol: (nz, ny, nx) float32; out_size (nx, ny, nz). Returns (nz', ny', nx')."""
     v = np.ascontiguousarray(vol, dtype=np.float32)
     nz, ny, nx = v.shape
@@ -461,7 +470,7 @@ def resample_linear(vol, in_spacing, in_origin, out_size, out_spacing, out_origi
     idir = eye if in_dir is None else (C.c_double * 9)(*np.asarray(in_dir, float).reshape(9))
     odir = eye if out_dir is None else (C.c_double * 9)(*np.asarray(out_dir, float).reshape(9))
     out = np.empty(int(np.prod(out_size)), dtype=np.float32)
-    lib().orc_resample_linear(_fp(v), _szp((nx, ny, nz)), _d3(in_spacing), _d3(in_origin), idir,
+    getattr(lib(), _fn)(_fp(v), _szp((nx, ny, nz)), _d3(in_spacing), _d3(in_origin), idir,
                               _szp(out_size), _d3(out_spacing), _d3(out_origin), odir,
                               C.byref(t), default, _fp(out))
     return out.reshape(out_size[2], out_size[1], out_size[0])

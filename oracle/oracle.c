@@ -1267,11 +1267,53 @@ void orc_fixture_commensurate(size_t n, float *out) {
  * first) -> transform_point -> input continuous index (mat_vec seeded 0.0)
  * -> linear_at, default outside. Geometry arrays: size[3], spacing[3],
  * origin[3], direction[9]. */
+/* interpolator.rs:37-54: nearest_at -- inside test first, then round half
+ * up (coord.rs:189-191, floor(c + 0.5)) clamped into the buffer. */
+static int nearest_at(const orc_problem *pb, const double c[3], double *v) {
+    if (!is_inside(c, pb->msize)) return 0;
+    size_t off = 0, stride = 1;
+    for (int d = 0; d < 3; ++d) {
+        int64_t i = (int64_t)floor(c[d] + 0.5);
+        if (i < 0) i = 0;
+        if (i > (int64_t)pb->msize[d] - 1) i = (int64_t)pb->msize[d] - 1;
+        off += (size_t)i * stride;
+        stride *= pb->msize[d];
+    }
+    *v = (double)pb->mvals[off];
+    return 1;
+}
+
+static void resample_with(const float *in, const size_t in_size[3], const double in_spacing[3],
+                          const double in_origin[3], const double in_dir[9],
+                          const size_t out_size[3], const double out_spacing[3],
+                          const double out_origin[3], const double out_dir[9],
+                          const orc_transform *t, double dflt, int nearest, float *out);
+
+/* ResampleImageFilter::execute (transform/resample.rs:226-290) with the
+ * nearest-neighbour interpolator. */
+void orc_resample_nearest(const float *in, const size_t in_size[3], const double in_spacing[3],
+                          const double in_origin[3], const double in_dir[9],
+                          const size_t out_size[3], const double out_spacing[3],
+                          const double out_origin[3], const double out_dir[9],
+                          const orc_transform *t, double dflt, float *out) {
+    resample_with(in, in_size, in_spacing, in_origin, in_dir, out_size, out_spacing, out_origin,
+                  out_dir, t, dflt, 1, out);
+}
+
 void orc_resample_linear(const float *in, const size_t in_size[3], const double in_spacing[3],
                          const double in_origin[3], const double in_dir[9],
                          const size_t out_size[3], const double out_spacing[3],
                          const double out_origin[3], const double out_dir[9],
                          const orc_transform *t, double dflt, float *out) {
+    resample_with(in, in_size, in_spacing, in_origin, in_dir, out_size, out_spacing, out_origin,
+                  out_dir, t, dflt, 0, out);
+}
+
+static void resample_with(const float *in, const size_t in_size[3], const double in_spacing[3],
+                          const double in_origin[3], const double in_dir[9],
+                          const size_t out_size[3], const double out_spacing[3],
+                          const double out_origin[3], const double out_dir[9],
+                          const orc_transform *t, double dflt, int nearest, float *out) {
     double o2p[9], i2p[9], p2i[9];
     orc_index_to_physical_matrix(out_dir, out_spacing, o2p);
     orc_index_to_physical_matrix(in_dir, in_spacing, i2p);
@@ -1293,7 +1335,8 @@ void orc_resample_linear(const float *in, const size_t in_size[3], const double 
         for (int d = 0; d < 3; ++d) diff[d] = mapped[d] - in_origin[d];
         mat_vec3(p2i, diff, c);
         double v;
-        out[o] = (float)(linear_at(&pb, c, &v) ? v : dflt);
+        const int ok = nearest ? nearest_at(&pb, c, &v) : linear_at(&pb, c, &v);
+        out[o] = (float)(ok ? v : dflt);
         for (int d = 0; d < 3; ++d) {
             if (++idx[d] < out_size[d]) break;
             idx[d] = 0;

@@ -203,6 +203,11 @@ void orc_resample_linear(const float *in, const size_t in_size[3], const double 
                          const size_t out_size[3], const double out_spacing[3],
                          const double out_origin[3], const double out_dir[9],
                          const orc_transform *t, double dflt, float *out);
+void orc_resample_nearest(const float *in, const size_t in_size[3], const double in_spacing[3],
+                         const double in_origin[3], const double in_dir[9],
+                         const size_t out_size[3], const double out_spacing[3],
+                         const double out_origin[3], const double out_dir[9],
+                         const orc_transform *t, double dflt, float *out);
 void orc_shrunk_geometry(const size_t size[3], const double spacing[3], const double origin[3],
                          const double dir[9], const size_t f[3], size_t out_size[3],
                          double out_spacing[3], double out_origin[3]);

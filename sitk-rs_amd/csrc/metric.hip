@@ -676,14 +676,40 @@ Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size
 
 Metric *metric_create_drawn(int kind, const Image &fixed, size_t n_draws, uint64_t seed,
                             const Image &moving, int precision, const CreateOptions &opt) {
-    return metric_create_impl(kind, fixed, nullptr, 0, nullptr, moving, nullptr, precision, true,
-                              n_draws, seed, opt);
+    return metric_create_impl(kind, fixed, nullptr, 0, opt.fixed_mask, moving, opt.moving_mask,
+                              precision, true, n_draws, seed, opt);
 }
 
 Metric *metric_create_with(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
                            const Image &moving, int precision, const CreateOptions &opt) {
-    return metric_create_impl(kind, fixed, indices, n_indices, nullptr, moving, nullptr, precision,
-                              false, 0, 0, opt);
+    return metric_create_impl(kind, fixed, indices, n_indices, opt.fixed_mask, moving,
+                              opt.moving_mask, precision, false, 0, 0, opt);
+}
+
+std::vector<int64_t> host_draws(size_t voxels, int strategy, double percentage, uint64_t seed) {
+    std::vector<int64_t> out;
+    if (voxels == 0) fail(SITKHIP_ERR_INVALID_ARGUMENT, "no voxels to draw from");
+    check_sampling_fraction(strategy, percentage, voxels);
+    if (strategy == 1) {
+        size_t stride = static_cast<size_t>(std::ceil(1.0 / percentage));
+        if (stride < 1) stride = 1;
+        out.reserve(voxels / stride + 1);
+        for (size_t v = 0; v < voxels; v += stride) out.push_back(static_cast<int64_t>(v));
+        return out;
+    }
+    if (strategy != 2) fail(SITKHIP_ERR_INVALID_ARGUMENT, "unknown sampling strategy");
+    const size_t count = static_cast<size_t>(static_cast<double>(voxels) * percentage);
+    out.resize(count);
+    uint64_t state = seed;
+    for (size_t i = 0; i < count; ++i) {
+        state += 0x9E3779B97F4A7C15ull;
+        uint64_t z = state;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        out[i] = static_cast<int64_t>(z % static_cast<uint64_t>(voxels));
+    }
+    return out;
 }
 
 Metric *metric_create_host(int kind, size_t n, const double *fixed_values,
@@ -1356,30 +1382,21 @@ extern "C" int sitkhip_draw_samples(size_t voxels, int strategy, double percenta
             *n = SIZE_MAX;
             return;
         }
-        if (voxels == 0) sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "no voxels to draw from");
-        sitkhip::check_sampling_fraction(strategy, percentage, voxels);
-        if (strategy == 1) {
+        // the count alone costs no list (strategy 1: ceil(voxels / stride))
+        if (!out && strategy == 1 && voxels > 0) {
+            sitkhip::check_sampling_fraction(strategy, percentage, voxels);
             size_t stride = static_cast<size_t>(std::ceil(1.0 / percentage));
             if (stride < 1) stride = 1;
-            size_t c = 0;
-            for (size_t v = 0; v < voxels; v += stride) {
-                if (out) out[c] = static_cast<int64_t>(v);
-                ++c;
-            }
-            *n = c;
+            *n = (voxels + stride - 1) / stride;
             return;
         }
-        if (strategy != 2) sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "unknown sampling strategy");
-        const size_t count = static_cast<size_t>(static_cast<double>(voxels) * percentage);
-        uint64_t state = seed;
-        for (size_t i = 0; i < count; ++i) {
-            state += 0x9E3779B97F4A7C15ull;
-            uint64_t z = state;
-            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-            z ^= z >> 31;
-            if (out) out[i] = static_cast<int64_t>(z % static_cast<uint64_t>(voxels));
+        if (!out && strategy == 2 && voxels > 0) {
+            sitkhip::check_sampling_fraction(strategy, percentage, voxels);
+            *n = static_cast<size_t>(static_cast<double>(voxels) * percentage);
+            return;
         }
-        *n = count;
+        const std::vector<int64_t> d = sitkhip::host_draws(voxels, strategy, percentage, seed);
+        if (out) std::memcpy(out, d.data(), d.size() * sizeof(int64_t));
+        *n = d.size();
     });
 }

@@ -35,6 +35,8 @@ struct CreateOptions {
     const double *f_i2p = nullptr;
     const double *m_p2i = nullptr;
     const double *points = nullptr;
+    const Mask *fixed_mask = nullptr;   // metric_create_with: gates the sample list
+    const Mask *moving_mask = nullptr;  // metric_create_with / _drawn
 };
 Metric *metric_create_drawn(int kind, const Image &fixed, size_t n_draws, uint64_t seed,
                             const Image &moving, int precision,
@@ -74,6 +76,11 @@ void shrunk_geometry(const sitkhip_geometry &g, const size_t f[3], sitkhip_geome
 Image *resample_linear(const Image &in, const sitkhip_geometry &og, const double *stages,
                        int nstage, double dflt, bool synchronous = true);
 Image *recursive_gaussian(const Image &in, const double sigma[3], bool synchronous = true);
+// A mask placed on grid og by the nearest-neighbour resample, default 0.
+Mask *resample_nearest_mask(const Mask &in, const sitkhip_geometry &og, const double *stages,
+                            int nstage, bool synchronous = true);
+// draw_samples (metric.rs:419-437) on the host: strategy 1 regular, 2 random.
+std::vector<int64_t> host_draws(size_t voxels, int strategy, double percentage, uint64_t seed);
 std::vector<Image *> recursive_gaussian_batch(const std::vector<const Image *> &in,
                                               const std::vector<std::array<double, 3>> &sigma);
 

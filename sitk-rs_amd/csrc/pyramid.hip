@@ -207,7 +207,7 @@ Image *resample_linear(const Image &in, const sitkhip_geometry &og, const double
 }
 
 Mask *resample_nearest_mask(const Mask &in, const sitkhip_geometry &og, const double *stages,
-                            int nstage) {
+                            int nstage, bool synchronous) {
     validate_geometry(og, "resample output");
     const size_t n = geometry_len(og);
     if (n >= (1ull << 32)) fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "output above 2^32 voxels");
@@ -223,7 +223,7 @@ Mask *resample_nearest_mask(const Mask &in, const sitkhip_geometry &og, const do
                            static_cast<uint8_t>(0), static_cast<uint32_t>(n));
         check_launch("resample_nearest3");
     }
-    sync();
+    if (synchronous) sync();
     return out;
 }
 

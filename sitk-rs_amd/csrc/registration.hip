@@ -344,6 +344,19 @@ static OptResult line_search(const sitkhip_registration_config &cfg, const std::
 }
 
 // ------------------------------------------------------------------- driver
+// The masks' sizes against their images (method.rs:2070-2083).
+static void check_masks(const sitkhip_registration_config &cfg, const Image &fixed,
+                        const Image &moving) {
+    auto same_size = [](const sitkhip_geometry &a, const sitkhip_geometry &b) {
+        return a.size[0] == b.size[0] && a.size[1] == b.size[1] && a.size[2] == b.size[2];
+    };
+    if (cfg.fixed_mask && !same_size(reinterpret_cast<const Mask *>(cfg.fixed_mask)->geom, fixed.geom))
+        fail(SITKHIP_ERR_MASK_SIZE_MISMATCH, "the fixed mask's size differs from the fixed image's");
+    if (cfg.moving_mask &&
+        !same_size(reinterpret_cast<const Mask *>(cfg.moving_mask)->geom, moving.geom))
+        fail(SITKHIP_ERR_MASK_SIZE_MISMATCH, "the moving mask's size differs from the moving image's");
+}
+
 static void check_config(const sitkhip_registration_config &cfg, int kind) {
     if (cfg.levels < 1 || cfg.levels > SITKHIP_MAX_LEVELS)
         fail(SITKHIP_ERR_INVALID_ARGUMENT, "levels must be 1..8");
@@ -423,7 +436,23 @@ static LevelInputs prepare_level_begin(const Image &fixed, const Image &moving,
     opt.adopt_moving = sm.owned ? sm.img : nullptr;
     opt.adopt_fixed = fl.owned || (fl.img == sf.img && sf.owned) ? fl.img : nullptr;
     check_sampling_fraction(cfg.sampling, cfg.sampling_percentage, nvox);
-    if (cfg.sampling == 2) {
+    // Masks (method.rs:2544-2559, 2760-2785, 3068-3110): the fixed mask placed
+    // on the level grid by the nearest-neighbour resample (default 0), the
+    // moving mask as given (the moving volume keeps its grid).
+    std::unique_ptr<Mask> level_mask;
+    if (cfg.fixed_mask) {
+        level_mask.reset(resample_nearest_mask(*reinterpret_cast<const Mask *>(cfg.fixed_mask),
+                                               out.grid, nullptr, 0, false));
+        opt.fixed_mask = level_mask.get();
+    }
+    opt.moving_mask = reinterpret_cast<const Mask *>(cfg.moving_mask);
+    if (cfg.sampling == 2 && level_mask) {
+        // the same draw on the host, filtered by the level mask at creation
+        // (FixedSamples::from_image_with, metric.rs:494-580)
+        const std::vector<int64_t> draws = host_draws(nvox, 2, cfg.sampling_percentage, cfg.seed);
+        out.metric.reset(metric_create_with(cfg.metric, fixed_level, draws.data(), draws.size(),
+                                            *sm.img, cfg.precision, opt));
+    } else if (cfg.sampling == 2) {
         // random draws (metric.rs:419-437), made and sorted on the device
         const size_t cnt = static_cast<size_t>(static_cast<double>(nvox) * cfg.sampling_percentage);
         out.metric.reset(metric_create_drawn(cfg.metric, fixed_level, cnt, cfg.seed, *sm.img,
@@ -488,6 +517,7 @@ void register_images(const Image &fixed, const Image &moving, const sitkhip_regi
                      sitkhip_transform &tr, sitkhip_registration_result &res) {
     const auto t0 = std::chrono::steady_clock::now();
     check_config(cfg, tr.kind);
+    check_masks(cfg, fixed, moving);
     std::memset(&res, 0, sizeof(res));
     Transform T = initial_transform(tr);
     std::vector<double> p(tr.params, tr.params + T.nparams);
@@ -629,6 +659,7 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
     for (size_t i = 0; i < n; ++i) {
         if (!fixed[i] || !moving[i]) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null image in batch");
         check_config(cfg[i], tr[i].kind);
+        check_masks(cfg[i], *fixed[i], *moving[i]);
         if (cfg[i].optimizer != SITKHIP_OPTIMIZER_REGULAR_STEP)
             fail(SITKHIP_ERR_INVALID_ARGUMENT,
                  "batched registration runs RegularStepGradientDescent (pair " + std::to_string(i) + ")");

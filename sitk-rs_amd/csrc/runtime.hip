@@ -552,6 +552,7 @@ const char *sitkhip_status_name(int s) {
         case SITKHIP_ERR_CONSTANT_INTENSITY: return "ConstantIntensity";
         case SITKHIP_ERR_TOO_MANY_BINS: return "TooManyHistogramBins";
         case SITKHIP_ERR_NOT_BUILT: return "NotBuilt";
+        case SITKHIP_ERR_MASK_SIZE_MISMATCH: return "MaskSizeMismatch";
         default: return "Unknown";
     }
 }

@@ -136,6 +136,8 @@ class RegistrationConfig(C.Structure):
         ("levels", C.c_int),
         ("shrink", (C.c_size_t * 3) * MAX_LEVELS),
         ("sigma", C.c_double * MAX_LEVELS),
+        ("fixed_mask", C.c_void_p),
+        ("moving_mask", C.c_void_p),
     ]
 
 
@@ -754,11 +756,22 @@ def resample_linear(img: DeviceImage, out_geometry: Geometry, stages=None,
 
 def resample_nearest(img: DeviceImage, out_geometry: Geometry, stages=None,
                      default_value: float = 0.0) -> DeviceImage:
+    """resample_nearest[_through] (crates/sitk/src/cuda/ops/pyramid.rs:739-790)."""
     sp, n, keep = _stage_arg(stages)
     h = C.c_void_p()
     _check(lib().sitkhip_resample_nearest(img._h, C.byref(out_geometry), sp, n, default_value,
                                           C.byref(h)))
     return DeviceImage(h)
+
+
+def mask_resample_nearest(mask: DeviceMask, out_geometry: Geometry, stages=None) -> DeviceMask:
+    """A mask placed on another grid by the nearest-neighbour resample with
+    default 0, then thresholded (level_mask_on_device,
+    crates/sitk/src/registration/method.rs:3068-3110)."""
+    sp, n, keep = _stage_arg(stages)
+    h = C.c_void_p()
+    _check(lib().sitkhip_mask_resample_nearest(mask._h, C.byref(out_geometry), sp, n, C.byref(h)))
+    return DeviceMask(h)
 
 
 # -------------------------------------------------------------------- registration
@@ -767,8 +780,13 @@ def registration_config(metric=MATTES, bins=50, sampling=0, sampling_percentage=
                         minimum_step=1e-4, iterations=100, relaxation=0.5,
                         gradient_tolerance=1e-4, learning_rate_mode=LR_ONCE,
                         scales_mode=SCALES_PHYSICAL_SHIFT, shrink_factors=((1, 1, 1),),
-                        sigmas=(0.0,)) -> RegistrationConfig:
+                        sigmas=(0.0,), fixed_mask=None, moving_mask=None) -> RegistrationConfig:
+    """fixed_mask / moving_mask: DeviceMask on the fixed / moving image's grid
+    (set_metric_fixed_mask / set_metric_moving_mask); the caller keeps them
+    alive while the configuration is used."""
     c = RegistrationConfig()
+    c.fixed_mask = fixed_mask._h.value if fixed_mask is not None else None
+    c.moving_mask = moving_mask._h.value if moving_mask is not None else None
     c.metric, c.bins, c.sampling = metric, bins, sampling
     c.sampling_percentage, c.seed, c.precision = sampling_percentage, seed, precision
     c.optimizer, c.learning_rate, c.minimum_step = optimizer, learning_rate, minimum_step

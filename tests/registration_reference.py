@@ -3,7 +3,10 @@ driving the CPU oracle's metric: RegularStepGradientDescent with physical-shift
 scales and a once-estimated learning rate
 (crates/sitk/src/registration/optimizer.rs:353-450,
 crates/sitk/src/registration/scales.rs:659-826,
-crates/sitk/src/registration/method.rs:3207-3233, 3357-3383).
+crates/sitk/src/registration/method.rs:3207-3233, 3357-3383), and gradient
+descent with the golden-section line search (optimizer.rs:458-785). The two
+optimizers are generic over the objective (rsgd, gd_line_search) so that the
+reference's own optimizer fixtures pin them (tests/test_optimizer_reference.py).
 
 Test infrastructure only: checks the device driver in sitk-rs_amd/csrc/registration.hip.
 """
@@ -64,6 +67,41 @@ class Estimator:
         return 1.0 if scale <= np.finfo(float).eps else self.max_step / scale
 
 
+def rsgd(evaluate, p0, learning_rate_of, minimum_step, iterations, grad_tol, scales=None,
+         relaxation=0.5):
+    """RegularStepGradientDescentOptimizer::optimize_with_lr (optimizer.rs:353-450):
+    evaluate(p) -> (value, gradient); learning_rate_of(scaled gradient) -> the
+    step-length scale. Returns (params, value, iterations, stop reason name)."""
+    n = len(p0)
+    scales = [1.0] * n if scales is None else list(scales)
+    p = list(p0)
+    v, g = evaluate(p)
+    prev = [0.0] * n
+    relax = 1.0
+    taken = 0
+    stop = "MaxIterations"
+    while True:
+        scaled = [g[k] / scales[k] for k in range(n)]
+        gm = math.sqrt(_kahan([x * x for x in scaled]))
+        if gm < grad_tol:
+            stop = "GradientConverged"
+            break
+        if _kahan([a * b for a, b in zip(scaled, prev)]) < 0.0:
+            relax *= relaxation
+        step = relax * learning_rate_of(scaled)
+        if step < minimum_step:
+            stop = "StepTooSmall"
+            break
+        if taken >= iterations:
+            break
+        factor = step / gm
+        p = [p[k] - factor * scaled[k] for k in range(n)]
+        prev = scaled
+        taken += 1
+        v, g = evaluate(p)
+    return p, v, taken, stop
+
+
 def register_rsgd(pb, geom, kind, p0, center, size, spacing, origin, minimum_step, iterations,
                   grad_tol, relaxation=0.5):
     """Returns (params, iterations, stop reason name)."""
@@ -76,35 +114,12 @@ def register_rsgd(pb, geom, kind, p0, center, size, spacing, origin, minimum_ste
         v, d, _ = pb.mattes_evaluate(geom, tt)
         return v, d
 
-    p = list(p0)
-    v, g = evaluate(p)
-    s0 = [g[k] / scales[k] for k in range(len(p))]
+    v, g = evaluate(list(p0))
+    s0 = [g[k] / scales[k] for k in range(len(p0))]
     gm0 = math.sqrt(sum(x * x for x in s0))
     lr = est.learning_rate(s0) * gm0
-    v, g = evaluate(p)
-    prev = [0.0] * len(p)
-    relax = 1.0
-    taken = 0
-    stop = "MaxIterations"
-    while True:
-        scaled = [g[k] / scales[k] for k in range(len(p))]
-        gm = math.sqrt(_kahan([x * x for x in scaled]))
-        if gm < grad_tol:
-            stop = "GradientConverged"
-            break
-        if _kahan([a * b for a, b in zip(scaled, prev)]) < 0.0:
-            relax *= relaxation
-        step = relax * lr
-        if step < minimum_step:
-            stop = "StepTooSmall"
-            break
-        if taken >= iterations:
-            break
-        factor = step / gm
-        p = [p[k] - factor * scaled[k] for k in range(len(p))]
-        prev = scaled
-        taken += 1
-        v, g = evaluate(p)
+    p, _, taken, stop = rsgd(evaluate, p0, lambda _s: lr, minimum_step, iterations, grad_tol,
+                             scales, relaxation)
     return np.array(p), taken, stop
 
 
@@ -167,6 +182,42 @@ class WindowConvergence:
         return c0 - c1
 
 
+def gd_line_search(evaluate, value_of, p0, learning_rate, iterations, scales=None, window=None,
+                   min_convergence=None, epsilon=0.01, max_line_iterations=20, lower=0.0,
+                   upper=5.0):
+    """GradientDescentLineSearchOptimizer::run (optimizer.rs:688-785): the
+    golden-section rate over [lr lower, lr, lr upper], carried to the next
+    iteration; the optional window convergence monitor; the lowest-value
+    iterate is returned. Returns (best params, best value, iterations, stop)."""
+    n = len(p0)
+    scales = [1.0] * n if scales is None else list(scales)
+    p = list(p0)
+    value, grad = evaluate(p)
+    best, best_p = value, list(p)
+    mon = WindowConvergence(window) if window is not None else None
+    lr = learning_rate
+    taken, stop = 0, "MaxIterations"
+    while True:
+        if value < best:
+            best, best_p = value, list(p)
+        if mon is not None:
+            mon.add(value)
+            cv = mon.value()
+            if cv is not None and cv <= min_convergence:
+                stop = "Converged"
+                break
+        if taken >= iterations:
+            break
+        d = [grad[k] / scales[k] for k in range(n)]
+        base = p
+        lr = golden_section(epsilon, max_line_iterations, lr * lower, lr, lr * upper, None, [0],
+                            lambda x: value_of([base[k] - x * d[k] for k in range(n)]))
+        p = [p[k] - lr * d[k] for k in range(n)]
+        taken += 1
+        value, grad = evaluate(p)
+    return best_p, best, taken, stop
+
+
 def register_gd_line_search(pb, geom, kind, p0, center, size, spacing, origin, iterations,
                             window=10, min_convergence=1e-6):
     """The estimated-rate line-search method (method.rs:1047-1057, 3361-3373):
@@ -184,28 +235,8 @@ def register_gd_line_search(pb, geom, kind, p0, center, size, spacing, origin, i
     def value_of(p):
         return pb.mattes_value(geom, O.make_transform(kind, p, center=center))
 
-    p = list(p0)
-    _, g = evaluate(p)
-    lr = est.learning_rate([g[k] / scales[k] for k in range(len(p))])
-    value, grad = evaluate(p)
-    best, best_p = value, list(p)
-    mon = WindowConvergence(window)
-    taken, stop = 0, "MaxIterations"
-    while True:
-        if value < best:
-            best, best_p = value, list(p)
-        mon.add(value)
-        cv = mon.value()
-        if cv is not None and cv <= min_convergence:
-            stop = "Converged"
-            break
-        if taken >= iterations:
-            break
-        d = [grad[k] / scales[k] for k in range(len(p))]
-        base = p
-        lr = golden_section(0.01, 20, lr * 0.0, lr, lr * 5.0, None, [0],
-                            lambda x: value_of([base[k] - x * d[k] for k in range(len(base))]))
-        p = [p[k] - lr * d[k] for k in range(len(p))]
-        taken += 1
-        value, grad = evaluate(p)
+    _, g = evaluate(list(p0))
+    lr = est.learning_rate([g[k] / scales[k] for k in range(len(p0))])
+    best_p, _, taken, stop = gd_line_search(evaluate, value_of, p0, lr, iterations, scales,
+                                            window, min_convergence)
     return np.array(best_p), taken, stop

@@ -166,3 +166,35 @@ def test_empty_cache_keeps_live_buffers(hip):
     tmp.free()
     hip.empty_cache()
     assert np.array_equal(img.to_host(), np.arange(4096, dtype=np.float32).reshape(16, 16, 16))
+
+
+def test_adam_steps_match_numpy(hip):
+    """Two Adam steps (Kingma & Ba 2015) of the device against the same float32
+    arithmetic in numpy: m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2,
+    p -= lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps), with the device's
+    own gradients of two different backward passes. The update of every
+    parameter agrees to 1e-5 of the learning rate (float32 rounding)."""
+    net = hip.CNN(max_batch=B, seed=13)
+    vols = _volumes(9)
+    for b in range(B):
+        net.set_input_raw(b, vols[2 * b], vols[2 * b + 1])
+    rng = np.random.default_rng(10)
+    lr, b1, b2, eps = np.float32(2e-3), np.float32(0.9), np.float32(0.999), np.float32(1e-8)
+    p = net.params()
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for t in (1, 2):
+        net.forward(B)
+        net.backward(B, rng.normal(0, 0.1, (B, 12)).astype(np.float32))
+        g = net.grads()
+        assert np.count_nonzero(g) > 0.9 * g.size
+        net.adam(lr=float(lr), beta1=float(b1), beta2=float(b2), eps=float(eps))
+        c1 = np.float32(1.0) - b1 ** np.float32(t)
+        c2 = np.float32(1.0) - b2 ** np.float32(t)
+        m = b1 * m + (np.float32(1.0) - b1) * g
+        v = b2 * v + (np.float32(1.0) - b2) * g * g
+        want = p - lr * (m / c1) / (np.sqrt(v / c2) + eps)
+        got = net.params()
+        err = np.max(np.abs((got - p) - (want - p)))
+        assert err <= 1e-5 * lr, (t, err)
+        p = got

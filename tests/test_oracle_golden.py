@@ -614,3 +614,139 @@ def test_explicit_point_derivative_matches_finite_difference():
         fd = (pb.ms_value(O.make_transform(O.EULER3D, pp, center=[7, 7, 7]))[0]
               - pb.ms_value(O.make_transform(O.EULER3D, pm, center=[7, 7, 7]))[0]) / (2 * h)
         assert abs(fd - an[k]) < 1e-4 * max(1.0, abs(an[k])), (k, fd, an[k])
+
+
+# ------------------------------------------------- nearest-neighbour resample
+def _nn(v, t, default=0.0, sp=(1.0, 1.0, 1.0), org=(0.0, 0.0, 0.0), out=None):
+    nz, ny, nx = v.shape
+    size, osp, oorg = out if out is not None else ((nx, ny, nz), sp, org)
+    return O.resample_nearest(v, sp, org, size, osp, oorg, t, default=default)
+
+
+def test_nearest_resample_identity_shift_and_half_ties():
+    """ResampleImageFilter with the nearest-neighbour interpolator
+    (transform/resample.rs:226-290, interpolator.rs:37-54): the identity
+    returns the input; a whole-voxel shift moves it and fills the default; a
+    shift of exactly half a voxel rounds every tie up (RoundHalfIntegerUp,
+    core/coord.rs:189-191: 2.5 -> 3, not to even); a 0.3-voxel shift picks the
+    nearer voxel's own value (no blending)."""
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=(5, 6, 7)).astype(np.float32)
+    ident = O.make_transform(O.TRANSLATION, [0.0, 0.0, 0.0])
+    assert np.array_equal(_nn(v, ident), v)
+    # out(x) = in(x + 2) along x: the last two columns fall outside
+    got = _nn(v, O.make_transform(O.TRANSLATION, [2.0, 0.0, 0.0]), default=-9.0)
+    assert np.array_equal(got[:, :, :5], v[:, :, 2:]) and np.all(got[:, :, 5:] == -9.0)
+    # c = i + 0.5 exactly: voxel i + 1, for even and odd i alike; the last
+    # column maps to size - 0.5, outside the half-open interval
+    got = _nn(v, O.make_transform(O.TRANSLATION, [0.5, 0.0, 0.0]), default=-9.0)
+    assert np.array_equal(got[:, :, :6], v[:, :, 1:]) and np.all(got[:, :, 6] == -9.0)
+    # c = i - 0.5: voxel i (the tie goes up), and c = -0.5 is inside
+    got = _nn(v, O.make_transform(O.TRANSLATION, [-0.5, 0.0, 0.0]), default=-9.0)
+    assert np.array_equal(got, v)
+    got = _nn(v, O.make_transform(O.TRANSLATION, [0.3, 0.0, -0.3]))
+    assert np.array_equal(got[:, :, :6], v[:, :, :6])
+    assert np.array_equal(got[:, :, 6], v[:, :, 6])
+
+
+def test_nearest_resample_onto_a_shrunk_grid_picks_the_containing_voxel():
+    """Onto a coarser grid (shrink 2 about the centre, as a pyramid level
+    places its mask): every output voxel takes the input voxel whose cell
+    contains its physical point, checked against a direct computation (odd
+    sizes put the coarse points on input voxels, away from the ties the first
+    test covers)."""
+    rng = np.random.default_rng(8)
+    v = rng.integers(0, 2, size=(13, 9, 11)).astype(np.float32)
+    sp, org = (0.9, 1.1, 1.3), (-2.0, 1.0, 0.5)
+    size, osp, oorg = O.shrunk_geometry((11, 9, 13), sp, org, (2, 2, 2))
+    got = O.resample_nearest(v, sp, org, size, osp, oorg,
+                             O.make_transform(O.TRANSLATION, [0.0, 0.0, 0.0]))
+    for k in range(size[2]):
+        for j in range(size[1]):
+            for i in range(size[0]):
+                p = np.asarray(oorg) + np.asarray(osp) * np.array([i, j, k])
+                c = (p - np.asarray(org)) / np.asarray(sp)
+                idx = np.clip(np.floor(c + 0.5).astype(int), 0, np.array([11, 9, 13]) - 1)
+                assert got[k, j, i] == v[idx[2], idx[1], idx[0]]
+
+
+# ------------------------------------------------------------------ Euler3D
+def test_euler3d_fixtures_of_the_reference():
+    """transform/parametric.rs:4105-4183: identity, single-axis quarter turns,
+    the centre is fixed, both composition orders orthonormal and different,
+    and the Jacobian at the identity is the so(3) generators applied to p."""
+    assert list(O.transform_point(O.make_transform(O.EULER3D, [0.0] * 6), [3.0, -4.0, 5.0])) == \
+        [3.0, -4.0, 5.0]
+    h = math.pi / 2
+    p = O.transform_point(O.make_transform(O.EULER3D, [0.0, 0.0, h, 0, 0, 0]), [1.0, 0.0, 7.0])
+    assert np.allclose(p, [0.0, 1.0, 7.0], atol=1e-12, rtol=0)
+    q = O.transform_point(O.make_transform(O.EULER3D, [h, 0.0, 0.0, 0, 0, 0]), [9.0, 1.0, 0.0])
+    assert np.allclose(q, [9.0, 0.0, 1.0], atol=1e-12, rtol=0)
+    r = O.transform_point(O.make_transform(O.EULER3D, [0.0, h, 0.0, 0, 0, 0]), [0.0, 4.0, 1.0])
+    assert np.allclose(r, [1.0, 4.0, 0.0], atol=1e-12, rtol=0)
+    c = [5.0, -2.0, 3.0]
+    assert np.allclose(O.transform_point(O.make_transform(O.EULER3D, [0.3, -0.5, 0.7, 0, 0, 0],
+                                                          center=c), c), c, atol=1e-12, rtol=0)
+    mats = []
+    for zyx in (False, True):
+        t = O.make_transform(O.EULER3D, [0.4, -0.6, 0.8, 0, 0, 0], compute_zyx=zyx)
+        m = O.transform_stage(t)[:9].reshape(3, 3)
+        assert np.allclose(m @ m.T, np.eye(3), atol=1e-12, rtol=0)
+        mats.append(m)
+    assert np.max(np.abs(mats[0] - mats[1])) > 1e-6
+    j = O.transform_jacobian(O.make_transform(O.EULER3D, [0.0] * 6), [2.0, 3.0, 5.0])
+    assert np.array_equal(j, np.array([[0.0, 5.0, -3.0, 1.0, 0.0, 0.0],
+                                       [-5.0, 0.0, 2.0, 0.0, 1.0, 0.0],
+                                       [3.0, -2.0, 0.0, 0.0, 0.0, 1.0]]))
+
+
+def test_euler3d_composition_orders_are_the_published_rotations():
+    """Default order Rz . Rx . Ry (parametric.rs:1030-1053) is scipy's
+    intrinsic 'ZXY' rotation; compute_zyx Rz . Ry . Rx is intrinsic 'ZYX'."""
+    from scipy.spatial.transform import Rotation
+
+    ax, ay, az = 0.37, -0.52, 0.81
+    t = O.make_transform(O.EULER3D, [ax, ay, az, 0, 0, 0])
+    assert np.allclose(O.transform_stage(t)[:9].reshape(3, 3),
+                       Rotation.from_euler("ZXY", [az, ax, ay]).as_matrix(), atol=1e-14, rtol=0)
+    t = O.make_transform(O.EULER3D, [ax, ay, az, 0, 0, 0], compute_zyx=True)
+    assert np.allclose(O.transform_stage(t)[:9].reshape(3, 3),
+                       Rotation.from_euler("ZYX", [az, ay, ax]).as_matrix(), atol=1e-14, rtol=0)
+
+
+# ------------------------------------------------------- masked moving range
+def _mask_fixture_volume(n, shift, outermost):
+    c = n / 2.0
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    x, y, z = i - c - shift, j - c, k - c
+    v = np.exp(-(x * x + y * y + z * z) / 32.0)
+    edge = (i == 0) | (j == 0) | (k == 0) | (i == n - 1) | (j == n - 1) | (k == n - 1)
+    return np.where(edge, outermost, v).astype(np.float32)
+
+
+def test_a_masked_out_voxel_does_not_size_the_moving_axis():
+    """tests/mattes_mask.rs:95-150: moving volumes that differ only in their
+    outermost layer (1000 vs 0.5) give different Mattes values without a mask
+    (the fixture is real), and bit-identical value, derivative and valid count
+    with a mask that excludes a two-voxel shell, at three poses."""
+    n, bins = 16, 32
+    fixed = _mask_fixture_volume(n, 0.0, 0.5)
+    outl = _mask_fixture_volume(n, 1.0, 1000.0)
+    tame = _mask_fixture_volume(n, 1.0, 0.5)
+    mask = np.zeros((n, n, n), np.uint8)
+    mask[2:n - 2, 2:n - 2, 2:n - 2] = 1
+    poses = [[0.0, 0.0, 0.0], [0.7, -0.4, 0.3], [-1.3, 0.9, -0.6]]
+    one = [1.0, 1.0, 1.0]
+    zero = [0.0, 0.0, 0.0]
+    for mm in (None, mask):
+        a = O.MetricProblem(fixed, one, zero, outl, one, zero, moving_mask=mm)
+        b = O.MetricProblem(fixed, one, zero, tame, one, zero, moving_mask=mm)
+        ga, gb = a.mattes_geometry(bins), b.mattes_geometry(bins)
+        for p in poses:
+            t = O.make_transform(O.TRANSLATION, p)
+            va, da, na = a.mattes_evaluate(ga, t)
+            vb, db, nb = b.mattes_evaluate(gb, t)
+            if mm is None:
+                assert va != vb, p
+            else:
+                assert va == vb and na == nb and np.array_equal(da, db), p
