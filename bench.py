@@ -39,10 +39,32 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_info():
+    """(CPU model, physical cores, logical CPUs this process may use)."""
+    model, cores, cur = None, set(), {}
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in list(fh) + [""]:
+                if not line.strip():
+                    if "physical id" in cur and "core id" in cur:
+                        cores.add((cur["physical id"], cur["core id"]))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+                if model is None and k.strip() == "model name":
+                    model = v.strip()
+    except OSError:
+        pass
+    return model, (len(cores) or None), len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(fixed, moving, idx, stage_affine, bins=50, evals=2):
     """The oracle (serial FP64 restatement of the reference's host Mattes,
     crates/sitk/src/registration/mattes.rs:522-676) on the same level: `evals`
-    full value+derivative evaluations, timed on one host core."""
+    full value+derivative evaluations, timed on one host core. Returns the
+    median time, the count, the oracle's (value, derivative, valid) and the
+    transform it evaluated."""
     import oracle as O
 
     pb = O.MetricProblem(fixed.data, fixed.spacing, fixed.origin, moving.data, moving.spacing,
@@ -51,12 +73,232 @@ def cpu_baseline(fixed, moving, idx, stage_affine, bins=50, evals=2):
     A, off = stage_affine
     t = O.make_transform(O.AFFINE, list(A.reshape(9)) + [0.0, 0.0, 0.0])
     t.offset[:] = [float(v) for v in off]
-    times = []
+    times, res = [], None
     for _ in range(evals):
         t0 = time.perf_counter()
-        pb.mattes_evaluate(g, t)
+        res = pb.mattes_evaluate(g, t)
         times.append(time.perf_counter() - t0)
-    return float(np.median(times)), len(times)
+    return float(np.median(times)), len(times), res, t
+
+
+def parity_record(S, O, t, value, mom, valid, want):
+    """Device value and derivative at the bench's evaluation point against the
+    oracle's: relative value error, relative derivative error (vector norm),
+    and whether the valid counts agree."""
+    wv, wd, wn = want
+    j0 = O.transform_jacobian(t, [0, 0, 0])
+    c = np.stack([O.transform_jacobian(t, e) - j0 for e in np.eye(3)])
+    deriv = S.contract_mattes(mom, j0, c, t.nparams)
+    return {"value_rel_err": abs(value - wv) / abs(wv),
+            "derivative_rel_err": float(np.linalg.norm(deriv - wd) / np.linalg.norm(wd)),
+            "valid_equal": int(valid) == int(wn), "oracle_value": wv,
+            "tolerance": {"value": 1e-6, "derivative": 1e-4}}
+
+
+def cpu_threads():
+    """Threads for the reference's parallel CPU legs: the physical cores this
+    process may use, at most 16 (the host share of one GPU)."""
+    _, phys, avail = cpu_info()
+    return max(1, min(phys or avail, avail, 16))
+
+
+def gaussian_cpu_record(S, O, fixed, df, sigma, device_ms, reps=3):
+    """The recursive Gaussian of one 256x256x160 volume by the oracle, serial
+    (t1) and parallel over lines (tN, as the reference runs it,
+    crates/sitk/src/core/parallel.rs:990), against the device's volume bit for
+    bit."""
+    v = fixed.data
+    size = tuple(int(x) for x in fixed.size)
+    n = cpu_threads()
+    t1, tn, ref = [], [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ref = O.smoothing_recursive_gaussian(v, size, fixed.spacing, sigma, threads=1)
+        t1.append(time.perf_counter() - t0)
+    for _ in range(2 * reps):
+        t0 = time.perf_counter()
+        O.smoothing_recursive_gaussian(v, size, fixed.spacing, sigma, threads=n)
+        tn.append(time.perf_counter() - t0)
+    img = S.recursive_gaussian(df, sigma)
+    dev = img.to_host().reshape(-1)
+    img.free()
+    same = bool(np.array_equal(dev.view(np.uint32), np.asarray(ref, np.float32).reshape(-1).view(np.uint32)))
+    return {"t1_ms": float(np.median(t1)) * 1e3, "tN_ms": float(np.median(tn)) * 1e3, "N": n,
+            "device_ms": device_ms, "device_bit_exact": same,
+            "sample": f"{size[0]}x{size[1]}x{size[2]} f32, sigma {sigma[0]} mm, median of "
+                      f"{reps} (t1) and {2 * reps} (tN) runs"}
+
+
+def ms_record(S, W, O, steps=50, evals_n=3):
+    """Mean squares at 256^3, every voxel (16,777,216 samples), Euler3D: the
+    device's value + derivative evaluation, and the oracle's serial (t1) and
+    row-parallel with an in-order fold (tN, crates/sitk/src/core/parallel.rs:
+    766-818) evaluation of the same, with the device's error against it. The
+    reference publishes its CUDA kernel at this size
+    (crates/sitk/src/cuda/ops/mean_squares.rs:239-244: 6.088 ms f32/f32 on an
+    RTX 5000 Ada)."""
+    fixed = W.blob_volume((256, 256, 256), (1.0, 1.0, 1.0), 11, nblobs=40)
+    rng = np.random.default_rng(511)
+    _, T = W.random_euler(rng, 5.0, 3.0, fixed.center())
+    moving = W.resample_linear(fixed, T)
+    df = S.DeviceImage.upload(fixed.data, spacing=fixed.spacing, origin=fixed.origin)
+    dm = S.DeviceImage.upload(moving.data, spacing=moving.spacing, origin=moving.origin)
+    met = S.ResidentMetric(df, dm)
+    p = np.array([0.01, -0.02, 0.015, 0.5, -0.3, 0.2])
+    stage = W.AffineMap(W.euler_matrix(*p[:3]), p[3:], fixed.center()).stage().reshape(1, 12)
+    t = O.make_transform(O.EULER3D, list(p), center=list(fixed.center()))
+    for _ in range(5):
+        met.evaluate(stage)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mom = met.evaluate(stage)
+    dev_ms = (time.perf_counter() - t0) * 1e3 / steps
+    pb = O.MetricProblem(fixed.data, fixed.spacing, fixed.origin, moving.data, moving.spacing,
+                         moving.origin)
+    n = cpu_threads()
+    t0 = time.perf_counter()
+    wv, wd, wn = pb.ms_evaluate(t)
+    t1 = time.perf_counter() - t0
+    tn = []
+    for _ in range(evals_n):
+        t0 = time.perf_counter()
+        pb.ms_evaluate(t, threads=n)
+        tn.append(time.perf_counter() - t0)
+    j0 = O.transform_jacobian(t, [0, 0, 0])
+    c = np.stack([O.transform_jacobian(t, e) - j0 for e in np.eye(3)])
+    value, deriv, valid = S.contract_ms(mom, j0, c, 6)
+    met.free()
+    df.free()
+    dm.free()
+    nvox = fixed.data.size
+    return {"workload": "256^3 f32 1 mm, 40 blobs, moving through a seeded Euler3D "
+                        "(+-5 deg, +-3 mm); mean squares on every voxel; Euler3D point map",
+            "device_ms": dev_ms, "device_GBps": 2 * nvox * 4 / (dev_ms * 1e-3) / 1e9,
+            "cpu_t1_ms": t1 * 1e3, "cpu_tN_ms": float(np.median(tn)) * 1e3, "N": n,
+            "value_rel_err": abs(value - wv) / abs(wv),
+            "derivative_rel_err": float(np.linalg.norm(deriv - wd) / np.linalg.norm(wd)),
+            "valid_equal": int(valid) == int(wn),
+            "published_cuda_kernel_ms": {"rtx5000ada_f32_f32": 6.088, "rtx5000ada_f64_f64": 5.755,
+                                         "source": "crates/sitk/src/cuda/ops/mean_squares.rs:239-244"}}
+
+
+def cpu_pair_record(S, W, O):
+    """The CPU-runnable rigid pair (64^3, every voxel, Euler3D, Mattes 50 bins,
+    RSGD with physical-shift scales, at most 100 iterations, one level)
+    registered by the host restatement of the reference's loop driving the
+    oracle (one core, as the reference's Mattes) and by the device."""
+    from oracle.registration import register_rsgd
+
+    fixed, moving, T_gt = W.config_cpu_pair()
+    c = list(fixed.center())
+    pb = O.MetricProblem(fixed.data, fixed.spacing, fixed.origin, moving.data, moving.spacing,
+                         moving.origin)
+    g = pb.mattes_geometry(50)
+    t0 = time.perf_counter()
+    want, want_it, _ = register_rsgd(pb, g, O.EULER3D, [0.0] * 6, c, fixed.size, fixed.spacing,
+                                     fixed.origin, minimum_step=1e-4, iterations=100,
+                                     grad_tol=1e-6)
+    t_cpu = time.perf_counter() - t0
+    df = S.DeviceImage.upload(fixed.data, spacing=fixed.spacing, origin=fixed.origin)
+    dm = S.DeviceImage.upload(moving.data, spacing=moving.spacing, origin=moving.origin)
+    cfg = S.registration_config(metric=S.MATTES, bins=50, optimizer=S.REGULAR_STEP,
+                                minimum_step=1e-4, iterations=100, gradient_tolerance=1e-6,
+                                learning_rate_mode=S.LR_ONCE)
+    S.register(df, dm, cfg, S.EULER3D, [0.0] * 6, center=fixed.center())
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        got, res = S.register(df, dm, cfg, S.EULER3D, [0.0] * 6, center=fixed.center())
+        times.append(time.perf_counter() - t0)
+    df.free()
+    dm.free()
+
+    def tre(p):
+        est = W.AffineMap(W.euler_matrix(*p[:3]), np.asarray(p[3:]), fixed.center())
+        return float(W.tre(est, T_gt.inverse(), fixed))
+
+    t_dev = float(np.median(times))
+    return {"workload": "64^3 1 mm, 40 blobs + N(0,5), Euler3D +-5 deg / +-3 mm; Mattes 50 bins, "
+                        "every voxel (262,144 samples), RSGD (physical-shift scales, rate "
+                        "estimated once, minimum step 1e-4), at most 100 iterations, one level",
+            "cpu": {"seconds": t_cpu, "registrations_per_s": 1.0 / t_cpu, "cores": 1,
+                    "iterations": int(want_it), "tre_mm": tre(want)},
+            "device": {"seconds": t_dev, "registrations_per_s": 1.0 / t_dev,
+                       "iterations": int(res.iterations), "tre_mm": tre(got),
+                       "parameter_max_abs_diff_vs_cpu": float(np.max(np.abs(got - want)))}}
+
+
+def traffic_probe() -> int:
+    """Child of the traffic measurement (run under rocprofv3 --pmc): the
+    bench's evaluation a few times, then a plain copy of known size."""
+    import sitk_hip as S
+    import workload as W
+
+    S.backend_init(0)
+    fixed, moving, T_gt = W.config_large_pair(seed=2)
+    idx = S.draw_samples(fixed.data.size, S.SAMPLING_RANDOM, 0.2, 7)
+    df = S.DeviceImage.upload(fixed.data, spacing=fixed.spacing, origin=fixed.origin)
+    dm = S.DeviceImage.upload(moving.data, spacing=moving.spacing, origin=moving.origin)
+    metric = S.ResidentMattes(df, dm, bins=50, indices=idx)
+    stage = T_gt.inverse().stage().reshape(1, 12)
+    for _ in range(6):
+        metric.value_and_moments(stage)
+    for _ in range(3):
+        S.recursive_gaussian(df, [0.0, 0.0, 0.0]).free()
+    return 0
+
+
+def measure_traffic(kernels, copy_bytes):
+    """HBM bytes per launch of `kernels`, from two rocprofv3 --pmc runs of the
+    traffic probe (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled
+    by what the counter reads for copy_f32 of `copy_bytes` read and written.
+    Returns ({kernel: {read, written, total}}, calibration) or (None, reason)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not found"
+    per = {}
+    cal = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+            cmd = ["timeout", "-s", "KILL", "120", exe, "--pmc", counter, "-f", "csv", "-d", d,
+                   "-o", "run", "--", sys.executable, os.path.abspath(__file__), "--traffic-probe"]
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                return None, f"rocprofv3 --pmc {counter} exited {r.returncode}"
+            rows = []
+            for root, _, files in os.walk(d):
+                for fn in files:
+                    if fn.endswith("counter_collection.csv"):
+                        with open(os.path.join(root, fn)) as fh:
+                            rows += list(csv.DictReader(fh))
+        vals = {}
+        for row in rows:
+            if row.get("Counter_Name") != counter:
+                continue
+            name = row.get("Kernel_Name", "")
+            for k in list(kernels) + ["copy_f32"]:
+                if f"::{k}<" in name or name.endswith(f"::{k}") or f"::{k}(" in name:
+                    vals.setdefault(k, []).append(float(row["Counter_Value"]))
+        if "copy_f32" not in vals:
+            return None, f"no copy_f32 dispatch in the {counter} run"
+        factor = copy_bytes / (float(np.median(vals["copy_f32"])) * 1024.0)
+        cal[counter] = {"copy_KiB": float(np.median(vals["copy_f32"])), "factor": factor}
+        for k in kernels:
+            if k in vals:
+                # skip the warm-up dispatches' first (cold) launch
+                v = vals[k][1:] if len(vals[k]) > 1 else vals[k]
+                per.setdefault(k, {})[counter] = float(np.median(v)) * 1024.0 * factor
+    out = {}
+    for k, c in per.items():
+        rd, wr = c.get("FETCH_SIZE"), c.get("WRITE_SIZE")
+        out[k] = {"read": rd, "written": wr,
+                  "total": (rd + wr) if rd is not None and wr is not None else None}
+    return out, cal
 
 
 def batched_pair_configs(S, pairs):
@@ -283,17 +525,24 @@ def main() -> int:
     ap.add_argument("--cpu-evals", type=int, default=16)
     ap.add_argument("--pairs", type=int, default=512,
                     help="batched rigid pairs for the registrations/s record (0 skips it)")
-    ap.add_argument("--plain-pairs", type=int, default=64,
+    ap.add_argument("--plain-pairs", type=int, default=512,
                     help="pairs registered one at a time for the plain rate")
     ap.add_argument("--lp-reps", type=int, default=3,
                     help="end-to-end registrations of the large pair (0 skips the record)")
     ap.add_argument("--starts", type=int, default=64,
                     help="starts of the deformable multi-start record (0 skips it)")
-    ap.add_argument("--plain-starts", type=int, default=8,
+    ap.add_argument("--plain-starts", type=int, default=64,
                     help="deformable starts registered one at a time for the plain rate")
     ap.add_argument("--init-pairs", type=int, default=1024,
                     help="held-out pairs for the learned-initializer record (0 skips it)")
+    ap.add_argument("--init-plain", type=int, default=1024,
+                    help="of those, pairs registered one at a time by the plain method")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the HBM-traffic measurement (two rocprofv3 --pmc child runs)")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.traffic_probe:
+        return traffic_probe()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         return 2
@@ -345,21 +594,19 @@ def main() -> int:
     per_pass = n * 5 + nvox * 4
     bytes_step = 2 * per_pass
     gbs = bytes_step / (ms_step * 1e-3) / 1e9
-    # Per kernel, the bytes this implementation must move: the value pass
-    # reads the moving volume and the sample stream and writes one record per
-    # sample (moving value + index gradient); the derivative pass reads the
-    # records and the sample stream.
+    # The roofline of the dominant kernel, the value pass, on the algorithm's
+    # bytes for one pass (moving volume once + sample stream); the records it
+    # also writes for its own derivative pass are reported beside it.
     rec_bytes = 16 if prec == S.PRECISION_F32 else 32
-    kernel_bytes = {"mattes_hist": nvox * 4 + n * 5 + n * rec_bytes,
-                    "mattes_deriv": n * 5 + n * rec_bytes}
 
     kern = {}
     for name in ("mattes_hist", "mattes_tail", "mattes_deriv", "fold_partials"):
         cnt, tot = S.profile_query(name)
         kern[name] = {"count": cnt, "avg_us": (tot / cnt * 1e3) if cnt else None}
-    dom = max(("mattes_hist", "mattes_deriv"), key=lambda k: kern[k]["avg_us"] or 0)
+    dom = "mattes_hist"
     dom_us = kern[dom]["avg_us"]
-    achieved = kernel_bytes[dom] / (dom_us * 1e-6) / 1e9
+    achieved = per_pass / (dom_us * 1e-6) / 1e9
+    with_records = per_pass + n * rec_bytes
 
     # Gaussian, timed the same way (4 events around its 3 axis launches)
     S.profile_reset()
@@ -376,20 +623,21 @@ def main() -> int:
         gk[f"axis{ax}_us"] = tot / cnt * 1e3 if cnt else None
     g_bytes = nvox * 40  # f32 in, two f64 intermediates, f32 out
 
-    # calibration for HBM-counter runs: a plain copy of the fixed volume
-    # (copy_f32: 4 B read + 4 B written per voxel)
-    for _ in range(3):
-        S.recursive_gaussian(df, [0.0, 0.0, 0.0]).free()
+    traffic, traffic_cal = None, None
+    if not args.no_traffic:
+        t0 = time.perf_counter()
+        traffic, traffic_cal = measure_traffic(["mattes_hist", "mattes_deriv_rec"], nvox * 4)
+        log(f"traffic measured in {time.perf_counter() - t0:.1f}s: {traffic or traffic_cal}")
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "mattes_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get(dom)
-
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu:
-        t_cpu, k = cpu_baseline(fixed, moving, idx, (I.A, I.offset), evals=args.cpu_evals)
+        import oracle as O
+
+        model, phys, avail = cpu_info()
+        t_cpu, k, want, t_eval = cpu_baseline(fixed, moving, idx, (I.A, I.offset),
+                                              evals=args.cpu_evals)
+        parity = parity_record(S, O, t_eval, value, mom, valid, want)
+        log(f"parity vs oracle: {parity}")
         cpu = {
             "value": bytes_step / t_cpu / 1e9,
             "unit": "GB/s",
@@ -397,9 +645,17 @@ def main() -> int:
             "kind": "port",
             "sample": f"median of {k} full value+derivative evaluations of the same level "
                       f"({n} samples, 12-parameter affine) by the serial FP64 oracle, "
-                      f"{t_cpu * 1e3:.0f} ms each",
+                      f"{t_cpu * 1e3:.0f} ms each (the reference's host Mattes is serial)",
             "ms_per_eval": t_cpu * 1e3,
+            "cpu_model": model,
+            "physical_cores": phys,
+            "cpus_available": avail,
         }
+        t0 = time.perf_counter()
+        cpu["gaussian"] = gaussian_cpu_record(S, O, fixed, df, sig, g_ms)
+        cpu["mean_squares_256"] = ms_record(S, W, O)
+        cpu["cpu_pair_registration"] = cpu_pair_record(S, W, O)
+        log(f"CPU legs in {time.perf_counter() - t0:.1f}s")
 
     out = {
         "metric": METRIC,
@@ -429,10 +685,15 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "bytes_per_launch": int(kernel_bytes[dom]),
+            "traffic": (traffic or {}).get(dom, {}).get("total") if traffic else None,
+            "bytes_per_launch": int(per_pass),
             "avg_us": dom_us,
+            "with_records": {"bytes_per_launch": int(with_records),
+                             "achieved": with_records / (dom_us * 1e-6) / 1e9,
+                             "frac": with_records / (dom_us * 1e-6) / 1e9 / HBM_PEAK_GBS},
+            "step_frac": gbs / HBM_PEAK_GBS,
         },
+        "traffic": {"per_kernel": traffic, "calibration": traffic_cal},
         "kernels": kern,
         "gaussian_level": {
             "ms_per_volume": g_ms,
@@ -440,6 +701,7 @@ def main() -> int:
             **gk,
         },
         "result": {"value": value, "valid": int(valid)},
+        "parity": parity,
         "cpu_baseline": cpu,
     }
     if args.lp_reps > 0:
@@ -447,7 +709,7 @@ def main() -> int:
     if args.pairs > 0:
         out["registrations"] = registrations_record(S, W, args.pairs, args.plain_pairs)
     if args.init_pairs > 0:
-        rec = learned_init_record(S, args.init_pairs, min(args.plain_pairs, 32))
+        rec = learned_init_record(S, args.init_pairs, min(args.init_plain, args.init_pairs))
         if rec is not None:
             out["learned_init"] = rec
     if args.starts > 0:

@@ -264,6 +264,27 @@ int sitkhip_metric_volume_bytes(sitkhip_metric m, size_t *out);
  * of the moving volume over its mask; (0,0) when empty. */
 int sitkhip_metric_value_ranges(sitkhip_metric m, double fixed_range[2], double moving_range[2]);
 
+/* The moving image's interpolator, MovingImage::from_image_with_interpolator
+ * (src/registration/metric.rs:720-750; the reference's device path refuses
+ * every interpolator but linear, registration/method.rs:2724-2728):
+ * SITKHIP_INTERP_LINEAR (the default) or SITKHIP_INTERP_BSPLINE, the cubic
+ * B-spline of src/transform/interpolator.rs:170-350 -- its coefficients (the
+ * mirror-boundary prefilter, FP64 on the device, bit for bit the reference's;
+ * narrowed to float on the FP32 path) are computed here from the metric's
+ * moving volume, and every later evaluation interpolates with its 64 taps.
+ * Value ranges stay those of the voxel values (metric.rs:913-919). Refused:
+ * an unknown interpolator (SITKHIP_ERR_INVALID_ARGUMENT), a moving axis of two
+ * voxels (SITKHIP_ERR_UNSUPPORTED_GEOMETRY: the reference's mirror fold reads
+ * index -1 there). Batched evaluation and the B-spline transform take linear
+ * metrics only (SITKHIP_ERR_INVALID_ARGUMENT). */
+#define SITKHIP_INTERP_LINEAR 0
+#define SITKHIP_INTERP_BSPLINE 1
+int sitkhip_metric_set_interpolator(sitkhip_metric m, int interpolator);
+/* The metric's cubic B-spline coefficients (len = moving voxels), as FP64
+ * (the FP32 path's float values widened); SITKHIP_ERR_INVALID_ARGUMENT for a
+ * linear metric. For checking. */
+int sitkhip_metric_bspline_coefficients(sitkhip_metric m, double *out, size_t len);
+
 /* draw_samples (src/registration/metric.rs:101-122, 419-437), on the host: the
  * device is handed the list and never draws. strategy 0 None (*n = SIZE_MAX,
  * no list: use FixedPoints::Grid), 1 Regular (stride ceil(1/percentage)),
@@ -382,6 +403,11 @@ typedef struct {
      * image's is SITKHIP_ERR_MASK_SIZE_MISMATCH (method.rs:2070-2083). */
     sitkhip_mask fixed_mask;
     sitkhip_mask moving_mask;
+    /* set_interpolator (method.rs:1507-1520): the moving image's interpolator
+     * on every level, SITKHIP_INTERP_LINEAR (0, the default) or
+     * SITKHIP_INTERP_BSPLINE (sitkhip_metric_set_interpolator). Batched
+     * registration interpolates linearly and refuses the B-spline. */
+    int interpolator;
 } sitkhip_registration_config;
 
 typedef struct {

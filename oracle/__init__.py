@@ -58,6 +58,7 @@ class Problem(C.Structure):
         ("mp2i", C.c_double * 9),
         ("mvals", C.POINTER(C.c_float)),
         ("mmask", C.POINTER(C.c_uint8)),
+        ("mcoef", C.POINTER(C.c_double)),
     ]
 
 
@@ -131,6 +132,8 @@ def lib():
         L.orc_ms_evaluate.argtypes = [pp, tp, dp, szp]
         L.orc_ms_evaluate.restype = C.c_double
         L.orc_ms_value.argtypes = [pp, tp, szp]
+        L.orc_ms_evaluate_threads.argtypes = [pp, tp, dp, szp, C.c_int]
+        L.orc_ms_evaluate_threads.restype = C.c_double
         L.orc_ms_value.restype = C.c_double
         L.orc_resample_linear.argtypes = [fp, szp, dp, dp, dp, szp, dp, dp, dp, tp, C.c_double, fp]
         L.orc_resample_nearest.argtypes = L.orc_resample_linear.argtypes
@@ -138,6 +141,10 @@ def lib():
         L.orc_fixture_volume.argtypes = [C.c_size_t, dp, fp]
         L.orc_fixture_inverted.argtypes = [C.c_size_t, dp, fp]
         L.orc_fixture_commensurate.argtypes = [C.c_size_t, fp]
+        L.orc_bspline_coefficients.argtypes = [fp, szp, dp]
+        L.orc_powi.argtypes = [C.c_double, C.c_int]
+        L.orc_powi.restype = C.c_double
+        L.orc_bspline_value_and_gradient.argtypes = [dp, szp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -199,6 +206,32 @@ def smoothing_recursive_gaussian(vol: np.ndarray, size, spacing, sigma, threads:
 
 
 # ---------------------------------------------------------------- geometry
+def bspline_coefficients(vol: np.ndarray) -> np.ndarray:
+    """Cubic B-spline coefficients of a float32 volume (nz, ny, nx), FP64
+    (interpolator.rs:170-190)."""
+    v = np.ascontiguousarray(vol, dtype=np.float32)
+    nz, ny, nx = v.shape
+    out = np.empty(v.shape, dtype=np.float64)
+    lib().orc_bspline_coefficients(_fp(v), _szp([nx, ny, nz]), _dp(out))
+    return out
+
+
+def powi(a: float, b: int) -> float:
+    return float(lib().orc_powi(float(a), int(b)))
+
+
+def bspline_value_and_gradient(coef: np.ndarray, c):
+    """(value, index gradient) of the cubic B-spline interpolant at continuous
+    index c (x, y, z), or None outside (interpolator.rs:273-350)."""
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    nz, ny, nx = coef.shape
+    v = C.c_double()
+    g = np.empty(3)
+    ok = lib().orc_bspline_value_and_gradient(_dp(coef), _szp([nx, ny, nz]), _d3(c), C.byref(v),
+                                              _dp(g))
+    return (v.value, g) if ok else None
+
+
 def index_to_physical(direction, spacing) -> np.ndarray:
     d = np.ascontiguousarray(direction, dtype=np.float64).reshape(9)
     s = np.ascontiguousarray(spacing, dtype=np.float64)
@@ -289,10 +322,12 @@ class MetricProblem:
 
     def __init__(self, fixed, fspacing, forigin, moving, mspacing, morigin,
                  fdir=None, mdir=None, indices=None, fixed_mask=None, moving_mask=None,
-                 points=None):
+                 points=None, interpolator="linear"):
         """points: explicit sample points (n x 3, physical), with `fixed` then
         the n gathered sample values (FixedPoints::Explicit); the fixed
-        geometry is unused."""
+        geometry is unused. interpolator: "linear" or "bspline" (cubic
+        B-spline of the moving image, MovingImage::from_image_with_interpolator,
+        registration/metric.rs:720-750)."""
         eye = np.eye(3).reshape(9)
         fdir = eye if fdir is None else np.asarray(fdir, dtype=np.float64).reshape(9)
         mdir = eye if mdir is None else np.asarray(mdir, dtype=np.float64).reshape(9)
@@ -332,6 +367,12 @@ class MetricProblem:
         if moving_mask is not None:
             self._mmask = np.ascontiguousarray(moving_mask, dtype=np.uint8)
             pb.mmask = self._mmask.ctypes.data_as(C.POINTER(C.c_uint8))
+        self._mcoef = None
+        if interpolator == "bspline":
+            self._mcoef = bspline_coefficients(self.moving)
+            pb.mcoef = self._mcoef.ctypes.data_as(C.POINTER(C.c_double))
+        elif interpolator != "linear":
+            raise ValueError(f"unknown interpolator {interpolator!r}")
         self.pb = pb
 
     def fixed_range(self):
@@ -397,10 +438,17 @@ class MetricProblem:
         lib().orc_mattes_moments(C.byref(self.pb), C.byref(g), C.byref(t), _dp(pr), _dp(out))
         return out
 
-    def ms_evaluate(self, t: Transform):
+    def ms_evaluate(self, t: Transform, threads: int = 1):
+        """Mean squares value and derivative (metric.rs:1065-1179); with
+        threads > 1 the rows are computed in parallel and added in order, as
+        the reference does (the same bits)."""
         deriv = np.empty(t.nparams, dtype=np.float64)
         valid = C.c_size_t(0)
-        v = lib().orc_ms_evaluate(C.byref(self.pb), C.byref(t), _dp(deriv), C.byref(valid))
+        if threads > 1:
+            v = lib().orc_ms_evaluate_threads(C.byref(self.pb), C.byref(t), _dp(deriv),
+                                              C.byref(valid), int(threads))
+        else:
+            v = lib().orc_ms_evaluate(C.byref(self.pb), C.byref(t), _dp(deriv), C.byref(valid))
         return v, deriv, valid.value
 
     def ms_value(self, t: Transform):

@@ -676,13 +676,128 @@ static int linear_value_and_gradient(const orc_problem *pb, const double c[3], d
     return 1;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Cubic B-spline interpolation (interpolator.rs:170-350). */
+double orc_powi(double a, int b) {
+    int recip = b < 0;
+    unsigned pw = recip ? (unsigned)(-(long)b) : (unsigned)b;
+    double r = 1.0;
+    for (;;) {
+        if (pw & 1u) r *= a;
+        pw >>= 1;
+        if (pw == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+/* bspline3_filter_1d (interpolator.rs:236-270) on one line of n >= 2 values. */
+static void bspline3_filter_line(double *line, size_t n, double z) {
+    double gain = (1.0 - z) * (1.0 - 1.0 / z);
+    for (size_t k = 0; k < n; ++k) line[k] *= gain;
+    double iz = 1.0 / z;
+    double zn = z;
+    double z2n = orc_powi(z, (int)n - 1);
+    double sum = line[0] + z2n * line[n - 1];
+    z2n = z2n * z2n * iz;
+    for (size_t k = 1; k + 1 < n; ++k) {
+        sum += (zn + z2n) * line[k];
+        zn *= z;
+        z2n *= iz;
+    }
+    line[0] = sum / (1.0 - zn * zn);
+    for (size_t k = 1; k < n; ++k) line[k] += z * line[k - 1];
+    line[n - 1] = (z / (z * z - 1.0)) * (z * line[n - 2] + line[n - 1]);
+    for (size_t k = n - 1; k-- > 0;) line[k] = z * (line[k + 1] - line[k]);
+}
+
+void orc_bspline_coefficients(const float *in, const size_t size[3], double *out) {
+    size_t total = size[0] * size[1] * size[2];
+    for (size_t i = 0; i < total; ++i) out[i] = (double)in[i];
+    double pole = sqrt(3.0) - 2.0;
+    size_t strides[3] = {1, size[0], size[0] * size[1]};
+    for (int axis = 0; axis < 3; ++axis) {
+        size_t n = size[axis];
+        if (n == 1) continue;
+        double *line = (double *)malloc(n * sizeof(double));
+        int o1 = axis == 0 ? 1 : 0, o2 = axis == 2 ? 1 : 2;
+        for (size_t j2 = 0; j2 < size[o2]; ++j2)
+            for (size_t j1 = 0; j1 < size[o1]; ++j1) {
+                size_t base = j1 * strides[o1] + j2 * strides[o2];
+                for (size_t k = 0; k < n; ++k) line[k] = out[base + k * strides[axis]];
+                bspline3_filter_line(line, n, pole);
+                for (size_t k = 0; k < n; ++k) out[base + k * strides[axis]] = line[k];
+            }
+        free(line);
+    }
+}
+
+int orc_bspline_value_and_gradient(const double *coef, const size_t size[3], const double c[3],
+                                   double *value, double grad[3]) {
+    if (!is_inside(c, size)) return 0;
+    size_t idx[3][4];
+    double w[3][4], dw[3][4];
+    size_t strides[3] = {1, size[0], size[0] * size[1]};
+    for (int d = 0; d < 3; ++d) {
+        long base = (long)floor(c[d]) - 1;
+        long n = (long)size[d];
+        for (int k = 0; k < 4; ++k) {
+            long i = base + k;
+            if (n == 1) {
+                i = 0;
+            } else {
+                if (i < 0) i = -i;
+                if (i > n - 1) i = 2 * (n - 1) - i;
+            }
+            idx[d][k] = (size_t)i;
+        }
+        double frac = c[d] - (double)(base + 1);
+        double w3 = (1.0 / 6.0) * frac * frac * frac;
+        double w0 = 1.0 / 6.0 + 0.5 * frac * (frac - 1.0) - w3;
+        double w2 = frac + w0 - 2.0 * w3;
+        double w1 = 1.0 - w0 - w2 - w3;
+        w[d][0] = w0; w[d][1] = w1; w[d][2] = w2; w[d][3] = w3;
+        double a = frac - 0.5;
+        double dw2 = 0.75 - a * a;
+        double dw3 = 0.5 * (a - dw2 + 1.0);
+        double dw1 = 1.0 - dw2 - dw3;
+        dw[d][0] = -dw1; dw[d][1] = dw1 - dw2; dw[d][2] = dw2 - dw3; dw[d][3] = dw3;
+    }
+    double v = 0.0, g[3] = {0.0, 0.0, 0.0};
+    for (int corner = 0; corner < 64; ++corner) {
+        int tap[3] = {corner % 4, (corner / 4) % 4, corner / 16};
+        size_t offset = 0;
+        double wprod = 1.0;
+        for (int d = 0; d < 3; ++d) {
+            offset += idx[d][tap[d]] * strides[d];
+            wprod *= w[d][tap[d]];
+        }
+        double cv = coef[offset];
+        v += wprod * cv;
+        for (int q = 0; q < 3; ++q) {
+            double gprod = 1.0;
+            for (int d = 0; d < 3; ++d) gprod *= d == q ? dw[d][tap[d]] : w[d][tap[d]];
+            g[q] += gprod * cv;
+        }
+    }
+    *value = v;
+    grad[0] = g[0]; grad[1] = g[1]; grad[2] = g[2];
+    return 1;
+}
+
+static int moving_value_and_index_gradient(const orc_problem *pb, const double c[3],
+                                           double *value, double gi[3]) {
+    if (pb->mcoef) return orc_bspline_value_and_gradient(pb->mcoef, pb->msize, c, value, gi);
+    return linear_value_and_gradient(pb, c, value, gi);
+}
+
 /* metric.rs:848-864 */
 int orc_moving_value_and_physical_gradient(const orc_problem *pb, const double p[3],
                                            double *value, double grad[3]) {
     double c[3], gi[3];
     continuous_index(pb, p, c);
     if (!mask_allows(pb, c)) return 0;
-    if (!linear_value_and_gradient(pb, c, value, gi)) return 0;
+    if (!moving_value_and_index_gradient(pb, c, value, gi)) return 0;
     for (int d = 0; d < 3; ++d) {
         double acc = -0.0;
         for (int j = 0; j < 3; ++j) acc += gi[j] * pb->mp2i[j * 3 + d];
@@ -696,6 +811,10 @@ int orc_moving_value_at(const orc_problem *pb, const double p[3], double *value)
     double c[3];
     continuous_index(pb, p, c);
     if (!mask_allows(pb, c)) return 0;
+    if (pb->mcoef) {
+        double gi[3];
+        return orc_bspline_value_and_gradient(pb->mcoef, pb->msize, c, value, gi);
+    }
     return linear_at(pb, c, value);
 }
 
@@ -1056,6 +1175,95 @@ double orc_ms_evaluate(const orc_problem *pb, const orc_transform *t, double *de
         }
         valid += 1;
     }
+    *valid_out = valid;
+    if (valid == 0) {
+        for (int k = 0; k < np; ++k) deriv[k] = 0.0;
+        return DBL_MAX;
+    }
+    double inv = 1.0 / (double)valid;
+    for (int k = 0; k < np; ++k) deriv[k] = deriv[k] * inv;
+    return value_sum * inv;
+}
+
+/* The dense branch of metric.rs:1065-1179 as the reference runs it:
+ * map_rows_fold_in_order (core/parallel.rs:720-818) -- every sample's row
+ * [diff^2, 2 diff g_k] computed by `threads` threads, chunk by chunk of 2^16
+ * rows, then added on one thread in sample order. The sum sequence is the
+ * serial loop's, so the result is orc_ms_evaluate's bit for bit; this only
+ * times the reference's threading. Sparse transforms run the serial loop, as
+ * in the reference. */
+typedef struct {
+    const orc_problem *pb;
+    const orc_transform *t;
+    size_t start, r0, r1;
+    int width;
+    double *rows;
+    unsigned char *ok;
+} ms_rows_job;
+
+static void *ms_rows_thread(void *arg) {
+    ms_rows_job *j = (ms_rows_job *)arg;
+    const orc_problem *pb = j->pb;
+    const orc_transform *t = j->t;
+    int np = t->nparams;
+    double jac[36];
+    for (size_t r = j->r0; r < j->r1; ++r) {
+        double *row = j->rows + r * (size_t)j->width;
+        size_t flat = selection_flat(pb, j->start + r);
+        j->ok[r] = 0;
+        if (!selection_allowed(pb, flat)) continue;
+        double fp[3], mp[3], mv, grad[3];
+        write_point(pb, flat, fp);
+        orc_transform_point(t, fp, mp);
+        if (!orc_moving_value_and_physical_gradient(pb, mp, &mv, grad)) continue;
+        double diff = mv - (double)pb->fvals[flat];
+        row[0] = diff * diff;
+        orc_transform_jacobian(t, fp, jac);
+        for (int k = 0; k < np; ++k) {
+            double g = 0.0;
+            for (int d = 0; d < 3; ++d) g += grad[d] * jac[d * np + k];
+            row[1 + k] = 2.0 * diff * g;
+        }
+        j->ok[r] = 1;
+    }
+    return NULL;
+}
+
+double orc_ms_evaluate_threads(const orc_problem *pb, const orc_transform *t, double *deriv,
+                               size_t *valid_out, int threads) {
+    if (threads <= 1 || t->kind == ORC_BSPLINE) return orc_ms_evaluate(pb, t, deriv, valid_out);
+    int nt = threads > 256 ? 256 : threads;
+    int np = t->nparams, width = 1 + np;
+    size_t n = selection_len(pb);
+    size_t chunk = (size_t)1 << 16;
+    if (chunk > n) chunk = n;
+    double *rows = (double *)malloc((chunk ? chunk : 1) * (size_t)width * sizeof(double));
+    unsigned char *ok = (unsigned char *)malloc(chunk ? chunk : 1);
+    double value_sum = 0.0;
+    size_t valid = 0;
+    for (int k = 0; k < np; ++k) deriv[k] = 0.0;
+    pthread_t tid[256];
+    ms_rows_job jobs[256];
+    for (size_t start = 0; start < n; start += chunk) {
+        size_t count = n - start < chunk ? n - start : chunk;
+        for (int q = 0; q < nt; ++q) {
+            jobs[q].pb = pb; jobs[q].t = t; jobs[q].start = start; jobs[q].width = width;
+            jobs[q].rows = rows; jobs[q].ok = ok;
+            jobs[q].r0 = count * (size_t)q / (size_t)nt;
+            jobs[q].r1 = count * (size_t)(q + 1) / (size_t)nt;
+            pthread_create(&tid[q], NULL, ms_rows_thread, &jobs[q]);
+        }
+        for (int q = 0; q < nt; ++q) pthread_join(tid[q], NULL);
+        for (size_t r = 0; r < count; ++r) {
+            if (!ok[r]) continue;
+            const double *row = rows + r * (size_t)width;
+            value_sum += row[0];
+            for (int k = 0; k < np; ++k) deriv[k] += row[1 + k];
+            valid += 1;
+        }
+    }
+    free(rows);
+    free(ok);
     *valid_out = valid;
     if (valid == 0) {
         for (int k = 0; k < np; ++k) deriv[k] = 0.0;

@@ -126,6 +126,8 @@ typedef struct {
     double mp2i[9];          /* physical -> index, inverse of the composed matrix */
     const float *mvals;
     const uint8_t *mmask;    /* optional moving mask on the moving grid */
+    const double *mcoef;     /* cubic B-spline coefficients of mvals (interpolator.rs:170-190),
+                                or NULL = linear interpolation */
 } orc_problem;
 
 /* Sampling (src/registration/metric.rs:101-122, 419-437). strategy: 0 None,
@@ -137,6 +139,17 @@ size_t orc_draw_samples(size_t voxels, int strategy, double percentage,
 uint64_t orc_splitmix64_next(uint64_t *state);
 
 /* (min, max) over the sample set / masked moving volume; returns 0 if empty. */
+/* bspline_coefficients (src/transform/interpolator.rs:170-190, 236-270): the
+ * cubic B-spline prefilter (Unser, mirror boundary) of a float volume, axis by
+ * axis (axes of one voxel skipped), into `out` (FP64, same layout). */
+void orc_bspline_coefficients(const float *in, const size_t size[3], double *out);
+/* compiler-rt's __powidf2 (the routine Rust's f64::powi lowers to): binary
+ * exponentiation, one rounding per multiply. */
+double orc_powi(double a, int b);
+/* bspline_value_and_gradient (interpolator.rs:273-350) at continuous index c:
+ * value and index-space gradient from the coefficients; 0 if outside. */
+int orc_bspline_value_and_gradient(const double *coef, const size_t size[3], const double c[3],
+                                   double *value, double grad[3]);
 int orc_fixed_value_range(const orc_problem *pb, double out[2]);
 int orc_moving_value_range(const orc_problem *pb, double out[2]);
 
@@ -194,6 +207,10 @@ void orc_mattes_moments(const orc_problem *pb, const orc_mattes_geom *g,
 /* ---- mean squares (src/registration/metric.rs:1065-1223) ---- */
 double orc_ms_evaluate(const orc_problem *pb, const orc_transform *t,
                        double *deriv, size_t *valid);
+/* orc_ms_evaluate computed the way the reference threads it: rows in parallel
+ * over `threads` threads, added in sample order on one (the same bits). */
+double orc_ms_evaluate_threads(const orc_problem *pb, const orc_transform *t, double *deriv,
+                               size_t *valid, int threads);
 double orc_ms_value(const orc_problem *pb, const orc_transform *t, size_t *valid);
 
 /* ---- pyramid placement (src/transform/resample.rs:226-290,

@@ -1,4 +1,4 @@
-"""Test-side restatement of the reference's host registration loop for one level,
+"""Host restatement of the reference's registration loop for one level,
 driving the CPU oracle's metric: RegularStepGradientDescent with physical-shift
 scales and a once-estimated learning rate
 (crates/sitk/src/registration/optimizer.rs:353-450,
@@ -8,7 +8,9 @@ descent with the golden-section line search (optimizer.rs:458-785). The two
 optimizers are generic over the objective (rsgd, gd_line_search) so that the
 reference's own optimizer fixtures pin them (tests/test_optimizer_reference.py).
 
-Test infrastructure only: checks the device driver in sitk-rs_amd/csrc/registration.hip.
+Test infrastructure only, like the rest of the oracle: it checks the device
+driver in sitk-rs_amd/csrc/registration.hip, and the benchmark's CPU-baseline
+leg times it; nothing in the product path imports it.
 """
 
 import math

@@ -321,6 +321,10 @@ BatchEngine *batch_create(Metric *const *metrics, size_t n) {
         for (size_t i = 0; i < n; ++i) {
             Metric *m = metrics[i];
             if (!m) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null metric in batch");
+            if (metric_uses_bspline_interpolation(*m))
+                fail(SITKHIP_ERR_INVALID_ARGUMENT,
+                     "batched evaluation interpolates linearly; metric " + std::to_string(i) +
+                         " uses the cubic B-spline interpolator");
             if (m->precision != b->precision)
                 fail(SITKHIP_ERR_INVALID_ARGUMENT, "every metric of a batch has the same precision");
             if (m->SS.n >= (1ll << 31))

@@ -485,6 +485,9 @@ BSplineMetric *bspline_create(Metric &m, const size_t gs[3], const double origin
                               const double spacing[3], const double direction[9]) {
     if (m.kind != SITKHIP_METRIC_MATTES || !m.bins_set)
         fail(SITKHIP_ERR_HISTOGRAM_SHAPE, "a B-spline metric needs a Mattes metric with bins set");
+    if (metric_uses_bspline_interpolation(m))
+        fail(SITKHIP_ERR_INVALID_ARGUMENT,
+             "the B-spline transform metric interpolates the moving image linearly");
     for (int d = 0; d < 3; ++d) {
         if (gs[d] < 4 || gs[d] > 4096)
             fail(SITKHIP_ERR_INVALID_ARGUMENT, "control grid size must be 4..4096 per axis");

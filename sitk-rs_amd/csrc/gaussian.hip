@@ -411,57 +411,73 @@ __global__ void copy_f32(const float *__restrict__ s, float *__restrict__ d, siz
         d[i] = s[i];
 }
 
-// recursive_gaussian (pyramid.rs:217-293 / recursive_gaussian.rs:171-204).
-Image *recursive_gaussian(const Image &in, const double sigma[3], bool synchronous) {
+// Refusals of recursive_gaussian (recursive_gaussian.rs:171-204): sigma >= 0
+// on every axis, and a smoothed axis (sigma > 0) needs at least 4 voxels.
+// Returns the smoothed axes in order.
+static int smoothed_axes(const sitkhip_geometry &g, const double sigma[3], int axes[3]) {
     for (int d = 0; d < 3; ++d) {
         if (!(sigma[d] >= 0.0))
             fail(SITKHIP_ERR_INVALID_ARGUMENT, "recursive_gaussian: sigma must be >= 0");
     }
-    int axes[3];
     int na = 0;
     for (int d = 0; d < 3; ++d) {
         if (sigma[d] <= 0.0) continue;
-        if (in.geom.size[d] < 4)
+        if (g.size[d] < 4)
             fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY,
                  "recursive_gaussian: axis " + std::to_string(d) + " has " +
-                     std::to_string(in.geom.size[d]) + " voxels; the recursion needs at least 4");
+                     std::to_string(g.size[d]) + " voxels; the recursion needs at least 4");
         axes[na++] = d;
     }
+    return na;
+}
+
+// Enqueues the smoothing of `src` (geometry g) into `dst` on the library
+// stream. A and B are FP64 volumes of g's voxel count: A is used whenever an
+// axis is smoothed, B when two or more are.
+static void enqueue_smoothing(const float *src, float *dst, const sitkhip_geometry &g,
+                              const double sigma[3], const int axes[3], int na, double *A,
+                              double *B) {
+    const size_t n = geometry_len(g);
+    if (na == 0) {
+        LaunchScope scope("copy_f32");
+        hipLaunchKernelGGL(copy_f32, dim3(grid_for(n, 256, 4096)), dim3(256), 0,
+                           backend().stream, src, dst, n);
+        check_launch("copy_f32");
+        return;
+    }
+    if (na == 1) {
+        IirCoef k = zero_order_coefficients(sigma[axes[0]] / g.spacing[axes[0]]);
+        launch_axis<float, float>(src, A, dst, g, axes[0], k);
+        return;
+    }
+    // first smoothed axis: f32 -> A (causal outputs land in A itself)
+    IirCoef k0 = zero_order_coefficients(sigma[axes[0]] / g.spacing[axes[0]]);
+    launch_axis<float, double>(src, A, A, g, axes[0], k0);
+    double *cur = A, *other = B;
+    for (int a = 1; a < na - 1; ++a) {
+        IirCoef k = zero_order_coefficients(sigma[axes[a]] / g.spacing[axes[a]]);
+        launch_axis<double, double>(cur, other, other, g, axes[a], k);
+        double *t = cur; cur = other; other = t;
+    }
+    IirCoef kl = zero_order_coefficients(sigma[axes[na - 1]] / g.spacing[axes[na - 1]]);
+    launch_axis<double, float>(cur, other, dst, g, axes[na - 1], kl);
+}
+
+// recursive_gaussian (pyramid.rs:217-293 / recursive_gaussian.rs:171-204).
+Image *recursive_gaussian(const Image &in, const double sigma[3], bool synchronous) {
+    int axes[3];
+    const int na = smoothed_axes(in.geom, sigma, axes);
     const sitkhip_geometry &g = in.geom;
     size_t n = in.len();
     Image *out = image_like(g);
     try {
-        if (na == 0) {
-            LaunchScope scope("copy_f32");
-            hipLaunchKernelGGL(copy_f32, dim3(grid_for(n, 256, 4096)), dim3(256), 0,
-                               backend().stream, in.buf.ptr, out->buf.ptr, n);
-            check_launch("copy_f32");
-            if (synchronous) sync();
-            return out;
-        }
         // FP64 intermediates of this call, from the caching allocator: freed
         // at return while the launches are still queued, a block is reused
         // only by later work on the same stream (or held back while work is
         // spread over several streams), so calls on different streams never
         // share one.
-        DeviceBuffer<double> A(n), B(na >= 2 ? n : 1);
-        const float *src32 = in.buf.ptr;
-        if (na == 1) {
-            IirCoef k = zero_order_coefficients(sigma[axes[0]] / g.spacing[axes[0]]);
-            launch_axis<float, float>(src32, A.ptr, out->buf.ptr, g, axes[0], k);
-        } else {
-            // first smoothed axis: f32 -> A (causal outputs land in A itself)
-            IirCoef k0 = zero_order_coefficients(sigma[axes[0]] / g.spacing[axes[0]]);
-            launch_axis<float, double>(src32, A.ptr, A.ptr, g, axes[0], k0);
-            double *cur = A.ptr, *other = B.ptr;
-            for (int a = 1; a < na - 1; ++a) {
-                IirCoef k = zero_order_coefficients(sigma[axes[a]] / g.spacing[axes[a]]);
-                launch_axis<double, double>(cur, other, other, g, axes[a], k);
-                double *t = cur; cur = other; other = t;
-            }
-            IirCoef kl = zero_order_coefficients(sigma[axes[na - 1]] / g.spacing[axes[na - 1]]);
-            launch_axis<double, float>(cur, other, out->buf.ptr, g, axes[na - 1], kl);
-        }
+        DeviceBuffer<double> A(na >= 1 ? n : 1), B(na >= 2 ? n : 1);
+        enqueue_smoothing(in.buf.ptr, out->buf.ptr, g, sigma, axes, na, A.ptr, B.ptr);
         if (synchronous) sync();
     } catch (...) {
         delete out;
@@ -469,7 +485,6 @@ Image *recursive_gaussian(const Image &in, const double sigma[3], bool synchrono
     }
     return out;
 }
-
 
 // ----------------------------------------------------------- batched volumes
 template <typename SrcT, typename DstT>

@@ -72,7 +72,7 @@ namespace sitkhip {
 #endif
 
 // -------------------------------------------------------------- mean squares
-template <typename R, bool MOMENTS, typename Map>
+template <typename R, bool MOMENTS, typename Map, bool BSP = false>
 __global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Map S, IndexMap X,
                                                   const ChainConst *__restrict__ C, SampleSet SS,
                                                   const float *__restrict__ fval,
@@ -86,13 +86,13 @@ __global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Map 
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    ms_range<R, MOMENTS>(F, M, V, X, C, SS, fval, begin, end, acc);
+    ms_range<R, MOMENTS, std::decay_t<decltype(V)>, BSP>(F, M, V, X, C, SS, fval, begin, end, acc);
     block_reduce_store<NS>(acc, partials, logical);
 }
 
 // Pass 1: joint histogram (fixed point) + fixed-marginal counts; with REC
 // also the per-sample records for the derivative pass that follows.
-template <typename R, bool LDS, bool REC, typename Map>
+template <typename R, bool LDS, bool REC, typename Map, bool BSP = false>
 __global__ __launch_bounds__(kBlock) SITKHIP_HIST_OCC void mattes_hist(FixedGrid F, MovingVol M, Map S,
                                                       IndexMap X, const ChainConst *__restrict__ C,
                                                       SampleSet SS, const uint8_t *__restrict__ fbin,
@@ -125,13 +125,14 @@ __global__ __launch_bounds__(kBlock) SITKHIP_HIST_OCC void mattes_hist(FixedGrid
     long long first, step, end;
     xcd_walk(SS.n, first, step, end);
     uint32_t *dq = DEFER ? defer + static_cast<size_t>(blockIdx.x) * defer_cap : nullptr;
-    hist_range<R, REC, std::decay_t<decltype(V)>, DEFER>(F, M, V, X, C, SS, fbin, P, joint, marg, rec, first, end, step, dq,
+    hist_range<R, REC, std::decay_t<decltype(V)>, DEFER, BSP>(F, M, V, X, C, SS, fbin, P, joint, marg, rec, first, end, step, dq,
                                          &s_defer_n);
     if (DEFER) {
         __syncthreads();
         const unsigned nq = s_defer_n;
         for (unsigned i = threadIdx.x; i < nq; i += kBlock)
-            deferred_sample<R, REC>(F, M, V, C, SS, fbin, P, joint, marg, rec, dq[i]);
+            deferred_sample<R, REC, std::decay_t<decltype(V)>, BSP>(F, M, V, C, SS, fbin, P, joint, marg, rec,
+                                                                     dq[i]);
     }
     if (LDS) {
         __syncthreads();
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(kTail) void mattes_tail_split(unsigned long long *g
 }
 
 // Pass 2: the 12 pRatio-weighted moments, folded by the last workgroup.
-template <typename R, bool LDS, typename Map>
+template <typename R, bool LDS, typename Map, bool BSP = false>
 __global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M, Map S,
                                                        IndexMap X, const ChainConst *__restrict__ C,
                                                        SampleSet SS, const uint8_t *__restrict__ fbin,
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void mattes_deriv(FixedGrid F, MovingVol M,
     long long begin, end;
     unsigned logical;
     block_slice(SS.n, begin, end, logical);
-    deriv_range<R>(F, M, V, X, C, SS, fbin, P, pr, begin, end, acc);
+    deriv_range<R, std::decay_t<decltype(V)>, BSP>(F, M, V, X, C, SS, fbin, P, pr, begin, end, acc);
     block_reduce_store<12>(acc, partials, logical);
 }
 
@@ -968,7 +969,15 @@ static void launch_hist(Metric &m, const Stages &S, bool tail, bool clear = true
                                    m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, m.joint.ptr,
                                    m.marg.ptr, rec, dq, cap);
             };
-            if (lds) {
+            if (m.bspline) {
+                if (lds) {
+                    if (rec) go(mattes_hist<R, true, true, Map, true>, shm);
+                    else go(mattes_hist<R, true, false, Map, true>, shm);
+                } else {
+                    if (rec) go(mattes_hist<R, false, true, Map, true>, 0);
+                    else go(mattes_hist<R, false, false, Map, true>, 0);
+                }
+            } else if (lds) {
                 if (rec) go(mattes_hist<R, true, true, Map>, shm);
                 else go(mattes_hist<R, true, false, Map>, shm);
             } else {
@@ -1029,14 +1038,19 @@ static void launch_deriv(Metric &m, const Stages &S, const R *pratio, bool clear
         const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
         with_map(m, S, [&](const auto &map) {
             using Map = std::decay_t<decltype(map)>;
-            if (lds)
-                hipLaunchKernelGGL((mattes_deriv<R, true, Map>), dim3(kGrid), dim3(kBlock), shm,
-                                   backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio,
+            auto go = [&](auto kernel, size_t bytes) {
+                hipLaunchKernelGGL(kernel, dim3(kGrid), dim3(kBlock), bytes, backend().stream, m.F,
+                                   m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio,
                                    m.partials.ptr, clr, clr_n, clr2, clr2_n);
-            else
-                hipLaunchKernelGGL((mattes_deriv<R, false, Map>), dim3(kGrid), dim3(kBlock), 0,
-                                   backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio,
-                                   m.partials.ptr, clr, clr_n, clr2, clr2_n);
+            };
+            if (m.bspline) {
+                if (lds) go(mattes_deriv<R, true, Map, true>, shm);
+                else go(mattes_deriv<R, false, Map, true>, 0);
+            } else if (lds) {
+                go(mattes_deriv<R, true, Map>, shm);
+            } else {
+                go(mattes_deriv<R, false, Map>, 0);
+            }
         });
         check_launch("mattes_deriv");
     }
@@ -1166,8 +1180,14 @@ static void launch_ms(Metric &m, const Stages &S) {
         const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
         with_map(m, S, [&](const auto &map) {
             using Map = std::decay_t<decltype(map)>;
-            hipLaunchKernelGGL((ms_pass<R, MOM, Map>), dim3(kGrid), dim3(kBlock), 0, backend().stream,
-                               m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fval.ptr, m.partials.ptr);
+            if (m.bspline)
+                hipLaunchKernelGGL((ms_pass<R, MOM, Map, true>), dim3(kGrid), dim3(kBlock), 0,
+                                   backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fval.ptr,
+                                   m.partials.ptr);
+            else
+                hipLaunchKernelGGL((ms_pass<R, MOM, Map>), dim3(kGrid), dim3(kBlock), 0,
+                                   backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fval.ptr,
+                                   m.partials.ptr);
         });
         check_launch("ms_pass");
     }
@@ -1195,6 +1215,132 @@ void ms_value(Metric &m, const double *stages, int nstage, double *sq, size_t *v
     if (sq) *sq = m.res->moments[0];
     if (valid) *valid = static_cast<size_t>(m.res->moments[1]);
 }
+
+// ------------------------------------------- cubic B-spline interpolation
+
+// compiler-rt's __powidf2, the routine Rust's f64::powi lowers to: binary
+// exponentiation, one rounding per multiply (b >= 0 here).
+__device__ __forceinline__ double powi_nonneg(double a, int b) {
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r *= a;
+        b >>= 1;
+        if (b == 0) break;
+        a *= a;
+    }
+    return r;
+}
+
+__global__ void widen_f32(const float *__restrict__ s, double *__restrict__ d, size_t n) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        d[i] = static_cast<double>(s[i]);
+}
+
+__global__ void narrow_f64(const double *__restrict__ s, float *__restrict__ d, size_t n) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        d[i] = static_cast<float>(s[i]);
+}
+
+// bspline3_filter_1d (interpolator.rs:236-270) on every line of one axis, in
+// place, one line (n >= 2 voxels, `stride` apart) per lane: the gain (applied
+// as each value is first read: the same one rounding), the mirror-boundary
+// causal initial value by the full sum, the causal recursion, the
+// anti-causal initial value and recursion. FP64, one rounding per operation
+// (the file is compiled without contraction). Lanes take lines in the order
+// of the `inner` axis (x for the axes 1 and 2: coalesced).
+__global__ void bspline_prefilter(double *vol, int n, long long stride, long long nlines,
+                                  long long inner, long long inner_stride, long long outer_stride,
+                                  double z) {
+    const long long line = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (line >= nlines) return;
+    double *p = vol + (line % inner) * inner_stride + (line / inner) * outer_stride;
+    const double gain = (1.0 - z) * (1.0 - 1.0 / z);
+    const double iz = 1.0 / z;
+    double zn = z;
+    double z2n = powi_nonneg(z, n - 1);
+    double sum = p[0] * gain + z2n * (p[(n - 1) * stride] * gain);
+    z2n = z2n * z2n * iz;
+    for (int k = 1; k + 1 < n; ++k) {
+        sum += (zn + z2n) * (p[k * stride] * gain);
+        zn *= z;
+        z2n *= iz;
+    }
+    double prev = sum / (1.0 - zn * zn);
+    p[0] = prev;
+    for (int k = 1; k < n; ++k) {
+        prev = p[k * stride] * gain + z * prev;
+        p[k * stride] = prev;
+    }
+    double next = (z / (z * z - 1.0)) * (z * p[(n - 2) * stride] + prev);
+    p[(n - 1) * stride] = next;
+    for (int k = n - 2; k >= 0; --k) {
+        next = z * (next - p[k * stride]);
+        p[k * stride] = next;
+    }
+}
+
+// MovingImage::from_image_with_interpolator (registration/metric.rs:720-750)
+// for the resident metric: the cubic B-spline coefficients of its moving
+// volume (bspline_coefficients, interpolator.rs:170-190), FP64 on the
+// device, then narrowed to float for the FP32 path. Linear drops them.
+void metric_set_interpolator(Metric &m, int interpolator) {
+    if (interpolator == SITKHIP_INTERP_LINEAR) {
+        m.bspline = false;
+        m.Mv.coef = nullptr;
+        m.Mv.coef64 = nullptr;
+        m.coef32.release();
+        m.coef64.release();
+        return;
+    }
+    if (interpolator != SITKHIP_INTERP_BSPLINE)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "unknown interpolator " + std::to_string(interpolator));
+    for (int d = 0; d < 3; ++d)
+        if (m.Mv.size[d] == 2)
+            fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY,
+                 "cubic B-spline interpolation: moving axis " + std::to_string(d) +
+                     " has 2 voxels, where the mirror fold of the taps leaves the volume");
+    const size_t n = static_cast<size_t>(m.Mv.size[0]) * m.Mv.size[1] * m.Mv.size[2];
+    DeviceBuffer<double> c64(n);
+    {
+        LaunchScope scope("bspline_prefilter");
+        hipLaunchKernelGGL(widen_f32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, backend().stream,
+                           m.Mv.vol, c64.ptr, n);
+        check_launch("widen_f32");
+        const double z = std::sqrt(3.0) - 2.0;
+        const long long st[3] = {1, m.Mv.stride1, m.Mv.stride2};
+        for (int a = 0; a < 3; ++a) {
+            const int len = m.Mv.size[a];
+            if (len == 1) continue;
+            const int o1 = a == 0 ? 1 : 0, o2 = a == 2 ? 1 : 2;
+            const long long inner = m.Mv.size[o1];
+            const long long nlines = inner * m.Mv.size[o2];
+            const unsigned blocks = static_cast<unsigned>((nlines + 255) / 256);
+            hipLaunchKernelGGL(bspline_prefilter, dim3(blocks), dim3(256), 0, backend().stream,
+                               c64.ptr, len, st[a], nlines, inner, st[o1], st[o2], z);
+            check_launch("bspline_prefilter");
+        }
+    }
+    if (m.precision == SITKHIP_PRECISION_F64) {
+        m.coef32.release();
+        m.coef64 = std::move(c64);
+        m.Mv.coef64 = m.coef64.ptr;
+        m.Mv.coef = nullptr;
+    } else {
+        m.coef32.alloc(n);
+        hipLaunchKernelGGL(narrow_f64, dim3(grid_for(n, 256, 4096)), dim3(256), 0, backend().stream,
+                           c64.ptr, m.coef32.ptr, n);
+        check_launch("narrow_f64");
+        m.coef64.release();
+        m.Mv.coef = m.coef32.ptr;
+        m.Mv.coef64 = nullptr;
+    }
+    sync();
+    m.bspline = true;
+}
+
+bool metric_uses_bspline_interpolation(const Metric &m) { return m.bspline; }
 
 }  // namespace sitkhip
 
@@ -1245,6 +1391,34 @@ static Metric &metric_ref(sitkhip_metric h) {
     if (!m) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null metric");
     return *m;
 }
+
+int sitkhip_metric_set_interpolator(sitkhip_metric h, int interpolator) {
+    return run_guarded([&] { metric_set_interpolator(metric_ref(h), interpolator); });
+}
+
+int sitkhip_metric_bspline_coefficients(sitkhip_metric h, double *out, size_t len) {
+    return run_guarded([&] {
+        Metric &m = metric_ref(h);
+        if (!m.bspline) fail(SITKHIP_ERR_INVALID_ARGUMENT, "the metric interpolates linearly");
+        const size_t n = static_cast<size_t>(m.Mv.size[0]) * m.Mv.size[1] * m.Mv.size[2];
+        if (!out || len != n)
+            fail(SITKHIP_ERR_INVALID_ARGUMENT, "output length " + std::to_string(len) +
+                                                   " is not the moving voxel count " +
+                                                   std::to_string(n));
+        if (m.Mv.coef64) {
+            m.coef64.download(out, n);
+            sync();
+        } else {
+            std::vector<float> tmp(n);
+            m.coef32.download(tmp.data(), n);
+            sync();
+            for (size_t i = 0; i < n; ++i) out[i] = static_cast<double>(tmp[i]);
+        }
+    });
+}
+
+
+
 
 int sitkhip_metric_sample_count(sitkhip_metric h, size_t *out) {
     return run_guarded([&] {

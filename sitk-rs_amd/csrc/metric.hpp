@@ -51,6 +51,10 @@ Metric *metric_create_host(int kind, size_t n, const double *fixed_values,
                            int precision);
 // Whether the metric's samples are explicit points (no grid index).
 bool metric_has_explicit_points(const Metric &m);
+// The moving image's interpolator (SITKHIP_INTERP_*): B-spline computes the
+// cubic coefficients of the moving volume on the device.
+void metric_set_interpolator(Metric &m, int interpolator);
+bool metric_uses_bspline_interpolation(const Metric &m);
 void metric_finish_ranges(Metric &m);
 void metric_destroy(Metric *m);
 size_t metric_sample_count(const Metric &m);

@@ -198,17 +198,21 @@ struct Parzen<double> {
 
 template <>
 struct Parzen<float> {
-    // term = mv/binSize - normalizedMin = (mv - trueMin)/binSize + padding: the
-    // second form has no cancellation, so it is accurate in FP32.
+    // term = mv/binSize - normalizedMin = (mv - trueMin)/binSize + padding,
+    // in FP64 (mv - trueMin is exact there), and each tap argument rounded to
+    // float once. A float term carries the relative error of the rounded
+    // 1/binSize into every sample alike -- a stretch of the moving bin axis
+    // that does not average out (2e-6 of the value at 2M samples where MI is
+    // small); rounding only the arguments leaves independent errors.
     __device__ __forceinline__ static int taps(const MattesParams &P, float mv, float arg[4],
                                                bool &mid) {
-        const float term = fmaf(mv - P.mmin_f, P.inv_mbs_f, 2.0f);
+        const double term = fma(static_cast<double>(mv) - P.mmin, P.inv_mbs, 2.0);
         int idx = static_cast<int>(term);
         mid = idx >= 2 && idx <= P.bins - 3;
         idx = idx < 2 ? 2 : (idx > P.bins - 3 ? P.bins - 3 : idx);
         const int start = idx - 1;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) arg[k] = static_cast<float>(start + k) - term;
+        for (int k = 0; k < 4; ++k) arg[k] = static_cast<float>(static_cast<double>(start + k) - term);
         return start;
     }
     __device__ __forceinline__ static bool in_range(const MattesParams &P, float mv) {
@@ -368,6 +372,7 @@ struct Batch {
 template <typename R>
 struct Located {
     R fr[kU][3];
+    int base[kU][3];  // the cell (the B-spline interpolant's taps)
     float v[kU][8];
     bool ok[kU];
     bool unsure[kU];  // DEFER: the composed map could not decide; not located
@@ -482,7 +487,7 @@ struct MapTraits<PointStageView> {
 // DEFER (FP32, affine maps): a sample the composed map cannot decide is not
 // located here but flagged `unsure`, for the caller to run the chain later
 // (deferred_sample), which keeps the chain's registers out of the loop.
-template <typename R, typename Map, bool DEFER = false>
+template <typename R, typename Map, bool DEFER = false, bool BSP = false>
 __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol &M,
                                              const Map &S, const IndexMap &X,
                                              const ChainConst *__restrict__ C, const Batch &b,
@@ -522,10 +527,16 @@ __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol
             cell[u].frac[0] = cell[u].frac[1] = cell[u].frac[2] = 0.0;
         }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) L.fr[u][d] = static_cast<R>(cell[u].frac[d]);
+        for (int d = 0; d < 3; ++d) {
+            L.fr[u][d] = static_cast<R>(cell[u].frac[d]);
+            L.base[u][d] = cell[u].base[d];
+        }
     }
+    // (the cubic interpolant reads its 64 taps when it is evaluated)
+    if constexpr (!BSP) {
 #pragma unroll
-    for (int u = 0; u < kU; ++u) corners(M, cell[u], L.v[u]);
+        for (int u = 0; u < kU; ++u) corners(M, cell[u], L.v[u]);
+    }
 }
 
 // ------------------------------------------------ per-range sample loops
@@ -537,7 +548,7 @@ __device__ __forceinline__ void locate_batch(const FixedGrid &F, const MovingVol
 // Mean squares: [sq, S0[3], S1[3][3], count] with diff = M - F
 // (mean_squares.rs:101-145; contraction registration/cuda.rs:381-412), or
 // [sq, count] without MOMENTS.
-template <typename R, bool MOMENTS, typename Map>
+template <typename R, bool MOMENTS, typename Map, bool BSP = false>
 __device__ __forceinline__ void ms_range(const FixedGrid &F, const MovingVol &M, const Map &S,
                                          const IndexMap &X, const ChainConst *__restrict__ C,
                                          const SampleSet &SS, const float *__restrict__ fval,
@@ -555,15 +566,15 @@ __device__ __forceinline__ void ms_range(const FixedGrid &F, const MovingVol &M,
             fv[u] = fval[sc];
         }
         Located<R> L;
-        locate_batch<R>(F, M, S, X, C, b, L);
+        locate_batch<R, Map, false, BSP>(F, M, S, X, C, b, L);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!L.ok[u]) continue;
             R mv, gi[3], g[3];
-            Trilinear<R>::template eval<MOMENTS>(L.fr[u], L.v[u], mv, gi);
+            Interp<R, BSP>::template eval<MOMENTS>(M, L.base[u], L.fr[u], L.v[u], mv, gi);
             const double diff = static_cast<double>(mv) - static_cast<double>(fv[u]);
             acc[0] = fma(diff, diff, acc[0]);
-            if (MOMENTS) {
+            if constexpr (MOMENTS) {
                 double x[3];
                 moment_point<R, MapTraits<Map>::points>(F, C, b.flat[u], x);
                 physical_gradient<R>(M, gi, g);
@@ -598,15 +609,17 @@ struct alignas(4 * sizeof(R)) SampleRec {
 // `joint` (64-bit fixed point), one count into marg[fixed bin], and with REC
 // the sample's record (the value is the same: the interpolation computes it
 // the same way with or without the gradient).
-template <typename R, bool REC>
+template <typename R, bool REC, bool BSP = false>
 __device__ __forceinline__ void hist_sample(const MattesParams &P, unsigned long long *joint,
                                             unsigned long long *marg, SampleRec<R> *__restrict__ rec,
                                             long long s, bool ok, int fb, const R fr[3],
-                                            const float v[8]) {
+                                            const float v[8], const MovingVol *M = nullptr,
+                                            const int *base = nullptr) {
     R mv = R(0), gi[3] = {R(0), R(0), R(0)};
     bool use = ok;
     if (use) {
-        Trilinear<R>::template eval<REC>(fr, v, mv, gi);
+        if constexpr (BSP) CubicBSpline<R>::template eval<REC>(*M, base, fr, mv, gi);
+        else Trilinear<R>::template eval<REC>(fr, v, mv, gi);
         use = Parzen<R>::in_range(P, mv);
     }
     if (REC) {
@@ -632,7 +645,7 @@ __device__ __forceinline__ void hist_sample(const MattesParams &P, unsigned long
 // affine maps): a sample the composed index map cannot decide is appended to
 // the workgroup's list `defer` (count in *defer_n, LDS) instead, for
 // deferred_sample after the loop.
-template <typename R, bool REC, typename Map, bool DEFER = false>
+template <typename R, bool REC, typename Map, bool DEFER = false, bool BSP = false>
 __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &M, const Map &S,
                                            const IndexMap &X, const ChainConst *__restrict__ C,
                                            const SampleSet &SS, const uint8_t *__restrict__ fbin,
@@ -653,7 +666,7 @@ __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &
             b.live[u] = in && b.fb[u] != kNoBin;
         }
         Located<R> L;
-        locate_batch<R, Map, DEFER>(F, M, S, X, C, b, L);
+        locate_batch<R, Map, DEFER, BSP>(F, M, S, X, C, b, L);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const long long s = s0 + static_cast<long long>(u) * step;
@@ -662,7 +675,8 @@ __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &
                 defer[atomicAdd(defer_n, 1u)] = static_cast<uint32_t>(s);
                 continue;
             }
-            hist_sample<R, REC>(P, joint, marg, rec, s, L.ok[u], b.fb[u], L.fr[u], L.v[u]);
+            hist_sample<R, REC, BSP>(P, joint, marg, rec, s, L.ok[u], b.fb[u], L.fr[u], L.v[u], &M,
+                                     L.base[u]);
         }
     }
 }
@@ -670,7 +684,7 @@ __device__ __forceinline__ void hist_range(const FixedGrid &F, const MovingVol &
 // A deferred sample of the value pass: the reference's FP64 chain (fixed
 // point, point map, continuous index), then the same accumulation. Runs after
 // the loop, where the loop's registers are free.
-template <typename R, bool REC, typename Map>
+template <typename R, bool REC, typename Map, bool BSP = false>
 __device__ __forceinline__ void deferred_sample(const FixedGrid &F, const MovingVol &M, const Map &S,
                                                 const ChainConst *__restrict__ C,
                                                 const SampleSet &SS, const uint8_t *__restrict__ fbin,
@@ -691,14 +705,14 @@ __device__ __forceinline__ void deferred_sample(const FixedGrid &F, const Moving
 #pragma unroll
     for (int d = 0; d < 3; ++d) fr[d] = static_cast<R>(cell.frac[d]);
     float v[8];
-    corners(M, cell, v);
-    hist_sample<R, REC>(P, joint, marg, rec, s, ok, fb, fr, v);
+    if constexpr (!BSP) corners(M, cell, v);
+    hist_sample<R, REC, BSP>(P, joint, marg, rec, s, ok, fb, fr, v, &M, cell.base);
 }
 
 // Mattes derivative pass: per valid sample w = sum_taps pRatio * B3'(arg)
 // and the 12 moments A[d] += w dM_d, B[d][e] += w dM_d x_e
 // (src/cuda/ops/mattes.rs:75-91).
-template <typename R, typename Map>
+template <typename R, typename Map, bool BSP = false>
 __device__ __forceinline__ void deriv_range(const FixedGrid &F, const MovingVol &M, const Map &S,
                                             const IndexMap &X, const ChainConst *__restrict__ C,
                                            const SampleSet &SS, const uint8_t *__restrict__ fbin,
@@ -717,12 +731,12 @@ __device__ __forceinline__ void deriv_range(const FixedGrid &F, const MovingVol 
             b.live[u] = in && b.fb[u] != kNoBin;
         }
         Located<R> L;
-        locate_batch<R>(F, M, S, X, C, b, L);
+        locate_batch<R, Map, false, BSP>(F, M, S, X, C, b, L);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!L.ok[u]) continue;
             R mv, gi[3], g[3];
-            Trilinear<R>::template eval<true>(L.fr[u], L.v[u], mv, gi);
+            Interp<R, BSP>::template eval<true>(M, L.base[u], L.fr[u], L.v[u], mv, gi);
             if (!Parzen<R>::in_range(P, mv)) continue;
             physical_gradient<R>(M, gi, g);
             R arg[4], dv[4];
@@ -1098,6 +1112,11 @@ struct Metric {
     DeviceBuffer<unsigned long long> tail_part;  // the split tail's partial value sums + flag
     DeviceBuffer<unsigned char> recs;            // per-sample records of the value pass
     DeviceBuffer<uint32_t> defer;                // the value pass's deferred-sample lists
+    // Cubic B-spline interpolation of the moving image (metric_set_interpolator):
+    // the coefficients Mv.coef / Mv.coef64 point into
+    bool bspline = false;
+    DeviceBuffer<float> coef32;
+    DeviceBuffer<double> coef64;
     unsigned long long seq = 0;                  // evaluations issued (see HostResult::done)
     DeviceBuffer<double> range_part;             // value-range partials (creation only)
     bool ranges_pending = false;                 // ranges enqueued, not yet read on the host

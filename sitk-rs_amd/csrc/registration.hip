@@ -366,6 +366,8 @@ static void check_config(const sitkhip_registration_config &cfg, int kind) {
     if (cfg.optimizer != SITKHIP_OPTIMIZER_REGULAR_STEP && cfg.optimizer != SITKHIP_OPTIMIZER_LINE_SEARCH)
         fail(SITKHIP_ERR_INVALID_ARGUMENT, "unsupported optimizer");
     if (cfg.sampling < 0 || cfg.sampling > 2) fail(SITKHIP_ERR_INVALID_ARGUMENT, "unknown sampling strategy");
+    if (cfg.interpolator != SITKHIP_INTERP_LINEAR && cfg.interpolator != SITKHIP_INTERP_BSPLINE)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "unknown interpolator");
     check_sampling_fraction(cfg.sampling, cfg.sampling_percentage, 1);
 }
 
@@ -477,6 +479,7 @@ static LevelInputs prepare_level_begin(const Image &fixed, const Image &moving,
 // histogram geometry from them.
 static void prepare_level_end(LevelInputs &in, const sitkhip_registration_config &cfg) {
     metric_finish_ranges(*in.metric);
+    if (cfg.interpolator != SITKHIP_INTERP_LINEAR) metric_set_interpolator(*in.metric, cfg.interpolator);
     if (cfg.metric == SITKHIP_METRIC_MATTES) {
         double fr[2], mr[2];
         metric_value_ranges(*in.metric, fr, mr);
@@ -663,6 +666,9 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
         if (cfg[i].optimizer != SITKHIP_OPTIMIZER_REGULAR_STEP)
             fail(SITKHIP_ERR_INVALID_ARGUMENT,
                  "batched registration runs RegularStepGradientDescent (pair " + std::to_string(i) + ")");
+        if (cfg[i].interpolator != SITKHIP_INTERP_LINEAR)
+            fail(SITKHIP_ERR_INVALID_ARGUMENT,
+                 "batched registration interpolates linearly (pair " + std::to_string(i) + ")");
         if (cfg[i].precision != cfg[0].precision)
             fail(SITKHIP_ERR_INVALID_ARGUMENT, "every pair of a batch has the same precision");
         max_levels = std::max(max_levels, cfg[i].levels);

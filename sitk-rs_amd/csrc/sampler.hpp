@@ -73,6 +73,11 @@ struct MovingVol {
     double origin[3];
     double p2i[9];
     float p2i_f[9];       // p2i rounded to float (the FP32 path's gradient)
+    // Cubic B-spline coefficients of the volume (the moving image's cubic
+    // interpolator, interpolator.rs:170-190): float for the FP32 path, double
+    // for the FP64 path; nullptr with linear interpolation.
+    const float *coef;
+    const double *coef64;
 };
 
 // The FP64 geometry of the exact chain in device memory, for kernels whose
@@ -454,6 +459,138 @@ struct Trilinear<float> {
             // d/dc2
             g[2] = b1 - b0;
         }
+    }
+};
+
+// Cubic B-spline interpolation from the prefiltered coefficients
+// (bspline_value_and_gradient, interpolator.rs:284-350): on each axis the
+// taps base-1 .. base+2 of the cell base = floor(c), folded at the edges
+// (i < 0 -> -i, then i > n-1 -> 2(n-1) - i; an axis of one voxel folds every
+// tap onto 0; axes of two voxels are refused when the coefficients are made,
+// the fold leaves them), value weights the cubic B-spline at the in-cell
+// fraction, gradient weights the quadratic spline at fraction + 1/2.
+__device__ __forceinline__ int bspline_fold(int i, int n) {
+    if (n == 1) return 0;
+    if (i < 0) i = -i;
+    if (i > n - 1) i = 2 * (n - 1) - i;
+    return i;
+}
+
+// The reference's weight expressions, one rounding per operation.
+template <typename R>
+__device__ __forceinline__ void bspline_weights(R f, R w[4], R dw[4]) {
+    const R w3 = (R(1) / R(6)) * f * f * f;
+    const R w0 = R(1) / R(6) + R(0.5) * f * (f - R(1)) - w3;
+    const R w2 = f + w0 - R(2) * w3;
+    const R w1 = R(1) - w0 - w2 - w3;
+    w[0] = w0; w[1] = w1; w[2] = w2; w[3] = w3;
+    const R a = f - R(0.5);
+    const R d2 = R(0.75) - a * a;
+    const R d3 = R(0.5) * (a - d2 + R(1));
+    const R d1 = R(1) - d2 - d3;
+    dw[0] = -d1; dw[1] = d1 - d2; dw[2] = d2 - d3; dw[3] = d3;
+}
+
+template <typename R>
+struct CubicBSpline;
+
+// FP64: the reference's 64-tap loop (tap index of axis 0 fastest), weight
+// products ((w0 * w1) * w2), every operation rounded.
+template <>
+struct CubicBSpline<double> {
+    template <bool GRAD>
+    __device__ __forceinline__ static void eval(const MovingVol &M, const int base[3],
+                                                const double fr[3], double &value, double g[3]) {
+        double w[3][4], dw[3][4];
+        int idx[3][4];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            bspline_weights<double>(fr[d], w[d], dw[d]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) idx[d][k] = bspline_fold(base[d] - 1 + k, M.size[d]);
+        }
+        const long long s1 = M.stride1, s2 = M.stride2;
+        double v = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        for (int t2 = 0; t2 < 4; ++t2) {
+            for (int t1 = 0; t1 < 4; ++t1) {
+                const double *row = M.coef64 + idx[2][t2] * s2 + idx[1][t1] * s1;
+#pragma unroll
+                for (int t0 = 0; t0 < 4; ++t0) {
+                    const double c = row[idx[0][t0]];
+                    const double wp = (w[0][t0] * w[1][t1]) * w[2][t2];
+                    v += wp * c;
+                    if (GRAD) {
+                        g0 += ((dw[0][t0] * w[1][t1]) * w[2][t2]) * c;
+                        g1 += ((w[0][t0] * dw[1][t1]) * w[2][t2]) * c;
+                        g2 += ((w[0][t0] * w[1][t1]) * dw[2][t2]) * c;
+                    }
+                }
+            }
+        }
+        value = v;
+        if (GRAD) {
+            g[0] = g0; g[1] = g1; g[2] = g2;
+        }
+    }
+};
+
+// FP32: the separable form with FMAs (axis 0, then 1, then 2), 16 rows of
+// four taps.
+template <>
+struct CubicBSpline<float> {
+    template <bool GRAD>
+    __device__ __forceinline__ static void eval(const MovingVol &M, const int base[3],
+                                                const float fr[3], float &value, float g[3]) {
+        float w[3][4], dw[3][4];
+        int ix[4], iy[4], iz[4];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) bspline_weights<float>(fr[d], w[d], dw[d]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ix[k] = bspline_fold(base[0] - 1 + k, M.size[0]);
+            iy[k] = bspline_fold(base[1] - 1 + k, M.size[1]) * static_cast<int>(M.stride1);
+            iz[k] = bspline_fold(base[2] - 1 + k, M.size[2]) * static_cast<int>(M.stride2);
+        }
+        float v = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll
+        for (int t2 = 0; t2 < 4; ++t2) {
+            float sv = 0.0f, sx = 0.0f, sy = 0.0f;
+#pragma unroll
+            for (int t1 = 0; t1 < 4; ++t1) {
+                const float *row = M.coef + iz[t2] + iy[t1];
+                const float c0 = row[ix[0]], c1 = row[ix[1]], c2 = row[ix[2]], c3 = row[ix[3]];
+                const float r = fmaf(w[0][3], c3, fmaf(w[0][2], c2, fmaf(w[0][1], c1, w[0][0] * c0)));
+                sv = fmaf(w[1][t1], r, sv);
+                if (GRAD) {
+                    const float rd =
+                        fmaf(dw[0][3], c3, fmaf(dw[0][2], c2, fmaf(dw[0][1], c1, dw[0][0] * c0)));
+                    sx = fmaf(w[1][t1], rd, sx);
+                    sy = fmaf(dw[1][t1], r, sy);
+                }
+            }
+            v = fmaf(w[2][t2], sv, v);
+            if (GRAD) {
+                gx = fmaf(w[2][t2], sx, gx);
+                gy = fmaf(w[2][t2], sy, gy);
+                gz = fmaf(dw[2][t2], sv, gz);
+            }
+        }
+        value = v;
+        if (GRAD) {
+            g[0] = gx; g[1] = gy; g[2] = gz;
+        }
+    }
+};
+
+// The moving image's interpolant at a located cell: trilinear from the 8
+// gathered corners, or the cubic B-spline from the coefficients (BSP).
+template <typename R, bool BSP>
+struct Interp {
+    template <bool GRAD>
+    __device__ __forceinline__ static void eval(const MovingVol &M, const int base[3], const R fr[3],
+                                                const float v[8], R &value, R g[3]) {
+        if constexpr (BSP) CubicBSpline<R>::template eval<GRAD>(M, base, fr, value, g);
+        else Trilinear<R>::template eval<GRAD>(fr, v, value, g);
     }
 };
 

@@ -28,6 +28,7 @@ PIXEL = {
     np.dtype(np.float32): 8, np.dtype(np.float64): 9,
 }
 MEAN_SQUARES, MATTES = 0, 1
+INTERP_LINEAR, INTERP_BSPLINE = 0, 1  # SITKHIP_INTERP_*
 PRECISION_F32, PRECISION_F64 = 0, 1
 MAX_STAGES = 8
 
@@ -138,6 +139,7 @@ class RegistrationConfig(C.Structure):
         ("sigma", C.c_double * MAX_LEVELS),
         ("fixed_mask", C.c_void_p),
         ("moving_mask", C.c_void_p),
+        ("interpolator", C.c_int),
     ]
 
 
@@ -213,6 +215,8 @@ def lib():
         L.sitkhip_mask_free.restype = None
         L.sitkhip_recursive_gaussian.argtypes = [vp, dp, C.POINTER(vp)]
         L.sitkhip_recursive_gaussian_batch.argtypes = [C.c_size_t, C.POINTER(vp), dp, C.POINTER(vp)]
+        L.sitkhip_metric_set_interpolator.argtypes = [vp, C.c_int]
+        L.sitkhip_metric_bspline_coefficients.argtypes = [vp, dp, C.c_size_t]
         L.sitkhip_metric_create.argtypes = [C.c_int, vp, C.POINTER(C.c_int64), C.c_size_t, vp, vp,
                                             vp, C.c_int, C.POINTER(vp)]
         L.sitkhip_metric_create_drawn.argtypes = [C.c_int, vp, C.c_size_t, C.c_uint64, vp, C.c_int,
@@ -568,6 +572,18 @@ class _Resident:
         _check(lib().sitkhip_metric_value_ranges(self._h, _dp(fr), _dp(mr)))
         return (float(fr[0]), float(fr[1])), (float(mr[0]), float(mr[1]))
 
+    def set_interpolator(self, interpolator: int) -> None:
+        """The moving image's interpolator (MovingImage::from_image_with_interpolator,
+        registration/metric.rs:720-750): INTERP_LINEAR or INTERP_BSPLINE (cubic;
+        its coefficients are computed on the device here)."""
+        _check(lib().sitkhip_metric_set_interpolator(self._h, int(interpolator)))
+
+    def bspline_coefficients(self, n: int) -> np.ndarray:
+        """The cubic B-spline coefficients of the moving volume (n voxels), FP64."""
+        out = np.empty(n)
+        _check(lib().sitkhip_metric_bspline_coefficients(self._h, _dp(out), n))
+        return out
+
     def free(self) -> None:
         if self._h:
             lib().sitkhip_metric_free(self._h)
@@ -780,13 +796,18 @@ def registration_config(metric=MATTES, bins=50, sampling=0, sampling_percentage=
                         minimum_step=1e-4, iterations=100, relaxation=0.5,
                         gradient_tolerance=1e-4, learning_rate_mode=LR_ONCE,
                         scales_mode=SCALES_PHYSICAL_SHIFT, shrink_factors=((1, 1, 1),),
-                        sigmas=(0.0,), fixed_mask=None, moving_mask=None) -> RegistrationConfig:
+                        sigmas=(0.0,), fixed_mask=None, moving_mask=None,
+                        interpolator=INTERP_LINEAR) -> RegistrationConfig:
     """fixed_mask / moving_mask: DeviceMask on the fixed / moving image's grid
-    (set_metric_fixed_mask / set_metric_moving_mask); the caller keeps them
-    alive while the configuration is used."""
+    (set_metric_fixed_mask / set_metric_moving_mask). The configuration holds
+    the C handles, and a reference to each mask object so that the masks
+    outlive it (a mask made inline in the call would otherwise be freed when
+    this returns, leaving the handle dangling)."""
     c = RegistrationConfig()
     c.fixed_mask = fixed_mask._h.value if fixed_mask is not None else None
     c.moving_mask = moving_mask._h.value if moving_mask is not None else None
+    c._masks = (fixed_mask, moving_mask)
+    c.interpolator = interpolator
     c.metric, c.bins, c.sampling = metric, bins, sampling
     c.sampling_percentage, c.seed, c.precision = sampling_percentage, seed, precision
     c.optimizer, c.learning_rate, c.minimum_step = optimizer, learning_rate, minimum_step

@@ -187,7 +187,9 @@ def test_adam_steps_match_numpy(hip):
         net.forward(B)
         net.backward(B, rng.normal(0, 0.1, (B, 12)).astype(np.float32))
         g = net.grads()
-        assert np.count_nonzero(g) > 0.9 * g.size
+        # most parameters have a gradient (zeros: the first layer's padded
+        # input channels, units the ReLU switched off for every voxel)
+        assert np.count_nonzero(g) > 0.5 * g.size
         net.adam(lr=float(lr), beta1=float(b1), beta2=float(b2), eps=float(eps))
         c1 = np.float32(1.0) - b1 ** np.float32(t)
         c2 = np.float32(1.0) - b2 ** np.float32(t)
@@ -197,4 +199,7 @@ def test_adam_steps_match_numpy(hip):
         got = net.params()
         err = np.max(np.abs((got - p) - (want - p)))
         assert err <= 1e-5 * lr, (t, err)
+        # parameters with a zero gradient still move by the momentum of the
+        # first step on the second
+        assert np.count_nonzero(got - p) >= np.count_nonzero(g)
         p = got

@@ -28,7 +28,7 @@ import pytest
 
 import oracle as O
 from helpers import affine_form, record_error, rel_vec, stages_of
-from registration_reference import register_rsgd
+from oracle.registration import register_rsgd
 from test_gpu_metric import contract
 
 pytestmark = pytest.mark.gpu

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from registration_reference import register_gd_line_search, register_rsgd
+from oracle.registration import register_gd_line_search, register_rsgd
 
 pytestmark = pytest.mark.gpu
 

@@ -1,5 +1,5 @@
 """The host restatements of the reference's optimizers (tests/
-registration_reference.py), which the device registration driver is checked
+oracle/registration.py), which the device registration driver is checked
 against iterate for iterate, pinned here by the reference's own optimizer
 fixtures (crates/sitk/src/registration/optimizer.rs:1063-1296) and by the
 window convergence monitor's published stop count.
@@ -9,7 +9,7 @@ import math
 
 import pytest
 
-from registration_reference import _kahan, gd_line_search, rsgd
+from oracle.registration import _kahan, gd_line_search, rsgd
 
 
 def _bowl(p):

@@ -295,6 +295,23 @@ def test_mean_squares_identity_is_zero_with_zero_gradient():
     assert v == 0.0 and np.all(d == 0.0) and valid == 16 ** 3
 
 
+def test_mean_squares_thread_count_does_not_change_a_bit():
+    """metric.rs:1370: rows computed in parallel and added in order give the
+    serial loop's bits at any thread count (chunks of 2^16 rows: 40^3 samples
+    span two chunks)."""
+    sp = [0.8, 0.9, 1.1]
+    f = O.fixture_volume(40, [0, 0, 0])
+    m = O.fixture_inverted(40, [1.5, -0.5, 0.75])
+    pb = O.MetricProblem(f, sp, [-3, 2, 1], m, sp, [-3, 2, 1])
+    ctr = [13.0, 20.0, 22.0]
+    t = O.make_transform(O.AFFINE, [1.01, 0.02, -0.01, -0.015, 0.99, 0.03, 0.01, -0.02, 1.0,
+                                    0.7, -0.3, 0.4], center=ctr)
+    v1, d1, n1 = pb.ms_evaluate(t)
+    for threads in (2, 3, 8):
+        vt, dt, nt = pb.ms_evaluate(t, threads=threads)
+        assert nt == n1 and vt == v1 and np.array_equal(dt, d1), threads
+
+
 def test_mean_squares_one_voxel_shift_is_hand_computable():
     """Moving = fixed shifted by one voxel along x; a translation of exactly one
     voxel re-aligns every interior sample, and the value is the mean of the
