@@ -134,6 +134,18 @@ int sitkhip_recursive_gaussian(sitkhip_image in, const double sigma[3], sitkhip_
 int sitkhip_recursive_gaussian_batch(size_t n, const sitkhip_image *in, const double *sigma,
                                      sitkhip_image *out);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sitkhip_recursive_gaussian into the caller's image `out`
+ * (the input's geometry, not the input itself), bit for bit the same volume,
+ * with no device allocation. The FP64 intermediates live in a static device
+ * scratch of 2 x SITKHIP_GAUSSIAN_INTO_MAX_VOXELS doubles (128 MiB) that each
+ * device the library runs on holds until the process ends; larger images are
+ * refused (SITKHIP_ERR_INVALID_ARGUMENT). Every call on one device shares that
+ * scratch, so calls that overlap on one device (several host threads) race. */
+#define SITKHIP_GAUSSIAN_INTO_MAX_VOXELS 8388608
+int sitkhip_recursive_gaussian_into(sitkhip_image in, const double sigma[3], sitkhip_image out);
+
 /* ------------------------------------------------------ pyramid placement */
 /* shrunk_geometry (src/cuda/ops/pyramid.rs:355-420, ITK ShrinkImageFilter): size
  * max(1, size / f), spacing * f, origin moved by Direction * diag(spacing) *

@@ -38,6 +38,10 @@ struct Backend {
     std::string name;
 };
 Backend &backend();
+// Looks up the device address of the static scratch of the allocation-free
+// smoothing entry point on `device` (current); called once per device when
+// the backend is initialised there (gaussian.hip).
+void gaussian_scratch_setup(int device);
 
 // While a StreamScope lives, backend() on this thread hands out `s` as the
 // library stream, so independent set-up work can be spread over several

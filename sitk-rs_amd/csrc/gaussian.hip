@@ -486,6 +486,54 @@ Image *recursive_gaussian(const Image &in, const double sigma[3], bool synchrono
     return out;
 }
 
+// ------------------------------------------------ smoothing without allocation
+// The FP64 intermediates of sitkhip_recursive_gaussian_into: two volumes of at
+// most kIntoMaxVoxels voxels (2 x 64 MiB), a static device array that every
+// device the library runs on holds until the process ends. Host code never
+// uses the array's name as an address: the device address of each device's
+// copy is looked up once, when the backend is initialised on that device, and
+// kept in g_scratch_addr.
+constexpr size_t kIntoMaxVoxels = SITKHIP_GAUSSIAN_INTO_MAX_VOXELS;
+static __device__ __align__(256) double scratch[2 * kIntoMaxVoxels];
+constexpr int kMaxDevices = 64;
+static double *g_scratch_addr[kMaxDevices] = {};
+
+void gaussian_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        fail(SITKHIP_ERR_NO_DEVICE, "device index " + std::to_string(device) +
+                                        " beyond the scratch table of " +
+                                        std::to_string(kMaxDevices) + " devices");
+    void *p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_addr[device] = static_cast<double *>(p);
+}
+
+// recursive_gaussian into a caller's image through the static scratch: no
+// device allocation. Every call on one device shares that device's scratch.
+static void recursive_gaussian_into(const Image &in, const double sigma[3], Image &out) {
+    int axes[3];
+    const int na = smoothed_axes(in.geom, sigma, axes);
+    if (!same_geometry(in.geom, out.geom))
+        fail(SITKHIP_ERR_INVALID_ARGUMENT,
+             "recursive_gaussian_into: the output image's geometry is not the input's");
+    if (in.buf.ptr == out.buf.ptr)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT,
+             "recursive_gaussian_into: the output image is the input image");
+    const size_t n = in.len();
+    if (n > kIntoMaxVoxels)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT,
+             "recursive_gaussian_into: " + std::to_string(n) + " voxels exceed the " +
+                 std::to_string(kIntoMaxVoxels) + " its scratch holds");
+    const int dev = backend().device;
+    double *base = (dev >= 0 && dev < kMaxDevices) ? g_scratch_addr[dev] : nullptr;
+    if (!base)
+        fail(SITKHIP_ERR_DRIVER, "recursive_gaussian_into: scratch not set up on device " +
+                                     std::to_string(dev));
+    enqueue_smoothing(in.buf.ptr, out.buf.ptr, in.geom, sigma, axes, na, base,
+                      base + kIntoMaxVoxels);
+    sync();
+}
+
 // ----------------------------------------------------------- batched volumes
 template <typename SrcT, typename DstT>
 static void launch_axis_batch(const std::vector<const SrcT *> &src, const std::vector<DstT *> &dst,
@@ -614,6 +662,17 @@ extern "C" int sitkhip_recursive_gaussian_batch(size_t n, const sitkhip_image *i
         std::vector<sitkhip::Image *> res = sitkhip::recursive_gaussian_batch(imgs, sig);
         sitkhip::sync();
         for (size_t i = 0; i < n; ++i) out[i] = reinterpret_cast<sitkhip_image>(res[i]);
+    });
+}
+
+extern "C" int sitkhip_recursive_gaussian_into(sitkhip_image in, const double sigma[3],
+                                               sitkhip_image out) {
+    return sitkhip::run_guarded([&] {
+        auto src = reinterpret_cast<sitkhip::Image *>(in);
+        auto dst = reinterpret_cast<sitkhip::Image *>(out);
+        if (!src || !sigma || !dst)
+            sitkhip::fail(SITKHIP_ERR_INVALID_ARGUMENT, "null image or sigma");
+        sitkhip::recursive_gaussian_into(*src, sigma, *dst);
     });
 }
 

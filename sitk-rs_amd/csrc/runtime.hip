@@ -245,6 +245,7 @@ static void init_backend_locked(int device) {
     // Default-priority, non-blocking stream: every operation of the library is
     // enqueued here and synchronised before the call returns.
     HIP_OK(hipStreamCreateWithFlags(&g_backend.stream, hipStreamNonBlocking));
+    gaussian_scratch_setup(device);
     g_backend_ready = true;
 }
 

@@ -215,6 +215,7 @@ def lib():
         L.sitkhip_mask_free.restype = None
         L.sitkhip_recursive_gaussian.argtypes = [vp, dp, C.POINTER(vp)]
         L.sitkhip_recursive_gaussian_batch.argtypes = [C.c_size_t, C.POINTER(vp), dp, C.POINTER(vp)]
+        L.sitkhip_recursive_gaussian_into.argtypes = [vp, dp, vp]
         L.sitkhip_metric_set_interpolator.argtypes = [vp, C.c_int]
         L.sitkhip_metric_bspline_coefficients.argtypes = [vp, dp, C.c_size_t]
         L.sitkhip_metric_create.argtypes = [C.c_int, vp, C.POINTER(C.c_int64), C.c_size_t, vp, vp,
@@ -458,6 +459,22 @@ def recursive_gaussian(src: DeviceImage, sigma) -> DeviceImage:
     h = C.c_void_p()
     _check(lib().sitkhip_recursive_gaussian(src._h, _dp(s), C.byref(h)))
     return DeviceImage(h)
+
+
+# SITKHIP_GAUSSIAN_INTO_MAX_VOXELS: the largest image recursive_gaussian_into takes
+GAUSSIAN_INTO_MAX_VOXELS = 8_388_608
+
+
+def recursive_gaussian_into(src: DeviceImage, sigma, out: DeviceImage) -> DeviceImage:
+    """recursive_gaussian into the caller's image `out` (the input's geometry),
+    allocating no device memory (no counterpart in the reference): for callers
+    that cannot allocate. At most GAUSSIAN_INTO_MAX_VOXELS voxels; calls that
+    overlap on one device share its scratch and race."""
+    s = np.ascontiguousarray(sigma, dtype=np.float64)
+    if s.size != 3:
+        raise CudaError("UnsupportedGeometry", "sigma must have 3 entries")
+    _check(lib().sitkhip_recursive_gaussian_into(src._h, _dp(s), out._h))
+    return out
 
 
 def recursive_gaussian_batch(srcs, sigmas) -> list:
