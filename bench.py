@@ -575,13 +575,21 @@ def main() -> int:
 
     for _ in range(args.warmup):
         metric.value_and_moments(stage)
-    # the timed steps run without per-kernel events (they add tens of us per
-    # step); the kernel durations come from a separate profiled loop below
+    # The timed steps run without per-kernel events (they add tens of us per
+    # step); the kernel durations come from a separate profiled loop below.
+    # The K steps are made inside the library, one after another, each
+    # waiting for its result as the reference's compiled optimizer loop does
+    # (a Python call per step would add the interpreter's own turnaround);
+    # the same K steps from Python are reported beside them.
+    metric.value_and_moments_repeat(stage, args.warmup)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        value, mom, valid = metric.value_and_moments(stage)
+    value, mom, valid = metric.value_and_moments_repeat(stage, args.steps)
     wall = time.perf_counter() - t0
     ms_step = wall * 1e3 / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        metric.value_and_moments(stage)
+    ms_step_py = (time.perf_counter() - t0) * 1e3 / args.steps
     S.profile_reset()
     S.profile_enable(True)
     for _ in range(min(args.steps, 100)):
@@ -665,6 +673,7 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_step,
+        "ms_per_step_python_loop": ms_step_py,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -336,6 +336,14 @@ int sitkhip_mattes_value(sitkhip_metric m, const double *stages, int nstage, dou
  * pass, with no host round trip between them. */
 int sitkhip_mattes_value_and_moments(sitkhip_metric m, const double *stages, int nstage,
                                      double *value, double moments[12], size_t *valid);
+/* No counterpart in the reference: sitkhip_mattes_value_and_moments `repeats`
+ * times in a row at the same stages, each evaluation waiting for its result
+ * as an optimizer's would; the last result is returned. For timing an
+ * evaluation the way compiled callers (the reference's optimizer loop) make
+ * it, without a foreign-function call per evaluation. */
+int sitkhip_mattes_value_and_moments_repeat(sitkhip_metric m, const double *stages, int nstage,
+                                            int repeats, double *value, double moments[12],
+                                            size_t *valid);
 
 /* ResidentMetric::evaluate (mean_squares.rs:329-439): the 14 moments
  * [sq, S0[3], S1[3][3] row-major [d][e], count] of the mean-squares metric,

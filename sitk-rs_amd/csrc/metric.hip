@@ -93,12 +93,13 @@ __global__ __launch_bounds__(kBlock) void ms_pass(FixedGrid F, MovingVol M, Map 
 // Pass 1: joint histogram (fixed point) + fixed-marginal counts; with REC
 // also the per-sample records for the derivative pass that follows.
 template <typename R, bool LDS, bool REC, typename Map, bool BSP = false>
-__global__ __launch_bounds__(kBlock) SITKHIP_HIST_OCC void mattes_hist(FixedGrid F, MovingVol M, Map S,
+__global__ __launch_bounds__(kHistBlock) SITKHIP_HIST_OCC void mattes_hist(FixedGrid F, MovingVol M, Map S,
                                                       IndexMap X, const ChainConst *__restrict__ C,
                                                       SampleSet SS, const uint8_t *__restrict__ fbin,
                                                       MattesParams P,
                                                       unsigned long long *__restrict__ g_joint,
                                                       unsigned long long *__restrict__ g_marg,
+                                                      unsigned long long *__restrict__ g_col,
                                                       SampleRec<R> *__restrict__ rec,
                                                       uint32_t *__restrict__ defer,
                                                       unsigned defer_cap) {
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(kBlock) SITKHIP_HIST_OCC void mattes_hist(FixedGrid
     unsigned long long *joint = LDS ? s_hist : g_joint;
     unsigned long long *marg = LDS ? s_hist + B * B : g_marg;
     if (LDS) {
-        for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0ull;
+        for (int i = threadIdx.x; i < nb; i += kHistBlock) s_hist[i] = 0ull;
         __syncthreads();
     }
     // FP32 with an affine map: samples the composed index map cannot decide
@@ -123,14 +124,14 @@ __global__ __launch_bounds__(kBlock) SITKHIP_HIST_OCC void mattes_hist(FixedGrid
     __shared__ double s_stage[kMaxStages * 12];
     const auto V = stage_to_lds(S, s_stage);  // ends with a barrier
     long long first, step, end;
-    xcd_walk(SS.n, first, step, end);
+    xcd_walk<kHistBlock>(SS.n, first, step, end);
     uint32_t *dq = DEFER ? defer + static_cast<size_t>(blockIdx.x) * defer_cap : nullptr;
     hist_range<R, REC, std::decay_t<decltype(V)>, DEFER, BSP>(F, M, V, X, C, SS, fbin, P, joint, marg, rec, first, end, step, dq,
                                          &s_defer_n);
     if (DEFER) {
         __syncthreads();
         const unsigned nq = s_defer_n;
-        for (unsigned i = threadIdx.x; i < nq; i += kBlock)
+        for (unsigned i = threadIdx.x; i < nq; i += kHistBlock)
             deferred_sample<R, REC, std::decay_t<decltype(V)>, BSP>(F, M, V, C, SS, fbin, P, joint, marg, rec,
                                                                      dq[i]);
     }
@@ -139,10 +140,20 @@ __global__ __launch_bounds__(kBlock) SITKHIP_HIST_OCC void mattes_hist(FixedGrid
         // The workgroups of one XCD finish together; each starts its flush at
         // its own bin so that they do not queue on the same addresses.
         const int rot = static_cast<int>((blockIdx.x / kHistCopies) * 193u % static_cast<unsigned>(nb));
-        for (int k = threadIdx.x; k < nb; k += kBlock) {
+        for (int k = threadIdx.x; k < nb; k += kHistBlock) {
             const int i = k + rot < nb ? k + rot : k + rot - nb;
             const unsigned long long v = s_hist[i];
             if (v) atomicAdd(i < B * B ? g_joint + i : g_marg + (i - B * B), v);
+        }
+        // for the split tail: this workgroup's moving-bin column sums into
+        // the XCD's copy (integers: the same totals as summing the folded
+        // histogram's columns)
+        if (g_col) {
+            for (int m = threadIdx.x; m < B; m += kHistBlock) {
+                unsigned long long c = 0;
+                for (int f = 0; f < B; ++f) c += s_hist[f * B + m];
+                if (c) atomicAdd(g_col + copy * B + m, c);
+            }
         }
     }
 }
@@ -162,27 +173,27 @@ __global__ __launch_bounds__(kTail) void mattes_tail_kernel(unsigned long long *
 }
 
 // The tail of a value+derivative evaluation split over T = gridDim.x
-// workgroups: each folds the whole histogram into its LDS and takes the
-// statistics, then computes the pRatio of its share of the bins and its exact
-// partial value sum (part[2b], part[2b + 1]); workgroup 0 writes the valid
-// count and joint mass, and part[2T] = 1 for a degenerate histogram. The
-// derivative pass that follows adds the partials (finish_split_tail) and
-// clears the histogram copies.
-__global__ __launch_bounds__(kTail) void mattes_tail_split(unsigned long long *g_joint,
-                                                            unsigned long long *g_marg, int B,
+// workgroups. The statistics come from the marginals alone -- the fixed
+// marginal and the moving-bin column sums the value pass added per XCD copy
+// (joint mass = sum of the column sums, valid = sum of the fixed marginal) --
+// so a workgroup folds the histogram copies only for its own share of the
+// bins, then computes their pRatio and its exact partial value sum
+// (part[2b], part[2b + 1]); workgroup 0 writes the valid count and joint mass,
+// and part[2T] = 1 for a degenerate histogram. The derivative pass that
+// follows adds the partials (finish_split_tail) and clears the copies.
+__global__ __launch_bounds__(kTail) void mattes_tail_split(const unsigned long long *g_joint,
+                                                            const unsigned long long *g_marg,
+                                                            const unsigned long long *g_col, int B,
                                                             double mbs, double *pratio,
                                                             float *pratio_f, TailOut *out,
                                                             unsigned long long *part) {
-    extern __shared__ unsigned long long s_hist[];
     __shared__ TailShared sh;
     const int t = threadIdx.x, nbin = B * B, T = static_cast<int>(gridDim.x);
-    tail_fold(g_joint, g_marg, B, s_hist);
-    __syncthreads();
-    const TailStats st = tail_stats(s_hist, s_hist + nbin, B, mbs, sh);
+    const TailStats st = tail_stats_marginals(g_marg, g_col, B, mbs, sh);
     const int per = (nbin + T - 1) / T;
     const int i0 = static_cast<int>(blockIdx.x) * per;
     const int i1 = i0 + per < nbin ? i0 + per : nbin;
-    const __int128 sum = tail_bins(s_hist, B, i0, i1, st, pratio, pratio_f, sh);
+    const __int128 sum = tail_bins_copies(g_joint, B, i0, i1, st, pratio, pratio_f, sh);
     if (t == 0) {
         part[2 * blockIdx.x] = static_cast<unsigned long long>(sum);
         part[2 * blockIdx.x + 1] = static_cast<unsigned long long>(sum >> 64);
@@ -192,6 +203,21 @@ __global__ __launch_bounds__(kTail) void mattes_tail_split(unsigned long long *g
             out->joint_sum = st.joint_sum;
         }
     }
+}
+
+// After the split tail of a value-only evaluation: the value from the
+// workgroups' exact partial sums (one lane), and the histogram copies, the
+// marginal copies and the column-sum copies cleared for the next evaluation.
+__global__ __launch_bounds__(256) void mattes_tail_finish(const unsigned long long *__restrict__ part,
+                                                          int T, TailOut *out,
+                                                          unsigned long long *__restrict__ joint,
+                                                          unsigned n_joint,
+                                                          unsigned long long *__restrict__ marg,
+                                                          unsigned n_marg) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) finish_split_tail(part, T, out);
+    const unsigned stride = gridDim.x * 256u;
+    for (unsigned i = blockIdx.x * 256u + threadIdx.x; i < n_joint; i += stride) joint[i] = 0ull;
+    for (unsigned i = blockIdx.x * 256u + threadIdx.x; i < n_marg; i += stride) marg[i] = 0ull;
 }
 
 // Pass 2: the 12 pRatio-weighted moments, folded by the last workgroup.
@@ -869,7 +895,9 @@ void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b) {
         check_launch("fixed_bins");
     }
     m.joint.alloc(static_cast<size_t>(kHistCopies) * B * B);
-    m.marg.alloc(static_cast<size_t>(kHistCopies) * B);
+    // the fixed-marginal copies, then the moving-bin column-sum copies (the
+    // split tail's statistics); cleared together
+    m.marg.alloc(static_cast<size_t>(kHistCopies) * B * 2);
     m.pratio.alloc(static_cast<size_t>(B) * B);
     m.pratio_f.alloc(static_cast<size_t>(B) * B);
     m.tail_part.alloc(2 * kMaxTailSplit + 1);
@@ -929,12 +957,12 @@ static int launch_tail_for_derivative(Metric &m, TailOut *out) {
         launch_tail(m, out, false);
         return 0;
     }
-    const size_t shm = static_cast<size_t>(B * B + B) * sizeof(unsigned long long);
     int T = (B * B + kTail - 1) / kTail;  // about one bin per lane
     if (T > kMaxTailSplit) T = kMaxTailSplit;
     LaunchScope scope("mattes_tail");
-    hipLaunchKernelGGL(mattes_tail_split, dim3(T), dim3(kTail), shm, backend().stream, m.joint.ptr,
-                       m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out, m.tail_part.ptr);
+    hipLaunchKernelGGL(mattes_tail_split, dim3(T), dim3(kTail), 0, backend().stream, m.joint.ptr,
+                       m.marg.ptr, m.marg.ptr + static_cast<size_t>(kHistCopies) * B, B, m.P.mbs,
+                       m.pratio.ptr, m.pratio_f.ptr, out, m.tail_part.ptr);
     check_launch("mattes_tail");
     return T;
 }
@@ -953,21 +981,26 @@ static void launch_hist(Metric &m, const Stages &S, bool tail, bool clear = true
     unsigned cap = 0;
     if (sizeof(R) == 4) {
         const long long chunk = (m.SS.n + 7) / 8;
-        const long long stride = static_cast<long long>(kHistGrid / 8) * kBlock;
-        cap = static_cast<unsigned>((chunk + stride - 1) / stride * kBlock);
+        const long long stride = static_cast<long long>(kHistGrid / 8) * kHistBlock;
+        cap = static_cast<unsigned>((chunk + stride - 1) / stride * kHistBlock);
         const size_t need = static_cast<size_t>(kHistGrid) * cap;
         if (m.defer.len < need) m.defer.alloc(need);
         dq = m.defer.ptr;
     }
+    // the split tail follows a value pass with records and a value-only pass
+    // that clears after itself (B <= 76): it reads the column sums this pass
+    // adds after the marginal copies
+    const bool split = B <= 76 && ((rec && !tail) || (tail && clear));
+    unsigned long long *col = split ? m.marg.ptr + static_cast<size_t>(kHistCopies) * B : nullptr;
     {
         LaunchScope scope("mattes_hist");
         const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
         with_map(m, S, [&](const auto &map) {
             using Map = std::decay_t<decltype(map)>;
             auto go = [&](auto kernel, size_t bytes) {
-                hipLaunchKernelGGL(kernel, dim3(kHistGrid), dim3(kBlock), bytes, backend().stream, m.F,
+                hipLaunchKernelGGL(kernel, dim3(kHistGrid), dim3(kHistBlock), bytes, backend().stream, m.F,
                                    m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, m.joint.ptr,
-                                   m.marg.ptr, rec, dq, cap);
+                                   m.marg.ptr, col, rec, dq, cap);
             };
             if (m.bspline) {
                 if (lds) {
@@ -988,6 +1021,16 @@ static void launch_hist(Metric &m, const Stages &S, bool tail, bool clear = true
         check_launch("mattes_hist");
     }
     if (!tail) return;
+    if (split) {
+        const int T = launch_tail_for_derivative(m, &m.res_dev->tail);
+        LaunchScope scope("mattes_tail_finish");
+        hipLaunchKernelGGL(mattes_tail_finish, dim3(16), dim3(256), 0, backend().stream,
+                           m.tail_part.ptr, T, &m.res_dev->tail, m.joint.ptr,
+                           static_cast<unsigned>(m.joint.len), m.marg.ptr,
+                           static_cast<unsigned>(m.marg.len));
+        check_launch("mattes_tail_finish");
+        return;
+    }
     launch_tail(m, &m.res_dev->tail, clear);
 }
 
@@ -1482,6 +1525,17 @@ int sitkhip_mattes_value_and_moments(sitkhip_metric h, const double *stages, int
                                      double *value, double moments[12], size_t *valid) {
     return run_guarded(
         [&] { mattes_value_and_moments(metric_ref(h), stages, nstage, value, moments, valid); });
+}
+
+int sitkhip_mattes_value_and_moments_repeat(sitkhip_metric h, const double *stages, int nstage,
+                                            int repeats, double *value, double moments[12],
+                                            size_t *valid) {
+    return run_guarded([&] {
+        if (repeats < 1) fail(SITKHIP_ERR_INVALID_ARGUMENT, "repeats must be >= 1");
+        Metric &m = metric_ref(h);
+        for (int r = 0; r < repeats; ++r)
+            mattes_value_and_moments(m, stages, nstage, value, moments, valid);
+    });
 }
 
 int sitkhip_ms_moments(sitkhip_metric h, const double *stages, int nstage, double moments[14]) {

@@ -28,14 +28,20 @@ constexpr int kBlock = SITKHIP_BLOCK;  // lanes per workgroup of the sample pass
 #define SITKHIP_GRID 1024
 #endif
 constexpr unsigned kGrid = SITKHIP_GRID;  // fixed: the reduction tree must not depend on n
-// The value pass's grid (its integer histogram does not depend on it): 128
-// workgroups per XCD. Twice as many keep more waves resident but double the
-// 64-bit atomics of the end-of-workgroup flushes, which cost more (measured
-// on the 2M-sample level: 26.6 us at 1024, 29.8 us at 2048).
+// The value pass's launch shape (its integer histogram does not depend on
+// it): 512 workgroups (64 per XCD) of 512 lanes, two per CU. Measured on the
+// 2M-sample level (value pass, us): 256 lanes x 1024 workgroups 26.5, x 2048
+// 30.0, x 512 32.7; 512 x 1024 30.0, x 512 25.8; 1024 x 512 30.3, x 256 26.1.
+// More workgroups add end-of-workgroup flushes (64-bit atomics into the
+// XCD's histogram copy); fewer waves per CU leave gathers exposed.
 #ifndef SITKHIP_HIST_GRID
-#define SITKHIP_HIST_GRID 1024
+#define SITKHIP_HIST_GRID 512
 #endif
 constexpr unsigned kHistGrid = SITKHIP_HIST_GRID;
+#ifndef SITKHIP_HIST_BLOCK
+#define SITKHIP_HIST_BLOCK 512
+#endif
+constexpr int kHistBlock = SITKHIP_HIST_BLOCK;
 constexpr int kLdsMaxBins = 90;       // bins^2 * 8 B of LDS per workgroup (<= 64 KB)
 constexpr double kFix = 4294967296.0; // 2^32 fixed-point units per histogram unit
 constexpr double kInvFix = 1.0 / 4294967296.0;
@@ -81,17 +87,20 @@ struct B3 {
     __device__ __forceinline__ static R d3(R u) { return R(-2) + R(2) * u - R(0.5) * u * u; }
     __device__ __forceinline__ static R d4(R u) { return R(2) + R(2) * u + R(0.5) * u * u; }
 };
-// FP32: the same polynomials, (2-a)^3/6 form and a multiply by 1/6 (no
-// correctly rounded division on the FP32 path).
+// FP32: the same polynomials times 6 ((2-a)^3 for the outer piece); the
+// division by 6 is folded into the fixed-point scale of to_fixed<float>, in
+// FP64. A float 1/6 would scale every weight by the same 1 + 3e-8: the joint
+// mass would carry it into the fixed marginal's normalisation
+// (mattes.rs:283-293) and shift the value by 3e-8 (2e-6 of an MI of 0.015).
 template <>
 __device__ __forceinline__ float B3<float>::inner(float a) {
     const float sq = a * a;
-    return fmaf(fmaf(3.0f, a, -6.0f), sq, 4.0f) * (1.0f / 6.0f);
+    return fmaf(fmaf(3.0f, a, -6.0f), sq, 4.0f);
 }
 template <>
 __device__ __forceinline__ float B3<float>::outer(float a) {
     const float t = 2.0f - a;
-    return t * t * t * (1.0f / 6.0f);
+    return t * t * t;
 }
 
 template <typename R>
@@ -155,18 +164,19 @@ __device__ __forceinline__ void block_slice(long long n, long long &begin, long 
 
 // The value pass's walk of the samples: XCD x (workgroup mod 8) takes the
 // x-th eighth of the samples and its G/8 workgroups sweep that range
-// together, kBlock consecutive samples each per step; a lane's samples are
-// first + lane + k * step. The moving voxels one XCD reads at a time then lie
+// together, BLK (the workgroup size) consecutive samples each per step; a
+// lane's samples are first + lane + k * step. The moving voxels one XCD reads at a time then lie
 // in a thin band of slices that stays in its L2 (an integer histogram does
 // not depend on which workgroup adds a sample).
+template <int BLK = kBlock>
 __device__ __forceinline__ void xcd_walk(long long n, long long &first, long long &step,
                                          long long &end) {
     const unsigned G = gridDim.x, b = blockIdx.x;
     const long long chunk = (n + 7) / 8;
     const long long xb = static_cast<long long>(b % 8u) * chunk;
     end = xb + chunk < n ? xb + chunk : n;
-    first = xb + static_cast<long long>(b / 8u) * kBlock;
-    step = static_cast<long long>(G / 8u) * kBlock;
+    first = xb + static_cast<long long>(b / 8u) * BLK;
+    step = static_cast<long long>(G / 8u) * BLK;
 }
 
 __device__ __forceinline__ uint32_t sample_flat(const SampleSet &S, long long s) {
@@ -221,15 +231,23 @@ struct Parzen<float> {
 };
 
 // B3 in [0, 2/3] (a rounding residue below zero becomes 0), scaled to 2^-32
-// units: every contribution fits in 32 bits.
+// units: every contribution fits in 32 bits. The FP32 weights arrive times 6
+// (B3<float>) and are scaled by 2^32 / 6 split into a float and its float
+// remainder, so the product is rounded once from a scale exact to 2^-48: the
+// rounding errors are then those of the individual weights, with no shared
+// bias.
 template <typename R>
 __device__ __forceinline__ unsigned long long to_fixed(R v) {
     const double d = static_cast<double>(v);
     return d > 0.0 ? static_cast<unsigned long long>(llrint(d * kFix)) : 0ull;
 }
+constexpr double kFixSixth = 4294967296.0 / 6.0;
+constexpr float kFixSixthHi = static_cast<float>(kFixSixth);
+constexpr float kFixSixthLo = static_cast<float>(kFixSixth - static_cast<double>(kFixSixthHi));
 template <>
 __device__ __forceinline__ unsigned long long to_fixed<float>(float v) {
-    return v > 0.0f ? static_cast<unsigned long long>(__float2uint_rn(v * 4294967296.0f)) : 0ull;
+    return v > 0.0f ? static_cast<unsigned long long>(__float2uint_rn(fmaf(v, kFixSixthHi, v * kFixSixthLo)))
+                    : 0ull;
 }
 // ------------------------------------------------------------ FP64 reduction
 // A double moved across lanes by DPP (two 32-bit halves); lanes with no
@@ -927,6 +945,7 @@ struct TailStats {
 struct TailShared {
     unsigned long long wu[kTail / 64][2];
     unsigned long long colsum[256];
+    unsigned long long fcount[256];
     double mm[256], fm[256], logfm[256];
     long long part[kTail / 64][2];
 };
@@ -998,9 +1017,52 @@ __device__ __forceinline__ TailStats tail_stats(const unsigned long long *hj,
     return st;
 }
 
+// The statistics of tail_stats from the marginal copies alone: the fixed
+// marginal (valid count) and the moving-bin column sums (joint mass), each
+// summed over the kHistCopies copies (exact integers: the same numbers as
+// from the folded histogram).
+__device__ __forceinline__ TailStats tail_stats_marginals(const unsigned long long *g_marg,
+                                                          const unsigned long long *g_col, int B,
+                                                          double mbs, TailShared &sh) {
+    const int t = threadIdx.x;
+    if (t < B) {
+        unsigned long long fc = 0, cc = 0;
+#pragma unroll
+        for (int x = 0; x < kHistCopies; ++x) {
+            fc += g_marg[x * B + t];
+            cc += g_col[x * B + t];
+        }
+        sh.fcount[t] = fc;
+        sh.colsum[t] = cc;
+    }
+    __syncthreads();
+    unsigned long long tot = 0, valid = 0;
+    for (int i = 0; i < B; ++i) {
+        tot += sh.colsum[i];
+        valid += sh.fcount[i];
+    }
+    TailStats st;
+    st.valid = valid;
+    st.joint_sum = static_cast<double>(tot) * kInvFix;
+    st.degenerate = valid == 0 || st.joint_sum < DBL_EPSILON;
+    st.inv_sum = st.degenerate ? 0.0 : 1.0 / st.joint_sum;
+    st.n_factor = st.degenerate ? 0.0 : 1.0 / (mbs * static_cast<double>(valid));
+    if (t < B) {
+        const double fm = static_cast<double>(sh.fcount[t]) * st.inv_sum;
+        sh.fm[t] = fm;
+        sh.logfm[t] = fm > DBL_EPSILON ? log(fm) : 0.0;
+        sh.mm[t] = static_cast<double>(sh.colsum[t]) * kInvFix * st.inv_sum;
+    }
+    __syncthreads();
+    return st;
+}
+
 // pRatio of bins [i0, i1) (lane t: i0 + t, i0 + t + kTail, ...) and the
 // workgroup's exact value sum over them (valid in lane 0 of wave 0 after
 // the barrier inside).
+// COPIES: hj holds the kHistCopies unfolded copies (B^2 apart), summed here
+// bin by bin.
+template <bool COPIES = false>
 __device__ __forceinline__ __int128 tail_bins(const unsigned long long *hj, int B, int i0, int i1,
                                               const TailStats &st, double *pratio, float *pratio_f,
                                               TailShared &sh) {
@@ -1009,7 +1071,13 @@ __device__ __forceinline__ __int128 tail_bins(const unsigned long long *hj, int 
     __int128 part = 0;
     for (int i = i0 + t; i < i1; i += kTail) {
         double pr_out = 0.0;
-        const unsigned long long q = hj[i];
+        unsigned long long q = 0;
+        if constexpr (COPIES) {
+#pragma unroll
+            for (int x = 0; x < kHistCopies; ++x) q += hj[static_cast<size_t>(x) * B * B + i];
+        } else {
+            q = hj[i];
+        }
         if (q && !st.degenerate) {
             const int f = i / B, m = i - f * B;
             const double jp = static_cast<double>(q) * kInvFix * st.inv_sum;
@@ -1035,6 +1103,12 @@ __device__ __forceinline__ __int128 tail_bins(const unsigned long long *hj, int 
             sum += static_cast<__int128>((static_cast<unsigned __int128>(static_cast<unsigned long long>(sh.part[w][1])) << 64) |
                                          static_cast<unsigned long long>(sh.part[w][0]));
     return sum;
+}
+
+__device__ __forceinline__ __int128 tail_bins_copies(const unsigned long long *g_joint, int B, int i0,
+                                                     int i1, const TailStats &st, double *pratio,
+                                                     float *pratio_f, TailShared &sh) {
+    return tail_bins<true>(g_joint, B, i0, i1, st, pratio, pratio_f, sh);
 }
 
 __device__ __forceinline__ double tail_value(__int128 sum, bool degenerate) {

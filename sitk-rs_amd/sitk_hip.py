@@ -217,6 +217,7 @@ def lib():
         L.sitkhip_recursive_gaussian_batch.argtypes = [C.c_size_t, C.POINTER(vp), dp, C.POINTER(vp)]
         L.sitkhip_recursive_gaussian_into.argtypes = [vp, dp, vp]
         L.sitkhip_metric_set_interpolator.argtypes = [vp, C.c_int]
+        L.sitkhip_mattes_value_and_moments_repeat.argtypes = [vp, dp, C.c_int, C.c_int, dp, dp, szp]
         L.sitkhip_metric_bspline_coefficients.argtypes = [vp, dp, C.c_size_t]
         L.sitkhip_metric_create.argtypes = [C.c_int, vp, C.POINTER(C.c_int64), C.c_size_t, vp, vp,
                                             vp, C.c_int, C.POINTER(vp)]
@@ -688,6 +689,17 @@ class ResidentMattes(_Resident):
         out = np.empty(12)
         _check(lib().sitkhip_mattes_value_and_moments(self._h, _dp(s), n, C.byref(v), _dp(out),
                                                       C.byref(valid)))
+        return v.value, out, valid.value
+
+    def value_and_moments_repeat(self, stages, repeats: int):
+        """value_and_moments `repeats` times in one call (timing without a
+        foreign call per evaluation); returns the last result."""
+        s, n = _stages(stages)
+        v = C.c_double()
+        valid = C.c_size_t()
+        out = np.empty(12)
+        _check(lib().sitkhip_mattes_value_and_moments_repeat(self._h, _dp(s), n, int(repeats),
+                                                             C.byref(v), _dp(out), C.byref(valid)))
         return v.value, out, valid.value
 
 
