@@ -14,6 +14,7 @@ Training is checked to lower the loss and to be deterministic run to run.
 import numpy as np
 import pytest
 
+from helpers import record_error
 from oracle import cnn as N
 
 pytestmark = pytest.mark.gpu
@@ -203,3 +204,19 @@ def test_adam_steps_match_numpy(hip):
         # first step on the second
         assert np.count_nonzero(got - p) >= np.count_nonzero(g)
         p = got
+
+
+def test_warp_inverse_labels_are_where_the_plain_optimizer_converges(hip):
+    """The training labels are the synthetic warps' inverses. Registered by
+    the plain refinement from those labels, 64 pairs of the training
+    distribution move by a median of well under the prediction error the
+    network reaches (3.3 mm median on held-out pairs): the plain optimizer's
+    converged parameters and the warp inverses are the same labels to that
+    precision."""
+    import initializer as I
+
+    gaps = I.label_gap(64)
+    record_error("learned initializer: warp-inverse label vs plain-optimizer convergence (64 pairs)",
+                 median_mm=float(np.median(gaps)), max_mm=float(np.max(gaps)))
+    assert np.median(gaps) < 0.1, np.median(gaps)
+    assert np.max(gaps) < 0.5, np.max(gaps)
