@@ -4,6 +4,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstddef>
 #include <cstdint>
@@ -141,6 +142,23 @@ struct LaunchScope {
         }
     }
 };
+
+// Enqueues `kernel` on the library stream, profiled under `name`. With the
+// profiler on, the two events ride on the dispatch packet itself
+// (hipExtLaunchKernelGGL), so the recorded span is the kernel's execution as a
+// kernel trace reports it, without the marker packets LaunchScope puts around
+// a launch (about 3 us more per launch on gfx950).
+template <typename F, typename... Args>
+void launch_kernel(const char *name, F kernel, dim3 grid, dim3 block, size_t shm, Args... args) {
+    if (profiling_enabled()) {
+        hipEvent_t a = profile_event(), b = profile_event();
+        hipExtLaunchKernelGGL(kernel, grid, block, static_cast<std::uint32_t>(shm), backend().stream, a, b,
+                              0u, args...);
+        profile_record(name, a, b);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shm, backend().stream, args...);
+    }
+}
 
 // Checks the launch itself (bad configuration) right after it is enqueued.
 void check_launch(const char *name);

@@ -208,9 +208,12 @@ __global__ __launch_bounds__(256) void iir_lines(const SrcT *__restrict__ src, d
 //     v = (((x*c0 + a*c1) + b*c2) + cc*c3) - (((p1*d1 + p2*d2) + p3*d3) + p4*d4),
 // with (c0..c3) = (n0..n3) for the causal lane and (m1..m4) for the anti-causal
 // lane (whose x is data[j+1] for output j), so the loop is lane-uniform.
-// Loads run PD steps ahead of their use.
+// Loads run one to two blocks of kPD steps ahead of their use.
 constexpr int kPairLines = 32;
-constexpr int kPD = 16;
+#ifndef SITKHIP_IIR_PD
+#define SITKHIP_IIR_PD 8
+#endif
+constexpr int kPD = SITKHIP_IIR_PD;
 
 template <typename SrcT, typename DstT>
 __device__ __forceinline__ void iir_pair_body(const SrcT *__restrict__ src, DstT *__restrict__ dst,
@@ -281,38 +284,70 @@ __device__ __forceinline__ void iir_pair_body(const SrcT *__restrict__ src, DstT
     const long long A = role ? s0 + static_cast<long long>(n) * st : s0;
     const long long W = role ? s0 + static_cast<long long>(n - 1) * st : s0;
 
-    SrcT cur[kPD], nxt[kPD];
-#pragma unroll
-    for (int j = 0; j < kPD; ++j) {
-        const int i = 4 + j;
-        cur[j] = i < n ? src[A + i * S] : SrcT(0);
-    }
-    for (int base = 4; base < n; base += kPD) {
+    // Loads run one block of kPD steps ahead, into two register blocks used
+    // in turn (no copies between them, which would wait for the loads). Every
+    // load is unconditional: a step past the line re-reads step n-1's value,
+    // which is never used.
+    auto load_block = [&](SrcT (&buf)[kPD], int base) {
 #pragma unroll
         for (int j = 0; j < kPD; ++j) {
-            const int i = base + kPD + j;
-            nxt[j] = i < n ? src[A + i * S] : SrcT(0);
+            const int i = base + j < n ? base + j : n - 1;
+            buf[j] = src[A + i * S];
         }
+    };
+    auto recur = [&](SrcT v) {
+        const double x = widen(v);
+        double o = x * c0 + a * c1 + b * c2 + cc * c3;
+        o -= p1 * d1 + p2 * d2 + p3 * d3 + p4 * d4;
+        cc = b; b = a; a = x;
+        p4 = p3; p3 = p2; p2 = p1; p1 = o;
+        return o;
+    };
+    // the workgroup is this one wave: before the first read of the partner's
+    // parked values only this wave's LDS writes need to have landed
+    auto lds_sync = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    };
+    auto run_block = [&](const SrcT (&buf)[kPD], int base) {
+        if (base + kPD <= H) {  // every step parks its value
 #pragma unroll
-        for (int j = 0; j < kPD; ++j) {
-            const int i = base + j;
-            if (i < n) {
-                const double x = widen(cur[j]);
-                double o = x * c0 + a * c1 + b * c2 + cc * c3;
-                o -= p1 * d1 + p2 * d2 + p3 * d3 + p4 * d4;
-                cc = b; b = a; a = x;
-                p4 = p3; p3 = p2; p2 = p1; p1 = o;
-                if (i < H) {
-                    mine[i * kPairLines + li] = o;
-                } else {
-                    if (i == H) __syncthreads();
-                    const double q = partner[(n - 1 - i) * kPairLines + li];
-                    if (act) dst[W + i * S] = narrow<DstT>(o + q);
+            for (int j = 0; j < kPD; ++j) mine[(base + j) * kPairLines + li] = recur(buf[j]);
+        } else if (base >= H && base + kPD <= n) {  // every step finishes an output
+            if (base == H) lds_sync();
+            double q[kPD];
+#pragma unroll
+            for (int j = 0; j < kPD; ++j) q[j] = partner[(n - 1 - base - j) * kPairLines + li];
+#pragma unroll
+            for (int j = 0; j < kPD; ++j) {
+                const double o = recur(buf[j]);
+                if (act) dst[W + (base + j) * S] = narrow<DstT>(o + q[j]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kPD; ++j) {
+                const int i = base + j;
+                if (i < n) {
+                    const double o = recur(buf[j]);
+                    if (i < H) {
+                        mine[i * kPairLines + li] = o;
+                    } else {
+                        if (i == H) lds_sync();
+                        const double q = partner[(n - 1 - i) * kPairLines + li];
+                        if (act) dst[W + i * S] = narrow<DstT>(o + q);
+                    }
                 }
             }
         }
-#pragma unroll
-        for (int j = 0; j < kPD; ++j) cur[j] = nxt[j];
+    };
+    SrcT bufA[kPD], bufB[kPD];
+    load_block(bufA, 4);
+    for (int base = 4; base < n; base += 2 * kPD) {
+        load_block(bufB, base + kPD);
+        run_block(bufA, base);
+        load_block(bufA, base + 2 * kPD);
+        run_block(bufB, base + kPD);
     }
     // odd n: the middle element was parked by both lanes at step H-1
     if ((n & 1) && role == 0 && act) {
@@ -388,19 +423,16 @@ static void launch_axis(const SrcT *src, double *tmp, DstT *dst, const sitkhip_g
     static const char *names[3] = {"gauss_iir_axis0", "gauss_iir_axis1", "gauss_iir_axis2"};
     const int H = (L.len + 1) / 2;
     const size_t lds = static_cast<size_t>(2) * H * kPairLines * sizeof(double);
-    LaunchScope scope(names[axis]);
     if (L.len >= 8 && lds <= max_lds_per_block()) {
         unsigned blocks = static_cast<unsigned>((L.nlines + kPairLines - 1) / kPairLines);
         if (lds > 64 * 1024)
             HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void *>(&iir_pair_lines<SrcT, DstT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        static_cast<int>(lds)));
-        hipLaunchKernelGGL((iir_pair_lines<SrcT, DstT>), dim3(blocks), dim3(64), lds,
-                           backend().stream, src, dst, L, k, H);
+        launch_kernel(names[axis], iir_pair_lines<SrcT, DstT>, dim3(blocks), dim3(64), lds, src, dst, L, k, H);
     } else {
         unsigned blocks = static_cast<unsigned>((L.nlines + 255) / 256);
-        hipLaunchKernelGGL((iir_lines<SrcT, DstT>), dim3(blocks), dim3(256), 0, backend().stream,
-                           src, tmp, dst, L, k);
+        launch_kernel(names[axis], iir_lines<SrcT, DstT>, dim3(blocks), dim3(256), 0, src, tmp, dst, L, k);
     }
     check_launch(names[axis]);
 }
@@ -439,9 +471,7 @@ static void enqueue_smoothing(const float *src, float *dst, const sitkhip_geomet
                               double *B) {
     const size_t n = geometry_len(g);
     if (na == 0) {
-        LaunchScope scope("copy_f32");
-        hipLaunchKernelGGL(copy_f32, dim3(grid_for(n, 256, 4096)), dim3(256), 0,
-                           backend().stream, src, dst, n);
+        launch_kernel("copy_f32", copy_f32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, src, dst, n);
         check_launch("copy_f32");
         return;
     }

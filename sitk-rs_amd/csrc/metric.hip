@@ -936,15 +936,9 @@ void launch_tail(Metric &m, TailOut *out, bool clear) {
     // staged histogram plus the tail's own ~15 KB of LDS within 64 KB
     const bool lds = B <= 76;
     const size_t shm = lds ? static_cast<size_t>(B * B + B) * sizeof(unsigned long long) : 0;
-    LaunchScope scope("mattes_tail");
-    if (lds)
-        hipLaunchKernelGGL((mattes_tail_kernel<true>), dim3(1), dim3(kTail), shm, backend().stream,
-                           m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out,
-                           clear ? 1 : 0);
-    else
-        hipLaunchKernelGGL((mattes_tail_kernel<false>), dim3(1), dim3(kTail), 0, backend().stream,
-                           m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out,
-                           clear ? 1 : 0);
+    launch_kernel("mattes_tail", lds ? mattes_tail_kernel<true> : mattes_tail_kernel<false>, dim3(1),
+                  dim3(kTail), shm, m.joint.ptr, m.marg.ptr, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr, out,
+                  clear ? 1 : 0);
     check_launch("mattes_tail");
 }
 
@@ -959,10 +953,9 @@ static int launch_tail_for_derivative(Metric &m, TailOut *out) {
     }
     int T = (B * B + kTail - 1) / kTail;  // about one bin per lane
     if (T > kMaxTailSplit) T = kMaxTailSplit;
-    LaunchScope scope("mattes_tail");
-    hipLaunchKernelGGL(mattes_tail_split, dim3(T), dim3(kTail), 0, backend().stream, m.joint.ptr,
-                       m.marg.ptr, m.marg.ptr + static_cast<size_t>(kHistCopies) * B, B, m.P.mbs,
-                       m.pratio.ptr, m.pratio_f.ptr, out, m.tail_part.ptr);
+    launch_kernel("mattes_tail", mattes_tail_split, dim3(T), dim3(kTail), 0, m.joint.ptr, m.marg.ptr,
+                  m.marg.ptr + static_cast<size_t>(kHistCopies) * B, B, m.P.mbs, m.pratio.ptr, m.pratio_f.ptr,
+                  out, m.tail_part.ptr);
     check_launch("mattes_tail");
     return T;
 }
@@ -993,14 +986,13 @@ static void launch_hist(Metric &m, const Stages &S, bool tail, bool clear = true
     const bool split = B <= 76 && ((rec && !tail) || (tail && clear));
     unsigned long long *col = split ? m.marg.ptr + static_cast<size_t>(kHistCopies) * B : nullptr;
     {
-        LaunchScope scope("mattes_hist");
         const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
         with_map(m, S, [&](const auto &map) {
             using Map = std::decay_t<decltype(map)>;
             auto go = [&](auto kernel, size_t bytes) {
-                hipLaunchKernelGGL(kernel, dim3(kHistGrid), dim3(kHistBlock), bytes, backend().stream, m.F,
-                                   m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, m.joint.ptr,
-                                   m.marg.ptr, col, rec, dq, cap);
+                launch_kernel("mattes_hist", kernel, dim3(kHistGrid), dim3(kHistBlock), bytes, m.F, m.Mv, map,
+                              X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, m.joint.ptr, m.marg.ptr, col, rec, dq,
+                              cap);
             };
             if (m.bspline) {
                 if (lds) {
@@ -1023,11 +1015,9 @@ static void launch_hist(Metric &m, const Stages &S, bool tail, bool clear = true
     if (!tail) return;
     if (split) {
         const int T = launch_tail_for_derivative(m, &m.res_dev->tail);
-        LaunchScope scope("mattes_tail_finish");
-        hipLaunchKernelGGL(mattes_tail_finish, dim3(16), dim3(256), 0, backend().stream,
-                           m.tail_part.ptr, T, &m.res_dev->tail, m.joint.ptr,
-                           static_cast<unsigned>(m.joint.len), m.marg.ptr,
-                           static_cast<unsigned>(m.marg.len));
+        launch_kernel("mattes_tail_finish", mattes_tail_finish, dim3(16), dim3(256), 0, m.tail_part.ptr, T,
+                      &m.res_dev->tail, m.joint.ptr, static_cast<unsigned>(m.joint.len), m.marg.ptr,
+                      static_cast<unsigned>(m.marg.len));
         check_launch("mattes_tail_finish");
         return;
     }
@@ -1040,10 +1030,9 @@ static std::atomic<unsigned long long> g_eval_seq{0};
 
 template <int NS>
 static void launch_fold(Metric &m) {
-    LaunchScope scope("fold_partials");
     m.seq = g_eval_seq.fetch_add(1) + 1;
-    hipLaunchKernelGGL((fold_partials<NS>), dim3(1), dim3(kBlock), 0, backend().stream,
-                       m.partials.ptr, kGrid, m.res_dev->moments, &m.res_dev->done, m.seq);
+    launch_kernel("fold_partials", fold_partials<NS>, dim3(1), dim3(kBlock), 0, m.partials.ptr, kGrid,
+                  m.res_dev->moments, &m.res_dev->done, m.seq);
     check_launch("fold_partials");
 }
 
@@ -1077,14 +1066,13 @@ static void launch_deriv(Metric &m, const Stages &S, const R *pratio, bool clear
     const size_t shm = static_cast<size_t>(B) * B * sizeof(R);
     const bool lds = shm + 2048 <= 64 * 1024;  // plus the reduction's static LDS
     {
-        LaunchScope scope("mattes_deriv");
         const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
         with_map(m, S, [&](const auto &map) {
             using Map = std::decay_t<decltype(map)>;
             auto go = [&](auto kernel, size_t bytes) {
-                hipLaunchKernelGGL(kernel, dim3(kGrid), dim3(kBlock), bytes, backend().stream, m.F,
-                                   m.Mv, map, X, m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio,
-                                   m.partials.ptr, clr, clr_n, clr2, clr2_n);
+                launch_kernel("mattes_deriv", kernel, dim3(kGrid), dim3(kBlock), bytes, m.F, m.Mv, map, X,
+                              m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio, m.partials.ptr, clr, clr_n, clr2,
+                              clr2_n);
             };
             if (m.bspline) {
                 if (lds) go(mattes_deriv<R, true, Map, true>, shm);
@@ -1105,11 +1093,10 @@ static void launch_deriv(Metric &m, const Stages &S, const R *pratio, bool clear
 template <typename R>
 static void launch_deriv_rec(Metric &m, const R *pratio, const SampleRec<R> *rec, int tail_T) {
     {
-        LaunchScope scope("mattes_deriv");
-        hipLaunchKernelGGL(mattes_deriv_rec<R>, dim3(kGrid), dim3(kBlock), 0, backend().stream, m.F, m.Mv,
-                           m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio, rec, m.partials.ptr, m.joint.ptr,
-                           static_cast<unsigned>(m.joint.len), m.marg.ptr, static_cast<unsigned>(m.marg.len),
-                           m.tail_part.ptr, tail_T, &m.res_dev->tail);
+        launch_kernel("mattes_deriv", mattes_deriv_rec<R>, dim3(kGrid), dim3(kBlock), 0, m.F, m.Mv,
+                      m.chain.ptr, m.SS, m.fbin.ptr, m.P, pratio, rec, m.partials.ptr, m.joint.ptr,
+                      static_cast<unsigned>(m.joint.len), m.marg.ptr, static_cast<unsigned>(m.marg.len),
+                      m.tail_part.ptr, tail_T, &m.res_dev->tail);
         check_launch("mattes_deriv");
     }
     launch_fold<12>(m);
@@ -1219,18 +1206,16 @@ void mattes_value_and_moments(Metric &m, const double *stages, int nstage, doubl
 template <typename R, bool MOM>
 static void launch_ms(Metric &m, const Stages &S) {
     {
-        LaunchScope scope(MOM ? "ms_moments" : "ms_value");
+        const char *name = MOM ? "ms_moments" : "ms_value";
         const IndexMap X = make_index_map(m.F, m.Mv, S.s, S.n);
         with_map(m, S, [&](const auto &map) {
             using Map = std::decay_t<decltype(map)>;
             if (m.bspline)
-                hipLaunchKernelGGL((ms_pass<R, MOM, Map, true>), dim3(kGrid), dim3(kBlock), 0,
-                                   backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fval.ptr,
-                                   m.partials.ptr);
+                launch_kernel(name, ms_pass<R, MOM, Map, true>, dim3(kGrid), dim3(kBlock), 0, m.F, m.Mv, map,
+                              X, m.chain.ptr, m.SS, m.fval.ptr, m.partials.ptr);
             else
-                hipLaunchKernelGGL((ms_pass<R, MOM, Map>), dim3(kGrid), dim3(kBlock), 0,
-                                   backend().stream, m.F, m.Mv, map, X, m.chain.ptr, m.SS, m.fval.ptr,
-                                   m.partials.ptr);
+                launch_kernel(name, ms_pass<R, MOM, Map>, dim3(kGrid), dim3(kBlock), 0, m.F, m.Mv, map, X,
+                              m.chain.ptr, m.SS, m.fval.ptr, m.partials.ptr);
         });
         check_launch("ms_pass");
     }
