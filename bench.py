@@ -366,6 +366,7 @@ def registrations_record(S, W, n_pairs, n_plain):
     ok_b = float(np.mean(tre_b <= 0.5))
     ok_p = float(np.mean(tre_p <= 0.5))
     its = np.array([r.iterations for r in res])
+    roof = batch_roofline(S, pairs, fixed, moving)
     return {
         "workload": f"{n_pairs} rigid pairs, edges U{{32..128}} per axis, 1 mm, 40 blobs, "
                     "Euler3D +-15 deg / +-15 mm; half mean squares (every voxel), "
@@ -376,6 +377,73 @@ def registrations_record(S, W, n_pairs, n_plain):
         "plain": {"pairs": k, "seconds": t_plain, "registrations_per_s": k / t_plain,
                   "success_fraction": ok_p, "successful_per_s": ok_p * k / t_plain},
         "speedup": (n_pairs / t_batch) / (k / t_plain),
+        "batch_roofline": roof,
+    }
+
+
+def batch_roofline(S, pairs, fixed, moving, reps=20):
+    """The batched engine's evaluation of every pair's finest level (mean
+    squares on every voxel, Mattes 32 bins on 25% random draws) at the inverse
+    of each generating warp, timed per kernel with the library's profiler.
+    Algorithmic bytes of the first pass (all pairs' values, histograms and
+    moments): mean squares N*4 (fixed value) + M*4 (moving volume), Mattes
+    n*(4 + 1) (sample index, fixed bin) + M*4; the second pass (the Mattes
+    derivative) reads the Mattes part again."""
+    metrics, stages = [], []
+    pass1 = pass2 = 0
+    for i, p in enumerate(pairs):
+        nvox = int(np.prod(p["fixed"].size))
+        if p["metric"] == "mattes":
+            idx = S.draw_samples(nvox, S.SAMPLING_RANDOM, 0.25, 17 + i)
+            m = S.ResidentMattes(fixed[i], moving[i], bins=32, indices=idx)
+            b = len(idx) * 5 + nvox * 4
+            pass1 += b
+            pass2 += b
+        else:
+            m = S.ResidentMetric(fixed[i], moving[i])
+            pass1 += nvox * 8
+        metrics.append(m)
+        stages.append(p["T_gt"].inverse().stage())
+    batch = S.Batch(metrics)
+    stages = np.stack(stages)
+    for _ in range(3):
+        batch.evaluate(stages)
+    # per-kernel times from direct launches (a graph replay is timed whole)
+    tile, grid = batch.tiling()
+    batch.set_tiling(tile, grid, use_graph=False)
+    S.profile_reset()
+    S.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.evaluate(stages)
+    wall_ms = (time.perf_counter() - t0) * 1e3 / reps
+    S.profile_enable(False)
+    kern = {}
+    for name in ("batch_pass1", "batch_tail", "batch_pass2", "batch_fold"):
+        cnt, tot = S.profile_query(name)
+        kern[name] = tot / cnt * 1e3 if cnt else None
+    batch.set_tiling(tile, grid, use_graph=True)
+    for _ in range(3):
+        batch.evaluate(stages)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.evaluate(stages)
+    graph_ms = (time.perf_counter() - t0) * 1e3 / reps
+    batch.free()
+    for m in metrics:
+        m.free()
+    p1_us = kern["batch_pass1"]
+    return {
+        "workload": f"{len(pairs)} pairs' finest level, one batched value + derivative evaluation",
+        "bytes_pass1": pass1,
+        "bytes_pass2": pass2,
+        "kernels_us": kern,
+        "tiling": {"tile_samples": tile, "grid": grid},
+        "ms_per_evaluation_graph": graph_ms,
+        "ms_per_evaluation_direct_profiled": wall_ms,
+        "pass1_GBps": pass1 / (p1_us * 1e-6) / 1e9 if p1_us else None,
+        "pass1_frac": pass1 / (p1_us * 1e-6) / 1e9 / 8000.0 if p1_us else None,
+        "evaluation_GBps": (pass1 + pass2) / (graph_ms * 1e-3) / 1e9,
     }
 
 

@@ -399,57 +399,34 @@ void batch_set_tiling(BatchEngine &b, const BatchTiling &t) {
 }
 
 // The four launches of one evaluation (plus the control copy), in stream
-// order. `profile`: bracket each launch with events (never under capture).
+// order. `profile`: time each launch with events (never under capture).
 template <typename R>
 static void enqueue(BatchEngine &b, bool profile) {
     hipStream_t st = backend().stream;
     HIP_OK(hipMemcpyAsync(b.ctl.ptr, b.ctl_host, b.n * sizeof(PairControl), hipMemcpyHostToDevice,
                           st));
+    // profiled launches carry their events on the dispatch (launch_kernel)
+    auto go = [&](const char *name, auto kernel, dim3 grid, dim3 block, size_t shm, auto... args) {
+        if (profile) launch_kernel(name, kernel, grid, block, shm, args...);
+        else hipLaunchKernelGGL(kernel, grid, block, shm, st, args...);
+        check_launch(name);
+    };
     const unsigned G = b.tiling.grid;
     if (b.ntiles_all > 0) {
         const unsigned g = std::min<unsigned>(G, static_cast<unsigned>((b.ntiles_all + 7) / 8 * 8));
-        if (profile) {
-            LaunchScope s("batch_pass1");
-            hipLaunchKernelGGL(batch_pass1<R>, dim3(g), dim3(kBlock), b.lds_hist, st, b.desc.ptr,
-                               b.tiles_all.ptr, b.seg_all, b.ctl.ptr);
-        } else {
-            hipLaunchKernelGGL(batch_pass1<R>, dim3(g), dim3(kBlock), b.lds_hist, st, b.desc.ptr,
-                               b.tiles_all.ptr, b.seg_all, b.ctl.ptr);
-        }
-        check_launch("batch_pass1");
+        go("batch_pass1", batch_pass1<R>, dim3(g), dim3(kBlock), b.lds_hist, b.desc.ptr, b.tiles_all.ptr,
+           b.seg_all, b.ctl.ptr);
     }
-    if (b.n_mattes > 0) {
-        if (profile) {
-            LaunchScope s("batch_tail");
-            hipLaunchKernelGGL(batch_tail, dim3(b.n_mattes), dim3(kTail), b.lds_tail, st, b.desc.ptr,
-                               b.mattes_pairs.ptr, b.ctl.ptr, b.res_dev);
-        } else {
-            hipLaunchKernelGGL(batch_tail, dim3(b.n_mattes), dim3(kTail), b.lds_tail, st, b.desc.ptr,
-                               b.mattes_pairs.ptr, b.ctl.ptr, b.res_dev);
-        }
-        check_launch("batch_tail");
-    }
+    if (b.n_mattes > 0)
+        go("batch_tail", batch_tail, dim3(b.n_mattes), dim3(kTail), b.lds_tail, b.desc.ptr,
+           b.mattes_pairs.ptr, b.ctl.ptr, b.res_dev);
     if (b.ntiles_mattes > 0) {
         const unsigned g = std::min<unsigned>(G, static_cast<unsigned>((b.ntiles_mattes + 7) / 8 * 8));
-        if (profile) {
-            LaunchScope s("batch_pass2");
-            hipLaunchKernelGGL(batch_pass2<R>, dim3(g), dim3(kBlock), b.lds_pr, st, b.desc.ptr,
-                               b.tiles_mattes.ptr, b.seg_mattes, b.ctl.ptr);
-        } else {
-            hipLaunchKernelGGL(batch_pass2<R>, dim3(g), dim3(kBlock), b.lds_pr, st, b.desc.ptr,
-                               b.tiles_mattes.ptr, b.seg_mattes, b.ctl.ptr);
-        }
-        check_launch("batch_pass2");
+        go("batch_pass2", batch_pass2<R>, dim3(g), dim3(kBlock), b.lds_pr, b.desc.ptr, b.tiles_mattes.ptr,
+           b.seg_mattes, b.ctl.ptr);
     }
-    if (profile) {
-        LaunchScope s("batch_fold");
-        hipLaunchKernelGGL(batch_fold, dim3(static_cast<unsigned>(b.n)), dim3(kBlock), 0, st,
-                           b.desc.ptr, b.ctl.ptr, b.res_dev);
-    } else {
-        hipLaunchKernelGGL(batch_fold, dim3(static_cast<unsigned>(b.n)), dim3(kBlock), 0, st,
-                           b.desc.ptr, b.ctl.ptr, b.res_dev);
-    }
-    check_launch("batch_fold");
+    go("batch_fold", batch_fold, dim3(static_cast<unsigned>(b.n)), dim3(kBlock), 0, b.desc.ptr, b.ctl.ptr,
+       b.res_dev);
 }
 
 static void enqueue_any(BatchEngine &b, bool profile) {

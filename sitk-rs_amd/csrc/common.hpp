@@ -44,35 +44,20 @@ Backend &backend();
 // the backend is initialised there (gaussian.hip).
 void gaussian_scratch_setup(int device);
 
-// While a StreamScope lives, backend() on this thread hands out `s` as the
-// library stream, so independent set-up work can be spread over several
-// streams; the caller orders those streams with events and drains them, with
-// device_defer_frees on meanwhile.
-class StreamScope {
-public:
-    explicit StreamScope(hipStream_t s);
-    ~StreamScope();
-    StreamScope(const StreamScope &) = delete;
-    StreamScope &operator=(const StreamScope &) = delete;
-
-private:
-    hipStream_t prev_;
-};
-
 // Device memory through a caching allocator: a freed block is kept in a
 // per-size-class free list and handed to the next request of that class
 // (hipMalloc and hipFree cost tens of microseconds and hipFree waits for the
 // device). Every operation of this library is ordered on its one stream, so a
 // block freed after an enqueued use and reused by a later enqueued operation
-// is never touched by both at once (work spread over several streams runs
-// with device_defer_frees on). sitkhip_empty_cache() returns the cached
-// blocks to the runtime.
+// is never touched by both at once (work recorded to be enqueued later, as a
+// PrepBatch does, runs with device_defer_frees on until it is enqueued).
+// sitkhip_empty_cache() returns the cached blocks to the runtime.
 void *device_alloc(size_t bytes);
 void device_free(void *ptr, size_t bytes);
 void device_empty_cache();
-// While on, freed blocks are held back instead of being reused (work on
-// several streams is not ordered by one stream); switching it off, once every
-// stream has drained, hands them to the free lists.
+// While on, freed blocks are held back instead of being reused (a use that
+// is recorded but not yet enqueued is not ordered by the stream); switching
+// it off, once every such use is enqueued, hands them to the free lists.
 void device_defer_frees(bool on);
 
 // Pinned, device-mapped host slots of `bytes` (<= 256) for small per-metric

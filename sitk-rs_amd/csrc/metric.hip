@@ -459,15 +459,18 @@ static void index_to_physical(const sitkhip_geometry &g, double out[9]) {
 // mod voxels (metric.rs:101-122, 419-437: the same draws as the host's), and
 // the multiset is laid out in ascending voxel order by a counting sort
 // (counts, exclusive scan, fill) -- the order the host path sorts into.
+__device__ __forceinline__ uint64_t draw_voxel(uint64_t seed, uint64_t voxels, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return z % voxels;
+}
+
 __global__ void draw_counts(uint64_t seed, uint64_t voxels, size_t n, uint32_t *__restrict__ counts) {
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        uint64_t z = seed + static_cast<uint64_t>(i + 1) * 0x9E3779B97F4A7C15ull;
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-        z ^= z >> 31;
-        atomicAdd(&counts[z % voxels], 1u);
-    }
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        atomicAdd(&counts[draw_voxel(seed, voxels, i)], 1u);
 }
 
 __global__ void fill_sorted(const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
@@ -504,6 +507,323 @@ static void device_draws(Metric &m, size_t voxels, size_t n, uint64_t seed) {
     }
 }
 
+// ------------------------------------------------------- batched level set-up
+// Each kernel below serves many jobs of one kind in one launch: workgroup b
+// (256 lanes) takes job job_of[b] and the (b - first)-th chunk of its items;
+// per item the arithmetic is that of the one-job kernel above it.
+constexpr int kPrepLanes = 256;
+constexpr int kPrepItems = 8;  // items per lane and workgroup chunk
+constexpr long long kPrepChunk = static_cast<long long>(kPrepLanes) * kPrepItems;
+constexpr long long kRangeChunk = static_cast<long long>(kPrepLanes) * 64;
+
+struct CopyJob {  // src == nullptr: fill with zeros
+    const uint32_t *src;
+    uint32_t *dst;
+    const uint8_t *src_tail;
+    uint8_t *dst_tail;
+    long long words, tail;
+    unsigned first;
+};
+struct DrawJob {
+    uint64_t seed, voxels, n;
+    uint32_t *counts;
+    unsigned first;
+};
+struct FillJob {
+    const uint32_t *counts, *offsets;
+    uint64_t voxels;
+    uint32_t base;  // the scan's value at the job's first voxel
+    uint32_t *out;
+    unsigned first;
+};
+struct GatherJob {
+    const uint32_t *idx;
+    long long n;
+    const float *src;
+    float *out;
+    unsigned first;
+};
+struct RangeJob {
+    const float *v;
+    const uint8_t *mask;
+    long long n;
+    double *out;
+    unsigned first, nblocks;
+};
+struct BinsJob {
+    const float *fval;
+    long long n;
+    int B;
+    double fbs, fnmin;
+    uint8_t *out;
+    unsigned first;
+};
+
+__global__ __launch_bounds__(kPrepLanes) void copy_batch(const CopyJob *__restrict__ jobs,
+                                                         const int *__restrict__ job_of) {
+    const CopyJob &J = jobs[job_of[blockIdx.x]];
+    const long long c = static_cast<long long>(blockIdx.x - J.first) * kPrepChunk;
+#pragma unroll
+    for (int k = 0; k < kPrepItems; ++k) {
+        const long long w = c + k * kPrepLanes + threadIdx.x;
+        if (w < J.words) J.dst[w] = J.src ? J.src[w] : 0u;
+    }
+    if (c == 0 && threadIdx.x < J.tail) J.dst_tail[threadIdx.x] = J.src ? J.src_tail[threadIdx.x] : 0u;
+}
+
+__global__ __launch_bounds__(kPrepLanes) void draw_counts_batch(const DrawJob *__restrict__ jobs,
+                                                                const int *__restrict__ job_of) {
+    const DrawJob &J = jobs[job_of[blockIdx.x]];
+    const long long c = static_cast<long long>(blockIdx.x - J.first) * kPrepChunk;
+#pragma unroll
+    for (int k = 0; k < kPrepItems; ++k) {
+        const long long i = c + k * kPrepLanes + threadIdx.x;
+        if (i < static_cast<long long>(J.n))
+            atomicAdd(&J.counts[draw_voxel(J.seed, J.voxels, static_cast<uint64_t>(i))], 1u);
+    }
+}
+
+__global__ __launch_bounds__(kPrepLanes) void fill_sorted_batch(const FillJob *__restrict__ jobs,
+                                                                const int *__restrict__ job_of) {
+    const FillJob &J = jobs[job_of[blockIdx.x]];
+    const long long c = static_cast<long long>(blockIdx.x - J.first) * kPrepChunk;
+#pragma unroll
+    for (int k = 0; k < kPrepItems; ++k) {
+        const long long v = c + k * kPrepLanes + threadIdx.x;
+        if (v < static_cast<long long>(J.voxels)) {
+            const uint32_t n = J.counts[v], o = J.offsets[v] - J.base;
+            for (uint32_t q = 0; q < n; ++q) J.out[o + q] = static_cast<uint32_t>(v);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kPrepLanes) void gather_batch(const GatherJob *__restrict__ jobs,
+                                                           const int *__restrict__ job_of) {
+    const GatherJob &J = jobs[job_of[blockIdx.x]];
+    const long long c = static_cast<long long>(blockIdx.x - J.first) * kPrepChunk;
+#pragma unroll
+    for (int k = 0; k < kPrepItems; ++k) {
+        const long long s = c + k * kPrepLanes + threadIdx.x;
+        if (s < J.n) J.out[s] = J.src[J.idx[s]];
+    }
+}
+
+// (min, max, count) of one chunk of kRangeChunk values per workgroup, into
+// partials[3 * b]; range_fold_batch folds a job's chunks in order. min and
+// max are selections and the count an integer: exact in any order.
+__global__ __launch_bounds__(kPrepLanes) void range_partials_batch(const RangeJob *__restrict__ jobs,
+                                                                   const int *__restrict__ job_of,
+                                                                   double *__restrict__ partials) {
+    __shared__ double smin[kPrepLanes], smax[kPrepLanes], scnt[kPrepLanes];
+    const RangeJob &J = jobs[job_of[blockIdx.x]];
+    const long long c = static_cast<long long>(blockIdx.x - J.first) * kRangeChunk;
+    const long long e = c + kRangeChunk < J.n ? c + kRangeChunk : J.n;
+    double lo = INFINITY, hi = -INFINITY, cnt = 0.0;
+    for (long long i = c + threadIdx.x; i < e; i += kPrepLanes) {
+        if (J.mask && !J.mask[i]) continue;
+        const double x = static_cast<double>(J.v[i]);
+        lo = fmin(lo, x);
+        hi = fmax(hi, x);
+        cnt += 1.0;
+    }
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    scnt[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int w = kPrepLanes / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+            scnt[threadIdx.x] += scnt[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x + 0] = smin[0];
+        partials[3 * blockIdx.x + 1] = smax[0];
+        partials[3 * blockIdx.x + 2] = scnt[0];
+    }
+}
+
+__global__ void range_fold_batch(const RangeJob *__restrict__ jobs, const double *__restrict__ partials) {
+    const RangeJob &J = jobs[blockIdx.x];
+    if (threadIdx.x != 0) return;
+    double lo = INFINITY, hi = -INFINITY, cnt = 0.0;
+    for (unsigned b = 0; b < J.nblocks; ++b) {
+        const double *p = partials + 3 * static_cast<size_t>(J.first + b);
+        lo = fmin(lo, p[0]);
+        hi = fmax(hi, p[1]);
+        cnt += p[2];
+    }
+    J.out[0] = lo;
+    J.out[1] = hi;
+    J.out[2] = cnt;
+}
+
+__global__ __launch_bounds__(kPrepLanes) void fixed_bins_batch(const BinsJob *__restrict__ jobs,
+                                                               const int *__restrict__ job_of) {
+    const BinsJob &J = jobs[job_of[blockIdx.x]];
+    const long long c = static_cast<long long>(blockIdx.x - J.first) * kPrepChunk;
+#pragma unroll
+    for (int k = 0; k < kPrepItems; ++k) {
+        const long long s = c + k * kPrepLanes + threadIdx.x;
+        if (s < J.n) {
+            const double v = static_cast<double>(J.fval[s]);
+            const double term = v / J.fbs - J.fnmin;
+            long long idx = static_cast<long long>(term);
+            if (idx < 2) idx = 2;
+            else if (idx > J.B - 3) idx = J.B - 3;
+            J.out[s] = static_cast<uint8_t>(idx);
+        }
+    }
+}
+
+// Appends job j's chunks to the workgroup map; returns its first workgroup.
+static unsigned map_chunks(std::vector<int> &job_of, long long items, long long chunk, int j) {
+    const unsigned first = static_cast<unsigned>(job_of.size());
+    const long long nb = (items + chunk - 1) / chunk;
+    job_of.insert(job_of.end(), static_cast<size_t>(nb), j);
+    if (job_of.size() >= (1ull << 31)) fail(SITKHIP_ERR_INVALID_ARGUMENT, "batched set-up too large");
+    return first;
+}
+
+template <typename Job, typename Kernel>
+static void launch_jobs(PrepBatch &pb, const char *name, Kernel kernel, const std::vector<Job> &jobs,
+                        const std::vector<int> &job_of) {
+    if (job_of.empty()) return;
+    const Job *dj = pb.upload(jobs);
+    const int *dm = pb.upload(job_of);
+    launch_kernel(name, kernel, dim3(static_cast<unsigned>(job_of.size())), dim3(kPrepLanes), 0, dj, dm);
+    check_launch(name);
+}
+
+static void flush_copies(PrepBatch &pb) {
+    std::vector<CopyJob> jobs;
+    std::vector<int> job_of;
+    auto add = [&](const void *src, void *dst, size_t bytes) {
+        if (bytes == 0) return;
+        if ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) % 4 != 0)
+            fail(SITKHIP_ERR_DRIVER, "batched set-up copy of unaligned memory");
+        CopyJob J{};
+        J.src = static_cast<const uint32_t *>(src);
+        J.dst = static_cast<uint32_t *>(dst);
+        J.words = static_cast<long long>(bytes / 4);
+        J.tail = static_cast<long long>(bytes % 4);
+        J.src_tail = src ? static_cast<const uint8_t *>(src) + 4 * J.words : nullptr;
+        J.dst_tail = static_cast<uint8_t *>(dst) + 4 * J.words;
+        J.first = map_chunks(job_of, J.words > 0 ? J.words : 1, kPrepChunk, static_cast<int>(jobs.size()));
+        jobs.push_back(J);
+    };
+    for (const auto &c : pb.copies) add(c.src, c.dst, c.bytes);
+    for (const auto &z : pb.zeros) add(nullptr, z.dst, z.bytes);
+    launch_jobs(pb, "copy_batch", copy_batch, jobs, job_of);
+    pb.copies.clear();
+    pb.zeros.clear();
+}
+
+// The draws of every job, laid out like device_draws: counts per voxel over
+// all jobs' grids end to end, one exclusive scan, then each job's samples in
+// ascending voxel order. Groups keep the scan's int item count and the
+// offsets' 32 bits in range.
+static void flush_draws(PrepBatch &pb) {
+    size_t i0 = 0;
+    const auto &D = pb.draws;
+    while (i0 < D.size()) {
+        size_t i1 = i0;
+        uint64_t vox = 0, draws = 0;
+        while (i1 < D.size() && (i1 == i0 || (vox + D[i1].voxels < (1ull << 31) &&
+                                             draws + D[i1].n < (1ull << 32)))) {
+            vox += D[i1].voxels;
+            draws += D[i1].n;
+            ++i1;
+        }
+        if (vox >= (1ull << 31) || draws >= (1ull << 32))
+            fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "device draws need a fixed grid below 2^31 voxels");
+        pb.keep.emplace_back(static_cast<size_t>(vox) * 2 * sizeof(uint32_t));
+        uint32_t *counts = reinterpret_cast<uint32_t *>(pb.keep.back().ptr);
+        uint32_t *offsets = counts + vox;
+        std::vector<DrawJob> dj;
+        std::vector<FillJob> fj;
+        std::vector<int> dmap, fmap;
+        uint64_t vbase = 0, nbase = 0;
+        for (size_t i = i0; i < i1; ++i) {
+            DrawJob d{D[i].seed, D[i].voxels, D[i].n, counts + vbase, 0};
+            d.first = map_chunks(dmap, static_cast<long long>(D[i].n), kPrepChunk, static_cast<int>(dj.size()));
+            if (D[i].n > 0) dj.push_back(d);
+            else dmap.resize(d.first);
+            FillJob f{counts + vbase, offsets + vbase, D[i].voxels, static_cast<uint32_t>(nbase), D[i].out, 0};
+            f.first = map_chunks(fmap, static_cast<long long>(D[i].voxels), kPrepChunk, static_cast<int>(fj.size()));
+            fj.push_back(f);
+            vbase += D[i].voxels;
+            nbase += D[i].n;
+        }
+        PrepBatch::Zero z{counts, static_cast<size_t>(vox) * sizeof(uint32_t)};
+        pb.zeros.push_back(z);
+        flush_copies(pb);
+        launch_jobs(pb, "draw_counts_batch", draw_counts_batch, dj, dmap);
+        size_t temp = 0;
+        HIP_OK(hipcub::DeviceScan::ExclusiveSum(nullptr, temp, counts, offsets, static_cast<int>(vox),
+                                                backend().stream));
+        pb.keep.emplace_back(temp);
+        HIP_OK(hipcub::DeviceScan::ExclusiveSum(pb.keep.back().ptr, temp, counts, offsets,
+                                                static_cast<int>(vox), backend().stream));
+        launch_jobs(pb, "fill_sorted_batch", fill_sorted_batch, fj, fmap);
+        i0 = i1;
+    }
+    pb.draws.clear();
+}
+
+void PrepBatch::flush() {
+    enqueue_placements(*this);
+    flush_copies(*this);
+    flush_draws(*this);
+    {
+        std::vector<GatherJob> jobs;
+        std::vector<int> job_of;
+        for (const auto &g : gathers) {
+            if (g.n <= 0) continue;
+            GatherJob J{g.idx, g.n, g.src, g.out, 0};
+            J.first = map_chunks(job_of, g.n, kPrepChunk, static_cast<int>(jobs.size()));
+            jobs.push_back(J);
+        }
+        launch_jobs(*this, "gather_batch", gather_batch, jobs, job_of);
+        gathers.clear();
+    }
+    if (!ranges.empty()) {
+        std::vector<RangeJob> jobs;
+        std::vector<int> job_of;
+        for (const auto &r : ranges) {
+            RangeJob J{r.v, r.mask, r.n, r.out, 0, 0};
+            J.first = map_chunks(job_of, r.n > 0 ? r.n : 1, kRangeChunk, static_cast<int>(jobs.size()));
+            J.nblocks = static_cast<unsigned>(job_of.size()) - J.first;
+            jobs.push_back(J);
+        }
+        const RangeJob *dj = upload(jobs);
+        const int *dm = upload(job_of);
+        keep.emplace_back(job_of.size() * 3 * sizeof(double));
+        double *part = reinterpret_cast<double *>(keep.back().ptr);
+        launch_kernel("range_partials_batch", range_partials_batch, dim3(static_cast<unsigned>(job_of.size())),
+                      dim3(kPrepLanes), 0, dj, dm, part);
+        check_launch("range_partials_batch");
+        launch_kernel("range_fold_batch", range_fold_batch, dim3(static_cast<unsigned>(jobs.size())), dim3(64),
+                      0, dj, static_cast<const double *>(part));
+        check_launch("range_fold_batch");
+        ranges.clear();
+    }
+    {
+        std::vector<BinsJob> jobs;
+        std::vector<int> job_of;
+        for (const auto &b : bins) {
+            if (b.n <= 0) continue;
+            BinsJob J{b.fval, b.n, b.B, b.fbs, b.fnmin, b.out, 0};
+            J.first = map_chunks(job_of, b.n, kPrepChunk, static_cast<int>(jobs.size()));
+            jobs.push_back(J);
+        }
+        launch_jobs(*this, "fixed_bins_batch", fixed_bins_batch, jobs, job_of);
+        bins.clear();
+    }
+}
+
 static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *indices,
                                   size_t n_indices, const Mask *fmask, const Image &moving,
                                   const Mask *mmask, int precision, bool draw_on_device,
@@ -527,6 +847,10 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
         fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "fixed mask geometry differs from the fixed image");
     if (mmask && !same_geometry(mmask->geom, mg))
         fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "moving mask geometry differs from the moving image");
+    PrepBatch *prep = opt.prep;
+    if (prep && (fmask || opt.points || !opt.defer_ranges))
+        fail(SITKHIP_ERR_INVALID_ARGUMENT,
+             "a batched set-up takes no fixed mask or explicit points and defers the ranges");
 
     auto *m = new Metric();
     try {
@@ -566,7 +890,13 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
             if (nf >= (1ull << 31))
                 fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY,
                      "device draws need a fixed grid below 2^31 voxels");
-            device_draws(*m, nf, n_draws, draw_seed);
+            if (prep) {
+                m->idx.alloc(n_draws);
+                prep->draws.push_back({static_cast<uint64_t>(nf), static_cast<uint64_t>(n_draws), draw_seed,
+                                       m->idx.ptr});
+            } else {
+                device_draws(*m, nf, n_draws, draw_seed);
+            }
         } else if (!grid) {
             list.reserve(n_indices);
             for (size_t s = 0; s < n_indices; ++s) {
@@ -616,8 +946,14 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
             m->fval = std::move(opt.adopt_fixed->buf);
         } else if (grid) {
             m->fval.alloc(m->nsamples);
-            HIP_OK(hipMemcpyAsync(m->fval.ptr, fixed.buf.ptr, nf * sizeof(float),
-                                  hipMemcpyDeviceToDevice, backend().stream));
+            if (prep)
+                prep->copies.push_back({fixed.buf.ptr, m->fval.ptr, nf * sizeof(float)});
+            else
+                HIP_OK(hipMemcpyAsync(m->fval.ptr, fixed.buf.ptr, nf * sizeof(float),
+                                      hipMemcpyDeviceToDevice, backend().stream));
+        } else if (m->nsamples > 0 && prep) {
+            m->fval.alloc(m->nsamples);
+            prep->gathers.push_back({m->idx.ptr, m->SS.n, fixed.buf.ptr, m->fval.ptr});
         } else if (m->nsamples > 0) {
             m->fval.alloc(m->nsamples);
             LaunchScope scope("gather_values");
@@ -631,6 +967,9 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
         const size_t nm = moving.len();
         if (opt.adopt_moving == &moving) {
             m->moving = std::move(opt.adopt_moving->buf);
+        } else if (prep) {
+            m->moving.alloc(nm);
+            prep->copies.push_back({moving.buf.ptr, m->moving.ptr, nm * sizeof(float)});
         } else {
             m->moving.alloc(nm);
             HIP_OK(hipMemcpyAsync(m->moving.ptr, moving.buf.ptr, nm * sizeof(float),
@@ -638,8 +977,11 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
         }
         if (mmask) {
             m->mmask.alloc(nm);
-            HIP_OK(hipMemcpyAsync(m->mmask.ptr, mmask->buf.ptr, nm, hipMemcpyDeviceToDevice,
-                                  backend().stream));
+            if (prep)
+                prep->copies.push_back({mmask->buf.ptr, m->mmask.ptr, nm});
+            else
+                HIP_OK(hipMemcpyAsync(m->mmask.ptr, mmask->buf.ptr, nm, hipMemcpyDeviceToDevice,
+                                      backend().stream));
         }
         m->Mv.vol = m->moving.ptr;
         m->Mv.mask = mmask ? m->mmask.ptr : nullptr;
@@ -666,12 +1008,15 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
             }
             // from a pinned slot: a copy from pageable memory would wait for
             // the stream to reach it
-            void *unused = nullptr;
-            m->chain_host = pinned_slot_alloc(sizeof(ChainConst), &unused);
+            void *chain_dev = nullptr;
+            m->chain_host = pinned_slot_alloc(sizeof(ChainConst), &chain_dev);
             std::memcpy(m->chain_host, &cc, sizeof(ChainConst));
             m->chain.alloc(1);
-            HIP_OK(hipMemcpyAsync(m->chain.ptr, m->chain_host, sizeof(ChainConst),
-                                  hipMemcpyHostToDevice, backend().stream));
+            if (prep)  // the kernel reads the pinned slot through its device view
+                prep->copies.push_back({chain_dev, m->chain.ptr, sizeof(ChainConst)});
+            else
+                HIP_OK(hipMemcpyAsync(m->chain.ptr, m->chain_host, sizeof(ChainConst),
+                                      hipMemcpyHostToDevice, backend().stream));
         }
 
         m->partials.alloc(kGrid * 14);
@@ -681,7 +1026,12 @@ static Metric *metric_create_impl(int kind, const Image &fixed, const int64_t *i
         m->res->done = 0;
 
         m->moving_voxels = static_cast<long long>(nm);
-        if (opt.ranges) {
+        if (opt.ranges && prep) {
+            prep->ranges.push_back({m->fval.ptr, nullptr, m->SS.n, m->res_dev->ranges});
+            prep->ranges.push_back({m->moving.ptr, m->Mv.mask, static_cast<long long>(nm),
+                                    m->res_dev->ranges + 3});
+            m->ranges_pending = true;
+        } else if (opt.ranges) {
             enqueue_ranges(*m, static_cast<long long>(nm));
             if (!opt.defer_ranges) {
                 sync();
@@ -862,7 +1212,7 @@ void mattes_bins_from_ranges(const double fr[2], const double mr[2], size_t bins
     out->moving_true_max = mr[1];
 }
 
-void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b) {
+void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b, PrepBatch *prep) {
     if (m.kind != SITKHIP_METRIC_MATTES) fail(SITKHIP_ERR_INVALID_ARGUMENT, "not a Mattes metric");
     if (b.bins < 5) fail(SITKHIP_ERR_TOO_FEW_BINS, "Mattes needs at least 5 histogram bins");
     if (b.bins > SITKHIP_MAX_BINS)
@@ -887,7 +1237,9 @@ void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b) {
     if (static_cast<double>(m.P.mmax_f) < b.moving_true_max)
         m.P.mmax_f = std::nextafter(m.P.mmax_f, INFINITY);
     m.fbin.alloc(m.nsamples);
-    if (m.nsamples > 0) {
+    if (prep) {
+        prep->bins.push_back({m.fval.ptr, m.SS.n, B, b.fixed_bin_size, b.fixed_normalized_min, m.fbin.ptr});
+    } else if (m.nsamples > 0) {
         LaunchScope scope("fixed_bins");
         hipLaunchKernelGGL(fixed_bins, dim3(grid_for(m.nsamples, 256, 4096)), dim3(256), 0,
                            backend().stream, m.SS, m.fval.ptr, B, b.fixed_bin_size,
@@ -901,8 +1253,13 @@ void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b) {
     m.pratio.alloc(static_cast<size_t>(B) * B);
     m.pratio_f.alloc(static_cast<size_t>(B) * B);
     m.tail_part.alloc(2 * kMaxTailSplit + 1);
-    HIP_OK(hipMemsetAsync(m.joint.ptr, 0, m.joint.bytes(), backend().stream));
-    HIP_OK(hipMemsetAsync(m.marg.ptr, 0, m.marg.bytes(), backend().stream));
+    if (prep) {
+        prep->zeros.push_back({m.joint.ptr, m.joint.bytes()});
+        prep->zeros.push_back({m.marg.ptr, m.marg.bytes()});
+    } else {
+        HIP_OK(hipMemsetAsync(m.joint.ptr, 0, m.joint.bytes(), backend().stream));
+        HIP_OK(hipMemsetAsync(m.marg.ptr, 0, m.marg.bytes(), backend().stream));
+    }
     m.bins_set = true;
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <array>
+#include <cstring>
 #include <vector>
 
 #include "common.hpp"
@@ -11,6 +12,57 @@
 namespace sitkhip {
 
 struct Metric;
+
+// The device work of setting up many metrics, one launch per kind of work
+// for all of them (the batched registration driver prepares a level of every
+// pair this way instead of some ten launches per pair). Creation calls given a
+// PrepBatch (CreateOptions::prep, resample_linear_into, mattes_set_bins) record
+// their work here; flush() enqueues each kind for every recorded item at once,
+// in dependency order: placements, copies and clears, draws (counted, scanned
+// and laid out in ascending voxel order), gathers, value ranges, fixed bins.
+// Records hold device addresses and geometry by value (the Image objects may
+// be gone by the flush); the device memory they point to must stay allocated
+// until the flushed work has run (device_defer_frees while recording). The
+// PrepBatch keeps its own descriptors until it is destroyed, which must be
+// after the stream has drained.
+struct PrepBatch {
+    struct Place {  // identity resample of in onto out's grid (the images' buffers and geometry)
+        const float *in;
+        sitkhip_geometry in_geom;
+        float *out;
+        sitkhip_geometry out_geom;
+    };
+    struct Copy { const void *src; void *dst; size_t bytes; };  // src device-readable
+    struct Zero { void *dst; size_t bytes; };
+    struct Draw { uint64_t voxels, n, seed; uint32_t *out; };
+    struct Gather { const uint32_t *idx; long long n; const float *src; float *out; };
+    struct Range { const float *v; const uint8_t *mask; long long n; double *out; };  // out[3]
+    struct Bins { const float *fval; long long n; int B; double fbs, fnmin; uint8_t *out; };
+    std::vector<Place> places;
+    std::vector<Copy> copies;
+    std::vector<Zero> zeros;
+    std::vector<Draw> draws;
+    std::vector<Gather> gathers;
+    std::vector<Range> ranges;
+    std::vector<Bins> bins;
+    // enqueues everything recorded so far (on the library stream) and clears
+    // the records
+    void flush();
+    // the flushed launches' descriptors, device copies and their host sources
+    std::vector<DeviceBuffer<unsigned char>> keep;
+    std::vector<std::vector<unsigned char>> keep_host;
+    // a device copy of v, kept with the batch
+    template <typename T>
+    T *upload(const std::vector<T> &v) {
+        std::vector<unsigned char> h(v.size() * sizeof(T));
+        if (!h.empty()) std::memcpy(h.data(), v.data(), h.size());
+        keep.emplace_back(h.size());
+        keep_host.push_back(std::move(h));
+        DeviceBuffer<unsigned char> &d = keep.back();
+        if (!keep_host.back().empty()) d.upload(keep_host.back().data(), keep_host.back().size());
+        return reinterpret_cast<T *>(d.ptr);
+    }
+};
 
 Metric *metric_create(int kind, const Image &fixed, const int64_t *indices, size_t n_indices,
                       const Mask *fmask, const Image &moving, const Mask *mmask, int precision);
@@ -37,6 +89,7 @@ struct CreateOptions {
     const double *points = nullptr;
     const Mask *fixed_mask = nullptr;   // metric_create_with: gates the sample list
     const Mask *moving_mask = nullptr;  // metric_create_with / _drawn
+    PrepBatch *prep = nullptr;          // record the device work there (no fixed mask)
 };
 Metric *metric_create_drawn(int kind, const Image &fixed, size_t n_draws, uint64_t seed,
                             const Image &moving, int precision,
@@ -61,7 +114,7 @@ size_t metric_sample_count(const Metric &m);
 void metric_value_ranges(Metric &m, double fixed_range[2], double moving_range[2]);
 void mattes_bins_from_ranges(const double fr[2], const double mr[2], size_t bins,
                              sitkhip_mattes_bins *out);
-void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b);
+void mattes_set_bins(Metric &m, const sitkhip_mattes_bins &b, PrepBatch *prep = nullptr);
 void mattes_value(Metric &m, const double *stages, int nstage, double *value, size_t *valid);
 void mattes_value_and_moments(Metric &m, const double *stages, int nstage, double *value,
                               double moments[12], size_t *valid);
@@ -79,6 +132,11 @@ void shrunk_geometry(const sitkhip_geometry &g, const size_t f[3], sitkhip_geome
 // C ABI entry points wait; internal callers that go on enqueueing need not).
 Image *resample_linear(const Image &in, const sitkhip_geometry &og, const double *stages,
                        int nstage, double dflt, bool synchronous = true);
+// The identity placement of `in` onto grid og (default 0) recorded in `prep`;
+// the returned image holds the values once prep has been flushed and run.
+Image *resample_linear_into(const Image &in, const sitkhip_geometry &og, PrepBatch &prep);
+// Enqueues the recorded placements, one launch for all (pyramid.hip).
+void enqueue_placements(PrepBatch &prep);
 Image *recursive_gaussian(const Image &in, const double sigma[3], bool synchronous = true);
 // A mask placed on grid og by the nearest-neighbour resample, default 0.
 Mask *resample_nearest_mask(const Mask &in, const sitkhip_geometry &og, const double *stages,

@@ -54,7 +54,8 @@ struct GridMap {
 };
 
 // cindex_of (pyramid.rs:479-560).
-__device__ __forceinline__ bool resample_cindex(const GridMap &G, const Stages &S, uint32_t o,
+template <typename Map>
+__device__ __forceinline__ bool resample_cindex(const GridMap &G, const Map &S, uint32_t o,
                                                 double c[3]) {
     double x[3], p[3];
     fixed_point(G.out, o, x);
@@ -77,10 +78,10 @@ __device__ __forceinline__ bool resample_cindex(const GridMap &G, const Stages &
     return inside;
 }
 
-__global__ void resample_linear3(const float *__restrict__ in, float *__restrict__ out, GridMap G,
-                                 Stages S, double dflt, uint32_t n_out) {
-    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= n_out) return;
+// Output voxel o of the linear resample (resample_linear3, pyramid.rs:562-600).
+template <typename Map>
+__device__ __forceinline__ float linear_voxel(const float *__restrict__ in, const GridMap &G,
+                                              const Map &S, double dflt, uint32_t o) {
     double c[3];
     double v = dflt;
     if (resample_cindex(G, S, o, c)) {
@@ -109,7 +110,36 @@ __global__ void resample_linear3(const float *__restrict__ in, float *__restrict
         }
         v = acc;
     }
-    out[o] = static_cast<float>(v);
+    return static_cast<float>(v);
+}
+
+__global__ void resample_linear3(const float *__restrict__ in, float *__restrict__ out, GridMap G,
+                                 Stages S, double dflt, uint32_t n_out) {
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n_out) return;
+    out[o] = linear_voxel(in, G, S, dflt, o);
+}
+
+// Many identity placements (prepare_level's resample through
+// AffineTransform::identity, method.rs:2462-2466) in one launch: workgroup b
+// takes placement job_of[b], output voxels (b - first) * 256 ... of it; the
+// per-voxel arithmetic is resample_linear3's with the one identity stage.
+struct PlaceJob {
+    const float *in;
+    float *out;
+    GridMap G;
+    uint32_t n_out;
+    unsigned first_block;
+};
+
+__global__ void place_linear_batch(const PlaceJob *__restrict__ jobs, const int *__restrict__ job_of) {
+    const PlaceJob &J = jobs[job_of[blockIdx.x]];
+    const uint32_t o = (blockIdx.x - J.first_block) * blockDim.x + threadIdx.x;
+    if (o >= J.n_out) return;
+    OneStage I;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) I.m[k] = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+    J.out[o] = linear_voxel(J.in, J.G, I, 0.0, o);
 }
 
 template <typename T>
@@ -204,6 +234,39 @@ Image *resample_linear(const Image &in, const sitkhip_geometry &og, const double
     }
     if (synchronous) sync();
     return out;
+}
+
+Image *resample_linear_into(const Image &in, const sitkhip_geometry &og, PrepBatch &prep) {
+    validate_geometry(og, "resample output");
+    const size_t n = geometry_len(og);
+    if (n >= (1ull << 32)) fail(SITKHIP_ERR_UNSUPPORTED_GEOMETRY, "output above 2^32 voxels");
+    (void)grid_map(in.geom, og);  // its refusals, now
+    Image *out = image_like(og);
+    prep.places.push_back({in.buf.ptr, in.geom, out->buf.ptr, og});
+    return out;
+}
+
+void enqueue_placements(PrepBatch &prep) {
+    if (prep.places.empty()) return;
+    std::vector<PlaceJob> jobs;
+    std::vector<int> job_of;
+    for (const PrepBatch::Place &p : prep.places) {
+        PlaceJob J{};
+        J.in = p.in;
+        J.out = p.out;
+        J.G = grid_map(p.in_geom, p.out_geom);
+        J.n_out = static_cast<uint32_t>(geometry_len(p.out_geom));
+        J.first_block = static_cast<unsigned>(job_of.size());
+        job_of.insert(job_of.end(), (J.n_out + 255) / 256, static_cast<int>(jobs.size()));
+        jobs.push_back(J);
+    }
+    if (job_of.empty()) return;
+    const PlaceJob *dj = prep.upload(jobs);
+    const int *dm = prep.upload(job_of);
+    launch_kernel("place_linear_batch", place_linear_batch, dim3(static_cast<unsigned>(job_of.size())),
+                  dim3(256), 0, dj, dm);
+    check_launch("place_linear_batch");
+    prep.places.clear();
 }
 
 Mask *resample_nearest_mask(const Mask &in, const sitkhip_geometry &og, const double *stages,

@@ -401,10 +401,14 @@ struct ImageHolder {
 // metric's value ranges are left pending (prepare_level_end reads them).
 // smoothed_fixed / smoothed_moving: the level's smoothed images when the
 // caller made them (taken over and freed here), else this smooths.
+// prep: the placement and the metric's set-up are recorded there (not for a
+// level with a fixed mask, which is set up at once).
 static LevelInputs prepare_level_begin(const Image &fixed, const Image &moving,
                                        const sitkhip_registration_config &cfg, int level,
                                        Image *smoothed_fixed = nullptr,
-                                       Image *smoothed_moving = nullptr) {
+                                       Image *smoothed_moving = nullptr,
+                                       PrepBatch *prep = nullptr) {
+    if (cfg.fixed_mask) prep = nullptr;
     const double s = cfg.sigma[level];
     const double sig[3] = {s, s, s};
     const size_t *f = cfg.shrink[level];
@@ -423,7 +427,8 @@ static LevelInputs prepare_level_begin(const Image &fixed, const Image &moving,
         sitkhip_geometry grid;
         long long off[3];
         shrunk_geometry(fixed.geom, f, grid, off);
-        fl.img = resample_linear(*sf.img, grid, nullptr, 0, 0.0, false);
+        fl.img = prep ? resample_linear_into(*sf.img, grid, *prep)
+                      : resample_linear(*sf.img, grid, nullptr, 0, 0.0, false);
         fl.owned = true;
     }
     const Image &fixed_level = *fl.img;
@@ -448,6 +453,7 @@ static LevelInputs prepare_level_begin(const Image &fixed, const Image &moving,
         opt.fixed_mask = level_mask.get();
     }
     opt.moving_mask = reinterpret_cast<const Mask *>(cfg.moving_mask);
+    opt.prep = prep;
     if (cfg.sampling == 2 && level_mask) {
         // the same draw on the host, filtered by the level mask at creation
         // (FixedSamples::from_image_with, metric.rs:494-580)
@@ -476,8 +482,9 @@ static LevelInputs prepare_level_begin(const Image &fixed, const Image &moving,
 }
 
 // After the stream has drained: the value ranges, and for Mattes the
-// histogram geometry from them.
-static void prepare_level_end(LevelInputs &in, const sitkhip_registration_config &cfg) {
+// histogram geometry from them (its fixed bins recorded in prep when given).
+static void prepare_level_end(LevelInputs &in, const sitkhip_registration_config &cfg,
+                              PrepBatch *prep = nullptr) {
     metric_finish_ranges(*in.metric);
     if (cfg.interpolator != SITKHIP_INTERP_LINEAR) metric_set_interpolator(*in.metric, cfg.interpolator);
     if (cfg.metric == SITKHIP_METRIC_MATTES) {
@@ -485,27 +492,8 @@ static void prepare_level_end(LevelInputs &in, const sitkhip_registration_config
         metric_value_ranges(*in.metric, fr, mr);
         sitkhip_mattes_bins b;
         mattes_bins_from_ranges(fr, mr, static_cast<size_t>(cfg.bins), &b);
-        mattes_set_bins(*in.metric, b);
+        mattes_set_bins(*in.metric, b, prep);
     }
-}
-
-// Streams for the level set-up of many pairs (the library stream and three
-// more, default priority) and the host threads that feed them, made once per
-// process.
-struct PrepLanes {
-    hipStream_t side[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t ready = nullptr;
-    HostPool pool{4};
-};
-
-static PrepLanes &prep_lanes() {
-    static PrepLanes *lanes = [] {
-        auto *l = new PrepLanes();
-        for (auto &s : l->side) HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        HIP_OK(hipEventCreateWithFlags(&l->ready, hipEventDisableTiming));
-        return l;
-    }();
-    return *lanes;
 }
 
 static LevelInputs prepare_level(const Image &fixed, const Image &moving,
@@ -705,50 +693,39 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
             sig.push_back({s, s, s});
         }
         std::vector<Image *> smoothed = recursive_gaussian_batch(vols, sig);
-        // Each pair's set-up is some ten small launches (resampling, draws and
-        // their counting sort, fixed values, ranges); spread over four streams,
-        // each fed by its own host thread, their kernels overlap instead of
-        // queueing one after another and the launches are made four at a
-        // time. The side streams start after the smoothing; blocks freed
-        // meanwhile are held back until every stream has drained.
-        const hipStream_t main_stream = backend().stream;
-        PrepLanes &lanes = prep_lanes();
-        const size_t L = m >= 8 ? 4 : 1;
-        if (L > 1) {
-            HIP_OK(hipEventRecord(lanes.ready, main_stream));
-            for (size_t l = 1; l < L; ++l) HIP_OK(hipStreamWaitEvent(lanes.side[l - 1], lanes.ready, 0));
-        }
-        auto drain_lanes = [&] {
-            for (size_t l = 1; l < L; ++l) HIP_OK(hipStreamSynchronize(lanes.side[l - 1]));
-            sync();
-        };
-        device_defer_frees(L > 1);
+        // Each pair's set-up (placement, sample draws and their counting sort,
+        // fixed values, copies, value ranges, then fixed bins) is recorded and
+        // enqueued as one launch per kind of work for every pair.
+        // Blocks freed while the work is only recorded (a pair's smoothed fixed
+        // image once placed, say) are held back until it is enqueued, so no
+        // allocation made meanwhile reuses them before the work reads them.
+        PrepBatch prep;
         auto tpe = now();
+        device_defer_frees(true);
         try {
-            lanes.pool.run(static_cast<int>(L), [&](int l) {
-                StreamScope scope(l == 0 ? main_stream : lanes.side[l - 1]);
-                for (size_t k = static_cast<size_t>(l); k < m; k += L) {
-                    const size_t i = who[k];
-                    Image *sf = nullptr, *sm = nullptr;
-                    if (vol_of[k] != SIZE_MAX) {
-                        sf = smoothed[vol_of[k]];
-                        sm = smoothed[vol_of[k] + 1];
-                        smoothed[vol_of[k]] = smoothed[vol_of[k] + 1] = nullptr;
-                    }
-                    inputs[k] = prepare_level_begin(*fixed[i], *moving[i], cfg[i], level, sf, sm);
-                    metrics[k] = inputs[k].metric.get();
+            for (size_t k = 0; k < m; ++k) {
+                const size_t i = who[k];
+                Image *sf = nullptr, *sm = nullptr;
+                if (vol_of[k] != SIZE_MAX) {
+                    sf = smoothed[vol_of[k]];
+                    sm = smoothed[vol_of[k] + 1];
+                    smoothed[vol_of[k]] = smoothed[vol_of[k] + 1] = nullptr;
                 }
-            });
+                inputs[k] = prepare_level_begin(*fixed[i], *moving[i], cfg[i], level, sf, sm, &prep);
+                metrics[k] = inputs[k].metric.get();
+            }
+            prep.flush();
             tpe = now();
-            drain_lanes();
+            sync();
+            for (size_t k = 0; k < m; ++k) prepare_level_end(inputs[k], cfg[who[k]], &prep);
+            prep.flush();
         } catch (...) {
-            for (size_t l = 1; l < L; ++l) (void)hipStreamSynchronize(lanes.side[l - 1]);
-            (void)hipStreamSynchronize(main_stream);
+            (void)hipStreamSynchronize(backend().stream);
             device_defer_frees(false);
+            for (Image *img : smoothed) delete img;
             throw;
         }
         device_defer_frees(false);
-        for (size_t k = 0; k < m; ++k) prepare_level_end(inputs[k], cfg[who[k]]);
         const auto tp1 = now();
         // Two engines over alternate pairs once the batch is large: the host
         // updates one half's optimizers while the device evaluates the other

@@ -249,27 +249,11 @@ static void init_backend_locked(int device) {
     g_backend_ready = true;
 }
 
-// the stream a StreamScope set on this thread (nullptr: the library stream)
-static thread_local hipStream_t t_stream = nullptr;
-static thread_local Backend t_view;
-
 Backend &backend() {
     std::lock_guard<std::mutex> lk(g_backend_mu);
     if (!g_backend_ready) init_backend_locked(0);
     HIP_OK(hipSetDevice(g_backend.device));
-    return t_stream ? t_view : g_backend;
-}
-
-StreamScope::StreamScope(hipStream_t s) : prev_(t_stream) {
-    const Backend b = backend();
-    t_view = b;
-    t_view.stream = s;
-    t_stream = s;
-}
-
-StreamScope::~StreamScope() {
-    t_stream = prev_;
-    t_view.stream = prev_;
+    return g_backend;
 }
 
 void sync() { HIP_OK(hipStreamSynchronize(backend().stream)); }

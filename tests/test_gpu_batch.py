@@ -203,9 +203,10 @@ def test_batched_registration_matches_one_at_a_time(hip):
 
 def test_batched_pyramid_registrations_match_one_at_a_time(hip):
     """Twelve pairs with two-level pyramids (shrink 2 and sigma 1, then full
-    resolution): from eight pairs on, the levels are set up over several
-    streams, so this checks that set-up against the one-at-a-time path for
-    every sampling mode (device draws, a regular grid, every voxel)."""
+    resolution): every level of every pair is set up by one launch per kind of
+    work (placement, draws, gathers, ranges, fixed bins), so this checks that
+    set-up against the one-at-a-time path for every sampling mode (device
+    draws, a regular grid, every voxel)."""
     import workload as W
 
     rng = np.random.default_rng(17)
@@ -234,6 +235,49 @@ def test_batched_pyramid_registrations_match_one_at_a_time(hip):
         assert np.max(np.abs(got[i] - want)) < 1e-6, (i, got[i], want)
         assert res[i].iterations == r1.iterations, (i, res[i].iterations, r1.iterations)
         assert res[i].valid_points == r1.valid_points
+
+
+def test_batched_masked_registrations_match_one_at_a_time(hip):
+    """Masked and unmasked pairs in one batch with two-level pyramids: a pair
+    with a fixed mask is set up at once (its level mask placed by the
+    nearest-neighbour resample), one with only a moving mask through the
+    batched set-up (its mask copied there), the rest batched as well; every
+    pair registers as it does alone."""
+    import workload as W
+
+    rng = np.random.default_rng(23)
+    fixed, moving, configs, kinds, p0, centres, keep = [], [], [], [], [], [], []
+    for i in range(10):
+        shp = (24 + 2 * (i % 3), 28 - (i % 4), 22 + (i % 2))
+        vol = W.blob_volume(shp, (1.0, 1.0, 1.0), seed=130 + i, nblobs=16)
+        (ang, t), T_gt = W.random_euler(rng, 4.0, 2.0, vol.center())
+        mov = W.resample_linear(vol, T_gt)
+        fixed.append(hip.DeviceImage.upload(vol.data, spacing=vol.spacing, origin=vol.origin))
+        moving.append(hip.DeviceImage.upload(mov.data, spacing=mov.spacing, origin=mov.origin))
+        fm = mm = None
+        if i % 3 == 0:  # both masks
+            fm = hip.DeviceMask.upload((vol.data > np.percentile(vol.data, 20)).astype(np.uint8),
+                                       spacing=vol.spacing, origin=vol.origin)
+        if i % 3 != 2:  # a moving mask
+            mm = hip.DeviceMask.upload((mov.data > np.percentile(mov.data, 10)).astype(np.uint8),
+                                       spacing=mov.spacing, origin=mov.origin)
+        keep += [fm, mm]
+        configs.append(hip.registration_config(metric=hip.MATTES if i % 2 == 0 else hip.MEAN_SQUARES,
+                                               bins=24, sampling=2 if i % 4 != 3 else 0,
+                                               sampling_percentage=0.3, seed=3 + i,
+                                               minimum_step=1e-3, iterations=25,
+                                               learning_rate_mode=hip.LR_ONCE,
+                                               shrink_factors=((2, 2, 2), (1, 1, 1)),
+                                               sigmas=(1.0, 0.0), fixed_mask=fm, moving_mask=mm))
+        kinds.append(hip.EULER3D)
+        p0.append([0.0] * 6)
+        centres.append(vol.center())
+    got, res = hip.register_batch(fixed, moving, configs, kinds, p0, centres)
+    for i in range(len(fixed)):
+        want, r1 = hip.register(fixed[i], moving[i], configs[i], kinds[i], p0[i], center=centres[i])
+        assert np.max(np.abs(got[i] - want)) < 1e-6, (i, got[i], want)
+        assert res[i].iterations == r1.iterations, (i, res[i].iterations, r1.iterations)
+        assert res[i].valid_points == r1.valid_points, i
 
 
 def test_probe_values_are_the_zero_iteration_values(hip):
