@@ -476,9 +476,11 @@ static void run_once(BatchEngine &b) {
 static void set_control(BatchEngine &b, const double *stages, const uint8_t *active) {
     if (!stages) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null stages");
     for (size_t i = 0; i < b.n; ++i) {
+        const int on = active ? (active[i] ? 1 : 0) : 1;
+        b.ctl_host[i].active = on;
+        if (!on) continue;  // the kernels skip the pair: its stage is not read
         std::memcpy(b.ctl_host[i].stage, stages + 12 * i, 12 * sizeof(double));
         b.ctl_host[i].X = make_index_map(b.hdesc[i].F, b.hdesc[i].M, stages + 12 * i, 1);
-        b.ctl_host[i].active = active ? (active[i] ? 1 : 0) : 1;
     }
 }
 

@@ -802,11 +802,15 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
             batch_launch(*h.eng, h.stages.data(), h.active.data());
             h.pending = true;
         }
+        double t_wait = 0.0, t_update = 0.0, t_launch = 0.0;  // host time per phase (trace)
         for (bool busy = true; busy;) {
             busy = false;
             for (Half &h : half) {
                 if (!h.pending) continue;
+                const auto tw = trace ? now() : tp3;
                 batch_collect(*h.eng, h.out);
+                const auto tu = trace ? now() : tp3;
+                if (trace) t_wait += ms(tw, tu);
                 ++h.rounds;
                 bool any = false;
                 for (size_t j = 0; j < h.ks.size(); ++j) {
@@ -826,7 +830,12 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
                         h.active[j] = 0;
                     }
                 }
+                const auto tl = trace ? now() : tp3;
                 if (any) batch_launch(*h.eng, h.stages.data(), h.active.data());
+                if (trace) {
+                    t_update += ms(tu, tl);
+                    t_launch += ms(tl, now());
+                }
                 h.pending = any;
                 busy = busy || any;
             }
@@ -837,10 +846,10 @@ void register_batch(size_t n, const Image *const *fixed, const Image *const *mov
             const auto tp4 = now();
             std::fprintf(stderr,
                          "register_batch level %d: %zu pairs, prepare %.1f ms (enqueued at %.1f ms), "
-                         "batch %.1f ms, tune %.1f ms, %d evaluations %.1f ms (%.3f ms each)\n",
+                         "batch %.1f ms, tune %.1f ms, %d evaluations %.1f ms (%.3f ms each; host "
+                         "waits %.1f, updates %.1f, launches %.1f ms)\n",
                          level, m, ms(tp0, tp1), ms(tp0, tpe), ms(tp1, tp2), ms(tp2, tp3), rounds,
-                         ms(tp3, tp4),
-                         ms(tp3, tp4) / rounds);
+                         ms(tp3, tp4), ms(tp3, tp4) / rounds, t_wait, t_update, t_launch);
         }
         for (size_t k = 0; k < m; ++k) {
             const size_t i = who[k];
