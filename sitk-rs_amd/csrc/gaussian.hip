@@ -593,12 +593,11 @@ static void launch_axis_batch(const std::vector<const SrcT *> &src, const std::v
     dv.upload(hv.data(), n);
     dmap.upload(map.data(), map.size());
     static const char *names[3] = {"gauss_iir_axis0", "gauss_iir_axis1", "gauss_iir_axis2"};
-    LaunchScope scope(names[axis]);
     if (lds > 64 * 1024)
         HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void *>(&iir_pair_lines_batch<SrcT, DstT>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    hipLaunchKernelGGL((iir_pair_lines_batch<SrcT, DstT>), dim3(blocks), dim3(64), lds, backend().stream,
-                       dv.ptr, dmap.ptr);
+    launch_kernel(names[axis], iir_pair_lines_batch<SrcT, DstT>, dim3(blocks), dim3(64), lds,
+                  static_cast<const BatchLines *>(dv.ptr), static_cast<const int *>(dmap.ptr));
     check_launch(names[axis]);
 }
 
