@@ -95,8 +95,16 @@ __device__ __forceinline__ OneStage load_stage(const PairControl &c) {
 
 // Pass 1: Mattes tiles scatter into an LDS histogram and fold it into their
 // pair's global histogram; mean-squares tiles reduce their 14 moments.
+#ifndef SITKHIP_BATCH_WAVES
+#define SITKHIP_BATCH_WAVES 0
+#endif
+#if SITKHIP_BATCH_WAVES > 0
+#define SITKHIP_BATCH_OCC __attribute__((amdgpu_waves_per_eu(SITKHIP_BATCH_WAVES)))
+#else
+#define SITKHIP_BATCH_OCC
+#endif
 template <typename R>
-__global__ __launch_bounds__(kBlock) void batch_pass1(const PairDesc *__restrict__ D,
+__global__ __launch_bounds__(kBlock) SITKHIP_BATCH_OCC void batch_pass1(const PairDesc *__restrict__ D,
                                                       const Tile *__restrict__ T, Segments seg,
                                                       const PairControl *__restrict__ C) {
     extern __shared__ unsigned long long s_hist[];

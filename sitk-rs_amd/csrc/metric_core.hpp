@@ -214,16 +214,22 @@ struct Parzen<float> {
     // 1/binSize into every sample alike -- a stretch of the moving bin axis
     // that does not average out (2e-6 of the value at 2M samples where MI is
     // small); rounding only the arguments leaves independent errors.
+    // The arguments (start + k) - term = (k - 1) - u with u = term - idx: the
+    // FP64 difference is exact (term and idx are within a factor of two) and
+    // rounded to float once per sample, then each argument is one float
+    // subtraction -- still independent roundings per sample.
     __device__ __forceinline__ static int taps(const MattesParams &P, float mv, float arg[4],
                                                bool &mid) {
         const double term = fma(static_cast<double>(mv) - P.mmin, P.inv_mbs, 2.0);
         int idx = static_cast<int>(term);
         mid = idx >= 2 && idx <= P.bins - 3;
         idx = idx < 2 ? 2 : (idx > P.bins - 3 ? P.bins - 3 : idx);
-        const int start = idx - 1;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) arg[k] = static_cast<float>(static_cast<double>(start + k) - term);
-        return start;
+        const float u = static_cast<float>(term - static_cast<double>(idx));
+        arg[0] = -1.0f - u;
+        arg[1] = -u;
+        arg[2] = 1.0f - u;
+        arg[3] = 2.0f - u;
+        return idx - 1;
     }
     __device__ __forceinline__ static bool in_range(const MattesParams &P, float mv) {
         return !(mv < P.mmin_f || mv > P.mmax_f);
@@ -244,10 +250,11 @@ __device__ __forceinline__ unsigned long long to_fixed(R v) {
 constexpr double kFixSixth = 4294967296.0 / 6.0;
 constexpr float kFixSixthHi = static_cast<float>(kFixSixth);
 constexpr float kFixSixthLo = static_cast<float>(kFixSixth - static_cast<double>(kFixSixthHi));
+// (a B3 weight is never negative; the max keeps the conversion defined)
 template <>
 __device__ __forceinline__ unsigned long long to_fixed<float>(float v) {
-    return v > 0.0f ? static_cast<unsigned long long>(__float2uint_rn(fmaf(v, kFixSixthHi, v * kFixSixthLo)))
-                    : 0ull;
+    v = fmaxf(v, 0.0f);
+    return static_cast<unsigned long long>(__float2uint_rn(fmaf(v, kFixSixthHi, v * kFixSixthLo)));
 }
 // ------------------------------------------------------------ FP64 reduction
 // A double moved across lanes by DPP (two 32-bit halves); lanes with no
@@ -730,6 +737,20 @@ __device__ __forceinline__ void deferred_sample(const FixedGrid &F, const Moving
     hist_sample<R, REC, BSP>(P, joint, marg, rec, s, ok, fb, fr, v, &M, cell.base);
 }
 
+// sum_k row[k] * dv[k] over the four taps: FP64 in the reference's order,
+// every product and sum rounded; FP32 as fused multiply-adds.
+template <typename R>
+__device__ __forceinline__ R tap_sum(const R *row, const R dv[4]) {
+    R w = R(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w += row[k] * dv[k];
+    return w;
+}
+template <>
+__device__ __forceinline__ float tap_sum<float>(const float *row, const float dv[4]) {
+    return fmaf(row[3], dv[3], fmaf(row[2], dv[2], fmaf(row[1], dv[1], row[0] * dv[0])));
+}
+
 // Mattes derivative pass: per valid sample w = sum_taps pRatio * B3'(arg)
 // and the 12 moments A[d] += w dM_d, B[d][e] += w dM_d x_e
 // (src/cuda/ops/mattes.rs:75-91).
@@ -765,9 +786,7 @@ __device__ __forceinline__ void deriv_range(const FixedGrid &F, const MovingVol 
             const int m0 = Parzen<R>::taps(P, mv, arg, mid);
             parzen_derivs<R>(arg, mid, dv);
             const R *row = pr + b.fb[u] * B + m0;
-            R w = R(0);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) w += row[k] * dv[k];
+            const R w = tap_sum<R>(row, dv);
             const double wd = static_cast<double>(w);
             double x[3];
             moment_point<R, MapTraits<Map>::points>(F, C, b.flat[u], x);
@@ -838,9 +857,7 @@ __device__ __forceinline__ void deriv_records(const FixedGrid &F, const MovingVo
             const int m0 = Parzen<R>::taps(P, q.r[u].mv, arg, mid);
             parzen_derivs<R>(arg, mid, dv);
             const R *row = pr + q.fb[u] * B + m0;
-            R w = R(0);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) w += row[k] * dv[k];
+            const R w = tap_sum<R>(row, dv);
             const double wd = static_cast<double>(w);
             double x[3];
             if (F.pts)

@@ -377,8 +377,12 @@ __device__ __forceinline__ void corners(const MovingVol &M, const Cell &cell, fl
         const int p = min(max(b0, 0), n0 - 2);
         const bool both_hi = b0 == n0 - 1;
         const bool both_lo = b0 < 0;
-        const int rows[4] = {y0 * s1 + z0 * s2 + p, y1 * s1 + z0 * s2 + p, y0 * s1 + z1 * s2 + p,
-                             y1 * s1 + z1 * s2 + p};
+        // non-negative (clamped) offsets below 2^31: unsigned, so the address
+        // is base + zero-extended offset (no sign extension per load)
+        const unsigned rows[4] = {static_cast<unsigned>(y0 * s1 + z0 * s2 + p),
+                                  static_cast<unsigned>(y1 * s1 + z0 * s2 + p),
+                                  static_cast<unsigned>(y0 * s1 + z1 * s2 + p),
+                                  static_cast<unsigned>(y1 * s1 + z1 * s2 + p)};
         FloatPair q[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) q[k] = *reinterpret_cast<const FloatPair *>(M.vol + rows[k]);
@@ -607,17 +611,13 @@ __device__ __forceinline__ void physical_gradient(const MovingVol &M, const R gi
     }
 }
 
+// FP32: the same sum with fused multiply-adds (three instructions per axis)
 template <>
 __device__ __forceinline__ void physical_gradient<float>(const MovingVol &M, const float gi[3],
                                                          float gp[3]) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        float acc = -0.0f;
-        acc += gi[0] * M.p2i_f[0 * 3 + d];
-        acc += gi[1] * M.p2i_f[1 * 3 + d];
-        acc += gi[2] * M.p2i_f[2 * 3 + d];
-        gp[d] = acc;
-    }
+    for (int d = 0; d < 3; ++d)
+        gp[d] = fmaf(gi[2], M.p2i_f[2 * 3 + d], fmaf(gi[1], M.p2i_f[1 * 3 + d], gi[0] * M.p2i_f[0 * 3 + d]));
 }
 
 }  // namespace sitkhip
