@@ -189,11 +189,18 @@ __global__ __launch_bounds__(kTail) void mattes_tail_split(const unsigned long l
                                                             unsigned long long *part) {
     __shared__ TailShared sh;
     const int t = threadIdx.x, nbin = B * B, T = static_cast<int>(gridDim.x);
-    const TailStats st = tail_stats_marginals(g_marg, g_col, B, mbs, sh);
     const int per = (nbin + T - 1) / T;
     const int i0 = static_cast<int>(blockIdx.x) * per;
     const int i1 = i0 + per < nbin ? i0 + per : nbin;
-    const __int128 sum = tail_bins_copies(g_joint, B, i0, i1, st, pratio, pratio_f, sh);
+    // the lane's first bin is loaded with the marginals: one round trip to
+    // memory instead of two
+    unsigned long long qx[kHistCopies];
+    first_bin_copies(g_joint, B, i0, i1, qx);
+    const TailStats st = tail_stats_marginals(g_marg, g_col, B, mbs, sh);
+    unsigned long long q0 = 0;
+#pragma unroll
+    for (int x = 0; x < kHistCopies; ++x) q0 += qx[x];
+    const __int128 sum = tail_bins_copies(g_joint, B, i0, i1, st, pratio, pratio_f, sh, &q0);
     if (t == 0) {
         part[2 * blockIdx.x] = static_cast<unsigned long long>(sum);
         part[2 * blockIdx.x + 1] = static_cast<unsigned long long>(sum >> 64);

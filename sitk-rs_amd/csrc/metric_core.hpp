@@ -1056,11 +1056,18 @@ __device__ __forceinline__ TailStats tail_stats_marginals(const unsigned long lo
         sh.colsum[t] = cc;
     }
     __syncthreads();
+    // every wave adds the B entries across its lanes (exact integers)
     unsigned long long tot = 0, valid = 0;
-    for (int i = 0; i < B; ++i) {
+    for (int i = t & 63; i < B; i += 64) {
         tot += sh.colsum[i];
         valid += sh.fcount[i];
     }
+    tot = wave_total_u64(tot);
+    valid = wave_total_u64(valid);
+    tot = (static_cast<unsigned long long>(static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(tot >> 32), 63))) << 32) |
+          static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned>(tot)), 63));
+    valid = (static_cast<unsigned long long>(static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(valid >> 32), 63))) << 32) |
+            static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned>(valid)), 63));
     TailStats st;
     st.valid = valid;
     st.joint_sum = static_cast<double>(tot) * kInvFix;
@@ -1082,17 +1089,21 @@ __device__ __forceinline__ TailStats tail_stats_marginals(const unsigned long lo
 // the barrier inside).
 // COPIES: hj holds the kHistCopies unfolded copies (B^2 apart), summed here
 // bin by bin.
+// `first`, when given, is the lane's first bin (i0 + lane) already summed by
+// the caller.
 template <bool COPIES = false>
 __device__ __forceinline__ __int128 tail_bins(const unsigned long long *hj, int B, int i0, int i1,
                                               const TailStats &st, double *pratio, float *pratio_f,
-                                              TailShared &sh) {
+                                              TailShared &sh, const unsigned long long *first = nullptr) {
     constexpr int kWaves = kTail / 64;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     __int128 part = 0;
     for (int i = i0 + t; i < i1; i += kTail) {
         double pr_out = 0.0;
         unsigned long long q = 0;
-        if constexpr (COPIES) {
+        if (first && i == i0 + t) {
+            q = *first;
+        } else if constexpr (COPIES) {
 #pragma unroll
             for (int x = 0; x < kHistCopies; ++x) q += hj[static_cast<size_t>(x) * B * B + i];
         } else {
@@ -1127,8 +1138,21 @@ __device__ __forceinline__ __int128 tail_bins(const unsigned long long *hj, int 
 
 __device__ __forceinline__ __int128 tail_bins_copies(const unsigned long long *g_joint, int B, int i0,
                                                      int i1, const TailStats &st, double *pratio,
-                                                     float *pratio_f, TailShared &sh) {
-    return tail_bins<true>(g_joint, B, i0, i1, st, pratio, pratio_f, sh);
+                                                     float *pratio_f, TailShared &sh,
+                                                     const unsigned long long *first = nullptr) {
+    return tail_bins<true>(g_joint, B, i0, i1, st, pratio, pratio_f, sh, first);
+}
+// The copies of the lane's first bin of [i0, i1) (zeros past i1), loaded
+// only: the caller sums them after issuing its other loads.
+__device__ __forceinline__ void first_bin_copies(const unsigned long long *g_joint, int B, int i0, int i1,
+                                                 unsigned long long (&q)[kHistCopies]) {
+    const int i = i0 + static_cast<int>(threadIdx.x);
+#pragma unroll
+    for (int x = 0; x < kHistCopies; ++x) q[x] = 0;
+    if (i < i1) {
+#pragma unroll
+        for (int x = 0; x < kHistCopies; ++x) q[x] = g_joint[static_cast<size_t>(x) * B * B + i];
+    }
 }
 
 __device__ __forceinline__ double tail_value(__int128 sum, bool degenerate) {
