@@ -669,6 +669,11 @@ int sitkhip_cnn_set_input(sitkhip_cnn net, int slot, sitkhip_image fixed, sitkhi
 /* The same from two host 64^3 float volumes (x fastest). */
 int sitkhip_cnn_set_input_raw(sitkhip_cnn net, int slot, const float *fixed64,
                               const float *moving64);
+/* Slots first_slot .. first_slot + n - 1 from n image pairs (fixed[b],
+ * moving[b]) as sitkhip_cnn_set_input does for each, the whole set in four
+ * launches (the 2n placements, the normalisations, the packing). */
+int sitkhip_cnn_set_inputs(sitkhip_cnn net, int first_slot, int n, const sitkhip_image *fixed,
+                           const sitkhip_image *moving);
 /* Forward pass over slots 0..batch-1; out: batch x 12 (may be NULL). */
 int sitkhip_cnn_forward(sitkhip_cnn net, int batch, float *out);
 /* Backward pass of the last forward from dL/dout (batch x 12). */

@@ -588,10 +588,14 @@ __global__ void cast_weights(const float *__restrict__ w, int co, int ci, uint16
     }
 }
 
-// Sum and sum of squares of a volume (FP64 partials per workgroup).
+// Sum and sum of squares of a volume (FP64 partials per workgroup); volume
+// blockIdx.y of consecutive volumes of n voxels, its partials 2 * gridDim.x
+// apart.
 __global__ __launch_bounds__(256) void moments_partials(const float *__restrict__ v, size_t n,
                                                         double *__restrict__ out) {
     __shared__ double s1[256], s2[256];
+    v += static_cast<size_t>(blockIdx.y) * n;
+    out += static_cast<size_t>(blockIdx.y) * 2 * gridDim.x;
     double a = 0.0, b = 0.0;
     for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * 256) {
@@ -617,10 +621,15 @@ __global__ __launch_bounds__(256) void moments_partials(const float *__restrict_
 
 // One input slot: channel `ch` of [64^3][8] = (v - mean) * inv_std as BF16;
 // channels 2..7 are zero.
-// The channel's normalisation from the 256 partial (sum, sum of squares)
-// pairs, added in order by one lane: norm = {mean, 1 / std} as floats.
-__global__ void moments_finish(const double *__restrict__ part, size_t nv, float *__restrict__ norm) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Each volume's normalisation from its 256 partial (sum, sum of squares)
+// pairs, added in order by one lane (lane t: volume t of nvol): norm[2t..] =
+// {mean, 1 / std} as floats.
+__global__ void moments_finish(const double *__restrict__ part, size_t nv, float *__restrict__ norm,
+                               int nvol) {
+    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= nvol) return;
+    part += static_cast<size_t>(t) * 512;
+    norm += 2 * t;
     double s1 = 0.0, s2 = 0.0;
     for (int b = 0; b < 256; ++b) {
         s1 += part[2 * b];
@@ -632,16 +641,23 @@ __global__ void moments_finish(const double *__restrict__ part, size_t nv, float
     norm[1] = static_cast<float>(1.0 / sqrt(var));
 }
 
-__global__ void pack_channel(const float *__restrict__ v, const float *__restrict__ norm, int ch,
-                             uint16_t *__restrict__ slot) {
+// Slots first .. first + gridDim.y - 1 from their two volumes (fixed, moving:
+// volumes 2b and 2b + 1 of `v`, normalisations 4b .. 4b + 3): voxel i of
+// slot b is {(f - mean_f) / std_f, (m - mean_m) / std_m, 0, ..., 0} as BF16.
+__global__ void pack_slots(const float *__restrict__ v, const float *__restrict__ norm,
+                           uint16_t *__restrict__ slots) {
+    constexpr int nv = kEdge * kEdge * kEdge;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kEdge * kEdge * kEdge) return;
-    const float mean = norm[0], inv_std = norm[1];
-    slot[static_cast<size_t>(i) * kCin0 + ch] = f2bf((v[i] - mean) * inv_std);
-    if (ch == 0) {
+    if (i >= nv) return;
+    const int b = blockIdx.y;
+    const float *f = v + static_cast<size_t>(2 * b) * nv;
+    const float *m = f + nv;
+    const float *q = norm + 4 * b;
+    uint16_t *o = slots + (static_cast<size_t>(b) * nv + i) * kCin0;
+    o[0] = f2bf((f[i] - q[0]) * q[1]);
+    o[1] = f2bf((m[i] - q[2]) * q[3]);
 #pragma unroll
-        for (int c = 2; c < kCin0; ++c) slot[static_cast<size_t>(i) * kCin0 + c] = 0;
-    }
+    for (int c = 2; c < kCin0; ++c) o[c] = 0;
 }
 
 // --------------------------------------------------------------- the net
@@ -905,26 +921,26 @@ void adam(Net &net, float lr, float b1, float b2, float eps) {
     sync();
 }
 
-// Input slot `slot` from two 64^3 float volumes already on the device.
-static void pack(Net &net, int slot, const float *fixed64, const float *moving64) {
-    if (slot < 0 || slot >= net.max_batch) fail(SITKHIP_ERR_INVALID_ARGUMENT, "slot out of range");
+// Input slots first .. first + n - 1 from 2n 64^3 float volumes already on
+// the device, consecutive: slot b's fixed volume is volume 2b, its moving
+// volume 2b + 1. The normalisations are computed on the device (no host round
+// trip), every volume's in one launch.
+static void pack(Net &net, int first, int n, const float *vols) {
+    if (first < 0 || n < 1 || first + n > net.max_batch)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "slot out of range");
     const size_t nv = static_cast<size_t>(kEdge) * kEdge * kEdge;
-    uint16_t *dst = net.act[0].ptr + static_cast<size_t>(slot) * nv * kCin0;
-    const float *src[2] = {fixed64, moving64};
-    // the normalisation is computed on the device: no host round trip
-    DeviceBuffer<double> part(2 * 256);
-    DeviceBuffer<float> norm(2);
-    for (int ch = 0; ch < 2; ++ch) {
-        hipLaunchKernelGGL(moments_partials, dim3(256), dim3(256), 0, backend().stream, src[ch], nv,
-                           part.ptr);
-        check_launch("moments_partials");
-        hipLaunchKernelGGL(moments_finish, dim3(1), dim3(64), 0, backend().stream, part.ptr, nv,
-                           norm.ptr);
-        check_launch("moments_finish");
-        hipLaunchKernelGGL(pack_channel, dim3((nv + 255) / 256), dim3(256), 0, backend().stream,
-                           src[ch], norm.ptr, ch, dst);
-        check_launch("pack_channel");
-    }
+    uint16_t *dst = net.act[0].ptr + static_cast<size_t>(first) * nv * kCin0;
+    DeviceBuffer<double> part(static_cast<size_t>(2 * n) * 2 * 256);
+    DeviceBuffer<float> norm(static_cast<size_t>(2 * n) * 2);
+    hipLaunchKernelGGL(moments_partials, dim3(256, static_cast<unsigned>(2 * n)), dim3(256), 0,
+                       backend().stream, vols, nv, part.ptr);
+    check_launch("moments_partials");
+    hipLaunchKernelGGL(moments_finish, dim3(static_cast<unsigned>((2 * n + 63) / 64)), dim3(64), 0,
+                       backend().stream, part.ptr, nv, norm.ptr, 2 * n);
+    check_launch("moments_finish");
+    hipLaunchKernelGGL(pack_slots, dim3(static_cast<unsigned>((nv + 255) / 256), static_cast<unsigned>(n)),
+                       dim3(256), 0, backend().stream, vols, norm.ptr, dst);
+    check_launch("pack_slots");
 }
 
 // The 64^3 grid spanning an image's physical extent (same origin corner and
@@ -945,20 +961,40 @@ static sitkhip_geometry grid64(const sitkhip_geometry &g) {
     return o;
 }
 
+// Slots first .. first + n - 1 from n image pairs: every image linearly
+// resampled (identity map) onto the 64^3 grid spanning its own extent -- all
+// 2n placements in one launch -- then z-scored and packed.
+void set_inputs(Net &net, int first, int n, const Image *const *fixed, const Image *const *moving) {
+    if (first < 0 || n < 1 || first + n > net.max_batch)
+        fail(SITKHIP_ERR_INVALID_ARGUMENT, "slot out of range");
+    const size_t nv = static_cast<size_t>(kEdge) * kEdge * kEdge;
+    DeviceBuffer<float> vols(static_cast<size_t>(2 * n) * nv);
+    PrepBatch prep;
+    for (int b = 0; b < n; ++b) {
+        for (int ch = 0; ch < 2; ++ch) {
+            const Image *im = ch ? moving[b] : fixed[b];
+            if (!im) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null image");
+            const sitkhip_geometry og = grid64(im->geom);
+            prep.places.push_back({im->buf.ptr, im->geom, vols.ptr + static_cast<size_t>(2 * b + ch) * nv, og});
+        }
+    }
+    enqueue_placements(prep);
+    pack(net, first, n, vols.ptr);
+    sync();  // the placement descriptors' host copies live as long as `prep`
+}
+
 void set_input(Net &net, int slot, const Image &fixed, const Image &moving) {
-    // enqueued only: the buffers are freed after the pack is enqueued, and a
-    // later use of the same blocks is ordered after it on the stream
-    std::unique_ptr<Image> f(resample_linear(fixed, grid64(fixed.geom), nullptr, 0, 0.0, false));
-    std::unique_ptr<Image> m(resample_linear(moving, grid64(moving.geom), nullptr, 0, 0.0, false));
-    pack(net, slot, f->buf.ptr, m->buf.ptr);
+    const Image *f = &fixed, *m = &moving;
+    set_inputs(net, slot, 1, &f, &m);
 }
 
 void set_input_raw(Net &net, int slot, const float *fixed_host, const float *moving_host) {
     const size_t nv = static_cast<size_t>(kEdge) * kEdge * kEdge;
-    DeviceBuffer<float> f(nv), m(nv);
-    f.upload(fixed_host, nv);
-    m.upload(moving_host, nv);
-    pack(net, slot, f.ptr, m.ptr);
+    DeviceBuffer<float> v(2 * nv);
+    HIP_OK(hipMemcpyAsync(v.ptr, fixed_host, nv * sizeof(float), hipMemcpyHostToDevice, backend().stream));
+    HIP_OK(hipMemcpyAsync(v.ptr + nv, moving_host, nv * sizeof(float), hipMemcpyHostToDevice,
+                          backend().stream));
+    pack(net, slot, 1, v.ptr);
     sync();  // the caller's host arrays may go once this returns
 }
 
@@ -1026,6 +1062,15 @@ int sitkhip_cnn_set_input(sitkhip_cnn h, int slot, sitkhip_image fixed, sitkhip_
         auto m = reinterpret_cast<Image *>(moving);
         if (!f || !m) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null image");
         cnn::set_input(net_ref(h), slot, *f, *m);
+    });
+}
+
+int sitkhip_cnn_set_inputs(sitkhip_cnn h, int first_slot, int n, const sitkhip_image *fixed,
+                           const sitkhip_image *moving) {
+    return run_guarded([&] {
+        if (!fixed || !moving) fail(SITKHIP_ERR_INVALID_ARGUMENT, "null image array");
+        cnn::set_inputs(net_ref(h), first_slot, n, reinterpret_cast<const Image *const *>(fixed),
+                        reinterpret_cast<const Image *const *>(moving));
     });
 }
 

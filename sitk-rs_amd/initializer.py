@@ -82,8 +82,7 @@ def predict(net: S.CNN, pairs) -> list:
     out = []
     for i in range(0, len(pairs), net.max_batch):
         chunk = pairs[i:i + net.max_batch]
-        for b, (f, m, _) in enumerate(chunk):
-            net.set_input(b, f, m)
+        net.set_inputs(0, [f for f, _, _ in chunk], [m for _, m, _ in chunk])
         y = net.forward(len(chunk))
         out.extend(affine_of(y[b], chunk[b][2]) for b in range(len(chunk)))
     return out

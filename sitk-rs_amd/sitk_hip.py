@@ -299,6 +299,7 @@ def lib():
         L.sitkhip_cnn_get_grads.argtypes = [vp, fp, C.c_size_t]
         L.sitkhip_cnn_set_input.argtypes = [vp, C.c_int, vp, vp]
         L.sitkhip_cnn_set_input_raw.argtypes = [vp, C.c_int, fp, fp]
+        L.sitkhip_cnn_set_inputs.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp), C.POINTER(vp)]
         L.sitkhip_cnn_forward.argtypes = [vp, C.c_int, fp]
         L.sitkhip_cnn_backward.argtypes = [vp, C.c_int, fp]
         L.sitkhip_cnn_adam.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
@@ -994,6 +995,15 @@ class CNN:
 
     def set_input(self, slot: int, fixed: DeviceImage, moving: DeviceImage) -> None:
         _check(lib().sitkhip_cnn_set_input(self._h, slot, fixed._h, moving._h))
+
+    def set_inputs(self, first: int, fixed, moving) -> None:
+        """Slots first .. first + len(fixed) - 1 from image pairs, in four launches."""
+        n = len(fixed)
+        if len(moving) != n:
+            raise ValueError("one moving image per fixed image")
+        fa = (C.c_void_p * n)(*[f._h for f in fixed])
+        ma = (C.c_void_p * n)(*[m._h for m in moving])
+        _check(lib().sitkhip_cnn_set_inputs(self._h, first, n, fa, ma))
 
     def set_input_raw(self, slot: int, fixed64, moving64) -> None:
         f = np.ascontiguousarray(fixed64, dtype=np.float32)

@@ -65,6 +65,45 @@ def _close(dev, ref, rel_step=2.0 ** -8, floor=1e-3):
     return not bad.any(), float(np.max(np.abs(dev - ref) - tol))
 
 
+def test_batched_inputs_are_the_per_pair_inputs(hip):
+    """set_inputs packs n pairs in four launches; every slot must hold what
+    set_input writes for that pair alone, and that is the general resampler's
+    output (identity map onto the 64^3 grid spanning the image) z-scored, as
+    set_input_raw packs it from host volumes."""
+    rng = np.random.default_rng(5)
+    shapes = [((40, 36, 30), (1.1, 0.9, 1.3)), ((33, 47, 29), (0.8, 1.0, 1.2)),
+              ((64, 64, 64), (1.0, 1.0, 1.0))]
+    fixed, moving = [], []
+    for shp, sp in shapes:
+        for lst in (fixed, moving):
+            v = rng.normal(100.0, 20.0, shp[::-1]).astype(np.float32)
+            lst.append(hip.DeviceImage.upload(v, spacing=sp, origin=tuple(rng.uniform(-5, 5, 3))))
+    n = len(shapes)
+    net = hip.CNN(max_batch=n + 1, seed=3)
+    for b in range(n):
+        net.set_input(b, fixed[b], moving[b])
+    one = net.activation(0, n)
+    net.set_inputs(0, fixed, moving)
+    many = net.activation(0, n)
+    assert np.array_equal(one.view(np.uint32), many.view(np.uint32))
+    # a set starting at slot 1
+    net.set_inputs(1, fixed[:1], moving[:1])
+    assert np.array_equal(net.activation(0, 2)[1].view(np.uint32), many[0].view(np.uint32))
+    # the general resampler's volumes packed from the host
+    vols = []
+    for img in (fixed[2], moving[2]):
+        g = img.geometry()
+        sp64 = [g.spacing[d] * g.size[d] / 64.0 for d in range(3)]
+        org = [g.origin[d] + 0.5 * (sp64[d] - g.spacing[d]) for d in range(3)]
+        r = hip.resample_linear(img, hip.Geometry.make((64, 64, 64), sp64, org))
+        vols.append(r.to_host())
+        r.free()
+    net.set_input_raw(0, vols[0], vols[1])
+    assert np.array_equal(net.activation(0, 1)[0].view(np.uint32), many[2].view(np.uint32))
+    for img in fixed + moving:
+        img.free()
+
+
 def test_input_packing_is_z_scored_bf16(net_run):
     x = net_run["acts"][0]
     assert x.shape == (B, 64, 64, 64, 8)
